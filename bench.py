@@ -1,0 +1,194 @@
+"""Exact triangle counting throughput on one MI355X.
+
+Default workload: R-MAT scale 22, edge factor 16, (a, b, c, d) = (0.57, 0.19,
+0.19, 0.05), seed 1, symmetrised, deduplicated, self-loops removed, ids
+compacted and relabelled by descending degree.  One step = one exact count
+of the whole graph by the hybrid engine (dense-tile kernel + wedge kernel)
+with the graph and its work lists already resident in HBM; every step's
+count is checked against the first and, once, against the CPU baseline.
+
+Prints progress to stderr and ONE JSON line to stdout.
+
+  python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--workload rmat22]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "btc-tc-artifact_amd"))
+
+METRIC = "exact-TC edges/s at R-MAT s22 (1 GPU); batched graphs/s + p99 at fixed QPS"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_workload(btc, name):
+    if name.startswith("rmat"):
+        scale = int(name[4:] or 22)
+        src, dst = btc.gen_rmat(scale, 16, 0.57, 0.19, 0.19, seed=1)
+        return src, dst, 1 << scale, {"workload": "rmat-s%d-ef16" % scale, "scale": scale, "edge_factor": 16,
+                                       "abcd": [0.57, 0.19, 0.19, 0.05], "seed": 1}
+    if name == "er":
+        src, dst = btc.gen_er(16384, 16.0, seed=1)
+        return src, dst, 16384, {"workload": "er-n16384-d16", "n": 16384, "avg_degree": 16, "seed": 1}
+    if name == "stencil":
+        src, dst = btc.gen_stencil3d(64, 64, 64)
+        return src, dst, 64 ** 3, {"workload": "stencil27-64x64x64"}
+    if name == "planted":
+        src, dst = btc.gen_planted(1 << 17, 128, 0.5, 4.0, seed=1)
+        return src, dst, 1 << 17, {"workload": "planted-2^17-c128-p0.5-noise4", "seed": 1}
+    raise SystemExit("unknown workload " + name)
+
+
+def cpu_baseline(g, gpu_total):
+    """The reference's own CPU verifier (oracle/_ref) on the same canonical L;
+    the oracle's C port when the reference build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    threads = int(os.environ.get("BTC_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    rowptr, col = g.csr()
+    t0 = time.perf_counter()
+    if oracle.ref_available():
+        kind = "reference"
+        total = oracle.ref_cpu_tc(rowptr, col, threads)
+    else:
+        kind = "port"
+        total = oracle.tc_merge(rowptr, col, threads)
+    dt = time.perf_counter() - t0
+    return {"value": g.m / dt, "unit": "edges/s", "cores": threads, "kind": kind,
+            "sample": "whole graph, same degree-ordered L as the GPU, one run (%.1f s)" % dt,
+            "seconds": dt, "total": int(total), "matches_gpu": int(total) == int(gpu_total)}
+
+
+def load_traffic(workload):
+    """HBM bytes per launch of the wedge kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration); None when no summary exists for this workload."""
+    path = os.path.join(ROOT, "profiles", "wedge_kernel_%s_traffic.json" % workload)
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path))["hbm_bytes_per_launch"]
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rmat22")
+    ap.add_argument("--tile-k", type=int, default=0)
+    ap.add_argument("--tau", type=int, default=-1)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--plain-runs", type=int, default=3, help="reference-semantics Bit-BSR yardstick runs (0 = skip)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the engine shards by row-block ranges by design, see DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import btc_amd as btc
+
+    rc = btc.lib().btc_device_init(0)
+    if rc != 0:
+        raise SystemExit("no HIP device: " + btc.lib().btc_last_error().decode())
+
+    t0 = time.perf_counter()
+    src, dst, n_in, config = make_workload(btc, args.workload)
+    log("generated %d pairs in %.2f s" % (src.size, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    g = btc.Graph.from_coo(src, dst, n_in)
+    canon_ms = (time.perf_counter() - t0) * 1e3
+    del src, dst
+    gi = g.info
+    log("canonical L: n=%d m=%d max_row=%d wedges=%d (%.0f ms)" % (gi["n"], gi["m"], gi["max_row"], gi["wedges"],
+                                                                   canon_ms))
+    t0 = time.perf_counter()
+    plan = btc.Plan(g, tile_k=args.tile_k, tau=args.tau, chunk_work=args.chunk)
+    build_ms = (time.perf_counter() - t0) * 1e3
+    pi = plan.info
+    log("plan: %s (%.0f ms)" % (pi, build_ms))
+
+    total0 = None
+    for _ in range(args.warmup):
+        total0, _, _ = plan.count()
+    kms, wms, totals = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t, _, ms = plan.count()
+        totals.append(t)
+        kms.append(ms)
+        wms.append(plan.last_timing()[1])
+    wall = time.perf_counter() - t0
+    if total0 is None:
+        total0 = totals[0]
+    assert all(t == total0 for t in totals), "count changed between steps"
+    ms_per_step = wall * 1e3 / args.steps
+    value = gi["m"] / (wall / args.steps)
+    log("engine: %d triangles, %.3f ms/step wall, kernel median %.3f ms, wedge kernel mean %.3f ms"
+        % (total0, ms_per_step, statistics.median(kms), statistics.mean(wms)))
+
+    # roofline of the dominant kernel (wedge kernel): merge-stream bytes of
+    # its edges, B = 4(n+1) + 4 m_s + 4 W_s, over its mean event time
+    wedge_ms = statistics.mean(wms)
+    b_alg = 4 * (gi["n"] + 1) + 4 * pi["sparse_edges"] + 4 * pi["sparse_merge_work"]
+    achieved = b_alg / (wedge_ms * 1e-3) / 1e9 if wedge_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(config["workload"]),
+                "kernel": "wedge_kernel", "kernel_ms": round(wedge_ms, 4), "alg_bytes": b_alg,
+                "stream_bytes": pi["stream_bytes"],
+                "stream_GBps": round(pi["stream_bytes"] / (wedge_ms * 1e-3) / 1e9, 1) if wedge_ms > 0 else None}
+
+    plain = None
+    if args.plain_runs > 0:
+        K = 128 if gi["n"] >= 50000 else btc.lite_tile_k(gi["n"], gi["m"])
+        t0 = time.perf_counter()
+        b = btc.BitBSR(g, K)
+        bms = (time.perf_counter() - t0) * 1e3
+        b.count()  # warmup
+        pms = []
+        for _ in range(args.plain_runs):
+            t, _, ms = b.count()
+            assert t == total0, ("reference-semantics path disagrees", t, total0)
+            pms.append(ms)
+        pmed = statistics.median(pms)
+        plain = {"tile_k": K, "kernel_ms": round(pmed, 3), "edges_per_s": gi["m"] / (pmed * 1e-3),
+                 "build_ms": round(bms, 1), "num_blocks": b.view["num_blocks"],
+                 "engine_speedup": round(pmed / statistics.median(kms), 2)}
+        log("plain 16x%d: %.3f ms (engine speedup %.2fx)" % (K, pmed, plain["engine_speedup"]))
+        b.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(g, total0)
+        log("cpu baseline: %s" % cpu)
+        assert cpu["matches_gpu"], "GPU count differs from the CPU baseline"
+
+    config.update({"n": gi["n"], "m": gi["m"], "ordering": "degree-descending", "tile_k": pi["tile_k"],
+                   "tau": pi["tau"], "dense_tiles": pi["dense_tiles"], "work_items": pi["work_items"]})
+    out = {
+        "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic", "config": config,
+        "triangles": int(total0), "triangles_per_s": total0 / (wall / args.steps),
+        "kernel_ms_median": statistics.median(kms), "canon_ms": round(canon_ms, 1), "plan_ms": round(build_ms, 1),
+        "roofline": roofline, "cpu_baseline": cpu, "plain_hybrid": plain,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

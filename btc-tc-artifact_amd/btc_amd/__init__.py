@@ -1,0 +1,340 @@
+"""ctypes binding of libbtc_amd.so, the gfx950 exact triangle-counting engine.
+
+Everything here calls the C ABI declared in include/btc_amd.h; the counting
+itself runs in the HIP kernels of that library.  If the library has not been
+built, every call raises BtcError (there is no CPU fallback).
+
+Graph           canonical lower triangle L (btc_graph_*)
+BitBSR          16 x K bit tiles of L and the reference-semantics count
+Plan            the hybrid engine (dense tiles + LDS-hashed wedge checks)
+count_cpu       the host verify path (reference -v 1)
+gen_*           seeded synthetic generators
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "BtcError", "lib", "lib_path", "Graph", "BitBSR", "Plan", "CountOpts", "count_cpu",
+    "gen_rmat", "gen_er", "gen_stencil2d", "gen_stencil3d", "gen_planted", "lite_tile_k",
+    "default_tau", "version", "CANON_COMPACT", "CANON_DEGREE_ORDER", "CANON_DEFAULT",
+    "VARIANT_AUTO", "VARIANT_BINSEARCH", "VARIANT_DIRECT",
+]
+
+CANON_COMPACT = 1
+CANON_DEGREE_ORDER = 2
+CANON_DEFAULT = 3
+VARIANT_AUTO = 0
+VARIANT_BINSEARCH = 1
+VARIANT_DIRECT = 2
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "lib", "libbtc_amd.so")
+_lib = None
+
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+vp = C.c_void_p
+
+
+class BtcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("btc error %d: %s" % (code, msg))
+        self.code = code
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("n", C.c_int32), ("n_input", C.c_int32), ("m", C.c_int64), ("max_row", C.c_int32),
+                ("flags", C.c_uint32), ("wedges", C.c_int64), ("merge_work", C.c_int64)]
+
+
+class BitBSRView(C.Structure):
+    _fields_ = [("indptr", vp), ("indices", vp), ("row_indices", vp), ("tiles", vp), ("n", C.c_int32),
+                ("n_row_blocks", C.c_int32), ("num_blocks", C.c_uint32), ("tile_k", C.c_int32),
+                ("words_per_tile", C.c_int32)]
+
+
+class CountOpts(C.Structure):
+    _fields_ = [("tile_k", C.c_int32), ("tau", C.c_int32), ("chunk_work", C.c_int32),
+                ("waves_per_eu", C.c_int32), ("lookup", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("sparse_edges", C.c_int64), ("dense_edges", C.c_int64), ("dense_tiles", C.c_int64),
+                ("wedge_checks", C.c_int64), ("work_items", C.c_int64), ("device_bytes", C.c_int64),
+                ("sparse_merge_work", C.c_int64), ("stream_bytes", C.c_int64), ("tile_k", C.c_int32),
+                ("tau", C.c_int32)]
+
+
+_SIGS = {
+    "btc_last_error": (C.c_char_p, []),
+    "btc_version": (C.c_char_p, []),
+    "btc_device_init": (C.c_int, [C.c_int32]),
+    "btc_gen_rmat": (C.c_int64, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, vp, vp]),
+    "btc_gen_er": (C.c_int64, [C.c_int32, C.c_double, C.c_uint64, vp, vp]),
+    "btc_gen_stencil2d": (C.c_int64, [C.c_int32, C.c_int32, vp, vp]),
+    "btc_gen_stencil3d": (C.c_int64, [C.c_int32, C.c_int32, C.c_int32, vp, vp]),
+    "btc_gen_planted": (C.c_int64, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_uint64, vp, vp]),
+    "btc_graph_from_coo": (C.c_int, [vp, vp, C.c_int64, C.c_int32, C.c_uint32, C.POINTER(vp)]),
+    "btc_graph_from_csr": (C.c_int, [vp, vp, C.c_int32, C.POINTER(vp)]),
+    "btc_graph_get_info": (C.c_int, [vp, C.POINTER(GraphInfo)]),
+    "btc_graph_copy_csr": (C.c_int, [vp, vp, vp]),
+    "btc_graph_copy_labels": (C.c_int, [vp, vp]),
+    "btc_graph_free": (None, [vp]),
+    "btc_bitbsr_build": (C.c_int, [vp, C.c_int32, vp, C.POINTER(vp)]),
+    "btc_bitbsr_get_view": (C.c_int, [vp, C.POINTER(BitBSRView)]),
+    "btc_bitbsr_copy_to_host": (C.c_int, [vp, vp, vp, vp, vp]),
+    "btc_bitbsr_free": (None, [vp]),
+    "btc_bitbsr_count": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, vp, C.POINTER(C.c_uint64), vp,
+                                   C.POINTER(C.c_float)]),
+    "btc_lite_tile_k": (C.c_int32, [C.c_int32, C.c_int64, C.c_int32]),
+    "btc_default_tau": (C.c_int32, [C.c_int32]),
+    "btc_count_opts_default": (None, [C.POINTER(CountOpts)]),
+    "btc_plan_create": (C.c_int, [vp, C.POINTER(CountOpts), vp, C.POINTER(vp)]),
+    "btc_plan_get_info": (C.c_int, [vp, C.POINTER(PlanInfo)]),
+    "btc_plan_count": (C.c_int, [vp, vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
+    "btc_plan_last_timing": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "btc_plan_free": (None, [vp]),
+    "btc_count": (C.c_int, [vp, C.POINTER(CountOpts), vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
+    "btc_count_cpu": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_uint64), vp]),
+}
+
+
+def lib():
+    """Loads libbtc_amd.so once; raises BtcError if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(lib_path):
+            raise BtcError(-1, "libbtc_amd.so not built (run __graft_entry__.build())")
+        L = C.CDLL(lib_path)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise BtcError(rc, lib().btc_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(vp) if a is not None and a.size else None
+
+
+def version():
+    return lib().btc_version().decode()
+
+
+def lite_tile_k(n, m, mode=0):
+    return int(lib().btc_lite_tile_k(n, m, mode))
+
+
+def default_tau(k):
+    return int(lib().btc_default_tau(k))
+
+
+def _gen(fn, *args):
+    cnt = fn(*args, None, None)
+    if cnt < 0:
+        raise BtcError(1, lib().btc_last_error().decode())
+    src = np.empty(cnt, np.int32)
+    dst = np.empty(cnt, np.int32)
+    fn(*args, _ptr(src), _ptr(dst))
+    return src, dst
+
+
+def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1):
+    return _gen(lib().btc_gen_rmat, scale, edge_factor, a, b, c, seed)
+
+
+def gen_er(n, avg_degree, seed=1):
+    return _gen(lib().btc_gen_er, n, avg_degree, seed)
+
+
+def gen_stencil2d(nx, ny):
+    return _gen(lib().btc_gen_stencil2d, nx, ny)
+
+
+def gen_stencil3d(nx, ny, nz):
+    return _gen(lib().btc_gen_stencil3d, nx, ny, nz)
+
+
+def gen_planted(n, community, p_in, noise_degree, seed=1):
+    return _gen(lib().btc_gen_planted, n, community, p_in, noise_degree, seed)
+
+
+class Graph:
+    """Canonical strictly-lower L of an undirected graph (host + HBM)."""
+
+    def __init__(self, handle):
+        self._h = vp(handle)
+
+    @classmethod
+    def from_coo(cls, src, dst, n=0, flags=CANON_DEFAULT):
+        src = np.ascontiguousarray(src, np.int32)
+        dst = np.ascontiguousarray(dst, np.int32)
+        if src.shape != dst.shape:
+            raise ValueError("src and dst differ in length")
+        h = vp()
+        _check(lib().btc_graph_from_coo(_ptr(src), _ptr(dst), src.size, n, flags, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_csr(cls, rowptr, col):
+        rowptr = np.ascontiguousarray(rowptr, np.int64)
+        col = np.ascontiguousarray(col, np.int32)
+        h = vp()
+        _check(lib().btc_graph_from_csr(_ptr(rowptr), _ptr(col), rowptr.size - 1, C.byref(h)))
+        return cls(h.value)
+
+    @property
+    def info(self):
+        gi = GraphInfo()
+        _check(lib().btc_graph_get_info(self._h, C.byref(gi)))
+        return {k: getattr(gi, k) for k, _ in GraphInfo._fields_}
+
+    @property
+    def n(self):
+        return self.info["n"]
+
+    @property
+    def m(self):
+        return self.info["m"]
+
+    def csr(self):
+        i = self.info
+        rp = np.empty(i["n"] + 1, np.int64)
+        col = np.empty(i["m"], np.int32)
+        _check(lib().btc_graph_copy_csr(self._h, _ptr(rp), _ptr(col)))
+        return rp, col
+
+    def labels(self):
+        lab = np.empty(self.info["n"], np.int32)
+        _check(lib().btc_graph_copy_labels(self._h, _ptr(lab)))
+        return lab
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().btc_graph_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _count_out(n, per_vertex):
+    total = C.c_uint64(0)
+    ms = C.c_float(0)
+    pv = np.zeros(n, np.int32) if per_vertex else None
+    return total, ms, pv
+
+
+class BitBSR:
+    """16 x K Bit-BSR of a Graph, resident in HBM."""
+
+    def __init__(self, graph, tile_k, stream=None):
+        self.graph = graph
+        h = vp()
+        _check(lib().btc_bitbsr_build(graph._h, tile_k, stream, C.byref(h)))
+        self._h = h
+
+    @property
+    def view(self):
+        v = BitBSRView()
+        _check(lib().btc_bitbsr_get_view(self._h, C.byref(v)))
+        return {k: getattr(v, k) for k, _ in BitBSRView._fields_}
+
+    def to_host(self):
+        v = self.view
+        indptr = np.empty(v["n_row_blocks"] + 1, np.uint32)
+        nb = v["num_blocks"]
+        indices = np.empty(nb, np.uint32)
+        rows = np.empty(nb, np.uint32)
+        tiles = np.empty(nb * v["words_per_tile"], np.uint64)
+        _check(lib().btc_bitbsr_copy_to_host(self._h, _ptr(indptr), _ptr(indices), _ptr(rows), _ptr(tiles)))
+        return indptr, indices, rows, tiles
+
+    def count(self, variant=VARIANT_AUTO, threshold_blocks=0, tau=-1, per_vertex=False, stream=None):
+        """Returns (total, per_vertex or None, kernel_ms)."""
+        total, ms, pv = _count_out(self.view["n"], per_vertex)
+        _check(lib().btc_bitbsr_count(self._h, variant, threshold_blocks, tau, stream, C.byref(total), _ptr(pv),
+                                      C.byref(ms)))
+        return total.value, pv, ms.value
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().btc_bitbsr_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_opts(tile_k=0, tau=-1, chunk_work=0, waves_per_eu=0, lookup=0):
+    o = CountOpts()
+    lib().btc_count_opts_default(C.byref(o))
+    o.tile_k, o.tau, o.chunk_work, o.waves_per_eu, o.lookup = tile_k, tau, chunk_work, waves_per_eu, lookup
+    return o
+
+
+class Plan:
+    """Hybrid engine plan for one Graph (device-resident work lists)."""
+
+    def __init__(self, graph, stream=None, **opts):
+        self.graph = graph
+        self.opts = make_opts(**opts)
+        h = vp()
+        _check(lib().btc_plan_create(graph._h, C.byref(self.opts), stream, C.byref(h)))
+        self._h = h
+
+    @property
+    def info(self):
+        pi = PlanInfo()
+        _check(lib().btc_plan_get_info(self._h, C.byref(pi)))
+        return {k: getattr(pi, k) for k, _ in PlanInfo._fields_}
+
+    def count(self, per_vertex=False, stream=None):
+        total, ms, pv = _count_out(self.graph.n, per_vertex)
+        _check(lib().btc_plan_count(self._h, stream, C.byref(total), _ptr(pv), C.byref(ms)))
+        return total.value, pv, ms.value
+
+    def last_timing(self):
+        """(dense-tile kernel ms, wedge kernel ms) of the last count."""
+        dm, wm = C.c_float(0), C.c_float(0)
+        _check(lib().btc_plan_last_timing(self._h, C.byref(dm), C.byref(wm)))
+        return dm.value, wm.value
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().btc_plan_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def count_cpu(graph, threads=0, per_vertex=False):
+    total, _, pv = _count_out(graph.n, per_vertex)
+    _check(lib().btc_count_cpu(graph._h, threads, C.byref(total), _ptr(pv)))
+    return total.value, pv

@@ -1,0 +1,555 @@
+// Bit-BSR builder and the reference-semantics tile count ("btc-plain") for
+// gfx950.
+//
+// Builder: the reference sorts 64-bit tile keys of every edge and then sets
+// bits with one atomicOr per edge after a binary search
+// (btc/operations/bcsr_16x128.h:137-241, bcsr_16x32.h:144-249).  Here L is
+// already a row-sorted CSR, so a row block's tiles are the 16-way merge of
+// its rows' column blocks: 16 lanes (one per row) step through their rows
+// together, the group minimum of the next column block is the next tile, and
+// each lane writes its row of that tile directly -- no sort, no atomics, no
+// memset of the tile array.  A wave handles 4 row blocks (4 x 16 lanes).  A
+// counting pass sizes the row blocks, a scan gives indptr, a fill pass
+// writes indices, row_indices and tiles.
+//
+// Count: one wave64 per mask tile S = (I, J_L), as one warp per tile in
+// tc_16x128_mma_v5.h:9-211 / tc_16x32_mma_v6.h:9-238: popcount(S) > tau
+// picks the dense path (wave-uniform two-pointer merge of the tile columns of
+// row blocks I and J_out, every matched pair contributing a full 16 x 16 block
+// of AND+popcount dot products, masked by S afterwards) or the sparse path
+// (per-lane merge, only the set mask bits of the lane are evaluated).  Lane
+// (r, q) = (lane / 4, lane % 4) owns row r of the tile and the 4 columns
+// 4q..4q+3 of each 16-column J_out slab, so every lane's partial sum belongs
+// to a single vertex 16I + r.
+#include "device_util.h"
+#include "internal.h"
+
+namespace btcx {
+
+// ------------------------------------------------------------ graph upload
+void ensure_device(btc_graph_s* g, hipStream_t s) {
+    if (g->d_rowptr.p) return;
+    BTC_REQUIRE(g->m < (int64_t)0xFFFFFFFFLL, BTC_ERR_OVERFLOW,
+                "nnz(L) = %lld needs 64-bit row offsets (limit 2^32 - 1)", (long long)g->m);
+    std::vector<uint32_t> rp32((size_t)g->n + 1);
+    for (size_t i = 0; i < rp32.size(); ++i) rp32[i] = (uint32_t)g->rowptr[i];
+    g->d_rowptr.alloc(rp32.size());
+    g->d_col.alloc((size_t)g->m);
+    BTC_HIP(hipMemcpyAsync(g->d_rowptr.p, rp32.data(), rp32.size() * 4, hipMemcpyHostToDevice, s));
+    if (g->m)
+        BTC_HIP(hipMemcpyAsync(g->d_col.p, g->col.data(), (size_t)g->m * 4, hipMemcpyHostToDevice, s));
+    BTC_HIP(hipStreamSynchronize(s));
+}
+
+int device_cu_count() {
+    int dev = 0;
+    BTC_HIP(hipGetDevice(&dev));
+    int cus = 0;
+    BTC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return cus;
+}
+
+// ------------------------------------------------------------------- scan
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* out, uint64_t* lds) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint64_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wid] = x;
+    __syncthreads();
+    uint64_t base = 0, total = 0;
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        if (w < wid) base += lds[w];
+        total += lds[w];
+    }
+    __syncthreads();
+    *out = base + x - v;
+    return total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_tile_sums(const uint32_t* __restrict__ in,
+                                                                size_t n, uint64_t* __restrict__ sums) {
+    __shared__ uint64_t lds[kScanThreads / 64];
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint64_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) v += in[base + k];
+    uint64_t ex;
+    uint64_t tot = block_exclusive_scan(v, &ex, lds);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_sums(uint64_t* __restrict__ sums, size_t nb,
+                                                           uint64_t* __restrict__ total_out) {
+    __shared__ uint64_t lds[kScanThreads / 64];
+    uint64_t carry = 0;
+    for (size_t b0 = 0; b0 < nb; b0 += kScanThreads) {
+        size_t b = b0 + threadIdx.x;
+        uint64_t v = b < nb ? sums[b] : 0;
+        uint64_t ex;
+        uint64_t tot = block_exclusive_scan(v, &ex, lds);
+        if (b < nb) sums[b] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_apply(const uint32_t* __restrict__ in, size_t n,
+                                                            const uint64_t* __restrict__ sums,
+                                                            uint64_t* __restrict__ out) {
+    __shared__ uint64_t lds[kScanThreads / 64];
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        s += v[k];
+    }
+    uint64_t ex;
+    block_exclusive_scan(s, &ex, lds);
+    uint64_t run = sums[blockIdx.x] + ex;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+}  // namespace
+
+void scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, hipStream_t s) {
+    size_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 0) {
+        BTC_HIP(hipMemsetAsync(out, 0, sizeof(uint64_t), s));
+        return;
+    }
+    DevBuf<uint64_t> sums;
+    sums.alloc(nb);
+    hipLaunchKernelGGL(scan_tile_sums, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, sums.p);
+    BTC_CHECK_LAUNCH("scan_tile_sums");
+    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanThreads), 0, s, sums.p, nb, out + n);
+    BTC_CHECK_LAUNCH("scan_sums");
+    hipLaunchKernelGGL(scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, sums.p, out);
+    BTC_CHECK_LAUNCH("scan_apply");
+    BTC_HIP(hipStreamSynchronize(s));  // sums is freed on return
+}
+
+// ---------------------------------------------------------------- builder
+namespace {
+
+// Bits of one 16 x K tile row held by a lane.
+template <int K>
+struct RowBits;
+
+template <>
+struct RowBits<32> {
+    uint32_t w = 0;
+    __device__ void set(int c) { w |= 1u << c; }
+    __device__ static RowBits load(const uint64_t* T, size_t tile, int r) {
+        RowBits x;
+        x.w = reinterpret_cast<const uint32_t*>(T + tile * 8)[r];
+        return x;
+    }
+    __device__ int and_popc(const RowBits& o) const { return __popc(w & o.w); }
+    __device__ uint32_t bits(int c0) const { return w >> c0; }
+};
+
+template <>
+struct RowBits<64> {
+    uint64_t w = 0;
+    __device__ void set(int c) { w |= 1ull << c; }
+    __device__ static RowBits load(const uint64_t* T, size_t tile, int r) {
+        RowBits x;
+        x.w = T[tile * 16 + r];
+        return x;
+    }
+    __device__ int and_popc(const RowBits& o) const { return __popcll(w & o.w); }
+    __device__ uint32_t bits(int c0) const { return (uint32_t)(w >> c0); }
+};
+
+template <>
+struct RowBits<128> {
+    uint64_t w0 = 0, w1 = 0;
+    __device__ void set(int c) {
+        if (c < 64) w0 |= 1ull << c;
+        else w1 |= 1ull << (c - 64);
+    }
+    __device__ static RowBits load(const uint64_t* T, size_t tile, int r) {
+        RowBits x;
+        const uint64_t* p = T + tile * 32 + 2 * r;
+        x.w0 = p[0];
+        x.w1 = p[1];
+        return x;
+    }
+    __device__ int and_popc(const RowBits& o) const {
+        return __popcll(w0 & o.w0) + __popcll(w1 & o.w1);
+    }
+    __device__ uint32_t bits(int c0) const {
+        return c0 < 64 ? (uint32_t)(w0 >> c0) : (uint32_t)(w1 >> (c0 - 64));
+    }
+};
+
+// FILL = false: counts[I] = number of tiles of row block I.
+// FILL = true: writes tile I's indices / row_indices / bits at offs[I].
+template <int K, bool FILL>
+__global__ __launch_bounds__(256) void build_kernel(int n, int n_rb, const uint32_t* __restrict__ rowptr,
+                                                     const int32_t* __restrict__ col,
+                                                     uint32_t* __restrict__ counts,
+                                                     const uint64_t* __restrict__ offs,
+                                                     uint32_t* __restrict__ indices,
+                                                     uint32_t* __restrict__ row_indices,
+                                                     uint64_t* __restrict__ tiles) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15;
+    const int64_t I = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4 + (lane >> 4);
+    const bool valid = I < n_rb;
+    const int64_t row = I * 16 + r;
+    uint32_t p = 0, e = 0;
+    if (valid && row < n) {
+        p = rowptr[row];
+        e = rowptr[row + 1];
+    }
+    uint64_t t = (FILL && valid) ? offs[I] : 0;
+    uint32_t cnt = 0;
+    for (;;) {
+        uint32_t nxt = p < e ? (uint32_t)col[p] / K : kNoTile;
+        uint32_t cur = group_min_u32(nxt, 16);
+        if (__all(cur == kNoTile)) break;
+        if (cur != kNoTile) {
+            RowBits<K> bits;
+            while (p < e && (uint32_t)col[p] / K == cur) {
+                bits.set((int)((uint32_t)col[p] % K));
+                ++p;
+            }
+            if (FILL) {
+                if (r == 0) {
+                    indices[t] = cur;
+                    row_indices[t] = (uint32_t)I;
+                }
+                if constexpr (K == 32) {
+                    uint32_t other = __shfl_xor(bits.w, 1, 16);
+                    if ((r & 1) == 0) tiles[t * 8 + r / 2] = (uint64_t)bits.w | ((uint64_t)other << 32);
+                } else if constexpr (K == 64) {
+                    tiles[t * 16 + r] = bits.w;
+                } else {
+                    tiles[t * 32 + 2 * r] = bits.w0;
+                    tiles[t * 32 + 2 * r + 1] = bits.w1;
+                }
+                ++t;
+            }
+            ++cnt;
+        }
+    }
+    if (!FILL && valid && r == 0) counts[I] = cnt;
+}
+
+__global__ void narrow_u64_to_u32(const uint64_t* __restrict__ in, uint32_t* __restrict__ out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint32_t)in[i];
+}
+
+template <int K>
+void build_impl(btc_graph_s* g, hipStream_t s, btc_bitbsr_s* b) {
+    const int n = g->n;
+    const int n_rb = (n + 15) / 16;
+    b->n = n;
+    b->n_rb = n_rb;
+    b->K = K;
+    b->wpt = K / 4;
+    b->max_row = g->max_row;
+    const unsigned grid = (unsigned)((n_rb + 15) / 16);
+    DevBuf<uint32_t> counts;
+    DevBuf<uint64_t> offs;
+    counts.alloc((size_t)n_rb);
+    offs.alloc((size_t)n_rb + 1);
+    if (grid) {
+        hipLaunchKernelGGL((build_kernel<K, false>), dim3(grid), dim3(256), 0, s, n, n_rb, g->d_rowptr.p,
+                           g->d_col.p, counts.p, nullptr, nullptr, nullptr, nullptr);
+        BTC_CHECK_LAUNCH("build_kernel<count>");
+    }
+    scan_u32_to_u64(counts.p, offs.p, (size_t)n_rb, s);
+    uint64_t nblk = 0;
+    BTC_HIP(hipMemcpyAsync(&nblk, offs.p + n_rb, 8, hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    BTC_REQUIRE(nblk < 0xFFFFFFFFull, BTC_ERR_OVERFLOW, "%llu tiles exceed the u32 tile index",
+                (unsigned long long)nblk);
+    b->nblk = (uint32_t)nblk;
+    b->indptr.alloc((size_t)n_rb + 1);
+    b->indices.alloc(nblk);
+    b->row_indices.alloc(nblk);
+    b->tiles.alloc(nblk * (size_t)b->wpt);
+    b->result.alloc(1);
+    hipLaunchKernelGGL(narrow_u64_to_u32, dim3((unsigned)((n_rb + 1 + 255) / 256)), dim3(256), 0, s, offs.p,
+                       b->indptr.p, (size_t)n_rb + 1);
+    BTC_CHECK_LAUNCH("narrow_u64_to_u32");
+    if (grid && nblk) {
+        hipLaunchKernelGGL((build_kernel<K, true>), dim3(grid), dim3(256), 0, s, n, n_rb, g->d_rowptr.p,
+                           g->d_col.p, nullptr, offs.p, b->indices.p, b->row_indices.p, b->tiles.p);
+        BTC_CHECK_LAUNCH("build_kernel<fill>");
+    }
+    BTC_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------------- count
+template <int K, bool BINSEARCH, bool PV>
+__global__ __launch_bounds__(256) void tile_count_kernel(
+    int n, int n_rb, const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
+    const uint32_t* __restrict__ row_indices, const uint64_t* __restrict__ tiles, uint32_t w0,
+    uint32_t w_end, const uint32_t* __restrict__ tile_list, int tau,
+    unsigned long long* __restrict__ result, int32_t* __restrict__ pv) {
+    constexpr int WPT = K / 4;
+    constexpr int SLABS = K / 16;
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = uniform(w0 + blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (w >= w_end) return;
+    const uint32_t t = tile_list ? uniform(tile_list[w]) : w;
+    uint32_t I;
+    if (BINSEARCH) {  // row block of tile t by binary search over indptr (tc_16x128_mma_v4.h:29-38)
+        uint32_t lo = 0, hi = (uint32_t)n_rb;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (indptr[mid + 1] <= t) lo = mid + 1;
+            else hi = mid;
+        }
+        I = uniform(lo);
+    } else {
+        I = uniform(row_indices[t]);
+    }
+    const uint32_t JL = uniform(indices[t]);
+    const uint32_t a0 = uniform(indptr[I]), a1 = uniform(indptr[I + 1]);
+
+    uint32_t c = lane < WPT ? (uint32_t)__popcll(tiles[(size_t)t * WPT + lane]) : 0u;
+    const bool dense = (int)wave_sum_u32(c) > tau;
+
+    const int r = lane >> 2, q = lane & 3;
+    const RowBits<K> S = RowBits<K>::load(tiles, t, r);
+    uint32_t sum = 0;
+    for (int it = 0; it < SLABS; ++it) {
+        const uint32_t Jo = JL * SLABS + it;
+        if (Jo >= (uint32_t)n_rb) break;
+        const uint32_t b0 = uniform(indptr[Jo]), b1 = uniform(indptr[Jo + 1]);
+        const uint32_t m4 = S.bits(it * 16 + 4 * q) & 0xFu;
+        if (dense) {
+            uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+            uint32_t pa = a0, pb = b0;
+            while (pa < a1 && pb < b1) {
+                const uint32_t ka = uniform(indices[pa]), kb = uniform(indices[pb]);
+                if (ka < kb) {
+                    ++pa;
+                } else if (ka > kb) {
+                    ++pb;
+                } else {
+                    const RowBits<K> A = RowBits<K>::load(tiles, pa, r);
+                    acc0 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 0));
+                    acc1 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 1));
+                    acc2 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 2));
+                    acc3 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 3));
+                    ++pa;
+                    ++pb;
+                }
+            }
+            sum += ((m4 & 1u) ? acc0 : 0u) + ((m4 & 2u) ? acc1 : 0u) + ((m4 & 4u) ? acc2 : 0u) +
+                   ((m4 & 8u) ? acc3 : 0u);
+        } else if (m4) {
+            uint32_t pa = a0, pb = b0;
+            while (pa < a1 && pb < b1) {
+                const uint32_t ka = indices[pa], kb = indices[pb];
+                if (ka < kb) {
+                    ++pa;
+                } else if (ka > kb) {
+                    ++pb;
+                } else {
+                    const RowBits<K> A = RowBits<K>::load(tiles, pa, r);
+                    uint32_t mm = m4;
+                    while (mm) {
+                        const int cc = __ffs(mm) - 1;
+                        sum += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + cc));
+                        mm &= mm - 1;
+                    }
+                    ++pa;
+                    ++pb;
+                }
+            }
+        }
+    }
+    if (PV) {
+        uint32_t rs = sum + __shfl_xor(sum, 1);
+        rs += __shfl_xor(rs, 2);
+        const int64_t v = (int64_t)I * 16 + r;
+        if (q == 0 && rs && v < n) atomicAdd(pv + v, (int32_t)rs);
+    }
+    const uint64_t tot = wave_sum_u64((uint64_t)sum);
+    if (lane == 0 && tot) atomicAdd(result, (unsigned long long)tot);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void tile_popcount_kernel(const uint64_t* __restrict__ tiles, uint32_t nblk,
+                                                             uint32_t* __restrict__ pop) {
+    constexpr int WPT = K / 4;
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (w >= nblk) return;
+    uint32_t c = lane < WPT ? (uint32_t)__popcll(tiles[(size_t)w * WPT + lane]) : 0u;
+    c = wave_sum_u32(c);
+    if (lane == 0) pop[w] = c;
+}
+
+// A dispatch holds at most 2^32 work-items, so long tile lists are split
+// into launches of kMaxTilesPerLaunch waves.
+constexpr uint32_t kMaxTilesPerLaunch = 1u << 24;
+
+template <int K, bool BS>
+void launch_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t nt, int tau, unsigned long long* res,
+              int32_t* pv, hipStream_t s) {
+    for (uint32_t w0 = 0; w0 < nt; w0 += kMaxTilesPerLaunch) {
+        const uint32_t chunk = std::min(kMaxTilesPerLaunch, nt - w0);
+        const unsigned grid = (chunk + 3) / 4;
+        const uint32_t end = w0 + chunk;
+        if (pv)
+            hipLaunchKernelGGL((tile_count_kernel<K, BS, true>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
+                               b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, end, list, tau,
+                               res, pv);
+        else
+            hipLaunchKernelGGL((tile_count_kernel<K, BS, false>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
+                               b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, end, list, tau,
+                               res, pv);
+        BTC_CHECK_LAUNCH("tile_count_kernel");
+    }
+}
+
+}  // namespace
+
+void bitbsr_build(btc_graph_s* g, int K, hipStream_t s, btc_bitbsr_s* b) {
+    ensure_device(g, s);
+    switch (K) {
+        case 32: build_impl<32>(g, s, b); break;
+        case 64: build_impl<64>(g, s, b); break;
+        case 128: build_impl<128>(g, s, b); break;
+        default: fail(BTC_ERR_ARG, "tile width K = %d (expected 32, 64 or 128)", K);
+    }
+}
+
+void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles, bool binsearch,
+                       int tau, unsigned long long* d_result, int32_t* d_pv, hipStream_t s) {
+    if (ntiles == 0) return;
+    BTC_REQUIRE(b->max_row <= 65536, BTC_ERR_OVERFLOW,
+                "longest row %d > 65536: per-lane 32-bit sums could overflow", b->max_row);
+    switch (b->K) {
+        case 32:
+            binsearch ? launch_k<32, true>(b, tile_list, ntiles, tau, d_result, d_pv, s)
+                      : launch_k<32, false>(b, tile_list, ntiles, tau, d_result, d_pv, s);
+            break;
+        case 64:
+            binsearch ? launch_k<64, true>(b, tile_list, ntiles, tau, d_result, d_pv, s)
+                      : launch_k<64, false>(b, tile_list, ntiles, tau, d_result, d_pv, s);
+            break;
+        default:
+            binsearch ? launch_k<128, true>(b, tile_list, ntiles, tau, d_result, d_pv, s)
+                      : launch_k<128, false>(b, tile_list, ntiles, tau, d_result, d_pv, s);
+            break;
+    }
+}
+
+void launch_tile_popcount(const btc_bitbsr_s* b, uint32_t* d_pop, hipStream_t s) {
+    if (b->nblk == 0) return;
+    const unsigned grid = (b->nblk + 3) / 4;
+    switch (b->K) {
+        case 32: hipLaunchKernelGGL(tile_popcount_kernel<32>, dim3(grid), dim3(256), 0, s, b->tiles.p, b->nblk, d_pop); break;
+        case 64: hipLaunchKernelGGL(tile_popcount_kernel<64>, dim3(grid), dim3(256), 0, s, b->tiles.p, b->nblk, d_pop); break;
+        default: hipLaunchKernelGGL(tile_popcount_kernel<128>, dim3(grid), dim3(256), 0, s, b->tiles.p, b->nblk, d_pop); break;
+    }
+    BTC_CHECK_LAUNCH("tile_popcount_kernel");
+}
+
+}  // namespace btcx
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int btc_bitbsr_build(btc_graph g, int32_t tile_k, void* stream, btc_bitbsr* out) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(g && out, BTC_ERR_ARG, "NULL argument");
+        *out = nullptr;
+        BTC_REQUIRE(tile_k == 32 || tile_k == 64 || tile_k == 128, BTC_ERR_ARG,
+                    "tile width K = %d (expected 32, 64 or 128)", tile_k);
+        auto b = std::make_unique<btc_bitbsr_s>();
+        btcx::bitbsr_build(g, tile_k, btcx::as_stream(stream), b.get());
+        *out = b.release();
+    });
+}
+
+int btc_bitbsr_get_view(btc_bitbsr b, btc_bitbsr_view* v) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(b && v, BTC_ERR_ARG, "NULL argument");
+        v->indptr = b->indptr.p;
+        v->indices = b->indices.p;
+        v->row_indices = b->row_indices.p;
+        v->tiles = b->tiles.p;
+        v->n = b->n;
+        v->n_row_blocks = b->n_rb;
+        v->num_blocks = b->nblk;
+        v->tile_k = b->K;
+        v->words_per_tile = b->wpt;
+    });
+}
+
+int btc_bitbsr_copy_to_host(btc_bitbsr b, uint32_t* indptr, uint32_t* indices, uint32_t* row_indices,
+                            uint64_t* tiles) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(b, BTC_ERR_ARG, "NULL handle");
+        if (indptr) BTC_HIP(hipMemcpy(indptr, b->indptr.p, ((size_t)b->n_rb + 1) * 4, hipMemcpyDeviceToHost));
+        if (b->nblk == 0) return;
+        if (indices) BTC_HIP(hipMemcpy(indices, b->indices.p, (size_t)b->nblk * 4, hipMemcpyDeviceToHost));
+        if (row_indices)
+            BTC_HIP(hipMemcpy(row_indices, b->row_indices.p, (size_t)b->nblk * 4, hipMemcpyDeviceToHost));
+        if (tiles)
+            BTC_HIP(hipMemcpy(tiles, b->tiles.p, (size_t)b->nblk * b->wpt * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+void btc_bitbsr_free(btc_bitbsr b) { delete b; }
+
+int btc_bitbsr_count(btc_bitbsr b, int32_t variant, int32_t threshold_blocks, int32_t tau, void* stream,
+                     uint64_t* total, int32_t* per_vertex, float* kernel_ms) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(b && total, BTC_ERR_ARG, "NULL argument");
+        BTC_REQUIRE(variant >= BTC_VARIANT_AUTO && variant <= BTC_VARIANT_DIRECT, BTC_ERR_ARG,
+                    "unknown variant %d", variant);
+        hipStream_t s = btcx::as_stream(stream);
+        if (tau < 0) tau = btc_default_tau(b->K);
+        if (threshold_blocks <= 0) threshold_blocks = b->K == 128 ? 2048 : 40000;
+        const bool bs = variant == BTC_VARIANT_BINSEARCH ||
+                        (variant == BTC_VARIANT_AUTO && b->nblk < (uint32_t)threshold_blocks);
+        btcx::DevBuf<int32_t> pv;
+        if (per_vertex) {
+            pv.alloc((size_t)b->n);
+            BTC_HIP(hipMemsetAsync(pv.p, 0, (size_t)b->n * 4, s));
+        }
+        BTC_HIP(hipMemsetAsync(b->result.p, 0, 8, s));
+        btcx::EventTimer tm(kernel_ms != nullptr);
+        tm.start(s);
+        btcx::launch_tile_count(b, nullptr, b->nblk, bs, tau, b->result.p, pv.p, s);
+        tm.stop(s);
+        unsigned long long h = 0;
+        BTC_HIP(hipMemcpyAsync(&h, b->result.p, 8, hipMemcpyDeviceToHost, s));
+        if (per_vertex && b->n)
+            BTC_HIP(hipMemcpyAsync(per_vertex, pv.p, (size_t)b->n * 4, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipStreamSynchronize(s));
+        if (kernel_ms) *kernel_ms = tm.ms();
+        *total = h;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// Internal definitions shared by the host (.cpp) and device (.hip) sources of
+// libbtc_amd.so.  Errors inside the library are raised as btcx::Error and
+// turned into status codes at the C ABI (btcx::guard); nothing exits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <new>
+#include <vector>
+
+#include "btc_amd.h"
+
+namespace btcx {
+
+struct Error {
+    int code;
+};
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+[[noreturn]] void fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define BTC_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            ::btcx::fail(e_ == hipErrorOutOfMemory ? BTC_ERR_NOMEM : BTC_ERR_HIP,            \
+                         "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+#define BTC_CHECK_LAUNCH(name)                                                                \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess)                                                                 \
+            ::btcx::fail(BTC_ERR_HIP, "launch of %s: %s", name, hipGetErrorString(e_));       \
+    } while (0)
+
+#define BTC_REQUIRE(cond, code, ...)          \
+    do {                                      \
+        if (!(cond)) ::btcx::fail(code, __VA_ARGS__); \
+    } while (0)
+
+// Runs f and maps library errors to status codes for the C ABI.
+template <class F>
+int guard(F&& f) {
+    clear_error();
+    try {
+        f();
+        return BTC_OK;
+    } catch (const Error& e) {
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return BTC_ERR_NOMEM;
+    } catch (...) {
+        set_error("unexpected exception");
+        return BTC_ERR_STATE;
+    }
+}
+
+// Owning device buffer (hipMalloc / hipFree).
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;  // keep a valid pointer for empty arrays
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            fail(BTC_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                 hipGetErrorString(e));
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Kernel timing with HIP events recorded on the launch stream.
+struct EventTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool on = false;
+    explicit EventTimer(bool enable);
+    ~EventTimer();
+    void start(hipStream_t s);
+    void stop(hipStream_t s);
+    float ms();  // synchronises on the stop event
+};
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace btcx
+
+// Canonical lower-triangular graph L (CSR, ascending columns) on host and HBM.
+struct btc_graph_s {
+    int32_t n = 0;
+    int32_t n_input = 0;
+    int64_t m = 0;
+    int32_t max_row = 0;
+    uint32_t flags = 0;
+    int64_t wedges = 0;
+    int64_t merge_work = 0;
+    std::vector<int64_t> rowptr;   // n + 1
+    std::vector<int32_t> col;      // m
+    std::vector<int32_t> labels;   // canonical -> input id
+    btcx::DevBuf<uint32_t> d_rowptr;  // n + 1 (m < 2^32 is enforced)
+    btcx::DevBuf<int32_t> d_col;      // m
+};
+
+namespace btcx {
+// Fills the derived statistics and uploads the CSR to HBM.
+void graph_finalize(btc_graph_s* g);
+}  // namespace btcx

@@ -1,0 +1,40 @@
+// Wave64 helpers for gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace btcx {
+
+constexpr int kWave = 64;
+
+// Make a value the compiler treats as per-lane into a wave-uniform one, so
+// index loads derived from it go through the scalar unit.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t group_min_u32(uint32_t v, int width) {
+    for (int o = width >> 1; o > 0; o >>= 1) {
+        uint32_t w = __shfl_xor(v, o, width);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
+
+}  // namespace btcx

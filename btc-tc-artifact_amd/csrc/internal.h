@@ -1,0 +1,49 @@
+// Declarations shared between the library's device-side translation units.
+#pragma once
+
+#include <memory>
+
+#include "common.h"
+
+// Bit-BSR of L with 16 x K tiles (K in {32, 64, 128}); see btc_amd.h.
+struct btc_bitbsr_s {
+    int32_t n = 0;
+    int32_t n_rb = 0;
+    int32_t K = 0;
+    int32_t wpt = 0;  // 64-bit words per tile = K / 4
+    uint32_t nblk = 0;
+    int32_t max_row = 0;
+    btcx::DevBuf<uint32_t> indptr;       // n_rb + 1
+    btcx::DevBuf<uint32_t> indices;      // nblk
+    btcx::DevBuf<uint32_t> row_indices;  // nblk
+    btcx::DevBuf<uint64_t> tiles;        // nblk * wpt
+    btcx::DevBuf<unsigned long long> result;
+};
+
+namespace btcx {
+
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;
+
+// Uploads L (rowptr narrowed to u32) to HBM on first use.
+void ensure_device(btc_graph_s* g, hipStream_t s);
+
+// out[0..n) = exclusive prefix sum of in[0..n), out[n] = total (u64).
+void scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, hipStream_t s);
+
+// Builds the 16 x K Bit-BSR of g on `s` into b (b's buffers are replaced).
+void bitbsr_build(btc_graph_s* g, int K, hipStream_t s, btc_bitbsr_s* b);
+
+// Count over mask tiles of b.  tile_list (device) selects ntiles mask tiles,
+// or all tiles when NULL.  tau < 0 sends every tile down the dense path.
+// Adds the total to *d_result and per-row sums to d_pv (when not NULL).
+void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles,
+                       bool binsearch, int tau, unsigned long long* d_result, int32_t* d_pv,
+                       hipStream_t s);
+
+// Per-tile popcounts (device, nblk entries).
+void launch_tile_popcount(const btc_bitbsr_s* b, uint32_t* d_pop, hipStream_t s);
+
+// Number of compute units of the current device.
+int device_cu_count();
+
+}  // namespace btcx

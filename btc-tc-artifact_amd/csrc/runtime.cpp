@@ -1,0 +1,99 @@
+// Error state, version, device selection, event timing and the scalar
+// dispatch rules shared by every path.
+#include <cstdarg>
+#include <cstring>
+
+#include "common.h"
+
+namespace btcx {
+
+static thread_local char g_err[1024] = {0};
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+void clear_error() { g_err[0] = 0; }
+
+void fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    throw Error{code};
+}
+
+EventTimer::EventTimer(bool enable) : on(enable) {
+    if (!on) return;
+    BTC_HIP(hipEventCreate(&a));
+    BTC_HIP(hipEventCreate(&b));
+}
+
+EventTimer::~EventTimer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+}
+
+void EventTimer::start(hipStream_t s) {
+    if (on) BTC_HIP(hipEventRecord(a, s));
+}
+
+void EventTimer::stop(hipStream_t s) {
+    if (on) BTC_HIP(hipEventRecord(b, s));
+}
+
+float EventTimer::ms() {
+    if (!on) return 0.f;
+    BTC_HIP(hipEventSynchronize(b));
+    float t = 0.f;
+    BTC_HIP(hipEventElapsedTime(&t, a, b));
+    return t;
+}
+
+}  // namespace btcx
+
+extern "C" {
+
+const char* btc_last_error(void) { return btcx::g_err; }
+
+const char* btc_version(void) { return "0.1.0-gfx950"; }
+
+int btc_device_init(int32_t device) {
+    return btcx::guard([&] {
+        int count = 0;
+        hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || count == 0) {
+            (void)hipGetLastError();
+            btcx::fail(BTC_ERR_HIP, "no HIP device available");
+        }
+        BTC_REQUIRE(device >= 0 && device < count, BTC_ERR_ARG, "device %d out of range [0,%d)",
+                    device, count);
+        BTC_HIP(hipSetDevice(device));
+    });
+}
+
+// apps/btc_tc_lite.cu:45-65: 16x32 if the average L-row is below 15 or the
+// graph has fewer than 50,000 rows, else 16x128; BTC_MODE 1/2 force 32/128.
+int32_t btc_lite_tile_k(int32_t n, int64_t m, int32_t mode) {
+    if (mode == 1) return 32;
+    if (mode == 2) return 128;
+    if (n <= 0) return 32;
+    float avg = (float)m / (float)n;
+    return (avg < 15.0f || n < 50000) ? 32 : 128;
+}
+
+// tau literals: tc_16x128_mma_v5.h:51 (512 of 2048 bits), tc_16x32_mma_v6.h:52
+// (64 of 512 bits); 16x64 is not in the reference and takes 12.5 % (128 of 1024).
+int32_t btc_default_tau(int32_t tile_k) {
+    switch (tile_k) {
+        case 128: return 512;
+        case 64: return 128;
+        case 32: return 64;
+        default: return -1;
+    }
+}
+
+}  // extern "C"

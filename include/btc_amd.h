@@ -1,0 +1,218 @@
+/*
+ * btc_amd.h -- C ABI of the MI355X (gfx950) exact triangle-counting engine.
+ *
+ * Plain C: pointers, sizes and integer status codes; no C++ or torch types.
+ * Every function returns BTC_OK (0) or a BTC_ERR_* code and never exits or
+ * aborts the process; btc_last_error() gives the message of the last failing
+ * call on the calling thread.
+ *
+ * Memory: inputs and outputs named "host" are caller-owned host buffers.
+ * Device memory is owned by the library and lives behind the opaque handles
+ * (btc_graph, btc_bitbsr, btc_plan, btc_batch); free each handle exactly once.
+ * A handle may be used by one host thread at a time.
+ *
+ * Streams: every entry point that launches GPU work takes a `stream`
+ * argument (a hipStream_t passed as void*, NULL = the default stream) and is
+ * synchronous with respect to the caller: it returns after the results are
+ * in the host buffers, like the reference's count calls
+ * (btc/operations/tc_16x128_mma_v5.h:213-252).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   preprocess_for_triangle_counting      btc/operations/convert.h:601-692
+ *   convert_coo_to_bcsr_16x128_gpu        btc/operations/bcsr_16x128.h:137-241
+ *   convert_coo_to_bcsr_16x32_gpu         btc/operations/bcsr_16x32.h:144-249
+ *   BCSR_16x128_Device / BCSR_16x32_Device btc/operations/bcsr_16x128.h:24-51,
+ *                                         btc/operations/bcsr_16x32.h:109-138
+ *   count_triangles_16x128_{v4,v5}        btc/operations/tc_16x128_mma_v4.h:222-260,
+ *                                         btc/operations/tc_16x128_mma_v5.h:213-252
+ *   count_triangles_16x32_{v5,v6}         btc/operations/tc_16x32_mma_v5.h:246-284,
+ *                                         btc/operations/tc_16x32_mma_v6.h:240-279
+ *   count_triangles_16x{128,32}_adaptive  btc/operations/tc_16x128_adaptive.h:12-29,
+ *                                         btc/operations/tc_16x32_adaptive.h:14-31
+ *   Lite tile policy                      apps/btc_tc_lite.cu:45-65
+ *   cpu_tc_intersection_lower_triangular  btc/operations/triangle_count_cpu.h:60-101
+ */
+#ifndef BTC_AMD_H
+#define BTC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+#define BTC_OK 0
+#define BTC_ERR_ARG 1      /* invalid argument (null pointer, bad size, id out of range) */
+#define BTC_ERR_HIP 2      /* HIP runtime error (launch, copy, no device) */
+#define BTC_ERR_NOMEM 3    /* device or host allocation failed / over the HBM budget */
+#define BTC_ERR_OVERFLOW 4 /* a 32-bit index or counter limit would be exceeded */
+#define BTC_ERR_STATE 5    /* handle used in a state that does not allow the call */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* btc_last_error(void);
+/* Library version string, "MAJOR.MINOR.PATCH-gfx950"; the reference is 0.1.0 (btc/btc.h:40-42). */
+const char* btc_version(void);
+/* Select the HIP device for calls from this thread; returns BTC_ERR_HIP if absent. */
+int btc_device_init(int32_t device);
+
+/* --------------------------------------------------------- synthetic inputs
+ * Seeded generators (host, OpenMP).  Every random draw is
+ * splitmix64(seed + (counter + 1) * 0x9E3779B97F4A7C15) for a counter fixed
+ * by the generator, so outputs are identical for any thread count.  Each
+ * returns the number of (src, dst) pairs; with src == dst == NULL only the
+ * count is computed (two-call pattern).  Pairs are undirected and may hold
+ * duplicates and self-loops; canonicalisation removes them.  Return -1 on a
+ * bad argument (see btc_last_error). */
+int64_t btc_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                     uint64_t seed, int32_t* src, int32_t* dst);
+int64_t btc_gen_er(int32_t n, double avg_degree, uint64_t seed, int32_t* src, int32_t* dst);
+int64_t btc_gen_stencil2d(int32_t nx, int32_t ny, int32_t* src, int32_t* dst); /* 9-point */
+int64_t btc_gen_stencil3d(int32_t nx, int32_t ny, int32_t nz, int32_t* src, int32_t* dst); /* 27-point */
+/* n vertices in communities of `community` vertices with intra-community
+ * edge probability p_in, plus G(n, noise_degree/(n-1)) noise. */
+int64_t btc_gen_planted(int32_t n, int32_t community, double p_in, double noise_degree,
+                        uint64_t seed, int32_t* src, int32_t* dst);
+
+/* ------------------------------------------------------ canonical graph L
+ * btc_graph_from_coo replaces preprocess_for_triangle_counting
+ * (btc/operations/convert.h:601-692): orient every pair to (max, min), drop
+ * self-loops, sort and deduplicate (convert.h:281-324), compact vertex ids
+ * in order (convert.h:387-425, flag BTC_CANON_COMPACT), and optionally relabel
+ * by descending degree with ties broken by id (BTC_CANON_DEGREE_ORDER, not in
+ * the reference; the triangle total is invariant under it).  The result is
+ * the strictly lower triangle L in CSR with ascending columns, resident in
+ * HBM.  Ids must lie in [0, n); n <= 0 means n = max id + 1. */
+#define BTC_CANON_COMPACT 1u
+#define BTC_CANON_DEGREE_ORDER 2u
+#define BTC_CANON_DEFAULT (BTC_CANON_COMPACT | BTC_CANON_DEGREE_ORDER)
+
+typedef struct btc_graph_s* btc_graph;
+
+typedef struct {
+    int32_t n;          /* vertices of L (after compaction) */
+    int32_t n_input;    /* vertex id space of the input */
+    int64_t m;          /* nnz(L) = undirected edges */
+    int32_t max_row;    /* max_i d_L(i) */
+    uint32_t flags;     /* canonicalisation flags used */
+    int64_t wedges;     /* sum_i d_L(i)(d_L(i)-1)/2 */
+    int64_t merge_work; /* W = sum over (i,j) in L of d_L(i) + d_L(j) */
+} btc_graph_info;
+
+int btc_graph_from_coo(const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n,
+                       uint32_t flags, btc_graph* out);
+/* Adopt an already canonical L (strictly lower, ascending, duplicate-free). */
+int btc_graph_from_csr(const int64_t* rowptr, const int32_t* col, int32_t n, btc_graph* out);
+int btc_graph_get_info(btc_graph g, btc_graph_info* info);
+/* Copy L to host: rowptr[n+1], col[m]. */
+int btc_graph_copy_csr(btc_graph g, int64_t* rowptr, int32_t* col);
+/* canon_to_input[v] = input id of canonical vertex v (n entries). */
+int btc_graph_copy_labels(btc_graph g, int32_t* canon_to_input);
+void btc_graph_free(btc_graph g);
+
+/* ------------------------------------------------------------- Bit-BSR
+ * 16 x K bit tiles of L, K in {32, 64, 128}; replaces
+ * convert_coo_to_bcsr_16x{128,32}_gpu (bcsr_16x128.h:137-241,
+ * bcsr_16x32.h:144-249).  Layout as in the reference: tile (I, J) holds rows
+ * 16I..16I+15 and columns K*J..K*J+K-1; bit p = r*K + c of a tile sits in
+ * 64-bit word p/64, bit p%64 (little-endian, so byte-identical to the
+ * reference's 32-bit words p/32, bit p%32).  Tile columns ascend within each
+ * row block; indptr[n_row_blocks] = num_blocks. */
+typedef struct btc_bitbsr_s* btc_bitbsr;
+
+typedef struct {
+    const uint32_t* indptr;      /* device, n_row_blocks + 1 */
+    const uint32_t* indices;     /* device, num_blocks: tile column J */
+    const uint32_t* row_indices; /* device, num_blocks: tile row block I */
+    const uint64_t* tiles;       /* device, num_blocks * words_per_tile */
+    int32_t n;
+    int32_t n_row_blocks;
+    uint32_t num_blocks;
+    int32_t tile_k;
+    int32_t words_per_tile; /* 16*K/64 */
+} btc_bitbsr_view;
+
+int btc_bitbsr_build(btc_graph g, int32_t tile_k, void* stream, btc_bitbsr* out);
+int btc_bitbsr_get_view(btc_bitbsr b, btc_bitbsr_view* v);
+int btc_bitbsr_copy_to_host(btc_bitbsr b, uint32_t* indptr, uint32_t* indices,
+                            uint32_t* row_indices, uint64_t* tiles);
+void btc_bitbsr_free(btc_bitbsr b);
+
+/* Reference-semantics hybrid count over a Bit-BSR ("btc-plain"): one wave per
+ * mask tile, in-kernel two-pointer matching of tile columns, popcount(tile) >
+ * tau routes the tile to the dense AND+popcount path, else the sparse per-bit
+ * path (tc_16x128_mma_v5.h:9-211, tc_16x32_mma_v6.h:9-238).
+ *   variant: BTC_VARIANT_AUTO picks BINSEARCH when num_blocks <
+ *            threshold_blocks, else DIRECT (tc_16x128_adaptive.h:12-29,
+ *            tc_16x32_adaptive.h:14-31); threshold_blocks <= 0 takes the
+ *            reference default for K (2048 for K=128, 40000 for K=32 and 64).
+ *   tau:     < 0 takes the default for K (512 for 128, 64 for 32, 128 for 64).
+ *   per_vertex: host int32[n] row sums of L .* (L L^T), or NULL.
+ *   kernel_ms: event time of the count kernel(s) on `stream`, or NULL. */
+#define BTC_VARIANT_AUTO 0
+#define BTC_VARIANT_BINSEARCH 1
+#define BTC_VARIANT_DIRECT 2
+int btc_bitbsr_count(btc_bitbsr b, int32_t variant, int32_t threshold_blocks, int32_t tau,
+                     void* stream, uint64_t* total, int32_t* per_vertex, float* kernel_ms);
+
+/* Lite tile policy (apps/btc_tc_lite.cu:45-65): returns 32 if m/n < 15 or
+ * n < 50000, else 128; mode 1 forces 32, mode 2 forces 128 (BTC_MODE). */
+int32_t btc_lite_tile_k(int32_t n, int64_t m, int32_t mode);
+/* Default tau for a tile width (512, 128, 64 for K = 128, 64, 32). */
+int32_t btc_default_tau(int32_t tile_k);
+
+/* ------------------------------------------------------ hybrid count engine
+ * The MI355X path for L .* (L L^T): edges of L in dense 16xK mask tiles
+ * (popcount > tau) are counted over LDS-staged bit tiles with 64-bit AND +
+ * popcount; every other edge (i, j) is counted as the wedges (i; j, k),
+ * k < j, k in N_L(i), checked against an LDS hash of N_L(j) by one
+ * workgroup per pivot j (work-stealing over pivot chunks).  Same result as
+ * btc_bitbsr_count and the CPU merge, bit for bit. */
+typedef struct {
+    int32_t tile_k;        /* dense tile width (32/64/128); 0 = auto */
+    int32_t tau;           /* dense threshold; < 0 = default for tile_k; > 16*K disables dense */
+    int32_t chunk_work;    /* wedge checks per work item; 0 = auto */
+    int32_t waves_per_eu;  /* occupancy hint for the persistent grid; 0 = auto */
+    int32_t lookup;        /* 0 = auto, 1 = LDS hash, 2 = LDS bitmap window */
+    int32_t reserved[3];
+} btc_count_opts;
+
+typedef struct btc_plan_s* btc_plan;
+
+typedef struct {
+    int64_t sparse_edges;      /* edges counted by the wedge path */
+    int64_t dense_edges;       /* edges inside dense mask tiles */
+    int64_t dense_tiles;       /* dense mask tiles */
+    int64_t wedge_checks;      /* hash lookups the wedge path performs */
+    int64_t work_items;        /* pivot chunks */
+    int64_t device_bytes;      /* HBM held by the plan */
+    int64_t sparse_merge_work; /* sum over sparse edges (i,j) of d_L(i) + d_L(j) */
+    int64_t stream_bytes;      /* bytes the wedge kernel reads: prefixes, pivot rows, lists, items */
+    int32_t tile_k;
+    int32_t tau;
+} btc_plan_info;
+
+void btc_count_opts_default(btc_count_opts* opts);
+int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_plan* out);
+int btc_plan_get_info(btc_plan p, btc_plan_info* info);
+int btc_plan_count(btc_plan p, void* stream, uint64_t* total, int32_t* per_vertex, float* kernel_ms);
+/* Event times of the last btc_plan_count with kernel_ms != NULL, split by
+ * kernel: dense-tile kernel and wedge kernel (0 when not launched). */
+int btc_plan_last_timing(btc_plan p, float* dense_ms, float* wedge_ms);
+void btc_plan_free(btc_plan p);
+/* One-shot: plan + count + free. */
+int btc_count(btc_graph g, const btc_count_opts* opts, void* stream, uint64_t* total,
+              int32_t* per_vertex, float* kernel_ms);
+
+/* --------------------------------------------------------- CPU verify path
+ * Host OpenMP merge over L (the reference's -v 1 path,
+ * triangle_count_cpu.h:60-101); per_vertex may be NULL.  Not used by any GPU
+ * entry point. */
+int btc_count_cpu(btc_graph g, int32_t threads, uint64_t* total, int32_t* per_vertex);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BTC_AMD_H */
