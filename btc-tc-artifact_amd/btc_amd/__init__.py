@@ -59,7 +59,8 @@ class BitBSRView(C.Structure):
 
 class CountOpts(C.Structure):
     _fields_ = [("tile_k", C.c_int32), ("tau", C.c_int32), ("chunk_work", C.c_int32),
-                ("waves_per_eu", C.c_int32), ("lookup", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("waves_per_eu", C.c_int32), ("lookup", C.c_int32), ("group_lanes", C.c_int32),
+                ("unroll", C.c_int32), ("schedule", C.c_int32)]
 
 
 class PlanInfo(C.Structure):
@@ -95,6 +96,7 @@ _SIGS = {
     "btc_count_opts_default": (None, [C.POINTER(CountOpts)]),
     "btc_plan_create": (C.c_int, [vp, C.POINTER(CountOpts), vp, C.POINTER(vp)]),
     "btc_plan_get_info": (C.c_int, [vp, C.POINTER(PlanInfo)]),
+    "btc_plan_tune": (C.c_int, [vp, C.POINTER(CountOpts)]),
     "btc_plan_count": (C.c_int, [vp, vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
     "btc_plan_last_timing": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "btc_plan_free": (None, [vp]),
@@ -288,10 +290,15 @@ class BitBSR:
             pass
 
 
-def make_opts(tile_k=0, tau=-1, chunk_work=0, waves_per_eu=0, lookup=0):
+def make_opts(tile_k=None, tau=None, chunk_work=None, waves_per_eu=None, lookup=None, group_lanes=None,
+              unroll=None, schedule=None):
+    """btc_count_opts with the library defaults, overridden where given."""
     o = CountOpts()
     lib().btc_count_opts_default(C.byref(o))
-    o.tile_k, o.tau, o.chunk_work, o.waves_per_eu, o.lookup = tile_k, tau, chunk_work, waves_per_eu, lookup
+    for k, v in dict(tile_k=tile_k, tau=tau, chunk_work=chunk_work, waves_per_eu=waves_per_eu, lookup=lookup,
+                     group_lanes=group_lanes, unroll=unroll, schedule=schedule).items():
+        if v is not None:
+            setattr(o, k, v)
     return o
 
 
@@ -310,6 +317,11 @@ class Plan:
         pi = PlanInfo()
         _check(lib().btc_plan_get_info(self._h, C.byref(pi)))
         return {k: getattr(pi, k) for k, _ in PlanInfo._fields_}
+
+    def tune(self, **opts):
+        """Changes launch-only options (waves_per_eu, lookup, group_lanes, unroll, schedule)."""
+        o = make_opts(**opts)
+        _check(lib().btc_plan_tune(self._h, C.byref(o)))
 
     def count(self, per_vertex=False, stream=None):
         total, ms, pv = _count_out(self.graph.n, per_vertex)

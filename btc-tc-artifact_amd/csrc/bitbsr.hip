@@ -34,7 +34,8 @@ void ensure_device(btc_graph_s* g, hipStream_t s) {
     std::vector<uint32_t> rp32((size_t)g->n + 1);
     for (size_t i = 0; i < rp32.size(); ++i) rp32[i] = (uint32_t)g->rowptr[i];
     g->d_rowptr.alloc(rp32.size());
-    g->d_col.alloc((size_t)g->m);
+    g->d_col.alloc((size_t)g->m + kColPad);
+    BTC_HIP(hipMemsetAsync(g->d_col.p + g->m, 0, kColPad * 4, s));
     BTC_HIP(hipMemcpyAsync(g->d_rowptr.p, rp32.data(), rp32.size() * 4, hipMemcpyHostToDevice, s));
     if (g->m)
         BTC_HIP(hipMemcpyAsync(g->d_col.p, g->col.data(), (size_t)g->m * 4, hipMemcpyHostToDevice, s));

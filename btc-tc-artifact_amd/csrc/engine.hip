@@ -33,94 +33,302 @@
 namespace btcx {
 namespace {
 
-constexpr int kTableSlots = 4096;  // 16 KiB of LDS per workgroup
-constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
+constexpr int kLdsWords = 4096;   // 16 KiB of LDS per workgroup
+constexpr int kWaveWords = 1024;  // per-wave slice of it for wave items
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+// Wave items: pivots with at most this many prefix lists.
+constexpr uint32_t kWaveItemLists = 64;
 
 __device__ __forceinline__ uint32_t hslot(uint32_t key, int shift) { return (key * 2654435761u) >> shift; }
 
-template <bool PV>
-__global__ __launch_bounds__(kThreads) void wedge_kernel(
-    const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const uint32_t* __restrict__ us,
-    const uint32_t* __restrict__ ue, const int32_t* __restrict__ ur, const uint4* __restrict__ items,
-    uint32_t nitems, uint32_t* __restrict__ counter, unsigned long long* __restrict__ result,
-    int32_t* __restrict__ pv) {
-    __shared__ uint32_t table[kTableSlots];
-    __shared__ uint32_t s_item;
-    const int lane = threadIdx.x & 63;
-    const uint32_t wid = uniform(threadIdx.x >> 6);
-    uint64_t acc = 0;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+// Membership tests for N_L(pivot).  Only ids inside the list's clipped
+// prefix, hence inside [min N_L(j), max N_L(j)], are tested.
+struct BitmapSet {  // bit (k - lo) of an LDS window
+    const uint32_t* bits;
+    uint32_t lo;
+    __device__ __forceinline__ uint32_t has(uint32_t k) const {
+        const uint32_t x = k - lo;
+        return (bits[x >> 5] >> (x & 31u)) & 1u;
+    }
+};
+
+struct HashSet {  // open addressing, linear probing, load <= 1/2
+    const uint32_t* slots;
+    int shift;
+    uint32_t mask;
+    __device__ __forceinline__ uint32_t has(uint32_t k) const {
+        uint32_t h = hslot(k, shift);
+        for (;;) {
+            const uint32_t v = slots[h];
+            if (v == k) return 1u;
+            if (v == kEmpty) return 0u;
+            h = (h + 1u) & mask;
+        }
+    }
+};
+
+struct SearchSet {  // binary search of the sorted pivot row in HBM
+    const int32_t* col;
+    uint32_t t0, t1;
+    __device__ __forceinline__ uint32_t has(uint32_t k) const {
+        uint32_t lo = t0, hi = t1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if ((uint32_t)col[mid] < k) lo = mid + 1;
+            else hi = mid;
+        }
+        return (lo < t1 && (uint32_t)col[lo] == k) ? 1u : 0u;
+    }
+};
+
+struct WedgeArgs {
+    const int32_t* col;
+    const uint32_t* us;
+    const uint32_t* ue;
+    const int32_t* ur;
+    const uint4* items;  // 2 x uint4 per item: {t0, t1, eb, ee}, {lo, hi, work, 0}
+    uint32_t n_block;    // items [0, n_block) are workgroup items
+    uint32_t n_items;    // items [n_block, n_items) are wave items
+    uint32_t* counters;  // [0] workgroup-item cursor, [1] wave-item cursor (schedule 0)
+    unsigned long long* result;
+    int32_t* pv;
+    int lookup;    // 0 = auto, 1 = never bitmap, 2 = never hash
+    int schedule;  // 0 = work stealing, 1 = static stride
+};
+
+// Barrier of the threads that share one LDS region: the workgroup (NT = 256)
+// or one wave (NT = 64).
+template <int NT>
+__device__ __forceinline__ void team_sync() {
+    if constexpr (NT == kThreads) {
         __syncthreads();
-        const uint32_t it = uniform(s_item);
-        if (it >= nitems) break;
-        const uint4 w = items[it];
-        const uint32_t j = uniform(w.x), eb = uniform(w.y), ee = uniform(w.z);
-        const uint32_t t0 = uniform(rowptr[j]), t1 = uniform(rowptr[j + 1]);
-        const uint32_t d = t1 - t0;  // >= 1 for every listed pivot; 0 would match nothing
-        if (d == 0) {
-        } else if (d <= (uint32_t)kTableSlots / 2) {
-            const int logS = max(6, 32 - __clz(2 * d - 1));
-            const int shift = 32 - logS;
-            const uint32_t mask = (1u << logS) - 1u;
-            for (uint32_t s = threadIdx.x; s <= mask; s += kThreads) table[s] = kEmpty;
-            __syncthreads();
-            for (uint32_t p = t0 + threadIdx.x; p < t1; p += kThreads) {
-                const uint32_t key = (uint32_t)col[p];
-                uint32_t h = hslot(key, shift);
-                while (atomicCAS(&table[h], kEmpty, key) != kEmpty) h = (h + 1u) & mask;
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// Streams the prefixes [us[e], ue[e]) of lists e in [eb, ee) through `set`.
+// A group of G lanes owns one list at a time (lists eb + g, eb + g + NT/G,
+// ...), reads it with aligned 16-byte loads (4 ids per lane, U loads in
+// flight) and masks the ids outside the prefix; the next list's bounds are
+// fetched before the current list is scanned.  (s0, s1) are the bounds of
+// the first list, loaded by the caller.
+template <int NT, int G, int U, bool PV, class Set>
+__device__ __forceinline__ uint64_t scan_lists(const Set& set, const WedgeArgs& a, uint32_t tid, uint32_t eb,
+                                               uint32_t ee, uint32_t s0, uint32_t s1) {
+    constexpr uint32_t NG = NT / G;
+    constexpr uint32_t STEP = 4 * G * U;
+    const uint32_t g = tid / G, gl = tid % G;
+    uint64_t acc = 0;
+    uint32_t e = eb + g;
+    while (e < ee) {
+        const uint32_t en = e + NG;
+        uint32_t n0 = 0, n1 = 0;
+        if (en < ee) {
+            n0 = a.us[en];
+            n1 = a.ue[en];
+        }
+        const uint32_t len = s1 - s0;
+        uint32_t hits = 0;
+        for (uint32_t q = (s0 & ~3u) + 4 * gl; q < s1; q += STEP) {
+            uint4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t qq = q + 4 * G * u;
+                v[u] = qq < s1 ? *reinterpret_cast<const uint4*>(a.col + qq) : make_uint4(0, 0, 0, 0);
             }
-            __syncthreads();
-            for (uint32_t e = eb + wid; e < ee; e += kWaves) {
-                const uint32_t s0 = uniform(us[e]), s1 = uniform(ue[e]);
-                uint32_t hits = 0;
-                for (uint32_t q = s0 + lane; q < s1; q += 64) {
-                    const uint32_t key = (uint32_t)col[q];
-                    uint32_t h = hslot(key, shift);
-                    for (;;) {
-                        const uint32_t v = table[h];
-                        if (v == key) {
-                            ++hits;
-                            break;
-                        }
-                        if (v == kEmpty) break;
-                        h = (h + 1u) & mask;
-                    }
-                }
-                acc += hits;
-                if (PV) {
-                    const uint32_t c = wave_sum_u32(hits);
-                    if (lane == 0 && c) atomicAdd(pv + ur[e], (int32_t)c);
-                }
-            }
-        } else {
-            // pivot row longer than the LDS table: binary search in HBM
-            for (uint32_t e = eb + wid; e < ee; e += kWaves) {
-                const uint32_t s0 = uniform(us[e]), s1 = uniform(ue[e]);
-                uint32_t hits = 0;
-                for (uint32_t q = s0 + lane; q < s1; q += 64) {
-                    const int32_t key = col[q];
-                    uint32_t lo = t0, hi = t1;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (col[mid] < key) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    hits += (lo < t1 && col[lo] == key) ? 1u : 0u;
-                }
-                acc += hits;
-                if (PV) {
-                    const uint32_t c = wave_sum_u32(hits);
-                    if (lane == 0 && c) atomicAdd(pv + ur[e], (int32_t)c);
-                }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t o = q + 4 * G * u - s0;  // offset of v[u].x in the prefix (may wrap)
+                if (o + 0 < len) hits += set.has(v[u].x);
+                if (o + 1 < len) hits += set.has(v[u].y);
+                if (o + 2 < len) hits += set.has(v[u].z);
+                if (o + 3 < len) hits += set.has(v[u].w);
             }
         }
-        __syncthreads();
+        acc += hits;
+        if (PV) {
+            uint32_t c = hits;
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, G);
+            if (gl == 0 && c) atomicAdd(a.pv + a.ur[e], (int32_t)c);
+        }
+        e = en;
+        s0 = n0;
+        s1 = n1;
+    }
+    return acc;
+}
+
+// One work item (pivot row [t0, t1) of col, lists [eb, ee)) processed by NT
+// threads that share W words of zeroed LDS.  N_L(j) goes into LDS as a
+// bitmap window when max - min < 32 W, else into a hash table when
+// d_L(j) <= W / 2, else stays in HBM for binary search.  The LDS words are
+// zero again on return.
+template <int NT, int W, int G, int U, bool PV>
+__device__ __forceinline__ uint64_t process_item(const WedgeArgs& a, uint4 A, uint4 B, uint32_t* lds,
+                                                 uint32_t tid) {
+    const uint32_t t0 = A.x, t1 = A.y, eb = A.z, ee = A.w, lo = B.x, hi = B.y;
+    const uint32_t d = t1 - t0;  // >= 1 for every listed pivot
+    // bounds of this thread's first list, fetched while the set is built
+    uint32_t s0 = 0, s1 = 0;
+    {
+        const uint32_t e = eb + tid / G;
+        if (e < ee) {
+            s0 = a.us[e];
+            s1 = a.ue[e];
+        }
+    }
+    uint64_t acc = 0;
+    if (hi - lo < 32u * W && a.lookup != 1) {
+        for (uint32_t p = t0 + tid; p < t1; p += NT) {
+            const uint32_t x = (uint32_t)a.col[p] - lo;
+            atomicOr(&lds[x >> 5], 1u << (x & 31u));
+        }
+        team_sync<NT>();
+        acc = scan_lists<NT, G, U, PV>(BitmapSet{lds, lo}, a, tid, eb, ee, s0, s1);
+        team_sync<NT>();
+        for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)a.col[p] - lo) >> 5] = 0;
+    } else if (d <= (uint32_t)W / 2 && a.lookup != 2) {
+        constexpr int kMaxLog = W == 4096 ? 12 : 10;
+        const int logS = min(kMaxLog, max(6, 32 - __clz(4 * d - 1)));  // S >= 4d while it fits
+        const int shift = 32 - logS;
+        const uint32_t mask = (1u << logS) - 1u;
+        for (uint32_t x = tid; x <= mask; x += NT) lds[x] = kEmpty;
+        team_sync<NT>();
+        for (uint32_t p = t0 + tid; p < t1; p += NT) {
+            const uint32_t key = (uint32_t)a.col[p];
+            uint32_t h = hslot(key, shift);
+            while (atomicCAS(&lds[h], kEmpty, key) != kEmpty) h = (h + 1u) & mask;
+        }
+        team_sync<NT>();
+        acc = scan_lists<NT, G, U, PV>(HashSet{lds, shift, mask}, a, tid, eb, ee, s0, s1);
+        team_sync<NT>();
+        for (uint32_t x = tid; x <= mask; x += NT) lds[x] = 0;
+    } else {
+        acc = scan_lists<NT, G, U, PV>(SearchSet{a.col, t0, t1}, a, tid, eb, ee, s0, s1);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ uint4 uniform4(uint4 v) {
+    return make_uint4(uniform(v.x), uniform(v.y), uniform(v.z), uniform(v.w));
+}
+
+// Persistent grid.  Phase 1: workgroup items (heavy pivots, sorted by work,
+// 16 KiB LDS each).  Phase 2: wave items (pivots with few lists), each wave
+// with its own 4 KiB slice.  schedule 1 strides statically over the sorted
+// items and prefetches the next item; schedule 0 takes items from an
+// atomic cursor.
+template <int G, int U, bool PV>
+__global__ __launch_bounds__(kThreads) void wedge_kernel(WedgeArgs a) {
+    __shared__ uint32_t lds[kLdsWords];
+    __shared__ uint32_t s_item;
+    const uint32_t tid = threadIdx.x;
+    const int lane = tid & 63;
+    const uint32_t wid = uniform(tid >> 6);
+    for (int i = tid; i < kLdsWords; i += kThreads) lds[i] = 0;
+    __syncthreads();
+    uint64_t acc = 0;
+
+    // phase 1: workgroup items
+    if (a.schedule == 1) {
+        uint32_t it = blockIdx.x;
+        uint4 A = make_uint4(0, 0, 0, 0), B = A;
+        if (it < a.n_block) {
+            A = uniform4(a.items[2 * it]);
+            B = uniform4(a.items[2 * it + 1]);
+        }
+        while (it < a.n_block) {
+            const uint32_t nx = it + gridDim.x;
+            uint4 nA = make_uint4(0, 0, 0, 0), nB = nA;
+            if (nx < a.n_block) {
+                nA = uniform4(a.items[2 * nx]);
+                nB = uniform4(a.items[2 * nx + 1]);
+            }
+            acc += process_item<kThreads, kLdsWords, G, U, PV>(a, A, B, lds, tid);
+            __syncthreads();
+            it = nx;
+            A = nA;
+            B = nB;
+        }
+    } else {
+        for (;;) {
+            if (tid == 0) s_item = atomicAdd(a.counters, 1u);
+            __syncthreads();
+            const uint32_t it = uniform(s_item);
+            if (it >= a.n_block) break;
+            acc += process_item<kThreads, kLdsWords, G, U, PV>(a, uniform4(a.items[2 * it]),
+                                                                uniform4(a.items[2 * it + 1]), lds, tid);
+            __syncthreads();
+        }
+    }
+
+    // phase 2: wave items, LDS split into four zeroed slices
+    uint32_t* slice = lds + wid * kWaveWords;
+    const uint32_t nwave = a.n_items - a.n_block;
+    if (a.schedule == 1) {
+        const uint32_t stride = gridDim.x * (kThreads / 64);
+        uint32_t it = blockIdx.x * (kThreads / 64) + wid;
+        uint4 A = make_uint4(0, 0, 0, 0), B = A;
+        if (it < nwave) {
+            A = uniform4(a.items[2 * (a.n_block + it)]);
+            B = uniform4(a.items[2 * (a.n_block + it) + 1]);
+        }
+        while (it < nwave) {
+            const uint32_t nx = it + stride;
+            uint4 nA = make_uint4(0, 0, 0, 0), nB = nA;
+            if (nx < nwave) {
+                nA = uniform4(a.items[2 * (a.n_block + nx)]);
+                nB = uniform4(a.items[2 * (a.n_block + nx) + 1]);
+            }
+            acc += process_item<64, kWaveWords, G, U, PV>(a, A, B, slice, (uint32_t)lane);
+            team_sync<64>();
+            it = nx;
+            A = nA;
+            B = nB;
+        }
+    } else {
+        for (;;) {
+            uint32_t it = 0;
+            if (lane == 0) it = atomicAdd(a.counters + 1, 1u);
+            it = uniform(__shfl(it, 0));
+            if (it >= nwave) break;
+            acc += process_item<64, kWaveWords, G, U, PV>(a, uniform4(a.items[2 * (a.n_block + it)]),
+                                                          uniform4(a.items[2 * (a.n_block + it) + 1]), slice,
+                                                          (uint32_t)lane);
+            team_sync<64>();
+        }
     }
     acc = wave_sum_u64(acc);
-    if (lane == 0 && acc) atomicAdd(result, (unsigned long long)acc);
+    if (lane == 0 && acc) atomicAdd(a.result, (unsigned long long)acc);
+}
+
+using WedgeFn = void (*)(WedgeArgs);
+
+template <int G, int U>
+WedgeFn pick_pv(bool pv) {
+    return pv ? wedge_kernel<G, U, true> : wedge_kernel<G, U, false>;
+}
+
+template <int G>
+WedgeFn pick_u(int u, bool pv) {
+    switch (u) {
+        case 1: return pick_pv<G, 1>(pv);
+        case 2: return pick_pv<G, 2>(pv);
+        default: return pick_pv<G, 4>(pv);
+    }
+}
+
+WedgeFn pick_wedge(int g, int u, bool pv) {
+    switch (g) {
+        case 16: return pick_u<16>(u, pv);
+        case 32: return pick_u<32>(u, pv);
+        default: return pick_u<8>(u, pv);
+    }
 }
 
 }  // namespace
@@ -130,13 +338,15 @@ struct btc_plan_s {
     btc_graph_s* g = nullptr;
     int K = 0;
     int tau = 0;
+    btc_count_opts launch{};  // launch-only options (btc_plan_tune)
     std::unique_ptr<btc_bitbsr_s> bsr;
     btcx::DevBuf<uint32_t> dense_list;
     uint32_t n_dense = 0;
     btcx::DevBuf<uint32_t> ustart, uend;
     btcx::DevBuf<int32_t> urow;
-    btcx::DevBuf<uint4> items;
+    btcx::DevBuf<uint4> items;  // 2 per item
     uint32_t nitems = 0;
+    uint32_t n_block_items = 0;
     btcx::DevBuf<unsigned long long> scratch;  // [0] total, [1] work-item counter
     btc_plan_info info{};
     float last_dense_ms = 0.f;
@@ -246,50 +456,90 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
             ue[e] = s1;
             ur[e] = i;
         }
-    // work items: consecutive lists of one pivot, about `chunk` checks each
-    std::vector<uint4> items;
+    // work items: consecutive lists of one pivot, about `chunk` checks each.
+    // Pivots with few lists become wave items; the rest workgroup items.
+    struct Item {
+        uint4 a, b;
+    };
+    std::vector<Item> block_items, wave_items;
     int64_t checks = 0, pivot_words = 0;
     for (int32_t j = 0; j < n; ++j) {
         int64_t e = off[j], e1 = off[j + 1];
+        if (e == e1) continue;
+        const uint32_t t0 = (uint32_t)rp[j], t1 = (uint32_t)rp[j + 1];
+        const uint32_t lo = (uint32_t)cl[rp[j]], hi = (uint32_t)cl[rp[j + 1] - 1];
+        const bool wave_fit = (hi - lo < 32u * kWaveWords) || (t1 - t0 <= (uint32_t)kWaveWords / 2);
+        const bool small = e1 - e <= (int64_t)kWaveItemLists && wave_fit;
         while (e < e1) {
             int64_t w = 0, f = e;
-            while (f < e1 && (w == 0 || w + (ue[f] - us[f]) <= chunk)) {
+            while (f < e1 && (small || w == 0 || w + (ue[f] - us[f]) <= chunk)) {
                 w += ue[f] - us[f];
                 ++f;
             }
             checks += w;
-            pivot_words += rp[j + 1] - rp[j];
-            items.push_back(make_uint4((uint32_t)j, (uint32_t)e, (uint32_t)f, (uint32_t)std::min<int64_t>(w, 0xFFFFFFFFLL)));
+            pivot_words += t1 - t0;
+            Item it{make_uint4(t0, t1, (uint32_t)e, (uint32_t)f),
+                    make_uint4(lo, hi, (uint32_t)std::min<int64_t>(w, 0xFFFFFFFFLL), 0u)};
+            (small ? wave_items : block_items).push_back(it);
             e = f;
         }
     }
-    std::stable_sort(items.begin(), items.end(), [](const uint4& a, const uint4& b) { return a.w > b.w; });
+    auto by_work = [](const Item& x, const Item& y) { return x.b.z > y.b.z; };
+    std::stable_sort(block_items.begin(), block_items.end(), by_work);
+    std::stable_sort(wave_items.begin(), wave_items.end(), by_work);
+    std::vector<Item> items(block_items);
+    items.insert(items.end(), wave_items.begin(), wave_items.end());
     p->nitems = (uint32_t)items.size();
+    p->n_block_items = (uint32_t)block_items.size();
     p->info.sparse_edges = sparse_edges;
     p->info.sparse_merge_work = sparse_work;
     p->info.wedge_checks = checks;
-    // prefix ids + pivot rows (4 B each), list entries (start, end, row: 12 B), items (16 B)
-    p->info.stream_bytes = 4 * (checks + pivot_words) + 12 * ne + 16 * (int64_t)items.size();
+    // prefix ids + pivot rows (4 B each), list entries (start, end, row: 12 B), items (32 B)
+    p->info.stream_bytes = 4 * (checks + pivot_words) + 12 * ne + 32 * (int64_t)items.size();
     p->info.work_items = (int64_t)items.size();
     p->ustart.alloc((size_t)ne);
     p->uend.alloc((size_t)ne);
     p->urow.alloc((size_t)ne);
-    p->items.alloc(items.size());
+    p->items.alloc(2 * items.size());
     if (ne) {
         BTC_HIP(hipMemcpyAsync(p->ustart.p, us.data(), (size_t)ne * 4, hipMemcpyHostToDevice, s));
         BTC_HIP(hipMemcpyAsync(p->uend.p, ue.data(), (size_t)ne * 4, hipMemcpyHostToDevice, s));
         BTC_HIP(hipMemcpyAsync(p->urow.p, ur.data(), (size_t)ne * 4, hipMemcpyHostToDevice, s));
     }
     if (!items.empty())
-        BTC_HIP(hipMemcpyAsync(p->items.p, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
+        BTC_HIP(hipMemcpyAsync(p->items.p, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
     BTC_HIP(hipStreamSynchronize(s));
 }
 
-template <bool PV>
-int occupancy_blocks() {
-    int nb = 0;
-    BTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, wedge_kernel<PV>, kThreads, 0));
-    return std::max(1, nb);
+}  // namespace
+
+void check_launch_opts(const btc_count_opts& o) {
+    BTC_REQUIRE(o.lookup >= 0 && o.lookup <= 2, BTC_ERR_ARG, "lookup %d (expected 0, 1 or 2)", o.lookup);
+    BTC_REQUIRE(o.group_lanes == 0 || o.group_lanes == 8 || o.group_lanes == 16 || o.group_lanes == 32,
+                BTC_ERR_ARG, "group_lanes %d (expected 0, 8, 16 or 32)", o.group_lanes);
+    BTC_REQUIRE(o.unroll == 0 || o.unroll == 1 || o.unroll == 2 || o.unroll == 4, BTC_ERR_ARG,
+                "unroll %d (expected 0, 1, 2 or 4)", o.unroll);
+    BTC_REQUIRE(o.schedule == 0 || o.schedule == 1, BTC_ERR_ARG, "schedule %d (expected 0 or 1)", o.schedule);
+    BTC_REQUIRE(o.waves_per_eu >= 0, BTC_ERR_ARG, "waves_per_eu %d < 0", o.waves_per_eu);
+}
+
+namespace {
+
+void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counters, int32_t* d_pv, hipStream_t s) {
+    const btc_count_opts& o = p->launch;
+    const int G = o.group_lanes ? o.group_lanes : 8;
+    const int U = o.unroll ? o.unroll : 4;
+    WedgeFn fn = pick_wedge(G, U, d_pv != nullptr);
+    int per_cu = 0;
+    BTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kThreads, 0));
+    per_cu = std::max(1, per_cu);
+    if (o.waves_per_eu > 0) per_cu = std::min(per_cu, (int)o.waves_per_eu);
+    const unsigned grid = (unsigned)(device_cu_count() * per_cu);
+    btc_graph_s* g = p->g;
+    WedgeArgs a{g->d_col.p, p->ustart.p, p->uend.p, p->urow.p, p->items.p, p->n_block_items, p->nitems,
+                d_counters, d_total, d_pv, (int)o.lookup, (int)o.schedule};
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
+    BTC_CHECK_LAUNCH("wedge_kernel");
 }
 
 void plan_count(btc_plan_s* p, hipStream_t s, uint64_t* total, int32_t* per_vertex, float* kernel_ms) {
@@ -303,25 +553,12 @@ void plan_count(btc_plan_s* p, hipStream_t s, uint64_t* total, int32_t* per_vert
     }
     BTC_HIP(hipMemsetAsync(p->scratch.p, 0, 2 * sizeof(unsigned long long), s));
     unsigned long long* d_total = p->scratch.p;
-    uint32_t* d_counter = reinterpret_cast<uint32_t*>(p->scratch.p + 1);
+    uint32_t* d_counters = reinterpret_cast<uint32_t*>(p->scratch.p + 1);
     EventTimer tm(kernel_ms != nullptr), tw(kernel_ms != nullptr);
     tm.start(s);
     if (p->n_dense) launch_tile_count(p->bsr.get(), p->dense_list.p, p->n_dense, false, -1, d_total, pv.p, s);
     tw.start(s);
-    if (p->nitems) {
-        const int cus = device_cu_count();
-        if (per_vertex) {
-            const unsigned grid = (unsigned)(cus * occupancy_blocks<true>());
-            hipLaunchKernelGGL(wedge_kernel<true>, dim3(grid), dim3(kThreads), 0, s, g->d_rowptr.p, g->d_col.p,
-                               p->ustart.p, p->uend.p, p->urow.p, p->items.p, p->nitems, d_counter, d_total, pv.p);
-        } else {
-            const unsigned grid = (unsigned)(cus * occupancy_blocks<false>());
-            hipLaunchKernelGGL(wedge_kernel<false>, dim3(grid), dim3(kThreads), 0, s, g->d_rowptr.p, g->d_col.p,
-                               p->ustart.p, p->uend.p, p->urow.p, p->items.p, p->nitems, d_counter, d_total,
-                               nullptr);
-        }
-        BTC_CHECK_LAUNCH("wedge_kernel");
-    }
+    if (p->nitems) launch_wedge(p, d_total, d_counters, pv.p, s);
     tm.stop(s);
     tw.stop(s);
     unsigned long long h = 0;
@@ -347,6 +584,7 @@ void btc_count_opts_default(btc_count_opts* o) {
     std::memset(o, 0, sizeof(*o));
     o->tile_k = 0;
     o->tau = -1;
+    o->schedule = 1;
 }
 
 int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_plan* out) {
@@ -363,6 +601,8 @@ int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_p
         p->g = g;
         p->K = o.tile_k ? o.tile_k : 64;
         p->tau = o.tau >= 0 ? o.tau : btc_default_tau(p->K);
+        btcx::check_launch_opts(o);
+        p->launch = o;
         btcx::ensure_device(g, s);
         std::vector<uint8_t> dense;
         btcx::classify_dense(p.get(), s, dense);
@@ -382,6 +622,14 @@ int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_p
                                p->bsr->tiles.bytes());
         p->info.device_bytes = bytes;
         *out = p.release();
+    });
+}
+
+int btc_plan_tune(btc_plan p, const btc_count_opts* opts) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(p && opts, BTC_ERR_ARG, "NULL argument");
+        btcx::check_launch_opts(*opts);
+        p->launch = *opts;
     });
 }
 

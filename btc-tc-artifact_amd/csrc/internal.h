@@ -23,6 +23,9 @@ struct btc_bitbsr_s {
 namespace btcx {
 
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;
+// The column array of L is padded so 16-byte loads that start inside a row
+// may run up to 3 ids past the last entry.
+constexpr size_t kColPad = 16;
 
 // Uploads L (rowptr narrowed to u32) to HBM on first use.
 void ensure_device(btc_graph_s* g, hipStream_t s);

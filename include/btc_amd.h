@@ -170,12 +170,14 @@ int32_t btc_default_tau(int32_t tile_k);
  * workgroup per pivot j (work-stealing over pivot chunks).  Same result as
  * btc_bitbsr_count and the CPU merge, bit for bit. */
 typedef struct {
-    int32_t tile_k;        /* dense tile width (32/64/128); 0 = auto */
-    int32_t tau;           /* dense threshold; < 0 = default for tile_k; > 16*K disables dense */
-    int32_t chunk_work;    /* wedge checks per work item; 0 = auto */
-    int32_t waves_per_eu;  /* occupancy hint for the persistent grid; 0 = auto */
-    int32_t lookup;        /* 0 = auto, 1 = LDS hash, 2 = LDS bitmap window */
-    int32_t reserved[3];
+    int32_t tile_k;       /* dense tile width (32/64/128); 0 = auto */
+    int32_t tau;          /* dense threshold; < 0 = default for tile_k; >= 16*K disables dense */
+    int32_t chunk_work;   /* wedge checks per work item; 0 = auto */
+    int32_t waves_per_eu; /* resident workgroups (4 waves) per CU of the persistent grid; 0 = max */
+    int32_t lookup;       /* 0 = auto, 1 = never the LDS bitmap, 2 = never the LDS hash */
+    int32_t group_lanes;  /* lanes streaming one prefix list: 8, 16 or 32; 0 = auto */
+    int32_t unroll;       /* 16-byte column loads in flight per lane: 1, 2 or 4; 0 = auto */
+    int32_t schedule;     /* 0 = work stealing on one counter, 1 = static stride over sorted items */
 } btc_count_opts;
 
 typedef struct btc_plan_s* btc_plan;
@@ -196,6 +198,9 @@ typedef struct {
 void btc_count_opts_default(btc_count_opts* opts);
 int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_plan* out);
 int btc_plan_get_info(btc_plan p, btc_plan_info* info);
+/* Changes the launch-only options of a plan (waves_per_eu, lookup,
+ * group_lanes, unroll, schedule); tile_k, tau and chunk_work need a new plan. */
+int btc_plan_tune(btc_plan p, const btc_count_opts* opts);
 int btc_plan_count(btc_plan p, void* stream, uint64_t* total, int32_t* per_vertex, float* kernel_ms);
 /* Event times of the last btc_plan_count with kernel_ms != NULL, split by
  * kernel: dense-tile kernel and wedge kernel (0 when not launched). */

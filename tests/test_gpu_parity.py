@@ -109,6 +109,45 @@ def test_engine_matches_oracle(gpu, orc, name, K, tau):
     assert info["sparse_edges"] + info["dense_edges"] == g.m
 
 
+@pytest.mark.parametrize("lookup", [0, 1, 2])
+@pytest.mark.parametrize("name", ["rmat15", "messy", "planted", "stencil3d"])
+def test_engine_lookup_modes(gpu, orc, name, lookup):
+    """Bitmap window, LDS hash and HBM binary search give the same counts."""
+    s, d, n = case(name, gpu)
+    for flags in (gpu.CANON_COMPACT, gpu.CANON_DEFAULT):
+        g = gpu.Graph.from_coo(s, d, n, flags)
+        want, want_pv = oracle_counts(orc, g)
+        t, pv, _ = gpu.Plan(g, lookup=lookup, tau=100000).count(per_vertex=True)
+        assert t == want
+        np.testing.assert_array_equal(pv, want_pv)
+
+
+def test_engine_launch_shapes(gpu, orc):
+    """Every group width / unroll / schedule / occupancy cap counts the same."""
+    for name in ("rmat15", "messy", "stencil2d"):
+        s, d, n = case(name, gpu)
+        g = gpu.Graph.from_coo(s, d, n)
+        want, want_pv = oracle_counts(orc, g)
+        p = gpu.Plan(g)
+        for G in (8, 16, 32):
+            for U in (1, 2, 4):
+                for sched in (0, 1):
+                    p.tune(group_lanes=G, unroll=U, schedule=sched, waves_per_eu=(0 if U != 2 else 1))
+                    t, pv, _ = p.count(per_vertex=(G == 16))
+                    assert t == want, (name, G, U, sched)
+                    if pv is not None:
+                        np.testing.assert_array_equal(pv, want_pv)
+
+
+def test_plan_tune_rejects_bad_options(gpu):
+    g = gpu.Graph.from_coo(np.array([1, 2, 2], np.int32), np.array([0, 0, 1], np.int32))
+    p = gpu.Plan(g)
+    for bad in ({"group_lanes": 12}, {"unroll": 3}, {"schedule": 2}, {"lookup": 5}):
+        with pytest.raises(gpu.BtcError):
+            p.tune(**bad)
+    assert p.count()[0] == 1
+
+
 @pytest.mark.parametrize("flags", [0, 1, 2, 3])
 def test_engine_any_labelling(gpu, orc, flags):
     s, d, n = case("rmat15", gpu)
