@@ -81,6 +81,25 @@ def load_traffic(workload):
         return None
 
 
+def copy_calibration(mb):
+    """One device-to-device copy of `mb` MiB (HIP's blit copy kernel), so a
+    rocprofv3 counter run can scale FETCH_SIZE / WRITE_SIZE by the bytes a
+    plain copy of known size reports."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy calibration: %d MiB device-to-device" % mb)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +111,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--plain-runs", type=int, default=3, help="reference-semantics Bit-BSR yardstick runs (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run one D2D copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -106,6 +127,8 @@ def main():
     if rc != 0:
         raise SystemExit("no HIP device: " + btc.lib().btc_last_error().decode())
 
+    if args.copy_calibration_mb > 0:
+        copy_calibration(args.copy_calibration_mb)
     t0 = time.perf_counter()
     src, dst, n_in, config = make_workload(btc, args.workload)
     log("generated %d pairs in %.2f s" % (src.size, time.perf_counter() - t0))

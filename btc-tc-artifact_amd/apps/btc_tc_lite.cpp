@@ -1,0 +1,119 @@
+// btc_tc_lite -- the reference's Lite CLI (apps/btc_tc_lite.cu:9-152) on the
+// MI355X engine: same flags (-i -v -w -r), same BTC_MODE tile override, same
+// stdout contract (btc_tc_lite.cu:141-149), so scripts that parse the
+// reference's output read this one unchanged.
+//
+// BTC_ENGINE=plain runs the reference-semantics Bit-BSR kernels with the
+// Lite tile policy; the default runs the hybrid engine (dense tiles of the
+// same width + LDS wedge checks).
+#include <btc/btc.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+int main(int argc, char** argv) try {
+    btc::CsrMatrix<int, float, btc::device_memory> A_csr;
+    btc::CooMatrix<int, float, btc::device_memory> A_coo;
+    btc::Config config = btc::program_options(argc, argv);
+    int mode = 0;
+    if (const char* env = std::getenv("BTC_MODE")) mode = std::atoi(env);
+    const char* eng = std::getenv("BTC_ENGINE");
+    const bool plain = eng && std::strcmp(eng, "plain") == 0;
+    btc::check(btc_device_init(0));
+
+    btc::read_matrix_file(A_csr, config.input_file);
+    btc::convert_csr_to_coo(A_coo, A_csr);
+    A_csr.free();
+
+    btc::CPUTimer timer;
+    timer.start();
+    btc::preprocess_for_triangle_counting(A_coo, false, true, config.input_file);
+    timer.stop();
+    const float t_pre = timer.elapsed();
+
+    std::printf("\n--------------btc Lite-Adaptive (O(1) Heuristic)----------------\n");
+    const float avg_degree = A_coo.num_rows ? (float)A_coo.num_entries / (float)A_coo.num_rows : 0.f;
+    std::printf("Stats: Rows=%d, NNZ=%zu, AvgDegree=%.2f\n", A_coo.num_rows, (size_t)A_coo.num_entries, avg_degree);
+    const int K = btc_lite_tile_k(A_coo.num_rows, A_coo.num_entries, mode);
+    if (mode == 1 || mode == 2)
+        std::printf("Decision: USE %d (Reason: Forced Mode)\n", K);
+    else if (K == 32)
+        std::printf("Decision: USE 32 (Reason: Lite V3 - Deg < 15 or Rows < 50k)\n");
+    else
+        std::printf("Decision: USE 128 (Reason: Lite V3 - Dense & Large)\n");
+
+    unsigned long long count = 0;
+    std::vector<float> kernel_times((size_t)std::max(config.repeat, 1), 0.f);
+    btc::CPUTimer timer_conv;
+    timer.start();
+    if (plain) {
+        auto run = [&](auto& d_bcsr, int threshold) {
+            timer_conv.start();
+            if constexpr (std::is_same_v<std::decay_t<decltype(d_bcsr)>, btc::BCSR_16x128_Device>)
+                btc::convert_coo_to_bcsr_16x128_gpu(d_bcsr, A_coo);
+            else
+                btc::convert_coo_to_bcsr_16x32_gpu(d_bcsr, A_coo);
+            timer_conv.stop();
+            for (int w = 0; w < config.warmup; ++w) {
+                float tmp = 0.f;
+                btc::count_bitbsr(d_bcsr, BTC_VARIANT_AUTO, threshold, &tmp);
+            }
+            for (int r = 0; r < config.repeat; ++r)
+                count = btc::count_bitbsr(d_bcsr, BTC_VARIANT_AUTO, threshold, &kernel_times[(size_t)r]);
+            d_bcsr.free();
+        };
+        if (K == 128) {
+            btc::BCSR_16x128_Device d;
+            run(d, 2048);
+        } else {
+            btc::BCSR_16x32_Device d;
+            run(d, 40000);
+        }
+    } else {
+        timer_conv.start();
+        btc::Graph g = btc::graph_from_lower_coo(A_coo);
+        btc_count_opts o;
+        btc_count_opts_default(&o);
+        o.tile_k = K;
+        btc_plan p = nullptr;
+        btc::check(btc_plan_create(g.h, &o, nullptr, &p));
+        timer_conv.stop();
+        uint64_t t = 0;
+        float tmp = 0.f;
+        for (int w = 0; w < config.warmup; ++w) btc::check(btc_plan_count(p, nullptr, &t, nullptr, &tmp));
+        for (int r = 0; r < config.repeat; ++r)
+            btc::check(btc_plan_count(p, nullptr, &t, nullptr, &kernel_times[(size_t)r]));
+        count = t;
+        btc_plan_free(p);
+    }
+    timer.stop();
+    std::vector<float> sorted(kernel_times.begin(), kernel_times.begin() + std::max(config.repeat, 0));
+    std::sort(sorted.begin(), sorted.end());
+    const float kernel_ms = sorted.empty() ? 0.f : sorted[sorted.size() / 2];
+    const float t_conv = timer_conv.elapsed(), t_e2e = timer.elapsed();
+
+    std::printf("[Preprocessing] time: %.4f ms\n", t_pre);
+    std::printf("[Time Breakdown] Convert: %.4f ms, Compute (Kernel): %.4f ms\n", t_conv, kernel_ms);
+    std::printf("[Total Time (Convert+Compute)] time: %f ms\n", t_e2e);
+    std::printf("[Total Time (Preprocess+Convert+Compute)] time: %f ms\n", t_pre + t_e2e);
+    std::printf("Triangles (GPU): %llu\n", count);
+    std::printf("Timing: warmup=%d, repeat=%d, median kernel reported\n", config.warmup, config.repeat);
+    std::printf("Kernel_runs:");
+    for (int r = 0; r < config.repeat; ++r) std::printf(" %.6f", kernel_times[(size_t)r]);
+    std::printf("\n");
+    if (config.verify) {
+        btc::CsrMatrix<int, float, btc::host_memory> L;
+        btc::convert_coo_to_csr(L, A_coo);
+        const size_t cpu = btc::cpu_tc_intersection_lower_triangular(L);
+        std::printf("Triangles (CPU): %zu\n", cpu);
+        std::printf("Verify: %s\n", cpu == count ? "PASS" : "FAIL");
+        if (cpu != count) return 2;
+    }
+    return 0;
+} catch (const btc::Error& e) {
+    std::fprintf(stderr, "btc_tc_lite: %s (status %d)\n", e.what(), e.code);
+    return 1;
+}

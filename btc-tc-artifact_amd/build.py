@@ -16,6 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "btc_amd", "lib")
 LIB = os.path.join(LIB_DIR, "libbtc_amd.so")
+BIN_DIR = os.path.join(HERE, "bin")
+APPS = os.path.join(HERE, "apps")
 
 ARCH = os.environ.get("BTC_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-Wall", "-Wno-unused-result",
@@ -72,7 +74,28 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed\n" + r.stdout)
     os.replace(tmp, LIB)
+    build_apps(verbose)
     return LIB
+
+
+def build_app(src, out, verbose=False):
+    """Links a host C++ program against libbtc_amd.so (rpath to btc_amd/lib)."""
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", HERE, src, "-o", out,
+           "-L", LIB_DIR, "-lbtc_amd", "-Wl,-rpath," + LIB_DIR]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("app build failed: %s\n%s" % (src, r.stdout))
+    return out
+
+
+def build_apps(verbose=False):
+    os.makedirs(BIN_DIR, exist_ok=True)
+    for f in sorted(os.listdir(APPS)):
+        if f.endswith(".cpp"):
+            build_app(os.path.join(APPS, f), os.path.join(BIN_DIR, f[:-4]), verbose)
 
 
 if __name__ == "__main__":
