@@ -16,7 +16,8 @@ import os
 import numpy as np
 
 __all__ = [
-    "BtcError", "lib", "lib_path", "Graph", "BitBSR", "Plan", "CountOpts", "count_cpu",
+    "BtcError", "lib", "lib_path", "Graph", "BitBSR", "Plan", "Batch", "CountOpts", "count_cpu",
+    "batch_workload",
     "gen_rmat", "gen_er", "gen_stencil2d", "gen_stencil3d", "gen_planted", "lite_tile_k",
     "default_tau", "version", "CANON_COMPACT", "CANON_DEGREE_ORDER", "CANON_DEFAULT",
     "VARIANT_AUTO", "VARIANT_BINSEARCH", "VARIANT_DIRECT",
@@ -70,6 +71,10 @@ class PlanInfo(C.Structure):
                 ("tau", C.c_int32)]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("graphs", C.c_int32), ("n", C.c_int32), ("m", C.c_int64), ("plan", PlanInfo)]
+
+
 _SIGS = {
     "btc_last_error": (C.c_char_p, []),
     "btc_version": (C.c_char_p, []),
@@ -102,6 +107,11 @@ _SIGS = {
     "btc_plan_free": (None, [vp]),
     "btc_count": (C.c_int, [vp, C.POINTER(CountOpts), vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
     "btc_count_cpu": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_uint64), vp]),
+    "btc_batch_create": (C.c_int, [vp, C.c_int32, C.POINTER(CountOpts), vp, C.POINTER(vp)]),
+    "btc_batch_get_info": (C.c_int, [vp, C.POINTER(BatchInfo)]),
+    "btc_batch_tune": (C.c_int, [vp, C.POINTER(CountOpts)]),
+    "btc_batch_count": (C.c_int, [vp, vp, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
+    "btc_batch_free": (None, [vp]),
 }
 
 
@@ -350,3 +360,81 @@ def count_cpu(graph, threads=0, per_vertex=False):
     total, _, pv = _count_out(graph.n, per_vertex)
     _check(lib().btc_count_cpu(graph._h, threads, C.byref(total), _ptr(pv)))
     return total.value, pv
+
+
+class Batch:
+    """Many graphs counted by one persistent launch (btc_batch_*)."""
+
+    def __init__(self, graphs, stream=None, **opts):
+        self.n_graphs = len(graphs)
+        self.opts = make_opts(**opts)
+        arr = (vp * self.n_graphs)(*[g._h.value for g in graphs])
+        h = vp()
+        _check(lib().btc_batch_create(arr, self.n_graphs, C.byref(self.opts), stream, C.byref(h)))
+        self._h = h
+
+    @property
+    def info(self):
+        bi = BatchInfo()
+        _check(lib().btc_batch_get_info(self._h, C.byref(bi)))
+        d = {k: getattr(bi, k) for k, _ in BatchInfo._fields_ if k != "plan"}
+        d["plan"] = {k: getattr(bi.plan, k) for k, _ in PlanInfo._fields_}
+        return d
+
+    def tune(self, **opts):
+        o = make_opts(**opts)
+        _check(lib().btc_batch_tune(self._h, C.byref(o)))
+
+    def count(self, stream=None):
+        """Returns (total, per-graph uint64 array, kernel_ms)."""
+        per = np.zeros(self.n_graphs, np.uint64)
+        total = C.c_uint64(0)
+        ms = C.c_float(0)
+        _check(lib().btc_batch_count(self._h, stream, _ptr(per), C.byref(total), C.byref(ms)))
+        return total.value, per, ms.value
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().btc_batch_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_workload(count=4096, seed=1, n_lo=256, n_hi=16384):
+    """The batched workload: `count` graphs with n log-uniform in
+    [n_lo, n_hi]; 40 % Erdos-Renyi (average degree uniform in [8, 32]), 40 %
+    R-MAT (scale ceil(log2 n), edge factor uniform in {8..16}, a, b, c =
+    0.57, 0.19, 0.19; pairs with an id >= n are dropped), 20 % 9-point
+    stencils (a x b, a = floor(sqrt(n)), b = n // a).  The spec is drawn with
+    numpy's PCG64 from `seed`; graph k's generator seed is drawn with it.
+    Returns a list of (src, dst, n, kind)."""
+    import math
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi)))))
+        r = rng.random()
+        gseed = int(rng.integers(1, 2 ** 62))
+        if r < 0.4:
+            src, dst = gen_er(n, float(rng.uniform(8, 32)), gseed)
+            kind = "er"
+        elif r < 0.8:
+            scale = max(1, math.ceil(math.log2(n)))
+            ef = int(rng.integers(8, 17))
+            src, dst = gen_rmat(scale, ef, 0.57, 0.19, 0.19, gseed)
+            keep = (src < n) & (dst < n)
+            src, dst = src[keep], dst[keep]
+            kind = "rmat"
+        else:
+            a = int(math.isqrt(n))
+            b = n // a
+            src, dst = gen_stencil2d(a, b)
+            n = a * b
+            kind = "stencil2d"
+        out.append((src, dst, n, kind))
+    return out

@@ -1,0 +1,154 @@
+// Batched counting: many graphs in one persistent launch.
+//
+// The canonical L of every graph is placed on the diagonal of one
+// block-diagonal L (graph g's vertices start at a multiple of 128, so no
+// 16 x K tile, K <= 128, holds rows or columns of two graphs), one plan is
+// built for it, and one enqueue of the dense-tile and wedge kernels counts
+// all graphs; the wedge kernel's static stride over work-sorted items
+// balances graphs of very different sizes.  Per-graph totals are segment
+// sums of the per-vertex counts, one workgroup per graph.  No reference
+// counterpart (the reference counts one graph per process).
+#include <algorithm>
+#include <cstring>
+
+#include "device_util.h"
+#include "internal.h"
+
+namespace btcx {
+constexpr int64_t kGraphAlign = 128;
+}
+
+struct btc_batch_s {
+    int32_t count = 0;
+    std::vector<int64_t> voff;  // count + 1 vertex offsets (multiples of 128)
+    btc_graph_s graph;          // the block-diagonal L
+    btc_plan plan = nullptr;
+    btcx::DevBuf<uint32_t> d_voff;
+    btcx::DevBuf<int32_t> d_pv;
+    btcx::DevBuf<unsigned long long> d_pg;
+    std::vector<uint64_t> h_pg;
+    ~btc_batch_s() {
+        if (plan) btc_plan_free(plan);
+    }
+};
+
+namespace btcx {
+namespace {
+
+// per_graph[g] = sum of pv over [voff[g], voff[g + 1]) (padding vertices are 0)
+__global__ __launch_bounds__(256) void graph_sums_kernel(const int32_t* __restrict__ pv,
+                                                         const uint32_t* __restrict__ voff,
+                                                         unsigned long long* __restrict__ per_graph) {
+    __shared__ unsigned long long part[4];
+    const uint32_t g = blockIdx.x;
+    uint64_t s = 0;
+    for (uint32_t v = voff[g] + threadIdx.x; v < voff[g + 1]; v += 256) s += (uint32_t)pv[v];
+    s = wave_sum_u64(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) per_graph[g] = part[0] + part[1] + part[2] + part[3];
+}
+
+}  // namespace
+}  // namespace btcx
+
+extern "C" {
+
+int btc_batch_create(const btc_graph* graphs, int32_t count, const btc_count_opts* opts, void* stream,
+                     btc_batch* out) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(out && (graphs || count == 0), BTC_ERR_ARG, "NULL argument");
+        *out = nullptr;
+        BTC_REQUIRE(count > 0, BTC_ERR_ARG, "a batch needs at least one graph");
+        auto b = std::make_unique<btc_batch_s>();
+        b->count = count;
+        b->voff.assign((size_t)count + 1, 0);
+        int64_t m = 0;
+        for (int32_t k = 0; k < count; ++k) {
+            BTC_REQUIRE(graphs[k], BTC_ERR_ARG, "graph %d is NULL", k);
+            const int64_t n = graphs[k]->n;
+            b->voff[k + 1] = b->voff[k] + (n + btcx::kGraphAlign - 1) / btcx::kGraphAlign * btcx::kGraphAlign;
+            m += graphs[k]->m;
+        }
+        const int64_t N = b->voff[count];
+        BTC_REQUIRE(N < (int64_t)INT32_MAX, BTC_ERR_OVERFLOW, "batch has %lld vertices (limit 2^31 - 1)",
+                    (long long)N);
+        btc_graph_s& G = b->graph;
+        G.n = G.n_input = (int32_t)N;
+        G.flags = 0;
+        G.rowptr.assign((size_t)N + 1, 0);
+        G.col.resize((size_t)m);
+        for (int32_t k = 0; k < count; ++k) {
+            const btc_graph_s* g = graphs[k];
+            const int64_t v0 = b->voff[k];
+            for (int32_t i = 0; i < g->n; ++i) G.rowptr[v0 + i + 1] = g->rowptr[i + 1] - g->rowptr[i];
+        }
+        for (int64_t v = 0; v < N; ++v) G.rowptr[v + 1] += G.rowptr[v];
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int32_t k = 0; k < count; ++k) {
+            const btc_graph_s* g = graphs[k];
+            const int64_t v0 = b->voff[k];
+            int64_t o = G.rowptr[v0];
+            for (int64_t p = 0; p < g->m; ++p) G.col[o + p] = g->col[p] + (int32_t)v0;
+        }
+        G.labels.resize((size_t)N);
+        for (int64_t v = 0; v < N; ++v) G.labels[v] = (int32_t)v;
+        btcx::graph_finalize(&G);
+        hipStream_t s = btcx::as_stream(stream);
+        btcx::check_status(btc_plan_create(&G, opts, stream, &b->plan));
+        std::vector<uint32_t> vo32(b->voff.begin(), b->voff.end());
+        b->d_voff.alloc(vo32.size());
+        b->d_pv.alloc((size_t)N);
+        b->d_pg.alloc((size_t)count);
+        b->h_pg.assign((size_t)count, 0);
+        BTC_HIP(hipMemcpyAsync(b->d_voff.p, vo32.data(), vo32.size() * 4, hipMemcpyHostToDevice, s));
+        BTC_HIP(hipStreamSynchronize(s));
+        *out = b.release();
+    });
+}
+
+int btc_batch_get_info(btc_batch b, btc_batch_info* info) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(b && info, BTC_ERR_ARG, "NULL argument");
+        info->graphs = b->count;
+        info->n = b->graph.n;
+        info->m = b->graph.m;
+        btcx::check_status(btc_plan_get_info(b->plan, &info->plan));
+    });
+}
+
+int btc_batch_tune(btc_batch b, const btc_count_opts* opts) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(b, BTC_ERR_ARG, "NULL handle");
+        btcx::check_status(btc_plan_tune(b->plan, opts));
+    });
+}
+
+int btc_batch_count(btc_batch b, void* stream, uint64_t* per_graph, uint64_t* total, float* kernel_ms) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(b && total, BTC_ERR_ARG, "NULL argument");
+        hipStream_t s = btcx::as_stream(stream);
+        BTC_HIP(hipMemsetAsync(b->d_pv.p, 0, b->d_pv.bytes(), s));
+        btcx::EventTimer tm(kernel_ms != nullptr), tw(kernel_ms != nullptr);
+        btcx::plan_enqueue(b->plan, s, b->d_pv.p, &tm, &tw);
+        hipLaunchKernelGGL(btcx::graph_sums_kernel, dim3((unsigned)b->count), dim3(256), 0, s, b->d_pv.p,
+                           b->d_voff.p, b->d_pg.p);
+        BTC_CHECK_LAUNCH("graph_sums_kernel");
+        if (kernel_ms) tm.stop(s);  // both kernels and the per-graph sums
+        unsigned long long t = 0;
+        BTC_HIP(hipMemcpyAsync(&t, b->plan->scratch.p, 8, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipMemcpyAsync(b->h_pg.data(), b->d_pg.p, (size_t)b->count * 8, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipStreamSynchronize(s));
+        if (kernel_ms) *kernel_ms = tm.ms();
+        uint64_t sum = 0;
+        for (auto x : b->h_pg) sum += x;
+        BTC_REQUIRE(sum == t, BTC_ERR_STATE, "per-graph sums %llu != total %llu", (unsigned long long)sum,
+                    (unsigned long long)t);
+        if (per_graph) std::copy(b->h_pg.begin(), b->h_pg.end(), per_graph);
+        *total = t;
+    });
+}
+
+void btc_batch_free(btc_batch b) { delete b; }
+
+}  // extern "C"

@@ -42,6 +42,11 @@ void clear_error();
         if (!(cond)) ::btcx::fail(code, __VA_ARGS__); \
     } while (0)
 
+// Rethrows a failed status of a nested C-ABI call (its message is kept).
+inline void check_status(int rc) {
+    if (rc != BTC_OK) throw Error{rc};
+}
+
 // Runs f and maps library errors to status codes for the C ABI.
 template <class F>
 int guard(F&& f) {

@@ -42,14 +42,16 @@ constexpr uint32_t kWaveItemLists = 64;
 
 __device__ __forceinline__ uint32_t hslot(uint32_t key, int shift) { return (key * 2654435761u) >> shift; }
 
-// Membership tests for N_L(pivot).  Only ids inside the list's clipped
-// prefix, hence inside [min N_L(j), max N_L(j)], are tested.
-struct BitmapSet {  // bit (k - lo) of an LDS window
-    const uint32_t* bits;
-    uint32_t lo;
+// Membership tests for N_L(pivot).  Ids of a list's clipped prefix lie in
+// [min N_L(j), max N_L(j)]; ids outside the prefix (partial 16-byte loads)
+// are replaced by sentinel(), which every set reports absent.
+struct BitmapSet {  // bit (k - base) of an LDS window; base is a multiple of 32
+    const uint32_t* words;  // LDS window minus base / 32 words: words[k >> 5] is k's word
+    uint32_t last;          // base + window bits - 1: never set
+    __device__ __forceinline__ uint32_t sentinel() const { return last; }
     __device__ __forceinline__ uint32_t has(uint32_t k) const {
-        const uint32_t x = k - lo;
-        return (bits[x >> 5] >> (x & 31u)) & 1u;
+        // v_bfe_u32 reads the bit offset from k's low 5 bits
+        return __builtin_amdgcn_ubfe(words[k >> 5], k, 1);
     }
 };
 
@@ -57,12 +59,13 @@ struct HashSet {  // open addressing, linear probing, load <= 1/2
     const uint32_t* slots;
     int shift;
     uint32_t mask;
+    __device__ __forceinline__ uint32_t sentinel() const { return kEmpty; }
     __device__ __forceinline__ uint32_t has(uint32_t k) const {
         uint32_t h = hslot(k, shift);
         for (;;) {
             const uint32_t v = slots[h];
+            if (v == kEmpty) return 0u;  // before the match test: the sentinel is kEmpty
             if (v == k) return 1u;
-            if (v == kEmpty) return 0u;
             h = (h + 1u) & mask;
         }
     }
@@ -71,6 +74,7 @@ struct HashSet {  // open addressing, linear probing, load <= 1/2
 struct SearchSet {  // binary search of the sorted pivot row in HBM
     const int32_t* col;
     uint32_t t0, t1;
+    __device__ __forceinline__ uint32_t sentinel() const { return kEmpty; }  // above every id
     __device__ __forceinline__ uint32_t has(uint32_t k) const {
         uint32_t lo = t0, hi = t1;
         while (lo < hi) {
@@ -111,25 +115,29 @@ __device__ __forceinline__ void team_sync() {
 }
 
 // Streams the prefixes [us[e], ue[e]) of lists e in [eb, ee) through `set`.
-// A group of G lanes owns one list at a time (lists eb + g, eb + g + NT/G,
-// ...), reads it with aligned 16-byte loads (4 ids per lane, U loads in
-// flight) and masks the ids outside the prefix; the next list's bounds are
-// fetched before the current list is scanned.  (s0, s1) are the bounds of
-// the first list, loaded by the caller.
+// A group of G lanes owns lists eb + g, eb + g + NT/G, ...; it reads a list
+// with aligned 16-byte loads (4 ids per lane, U loads in flight) and
+// replaces ids outside the prefix by the set's sentinel.  The next list's
+// bounds are fetched before the current list is scanned; list bounds are
+// read once per launch, so they are loaded non-temporally and leave the
+// caches to the column ids, which are re-read for every pivot of a row.
+// (s0, s1) are the bounds of the group's first list, loaded by the caller.
 template <int NT, int G, int U, bool PV, class Set>
 __device__ __forceinline__ uint64_t scan_lists(const Set& set, const WedgeArgs& a, uint32_t tid, uint32_t eb,
                                                uint32_t ee, uint32_t s0, uint32_t s1) {
     constexpr uint32_t NG = NT / G;
     constexpr uint32_t STEP = 4 * G * U;
     const uint32_t g = tid / G, gl = tid % G;
+    // 32-bit byte offsets (nnz(L) + padding < 2^30 is enforced by the planner)
+    const char* colb = reinterpret_cast<const char*>(a.col);
+    const uint32_t sent = set.sentinel();
     uint64_t acc = 0;
-    uint32_t e = eb + g;
-    while (e < ee) {
+    for (uint32_t e = eb + g; e < ee; e += NG) {
         const uint32_t en = e + NG;
         uint32_t n0 = 0, n1 = 0;
         if (en < ee) {
-            n0 = a.us[en];
-            n1 = a.ue[en];
+            n0 = __builtin_nontemporal_load(a.us + en);
+            n1 = __builtin_nontemporal_load(a.ue + en);
         }
         const uint32_t len = s1 - s0;
         uint32_t hits = 0;
@@ -138,15 +146,16 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const WedgeArgs& 
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t qq = q + 4 * G * u;
-                v[u] = qq < s1 ? *reinterpret_cast<const uint4*>(a.col + qq) : make_uint4(0, 0, 0, 0);
+                v[u] = qq < s1 ? *reinterpret_cast<const uint4*>(colb + (qq << 2))
+                               : make_uint4(sent, sent, sent, sent);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t o = q + 4 * G * u - s0;  // offset of v[u].x in the prefix (may wrap)
-                if (o + 0 < len) hits += set.has(v[u].x);
-                if (o + 1 < len) hits += set.has(v[u].y);
-                if (o + 2 < len) hits += set.has(v[u].z);
-                if (o + 3 < len) hits += set.has(v[u].w);
+                const uint32_t o = q + 4 * G * u - s0;  // prefix offset of v[u].x (wraps below s0)
+                hits += set.has(o + 0 < len ? v[u].x : sent);
+                hits += set.has(o + 1 < len ? v[u].y : sent);
+                hits += set.has(o + 2 < len ? v[u].z : sent);
+                hits += set.has(o + 3 < len ? v[u].w : sent);
             }
         }
         acc += hits;
@@ -156,7 +165,6 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const WedgeArgs& 
             for (int o = G / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, G);
             if (gl == 0 && c) atomicAdd(a.pv + a.ur[e], (int32_t)c);
         }
-        e = en;
         s0 = n0;
         s1 = n1;
     }
@@ -178,20 +186,22 @@ __device__ __forceinline__ uint64_t process_item(const WedgeArgs& a, uint4 A, ui
     {
         const uint32_t e = eb + tid / G;
         if (e < ee) {
-            s0 = a.us[e];
-            s1 = a.ue[e];
+            s0 = __builtin_nontemporal_load(a.us + e);
+            s1 = __builtin_nontemporal_load(a.ue + e);
         }
     }
     uint64_t acc = 0;
-    if (hi - lo < 32u * W && a.lookup != 1) {
+    const uint32_t base = lo & ~31u;
+    if (hi - base < 32u * W - 1u && a.lookup != 1) {  // the window's last bit stays 0 (sentinel)
+        const uint32_t bw = base >> 5;
         for (uint32_t p = t0 + tid; p < t1; p += NT) {
-            const uint32_t x = (uint32_t)a.col[p] - lo;
-            atomicOr(&lds[x >> 5], 1u << (x & 31u));
+            const uint32_t k = (uint32_t)a.col[p];
+            atomicOr(&lds[(k >> 5) - bw], 1u << (k & 31u));
         }
         team_sync<NT>();
-        acc = scan_lists<NT, G, U, PV>(BitmapSet{lds, lo}, a, tid, eb, ee, s0, s1);
+        acc = scan_lists<NT, G, U, PV>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, s0, s1);
         team_sync<NT>();
-        for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)a.col[p] - lo) >> 5] = 0;
+        for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)a.col[p] >> 5) - bw] = 0;
     } else if (d <= (uint32_t)W / 2 && a.lookup != 2) {
         constexpr int kMaxLog = W == 4096 ? 12 : 10;
         const int logS = min(kMaxLog, max(6, 32 - __clz(4 * d - 1)));  // S >= 4d while it fits
@@ -333,25 +343,6 @@ WedgeFn pick_wedge(int g, int u, bool pv) {
 
 }  // namespace
 }  // namespace btcx
-
-struct btc_plan_s {
-    btc_graph_s* g = nullptr;
-    int K = 0;
-    int tau = 0;
-    btc_count_opts launch{};  // launch-only options (btc_plan_tune)
-    std::unique_ptr<btc_bitbsr_s> bsr;
-    btcx::DevBuf<uint32_t> dense_list;
-    uint32_t n_dense = 0;
-    btcx::DevBuf<uint32_t> ustart, uend;
-    btcx::DevBuf<int32_t> urow;
-    btcx::DevBuf<uint4> items;  // 2 per item
-    uint32_t nitems = 0;
-    uint32_t n_block_items = 0;
-    btcx::DevBuf<unsigned long long> scratch;  // [0] total, [1] work-item counter
-    btc_plan_info info{};
-    float last_dense_ms = 0.f;
-    float last_wedge_ms = 0.f;
-};
 
 namespace btcx {
 namespace {
@@ -528,7 +519,7 @@ namespace {
 void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counters, int32_t* d_pv, hipStream_t s) {
     const btc_count_opts& o = p->launch;
     const int G = o.group_lanes ? o.group_lanes : 8;
-    const int U = o.unroll ? o.unroll : 4;
+    const int U = o.unroll ? o.unroll : 1;
     WedgeFn fn = pick_wedge(G, U, d_pv != nullptr);
     int per_cu = 0;
     BTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kThreads, 0));
@@ -542,27 +533,35 @@ void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counte
     BTC_CHECK_LAUNCH("wedge_kernel");
 }
 
+}  // namespace
+
+void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, EventTimer* tw) {
+    BTC_REQUIRE(!d_pv || p->g->max_row <= 65536, BTC_ERR_OVERFLOW,
+                "longest row %d > 65536: per-vertex counts could overflow int32", p->g->max_row);
+    BTC_HIP(hipMemsetAsync(p->scratch.p, 0, 2 * sizeof(unsigned long long), s));
+    unsigned long long* d_total = p->scratch.p;
+    uint32_t* d_counters = reinterpret_cast<uint32_t*>(p->scratch.p + 1);
+    if (tm) tm->start(s);
+    if (p->n_dense) launch_tile_count(p->bsr.get(), p->dense_list.p, p->n_dense, false, -1, d_total, d_pv, s);
+    if (tw) tw->start(s);
+    if (p->nitems) launch_wedge(p, d_total, d_counters, d_pv, s);
+    if (tm) tm->stop(s);
+    if (tw) tw->stop(s);
+}
+
+namespace {
+
 void plan_count(btc_plan_s* p, hipStream_t s, uint64_t* total, int32_t* per_vertex, float* kernel_ms) {
     btc_graph_s* g = p->g;
-    BTC_REQUIRE(!per_vertex || g->max_row <= 65536, BTC_ERR_OVERFLOW,
-                "longest row %d > 65536: per-vertex counts could overflow int32", g->max_row);
     DevBuf<int32_t> pv;
     if (per_vertex) {
         pv.alloc((size_t)g->n);
         BTC_HIP(hipMemsetAsync(pv.p, 0, (size_t)g->n * 4, s));
     }
-    BTC_HIP(hipMemsetAsync(p->scratch.p, 0, 2 * sizeof(unsigned long long), s));
-    unsigned long long* d_total = p->scratch.p;
-    uint32_t* d_counters = reinterpret_cast<uint32_t*>(p->scratch.p + 1);
     EventTimer tm(kernel_ms != nullptr), tw(kernel_ms != nullptr);
-    tm.start(s);
-    if (p->n_dense) launch_tile_count(p->bsr.get(), p->dense_list.p, p->n_dense, false, -1, d_total, pv.p, s);
-    tw.start(s);
-    if (p->nitems) launch_wedge(p, d_total, d_counters, pv.p, s);
-    tm.stop(s);
-    tw.stop(s);
+    plan_enqueue(p, s, pv.p, &tm, &tw);
     unsigned long long h = 0;
-    BTC_HIP(hipMemcpyAsync(&h, d_total, 8, hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipMemcpyAsync(&h, p->scratch.p, 8, hipMemcpyDeviceToHost, s));
     if (per_vertex && g->n)
         BTC_HIP(hipMemcpyAsync(per_vertex, pv.p, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
     BTC_HIP(hipStreamSynchronize(s));
@@ -604,6 +603,9 @@ int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_p
         btcx::check_launch_opts(o);
         p->launch = o;
         btcx::ensure_device(g, s);
+        BTC_REQUIRE((uint64_t)g->m + btcx::kColPad < (1ull << 30), BTC_ERR_OVERFLOW,
+                    "nnz(L) = %lld: the wedge kernel addresses L with 32-bit byte offsets (limit 2^30 - %zu)",
+                    (long long)g->m, btcx::kColPad);
         std::vector<uint8_t> dense;
         btcx::classify_dense(p.get(), s, dense);
         int64_t chunk = o.chunk_work > 0 ? o.chunk_work : 0;

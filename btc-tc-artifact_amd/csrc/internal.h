@@ -20,11 +20,30 @@ struct btc_bitbsr_s {
     btcx::DevBuf<unsigned long long> result;
 };
 
+struct btc_plan_s {
+    btc_graph_s* g = nullptr;
+    int K = 0;
+    int tau = 0;
+    btc_count_opts launch{};  // launch-only options (btc_plan_tune)
+    std::unique_ptr<btc_bitbsr_s> bsr;
+    btcx::DevBuf<uint32_t> dense_list;
+    uint32_t n_dense = 0;
+    btcx::DevBuf<uint32_t> ustart, uend;
+    btcx::DevBuf<int32_t> urow;
+    btcx::DevBuf<uint4> items;  // 2 per item
+    uint32_t nitems = 0;
+    uint32_t n_block_items = 0;
+    btcx::DevBuf<unsigned long long> scratch;  // [0] total, [1] work-item counter
+    btc_plan_info info{};
+    float last_dense_ms = 0.f;
+    float last_wedge_ms = 0.f;
+};
+
 namespace btcx {
 
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;
-// The column array of L is padded so 16-byte loads that start inside a row
-// may run up to 3 ids past the last entry.
+// The column array of L is padded so that 16-byte loads that start inside
+// a row may run up to 3 ids past the last entry.
 constexpr size_t kColPad = 16;
 
 // Uploads L (rowptr narrowed to u32) to HBM on first use.
@@ -48,5 +67,11 @@ void launch_tile_popcount(const btc_bitbsr_s* b, uint32_t* d_pop, hipStream_t s)
 
 // Number of compute units of the current device.
 int device_cu_count();
+
+// Enqueues one exact count of plan p on stream s: zeroes the plan's total,
+// runs the dense-tile and wedge kernels; the u64 total lands in
+// p->scratch.p[0] and per-row sums are added into d_pv when not NULL.
+// tm times both kernels, tw the wedge kernel (either may be NULL).
+void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, EventTimer* tw);
 
 }  // namespace btcx

@@ -210,6 +210,30 @@ void btc_plan_free(btc_plan p);
 int btc_count(btc_graph g, const btc_count_opts* opts, void* stream, uint64_t* total,
               int32_t* per_vertex, float* kernel_ms);
 
+/* ------------------------------------------------------------- batching
+ * Many graphs counted by one persistent launch: their L's are placed on the
+ * diagonal of one block-diagonal L (each graph's vertices start at a
+ * multiple of 128 so no tile spans two graphs), planned once and counted
+ * by one enqueue of the engine's kernels; per-graph totals are segment sums
+ * of per-vertex counts.  The input graphs may be freed after creation.  No
+ * counterpart in the reference, which counts one graph per process. */
+typedef struct btc_batch_s* btc_batch;
+
+typedef struct {
+    int32_t graphs;
+    int32_t n;  /* vertices of the block-diagonal L, padding included */
+    int64_t m;
+    btc_plan_info plan;
+} btc_batch_info;
+
+int btc_batch_create(const btc_graph* graphs, int32_t count, const btc_count_opts* opts, void* stream,
+                     btc_batch* out);
+int btc_batch_get_info(btc_batch b, btc_batch_info* info);
+int btc_batch_tune(btc_batch b, const btc_count_opts* opts);
+/* per_graph: host uint64[count] or NULL; total: sum over the batch. */
+int btc_batch_count(btc_batch b, void* stream, uint64_t* per_graph, uint64_t* total, float* kernel_ms);
+void btc_batch_free(btc_batch b);
+
 /* --------------------------------------------------------- CPU verify path
  * Host OpenMP merge over L (the reference's -v 1 path,
  * triangle_count_cpu.h:60-101); per_vertex may be NULL.  Not used by any GPU

@@ -1,0 +1,60 @@
+"""Batched counting: every graph's count from the one persistent launch
+equals its own oracle count, and equals the count of the same graph alone."""
+import numpy as np
+import pytest
+
+import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_total(orc, g):
+    rp, c = g.csr()
+    return orc.tc_merge(rp, c, threads=4)
+
+
+def test_batch_workload_matches_oracle(gpu, orc):
+    work = gpu.batch_workload(count=96, seed=7, n_lo=64, n_hi=3000)
+    gs = [gpu.Graph.from_coo(s, d, n) for s, d, n, _ in work]
+    want = np.array([oracle_total(orc, g) for g in gs], np.uint64)
+    b = gpu.Batch(gs)
+    total, per, ms = b.count()
+    np.testing.assert_array_equal(per, want)
+    assert total == int(want.sum()) and ms > 0
+    assert b.info["graphs"] == 96 and b.info["n"] % 128 == 0
+    # the same graph alone gives the same count (one launch per graph)
+    for k in (0, 17, 95):
+        assert gpu.Plan(gs[k]).count()[0] == want[k]
+    # launch shapes do not change any count
+    for sched in (0, 1):
+        b.tune(schedule=sched, group_lanes=16)
+        assert np.array_equal(b.count()[1], want)
+
+
+@pytest.mark.parametrize("tau", [-1, 0, 100000])
+def test_batch_mixed_extremes(gpu, orc, tau):
+    """Empty graphs, a single triangle, cliques, stars and a dense planted
+    graph in one batch; tau = 0 sends every tile down the dense path."""
+    parts = [
+        (np.zeros(0, np.int32), np.zeros(0, np.int32), 5),
+        (np.array([1, 2, 2], np.int32), np.array([0, 0, 1], np.int32), 3),
+        graphs.complete(40) + (40,),
+        graphs.star(300) + (300,),
+        gpu.gen_planted(2048, 128, 0.5, 4.0, seed=3) + (2048,),
+        graphs.complete(7) + (7,),
+    ]
+    gs = [gpu.Graph.from_coo(s, d, n) for s, d, n in parts]
+    want = np.array([oracle_total(orc, g) for g in gs], np.uint64)
+    total, per, _ = gpu.Batch(gs, tau=tau).count()
+    np.testing.assert_array_equal(per, want)
+    assert want[2] == 40 * 39 * 38 // 6 and want[3] == 0 and want[1] == 1
+
+
+def test_batch_graphs_may_be_freed(gpu, orc):
+    work = gpu.batch_workload(count=10, seed=3, n_lo=100, n_hi=500)
+    gs = [gpu.Graph.from_coo(s, d, n) for s, d, n, _ in work]
+    want = [oracle_total(orc, g) for g in gs]
+    b = gpu.Batch(gs)
+    for g in gs:
+        g.close()
+    assert b.count()[1].tolist() == want
