@@ -13,6 +13,8 @@ Prints progress to stderr and ONE JSON line to stdout.
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import statistics
 import sys
@@ -100,6 +102,125 @@ def copy_calibration(mb):
     log("copy calibration: %d MiB device-to-device" % mb)
 
 
+def batched_leg(btc, count, steps, sample):
+    """4096 graphs (40 % ER, 40 % R-MAT, 20 % 9-point stencils, n log-uniform
+    in [256, 16384]) counted by one persistent launch, against one launch per
+    graph on a sample of them (each with its plan resident)."""
+    t0 = time.perf_counter()
+    work = btc.batch_workload(count, seed=1)
+    graphs = [btc.Graph.from_coo(s, d, n) for s, d, n, _ in work]
+    del work
+    t_canon = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch = btc.Batch(graphs)
+    t_build = time.perf_counter() - t0
+    info = batch.info
+    batch.count()
+    wall = []
+    per = None
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        total, per, ms = batch.count()
+        wall.append(time.perf_counter() - t1)
+    t_batch = statistics.median(wall)
+    log("batched: %d graphs, %d edges, %.3f ms per launch (%.0f graphs/s); build %.1f s, canon %.1f s"
+        % (count, info["m"], t_batch * 1e3, count / t_batch, t_build, t_canon))
+    # one launch per graph, plans resident, on an evenly spaced sample
+    idx = list(range(0, count, max(1, count // sample)))[:sample]
+    plans = [btc.Plan(graphs[k]) for k in idx]
+    for p in plans[:8]:
+        p.count()
+    t1 = time.perf_counter()
+    single = [p.count()[0] for p in plans]
+    t_single = (time.perf_counter() - t1) / len(plans)
+    exact = all(int(per[k]) == single[j] for j, k in enumerate(idx))
+    log("one launch per graph: %.3f ms per graph (%.0f graphs/s) on %d graphs; exact %s"
+        % (t_single * 1e3, 1 / t_single, len(plans), exact))
+    return {"graphs": count, "edges": info["m"], "ms_per_launch": round(t_batch * 1e3, 3),
+            "graphs_per_s": count / t_batch, "edges_per_s": info["m"] / t_batch,
+            "single_launch_graphs_per_s": 1.0 / t_single, "speedup_vs_single": round(t_single * count / t_batch, 2),
+            "single_sample": len(plans), "exact_vs_single": exact, "triangles": int(total),
+            "build_s": round(t_build, 2), "canon_s": round(t_canon, 2), "work_items": info["plan"]["work_items"]}
+
+
+def serving_leg(btc, requests, seed=1):
+    """Poisson stream of exact-count queries: R-MAT graphs (edge factor 16)
+    with n log-normal (median 2^13, sigma 1 in ln n, capped at 2^20), 30 %
+    of requests repeat a graph submitted earlier.  A closed burst measures
+    saturation; the open-loop run uses 70 % of it.  Latency = done - submit
+    on the server clock."""
+    import math
+    import threading
+
+    rng = np.random.default_rng(seed)
+
+    def stream(nreq, key0):
+        reqs, keys = [], []
+        for _ in range(nreq):
+            if keys and rng.random() < 0.3:
+                reqs.append((int(rng.choice(keys)), None))
+            else:
+                n = int(min(1 << 20, max(64, round(math.exp(rng.normal(math.log(1 << 13), 1.0))))))
+                sc = max(1, math.ceil(math.log2(n)))
+                s, d = btc.gen_rmat(sc, 16, 0.57, 0.19, 0.19, seed=int(rng.integers(1, 2 ** 62)))
+                keep = (s < n) & (d < n)
+                key = key0 + len(keys)
+                keys.append(key)
+                reqs.append((key, (s[keep], d[keep], n)))
+        return reqs
+
+    def run(reqs, qps):
+        sv = btc.Server()
+        gaps = rng.exponential(1.0 / qps, len(reqs)) if qps else np.zeros(len(reqs))
+        t_start = time.perf_counter()
+
+        def submit():
+            t = 0.0
+            for (key, g), gap in zip(reqs, gaps):
+                t += gap
+                delay = t - (time.perf_counter() - t_start)
+                if delay > 0:
+                    time.sleep(delay)
+                if g is None:
+                    sv.submit(key)
+                else:
+                    sv.submit(key, g[0], g[1], g[2])
+
+        th = threading.Thread(target=submit)
+        th.start()
+        res = []
+        while len(res) < len(reqs):
+            res += sv.poll(wait_ms=20)
+        th.join()
+        st = sv.stats()
+        sv.close()
+        return res, st
+
+    burst = stream(max(64, requests // 3), 1 << 40)
+    res, _ = run(burst, 0)
+    span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    sat = len(res) / span
+    qps = 0.7 * sat
+    reqs = stream(requests, 2 << 40)
+    res, st = run(reqs, qps)
+    lat = np.array(sorted(r["t_done"] - r["t_submit"] for r in res))
+    span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    graphs = {}
+    for key, g in reqs:
+        if g is not None:
+            graphs[key] = g
+    want = {k: btc.Plan(btc.Graph.from_coo(*g)).count()[0] for k, g in graphs.items()}
+    exact = all(r["status"] == 0 and r["triangles"] == want[r["graph_key"]] for r in res)
+    out = {"requests": len(res), "qps": round(qps, 1), "saturation_graphs_per_s": round(sat, 1),
+           "graphs_per_s": round(len(res) / span, 1), "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+           "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3), "exact_alone_vs_mix": exact,
+           "resident_hits": st["resident_hits"], "launches": st["launches"],
+           "mean_batch": round(st["launched_requests"] / max(1, st["launches"]), 2),
+           "distinct_graphs": len(graphs)}
+    log("serving: %s" % out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +232,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--plain-runs", type=int, default=3, help="reference-semantics Bit-BSR yardstick runs (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=4096, help="graphs in the batched leg (0 = skip)")
+    ap.add_argument("--batch-sample", type=int, default=256, help="graphs timed one launch each")
+    ap.add_argument("--serve", type=int, default=600, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one D2D copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -200,6 +324,9 @@ def main():
         log("cpu baseline: %s" % cpu)
         assert cpu["matches_gpu"], "GPU count differs from the CPU baseline"
 
+    batched = batched_leg(btc, args.batch, max(3, args.steps // 2), args.batch_sample) if args.batch > 0 else None
+    serving = serving_leg(btc, args.serve) if args.serve > 0 else None
+
     config.update({"n": gi["n"], "m": gi["m"], "ordering": "degree-descending", "tile_k": pi["tile_k"],
                    "tau": pi["tau"], "dense_tiles": pi["dense_tiles"], "work_items": pi["work_items"]})
     out = {
@@ -209,6 +336,7 @@ def main():
         "triangles": int(total0), "triangles_per_s": total0 / (wall / args.steps),
         "kernel_ms_median": statistics.median(kms), "canon_ms": round(canon_ms, 1), "plan_ms": round(build_ms, 1),
         "roofline": roofline, "cpu_baseline": cpu, "plain_hybrid": plain,
+        "batched": batched, "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

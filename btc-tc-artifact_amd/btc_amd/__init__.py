@@ -17,7 +17,7 @@ import numpy as np
 
 __all__ = [
     "BtcError", "lib", "lib_path", "Graph", "BitBSR", "Plan", "Batch", "CountOpts", "count_cpu",
-    "batch_workload",
+    "batch_workload", "Server",
     "gen_rmat", "gen_er", "gen_stencil2d", "gen_stencil3d", "gen_planted", "lite_tile_k",
     "default_tau", "version", "CANON_COMPACT", "CANON_DEGREE_ORDER", "CANON_DEFAULT",
     "VARIANT_AUTO", "VARIANT_BINSEARCH", "VARIANT_DIRECT",
@@ -71,6 +71,24 @@ class PlanInfo(C.Structure):
                 ("tau", C.c_int32)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_batch", C.c_int32), ("prep_threads", C.c_int32),
+                ("canon_flags", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("graph_key", C.c_uint64), ("triangles", C.c_uint64),
+                ("status", C.c_int32), ("resident_hit", C.c_int32), ("batch_size", C.c_int32),
+                ("reserved", C.c_int32), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "resident_hits", "prepared",
+                                         "evictions", "launches", "launched_requests", "resident_bytes",
+                                         "resident_graphs")] + [("clock", C.c_double)]
+
+
 class BatchInfo(C.Structure):
     _fields_ = [("graphs", C.c_int32), ("n", C.c_int32), ("m", C.c_int64), ("plan", PlanInfo)]
 
@@ -112,6 +130,12 @@ _SIGS = {
     "btc_batch_tune": (C.c_int, [vp, C.POINTER(CountOpts)]),
     "btc_batch_count": (C.c_int, [vp, vp, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
     "btc_batch_free": (None, [vp]),
+    "btc_server_opts_default": (None, [C.POINTER(ServerOpts)]),
+    "btc_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(vp)]),
+    "btc_server_submit": (C.c_int, [vp, C.c_uint64, vp, vp, C.c_int64, C.c_int32, C.POINTER(C.c_uint64)]),
+    "btc_server_poll": (C.c_int, [vp, C.POINTER(Result), C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
+    "btc_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
+    "btc_server_free": (None, [vp]),
 }
 
 
@@ -438,3 +462,53 @@ def batch_workload(count=4096, seed=1, n_lo=256, n_hi=16384):
             kind = "stencil2d"
         out.append((src, dst, n, kind))
     return out
+
+
+class Server:
+    """Request loop with continuous batching (btc_server_*)."""
+
+    def __init__(self, hbm_budget_bytes=None, max_batch=None, prep_threads=None, canon_flags=None):
+        o = ServerOpts()
+        lib().btc_server_opts_default(C.byref(o))
+        for k, v in dict(hbm_budget_bytes=hbm_budget_bytes, max_batch=max_batch, prep_threads=prep_threads,
+                         canon_flags=canon_flags).items():
+            if v is not None:
+                setattr(o, k, v)
+        h = vp()
+        _check(lib().btc_server_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self._buf = (Result * 4096)()
+
+    def submit(self, key, src=None, dst=None, n=0):
+        """Queues a count of graph `key`; the edges are copied (omit them for
+        a key submitted before).  Returns the request id."""
+        rid = C.c_uint64(0)
+        if src is None:
+            _check(lib().btc_server_submit(self._h, key, None, None, 0, n, C.byref(rid)))
+        else:
+            src = np.ascontiguousarray(src, np.int32)
+            dst = np.ascontiguousarray(dst, np.int32)
+            _check(lib().btc_server_submit(self._h, key, _ptr(src), _ptr(dst), src.size, n, C.byref(rid)))
+        return rid.value
+
+    def poll(self, wait_ms=0):
+        got = C.c_int32(0)
+        _check(lib().btc_server_poll(self._h, self._buf, len(self._buf), wait_ms, C.byref(got)))
+        return [{k: getattr(self._buf[i], k) for k, _ in Result._fields_ if k != "reserved"}
+                for i in range(got.value)]
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().btc_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().btc_server_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

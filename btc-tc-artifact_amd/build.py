@@ -69,7 +69,7 @@ def build(jobs=None, verbose=False):
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: compile_one(s, verbose), srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fopenmp", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    cmd = [hipcc(), "-shared", "-fopenmp", "-pthread", "--offload-arch=" + ARCH, "-o", tmp] + objs
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed\n" + r.stdout)

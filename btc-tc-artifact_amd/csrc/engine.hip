@@ -86,19 +86,32 @@ struct SearchSet {  // binary search of the sorted pivot row in HBM
     }
 };
 
-struct WedgeArgs {
+// Device view of one plan's work lists.
+struct PlanDev {
     const int32_t* col;
     const uint32_t* us;
     const uint32_t* ue;
     const int32_t* ur;
     const uint4* items;  // 2 x uint4 per item: {t0, t1, eb, ee}, {lo, hi, work, 0}
-    uint32_t n_block;    // items [0, n_block) are workgroup items
-    uint32_t n_items;    // items [n_block, n_items) are wave items
-    uint32_t* counters;  // [0] workgroup-item cursor, [1] wave-item cursor (schedule 0)
+    uint32_t n_block;    // items [0, n_block) are workgroup items, the rest wave items
+    uint32_t n_items;
     unsigned long long* result;
     int32_t* pv;
-    int lookup;    // 0 = auto, 1 = never bitmap, 2 = never hash
-    int schedule;  // 0 = work stealing, 1 = static stride
+};
+
+// One launch counts one plan (nplans == 1: p0) or several (plans[], with map
+// giving each global item its plan and local item: workgroup items of all
+// plans first, then wave items).
+struct WedgeArgs {
+    PlanDev p0;
+    const PlanDev* plans;
+    const uint2* map;
+    uint32_t nplans;
+    uint32_t n_block;    // global workgroup items
+    uint32_t n_items;    // global items
+    uint32_t* counters;  // [0] workgroup-item cursor, [1] wave-item cursor (schedule 0)
+    int lookup;          // 0 = auto, 1 = never bitmap, 2 = never hash
+    int schedule;        // 0 = work stealing, 1 = static stride
 };
 
 // Barrier of the threads that share one LDS region: the workgroup (NT = 256)
@@ -123,7 +136,7 @@ __device__ __forceinline__ void team_sync() {
 // caches to the column ids, which are re-read for every pivot of a row.
 // (s0, s1) are the bounds of the group's first list, loaded by the caller.
 template <int NT, int G, int U, bool PV, class Set>
-__device__ __forceinline__ uint64_t scan_lists(const Set& set, const WedgeArgs& a, uint32_t tid, uint32_t eb,
+__device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a, uint32_t tid, uint32_t eb,
                                                uint32_t ee, uint32_t s0, uint32_t s1) {
     constexpr uint32_t NG = NT / G;
     constexpr uint32_t STEP = 4 * G * U;
@@ -177,7 +190,7 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const WedgeArgs& 
 // d_L(j) <= W / 2, else stays in HBM for binary search.  The LDS words are
 // zero again on return.
 template <int NT, int W, int G, int U, bool PV>
-__device__ __forceinline__ uint64_t process_item(const WedgeArgs& a, uint4 A, uint4 B, uint32_t* lds,
+__device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, uint4 A, uint4 B, uint32_t* lds,
                                                  uint32_t tid) {
     const uint32_t t0 = A.x, t1 = A.y, eb = A.z, ee = A.w, lo = B.x, hi = B.y;
     const uint32_t d = t1 - t0;  // >= 1 for every listed pivot
@@ -192,7 +205,7 @@ __device__ __forceinline__ uint64_t process_item(const WedgeArgs& a, uint4 A, ui
     }
     uint64_t acc = 0;
     const uint32_t base = lo & ~31u;
-    if (hi - base < 32u * W - 1u && a.lookup != 1) {  // the window's last bit stays 0 (sentinel)
+    if (hi - base < 32u * W - 1u && lookup != 1) {  // the window's last bit stays 0 (sentinel)
         const uint32_t bw = base >> 5;
         for (uint32_t p = t0 + tid; p < t1; p += NT) {
             const uint32_t k = (uint32_t)a.col[p];
@@ -202,7 +215,7 @@ __device__ __forceinline__ uint64_t process_item(const WedgeArgs& a, uint4 A, ui
         acc = scan_lists<NT, G, U, PV>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, s0, s1);
         team_sync<NT>();
         for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)a.col[p] >> 5) - bw] = 0;
-    } else if (d <= (uint32_t)W / 2 && a.lookup != 2) {
+    } else if (d <= (uint32_t)W / 2 && lookup != 2) {
         constexpr int kMaxLog = W == 4096 ? 12 : 10;
         const int logS = min(kMaxLog, max(6, 32 - __clz(4 * d - 1)));  // S >= 4d while it fits
         const int shift = 32 - logS;
@@ -228,6 +241,43 @@ __device__ __forceinline__ uint4 uniform4(uint4 v) {
     return make_uint4(uniform(v.x), uniform(v.y), uniform(v.z), uniform(v.w));
 }
 
+__device__ __forceinline__ PlanDev uniform_plan(const PlanDev* p) {
+    PlanDev q = *p;
+    q.n_block = uniform(q.n_block);
+    q.n_items = uniform(q.n_items);
+    return q;
+}
+
+// Item t of the launch: its plan and its two descriptor words.  Wave items
+// are numbered after all workgroup items.
+struct Item {
+    PlanDev P;
+    uint4 A, B;
+};
+
+__device__ __forceinline__ Item fetch_item(const WedgeArgs& a, uint32_t t) {
+    Item r;
+    uint32_t li = t;
+    if (a.nplans == 1) {
+        r.P = a.p0;  // single plan: the launch numbering is the plan's own
+    } else {
+        const uint2 m = a.map[t];
+        r.P = uniform_plan(a.plans + uniform(m.x));
+        li = uniform(m.y);
+    }
+    r.A = uniform4(r.P.items[2 * li]);
+    r.B = uniform4(r.P.items[2 * li + 1]);
+    return r;
+}
+
+// Adds a team's partial counts into a plan's total (multi-plan launches
+// flush after every item; a single-plan launch once at the end).
+__device__ __forceinline__ void flush(uint64_t& acc, unsigned long long* result) {
+    const uint64_t s = wave_sum_u64(acc);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(result, (unsigned long long)s);
+    acc = 0;
+}
+
 // Persistent grid.  Phase 1: workgroup items (heavy pivots, sorted by work,
 // 16 KiB LDS each).  Phase 2: wave items (pivots with few lists), each wave
 // with its own 4 KiB slice.  schedule 1 strides statically over the sorted
@@ -240,6 +290,7 @@ __global__ __launch_bounds__(kThreads) void wedge_kernel(WedgeArgs a) {
     const uint32_t tid = threadIdx.x;
     const int lane = tid & 63;
     const uint32_t wid = uniform(tid >> 6);
+    const bool multi = a.nplans != 1;
     for (int i = tid; i < kLdsWords; i += kThreads) lds[i] = 0;
     __syncthreads();
     uint64_t acc = 0;
@@ -247,23 +298,17 @@ __global__ __launch_bounds__(kThreads) void wedge_kernel(WedgeArgs a) {
     // phase 1: workgroup items
     if (a.schedule == 1) {
         uint32_t it = blockIdx.x;
-        uint4 A = make_uint4(0, 0, 0, 0), B = A;
-        if (it < a.n_block) {
-            A = uniform4(a.items[2 * it]);
-            B = uniform4(a.items[2 * it + 1]);
-        }
+        Item cur;
+        if (it < a.n_block) cur = fetch_item(a, it);
         while (it < a.n_block) {
             const uint32_t nx = it + gridDim.x;
-            uint4 nA = make_uint4(0, 0, 0, 0), nB = nA;
-            if (nx < a.n_block) {
-                nA = uniform4(a.items[2 * nx]);
-                nB = uniform4(a.items[2 * nx + 1]);
-            }
-            acc += process_item<kThreads, kLdsWords, G, U, PV>(a, A, B, lds, tid);
+            Item nxt;
+            if (nx < a.n_block) nxt = fetch_item(a, nx);
+            acc += process_item<kThreads, kLdsWords, G, U, PV>(cur.P, a.lookup, cur.A, cur.B, lds, tid);
+            if (multi) flush(acc, cur.P.result);
             __syncthreads();
             it = nx;
-            A = nA;
-            B = nB;
+            cur = nxt;
         }
     } else {
         for (;;) {
@@ -271,8 +316,9 @@ __global__ __launch_bounds__(kThreads) void wedge_kernel(WedgeArgs a) {
             __syncthreads();
             const uint32_t it = uniform(s_item);
             if (it >= a.n_block) break;
-            acc += process_item<kThreads, kLdsWords, G, U, PV>(a, uniform4(a.items[2 * it]),
-                                                                uniform4(a.items[2 * it + 1]), lds, tid);
+            const Item cur = fetch_item(a, it);
+            acc += process_item<kThreads, kLdsWords, G, U, PV>(cur.P, a.lookup, cur.A, cur.B, lds, tid);
+            if (multi) flush(acc, cur.P.result);
             __syncthreads();
         }
     }
@@ -283,23 +329,17 @@ __global__ __launch_bounds__(kThreads) void wedge_kernel(WedgeArgs a) {
     if (a.schedule == 1) {
         const uint32_t stride = gridDim.x * (kThreads / 64);
         uint32_t it = blockIdx.x * (kThreads / 64) + wid;
-        uint4 A = make_uint4(0, 0, 0, 0), B = A;
-        if (it < nwave) {
-            A = uniform4(a.items[2 * (a.n_block + it)]);
-            B = uniform4(a.items[2 * (a.n_block + it) + 1]);
-        }
+        Item cur;
+        if (it < nwave) cur = fetch_item(a, a.n_block + it);
         while (it < nwave) {
             const uint32_t nx = it + stride;
-            uint4 nA = make_uint4(0, 0, 0, 0), nB = nA;
-            if (nx < nwave) {
-                nA = uniform4(a.items[2 * (a.n_block + nx)]);
-                nB = uniform4(a.items[2 * (a.n_block + nx) + 1]);
-            }
-            acc += process_item<64, kWaveWords, G, U, PV>(a, A, B, slice, (uint32_t)lane);
+            Item nxt;
+            if (nx < nwave) nxt = fetch_item(a, a.n_block + nx);
+            acc += process_item<64, kWaveWords, G, U, PV>(cur.P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
+            if (multi) flush(acc, cur.P.result);
             team_sync<64>();
             it = nx;
-            A = nA;
-            B = nB;
+            cur = nxt;
         }
     } else {
         for (;;) {
@@ -307,14 +347,13 @@ __global__ __launch_bounds__(kThreads) void wedge_kernel(WedgeArgs a) {
             if (lane == 0) it = atomicAdd(a.counters + 1, 1u);
             it = uniform(__shfl(it, 0));
             if (it >= nwave) break;
-            acc += process_item<64, kWaveWords, G, U, PV>(a, uniform4(a.items[2 * (a.n_block + it)]),
-                                                          uniform4(a.items[2 * (a.n_block + it) + 1]), slice,
-                                                          (uint32_t)lane);
+            const Item cur = fetch_item(a, a.n_block + it);
+            acc += process_item<64, kWaveWords, G, U, PV>(cur.P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
+            if (multi) flush(acc, cur.P.result);
             team_sync<64>();
         }
     }
-    acc = wave_sum_u64(acc);
-    if (lane == 0 && acc) atomicAdd(a.result, (unsigned long long)acc);
+    if (!multi) flush(acc, a.p0.result);
 }
 
 using WedgeFn = void (*)(WedgeArgs);
@@ -516,24 +555,81 @@ void check_launch_opts(const btc_count_opts& o) {
 
 namespace {
 
-void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counters, int32_t* d_pv, hipStream_t s) {
-    const btc_count_opts& o = p->launch;
+WedgeFn wedge_fn(const btc_count_opts& o, bool pv, unsigned* grid) {
     const int G = o.group_lanes ? o.group_lanes : 8;
     const int U = o.unroll ? o.unroll : 1;
-    WedgeFn fn = pick_wedge(G, U, d_pv != nullptr);
+    WedgeFn fn = pick_wedge(G, U, pv);
     int per_cu = 0;
     BTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kThreads, 0));
     per_cu = std::max(1, per_cu);
     if (o.waves_per_eu > 0) per_cu = std::min(per_cu, (int)o.waves_per_eu);
-    const unsigned grid = (unsigned)(device_cu_count() * per_cu);
-    btc_graph_s* g = p->g;
-    WedgeArgs a{g->d_col.p, p->ustart.p, p->uend.p, p->urow.p, p->items.p, p->n_block_items, p->nitems,
-                d_counters, d_total, d_pv, (int)o.lookup, (int)o.schedule};
+    *grid = (unsigned)(device_cu_count() * per_cu);
+    return fn;
+}
+
+PlanDev plan_dev(btc_plan_s* p, unsigned long long* d_total, int32_t* d_pv) {
+    return PlanDev{p->g->d_col.p, p->ustart.p, p->uend.p, p->urow.p, p->items.p, p->n_block_items, p->nitems,
+                   d_total, d_pv};
+}
+
+void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counters, int32_t* d_pv, hipStream_t s) {
+    unsigned grid = 0;
+    WedgeFn fn = wedge_fn(p->launch, d_pv != nullptr, &grid);
+    WedgeArgs a{plan_dev(p, d_total, d_pv), nullptr, nullptr, 1u, p->n_block_items, p->nitems, d_counters,
+                (int)p->launch.lookup, (int)p->launch.schedule};
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
     BTC_CHECK_LAUNCH("wedge_kernel");
 }
 
 }  // namespace
+
+void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, MultiWorkspace& ws) {
+    if (k <= 0) return;
+    if (ws.totals.n < (size_t)k) ws.totals.alloc((size_t)k);
+    if (ws.counters.n < 2) ws.counters.alloc(2);
+    BTC_HIP(hipMemsetAsync(ws.totals.p, 0, (size_t)k * 8, s));
+    BTC_HIP(hipMemsetAsync(ws.counters.p, 0, 8, s));
+    std::vector<PlanDev> pd((size_t)k);
+    uint32_t nb = 0, ni = 0;
+    for (int j = 0; j < k; ++j) {
+        pd[(size_t)j] = plan_dev(P[j], ws.totals.p + j, nullptr);
+        nb += P[j]->n_block_items;
+        ni += P[j]->nitems;
+    }
+    for (int j = 0; j < k; ++j)
+        if (P[j]->n_dense)
+            launch_tile_count(P[j]->bsr.get(), P[j]->dense_list.p, P[j]->n_dense, false, -1, ws.totals.p + j,
+                              nullptr, s);
+    if (ni) {
+        unsigned grid = 0;
+        WedgeFn fn = wedge_fn(P[0]->launch, false, &grid);
+        WedgeArgs a{pd[0], nullptr, nullptr, 1u, nb, ni, ws.counters.p, (int)P[0]->launch.lookup,
+                    (int)P[0]->launch.schedule};
+        if (k > 1) {
+            ws.h_map.clear();
+            ws.h_map.reserve(ni);
+            for (int j = 0; j < k; ++j)
+                for (uint32_t li = 0; li < P[j]->n_block_items; ++li) ws.h_map.push_back(make_uint2((uint32_t)j, li));
+            for (int j = 0; j < k; ++j)
+                for (uint32_t li = P[j]->n_block_items; li < P[j]->nitems; ++li)
+                    ws.h_map.push_back(make_uint2((uint32_t)j, li));
+            const size_t pbytes = pd.size() * sizeof(PlanDev);
+            if (ws.plans.n < pbytes) ws.plans.alloc(pbytes);
+            if (ws.map.n < ni) ws.map.alloc(ni);
+            BTC_HIP(hipMemcpyAsync(ws.plans.p, pd.data(), pbytes, hipMemcpyHostToDevice, s));
+            BTC_HIP(hipMemcpyAsync(ws.map.p, ws.h_map.data(), (size_t)ni * sizeof(uint2), hipMemcpyHostToDevice, s));
+            a.plans = reinterpret_cast<const PlanDev*>(ws.plans.p);
+            a.map = ws.map.p;
+            a.nplans = (uint32_t)k;
+        }
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
+        BTC_CHECK_LAUNCH("wedge_kernel");
+    }
+    std::vector<unsigned long long> h((size_t)k);
+    BTC_HIP(hipMemcpyAsync(h.data(), ws.totals.p, (size_t)k * 8, hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < k; ++j) totals[j] = h[(size_t)j];
+}
 
 void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, EventTimer* tw) {
     BTC_REQUIRE(!d_pv || p->g->max_row <= 65536, BTC_ERR_OVERFLOW,

@@ -68,6 +68,20 @@ void launch_tile_popcount(const btc_bitbsr_s* b, uint32_t* d_pop, hipStream_t s)
 // Number of compute units of the current device.
 int device_cu_count();
 
+// Device buffers reused across multi-plan launches.
+struct MultiWorkspace {
+    DevBuf<unsigned char> plans;  // device array of plan descriptors
+    DevBuf<uint2> map;            // launch item -> {plan, item of the plan}
+    DevBuf<unsigned long long> totals;
+    DevBuf<uint32_t> counters;
+    std::vector<uint2> h_map;
+};
+
+// Counts k plans with one persistent wedge-kernel launch (plus a dense-tile
+// launch per plan that has dense tiles) on stream s; totals[j] = triangles
+// of plans[j].  Synchronous.  Launch options are those of plans[0].
+void plans_count(btc_plan_s* const* plans, int k, hipStream_t s, uint64_t* totals, MultiWorkspace& ws);
+
 // Enqueues one exact count of plan p on stream s: zeroes the plan's total,
 // runs the dense-tile and wedge kernels; the u64 total lands in
 // p->scratch.p[0] and per-row sums are added into d_pv when not NULL.

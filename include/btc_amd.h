@@ -234,6 +234,56 @@ int btc_batch_tune(btc_batch b, const btc_count_opts* opts);
 int btc_batch_count(btc_batch b, void* stream, uint64_t* per_graph, uint64_t* total, float* kernel_ms);
 void btc_batch_free(btc_batch b);
 
+/* -------------------------------------------------------------- serving
+ * A request loop with continuous batching.  btc_server_submit queues an
+ * exact-count query for a graph identified by the caller's key; a graph not
+ * yet resident is copied, canonicalised and planned by a pool of host prep
+ * threads on their own HIP streams, admitted only when the resident set
+ * stays under hbm_budget_bytes (idle graphs are evicted least recently used
+ * first); later queries for a resident key skip all of that.  A GPU worker
+ * counts every request whose plan is resident in one persistent launch over
+ * all their plans, while the prep threads keep preparing new arrivals.  Each
+ * result is exact and the same as the graph's count alone.  Times are
+ * seconds on the server's clock.  No counterpart in the reference. */
+typedef struct btc_server_s* btc_server;
+
+typedef struct {
+    int64_t hbm_budget_bytes; /* resident graphs + plans; default 64 GiB */
+    int32_t max_batch;        /* requests per launch; default 512 */
+    int32_t prep_threads;     /* host threads that canonicalise and plan; default 8 */
+    int32_t canon_flags;      /* default BTC_CANON_DEFAULT */
+    int32_t reserved;
+} btc_server_opts;
+
+typedef struct {
+    uint64_t request_id;
+    uint64_t graph_key;
+    uint64_t triangles;
+    int32_t status;       /* BTC_OK or the error of canonicalisation / planning / counting */
+    int32_t resident_hit; /* 1: the graph was resident or already being prepared */
+    int32_t batch_size;   /* requests in the launch that counted it */
+    int32_t reserved;
+    double t_submit, t_start, t_done;
+} btc_result;
+
+typedef struct {
+    int64_t submitted, completed, failed, resident_hits, prepared, evictions, launches, launched_requests;
+    int64_t resident_bytes, resident_graphs;
+    double clock;
+} btc_server_stats;
+
+void btc_server_opts_default(btc_server_opts* opts);
+int btc_server_create(const btc_server_opts* opts, btc_server* out);
+/* src/dst/n describe the graph (copied); they may be NULL/0 when graph_key
+ * was submitted before. */
+int btc_server_submit(btc_server s, uint64_t graph_key, const int32_t* src, const int32_t* dst, int64_t nnz,
+                      int32_t n, uint64_t* request_id);
+/* Moves up to `capacity` finished results into out, waiting up to wait_ms
+ * for the first one; *count = results returned. */
+int btc_server_poll(btc_server s, btc_result* out, int32_t capacity, int32_t wait_ms, int32_t* count);
+int btc_server_get_stats(btc_server s, btc_server_stats* stats);
+void btc_server_free(btc_server s);
+
 /* --------------------------------------------------------- CPU verify path
  * Host OpenMP merge over L (the reference's -v 1 path,
  * triangle_count_cpu.h:60-101); per_vertex may be NULL.  Not used by any GPU
