@@ -17,7 +17,8 @@ import numpy as np
 
 __all__ = [
     "BtcError", "lib", "lib_path", "Graph", "BitBSR", "Plan", "Batch", "CountOpts", "count_cpu",
-    "batch_workload", "Server",
+    "batch_workload", "Server", "autotune", "tune_lookup", "tune_table_save", "tune_table_load",
+    "tune_table_clear",
     "gen_rmat", "gen_er", "gen_stencil2d", "gen_stencil3d", "gen_planted", "lite_tile_k",
     "default_tau", "version", "CANON_COMPACT", "CANON_DEGREE_ORDER", "CANON_DEFAULT",
     "VARIANT_AUTO", "VARIANT_BINSEARCH", "VARIANT_DIRECT",
@@ -89,6 +90,11 @@ class ServerStats(C.Structure):
                                          "resident_graphs")] + [("clock", C.c_double)]
 
 
+class TuneOpts(C.Structure):
+    _fields_ = [("budget_ms", C.c_int32), ("reps", C.c_int32), ("tile_ks_mask", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 class BatchInfo(C.Structure):
     _fields_ = [("graphs", C.c_int32), ("n", C.c_int32), ("m", C.c_int64), ("plan", PlanInfo)]
 
@@ -130,6 +136,12 @@ _SIGS = {
     "btc_batch_tune": (C.c_int, [vp, C.POINTER(CountOpts)]),
     "btc_batch_count": (C.c_int, [vp, vp, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
     "btc_batch_free": (None, [vp]),
+    "btc_tune_opts_default": (None, [C.POINTER(TuneOpts)]),
+    "btc_autotune": (C.c_int, [vp, C.POINTER(TuneOpts), vp, C.POINTER(CountOpts), C.POINTER(C.c_float)]),
+    "btc_tune_lookup": (C.c_int, [vp, C.POINTER(CountOpts)]),
+    "btc_tune_table_save": (C.c_int, [C.c_char_p]),
+    "btc_tune_table_load": (C.c_int, [C.c_char_p]),
+    "btc_tune_table_clear": (None, []),
     "btc_server_opts_default": (None, [C.POINTER(ServerOpts)]),
     "btc_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(vp)]),
     "btc_server_submit": (C.c_int, [vp, C.c_uint64, vp, vp, C.c_int64, C.c_int32, C.POINTER(C.c_uint64)]),
@@ -512,3 +524,39 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def _opts_dict(o):
+    return {k: getattr(o, k) for k, _ in CountOpts._fields_}
+
+
+def autotune(graph, budget_ms=None, reps=None, tile_ks_mask=None, stream=None):
+    """Times candidate options on the device; returns (best options dict, ms)."""
+    t = TuneOpts()
+    lib().btc_tune_opts_default(C.byref(t))
+    for k, v in dict(budget_ms=budget_ms, reps=reps, tile_ks_mask=tile_ks_mask).items():
+        if v is not None:
+            setattr(t, k, v)
+    best = CountOpts()
+    ms = C.c_float(0)
+    _check(lib().btc_autotune(graph._h, C.byref(t), stream, C.byref(best), C.byref(ms)))
+    return _opts_dict(best), ms.value
+
+
+def tune_lookup(graph):
+    """Tuned options for the graph's feature class, or None."""
+    o = CountOpts()
+    rc = lib().btc_tune_lookup(graph._h, C.byref(o))
+    return _opts_dict(o) if rc == 0 else None
+
+
+def tune_table_save(path):
+    _check(lib().btc_tune_table_save(str(path).encode()))
+
+
+def tune_table_load(path):
+    _check(lib().btc_tune_table_load(str(path).encode()))
+
+
+def tune_table_clear():
+    lib().btc_tune_table_clear()

@@ -131,6 +131,9 @@ struct btc_graph_s {
 };
 
 namespace btcx {
-// Fills the derived statistics and uploads the CSR to HBM.
+// Fills the derived statistics (rows, wedges, merge work) of g.
 void graph_finalize(btc_graph_s* g);
+// Fills the auto fields of o from the autotuner's table when an entry
+// matches g's features; returns whether one did.
+bool tune_table_apply(const btc_graph_s* g, btc_count_opts* o);
 }  // namespace btcx

@@ -388,10 +388,67 @@ namespace {
 
 // Marks the L entries that fall into dense tiles and uploads the dense tile
 // list.  Leaves p->bsr empty when no tile is dense.
+// Graphs up to this many edges are classified on the host (tile counts from
+// the CSR), so a graph without dense tiles never builds a Bit-BSR.
+constexpr int64_t kHostClassifyEdges = 8 << 20;
+
+// Host classification: tiles of each row block enumerated in (I, J) order --
+// the order of the GPU builder -- so tile t here is tile t of the Bit-BSR.
+void classify_dense_host(btc_plan_s* p, hipStream_t s, std::vector<uint8_t>& dense) {
+    btc_graph_s* g = p->g;
+    const int K = p->K;
+    const int32_t n_rb = (g->n + 15) / 16;
+    std::vector<uint32_t> ntiles((size_t)n_rb + 1, 0);
+    // pass 1: tiles per row block; pass 2: dense tiles and flags
+    auto row_block = [&](int32_t I, bool mark, uint32_t t0, std::vector<uint32_t>* list, int64_t* dedges) {
+        thread_local std::vector<uint32_t> cb;
+        cb.clear();
+        const int32_t r0 = I * 16, r1 = std::min(g->n, r0 + 16);
+        for (int64_t q = g->rowptr[r0]; q < g->rowptr[r1]; ++q) cb.push_back((uint32_t)g->col[q] / K);
+        std::sort(cb.begin(), cb.end());
+        uint32_t tiles = 0;
+        for (size_t a = 0; a < cb.size();) {
+            size_t b = a;
+            while (b < cb.size() && cb[b] == cb[a]) ++b;
+            if (mark && (int)(b - a) > p->tau) {
+                list->push_back(t0 + tiles);
+                *dedges += (int64_t)(b - a);
+                for (int32_t r = r0; r < r1; ++r)
+                    for (int64_t q = g->rowptr[r]; q < g->rowptr[r + 1]; ++q)
+                        if ((uint32_t)g->col[q] / K == cb[a]) dense[q] = 1;
+            }
+            ++tiles;
+            a = b;
+        }
+        return tiles;
+    };
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int32_t I = 0; I < n_rb; ++I) ntiles[I + 1] = row_block(I, false, 0, nullptr, nullptr);
+    for (int32_t I = 0; I < n_rb; ++I) ntiles[I + 1] += ntiles[I];
+    std::vector<uint32_t> list;
+    int64_t dense_edges = 0;
+    for (int32_t I = 0; I < n_rb; ++I) row_block(I, true, ntiles[I], &list, &dense_edges);
+    p->n_dense = (uint32_t)list.size();
+    p->info.dense_tiles = p->n_dense;
+    p->info.dense_edges = dense_edges;
+    if (list.empty()) return;
+    p->bsr = std::make_unique<btc_bitbsr_s>();
+    bitbsr_build(g, K, s, p->bsr.get());
+    BTC_REQUIRE(p->bsr->nblk == ntiles[n_rb], BTC_ERR_STATE, "tile count mismatch (%u vs %u)", p->bsr->nblk,
+                ntiles[n_rb]);
+    p->dense_list.alloc(list.size());
+    BTC_HIP(hipMemcpyAsync(p->dense_list.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
+    BTC_HIP(hipStreamSynchronize(s));
+}
+
 void classify_dense(btc_plan_s* p, hipStream_t s, std::vector<uint8_t>& dense) {
     btc_graph_s* g = p->g;
     dense.assign((size_t)g->m, 0);
     if (p->tau >= 16 * p->K || g->m == 0) return;
+    if (g->m <= kHostClassifyEdges) {
+        classify_dense_host(p, s, dense);
+        return;
+    }
     p->bsr = std::make_unique<btc_bitbsr_s>();
     bitbsr_build(g, p->K, s, p->bsr.get());
     btc_bitbsr_s* b = p->bsr.get();
@@ -527,17 +584,20 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
     // prefix ids + pivot rows (4 B each), list entries (start, end, row: 12 B), items (32 B)
     p->info.stream_bytes = 4 * (checks + pivot_words) + 12 * ne + 32 * (int64_t)items.size();
     p->info.work_items = (int64_t)items.size();
-    p->ustart.alloc((size_t)ne);
-    p->uend.alloc((size_t)ne);
-    p->urow.alloc((size_t)ne);
-    p->items.alloc(2 * items.size());
-    if (ne) {
-        BTC_HIP(hipMemcpyAsync(p->ustart.p, us.data(), (size_t)ne * 4, hipMemcpyHostToDevice, s));
-        BTC_HIP(hipMemcpyAsync(p->uend.p, ue.data(), (size_t)ne * 4, hipMemcpyHostToDevice, s));
-        BTC_HIP(hipMemcpyAsync(p->urow.p, ur.data(), (size_t)ne * 4, hipMemcpyHostToDevice, s));
-    }
-    if (!items.empty())
-        BTC_HIP(hipMemcpyAsync(p->items.p, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
+    // one device allocation: items (16-byte aligned) first, then us, ue, ur
+    const size_t ib = items.size() * sizeof(Item), lb = (size_t)ne * 4;
+    std::vector<unsigned char> host(ib + 3 * lb);
+    std::memcpy(host.data(), items.data(), ib);
+    std::memcpy(host.data() + ib, us.data(), lb);
+    std::memcpy(host.data() + ib + lb, ue.data(), lb);
+    std::memcpy(host.data() + ib + 2 * lb, ur.data(), lb);
+    p->lists.alloc(host.size());
+    if (!host.empty())
+        BTC_HIP(hipMemcpyAsync(p->lists.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    p->items_p = reinterpret_cast<const uint4*>(p->lists.p);
+    p->ustart_p = reinterpret_cast<const uint32_t*>(p->lists.p + ib);
+    p->uend_p = reinterpret_cast<const uint32_t*>(p->lists.p + ib + lb);
+    p->urow_p = reinterpret_cast<const int32_t*>(p->lists.p + ib + 2 * lb);
     BTC_HIP(hipStreamSynchronize(s));
 }
 
@@ -568,7 +628,7 @@ WedgeFn wedge_fn(const btc_count_opts& o, bool pv, unsigned* grid) {
 }
 
 PlanDev plan_dev(btc_plan_s* p, unsigned long long* d_total, int32_t* d_pv) {
-    return PlanDev{p->g->d_col.p, p->ustart.p, p->uend.p, p->urow.p, p->items.p, p->n_block_items, p->nitems,
+    return PlanDev{p->g->d_col.p, p->ustart_p, p->uend_p, p->urow_p, p->items_p, p->n_block_items, p->nitems,
                    d_total, d_pv};
 }
 
@@ -689,6 +749,7 @@ int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_p
         btc_count_opts o;
         btc_count_opts_default(&o);
         if (opts) o = *opts;
+        btcx::tune_table_apply(g, &o);  // auto fields from the autotuner's table, when it has this graph's class
         BTC_REQUIRE(o.tile_k == 0 || o.tile_k == 32 || o.tile_k == 64 || o.tile_k == 128, BTC_ERR_ARG,
                     "tile width K = %d (expected 0, 32, 64 or 128)", o.tile_k);
         hipStream_t s = btcx::as_stream(stream);
@@ -713,8 +774,7 @@ int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_p
         p->scratch.alloc(2);
         p->info.tile_k = p->K;
         p->info.tau = p->tau;
-        int64_t bytes = (int64_t)(p->ustart.bytes() + p->uend.bytes() + p->urow.bytes() + p->items.bytes() +
-                                  p->dense_list.bytes() + p->scratch.bytes());
+        int64_t bytes = (int64_t)(p->lists.bytes() + p->dense_list.bytes() + p->scratch.bytes());
         if (p->bsr)
             bytes += (int64_t)(p->bsr->indptr.bytes() + p->bsr->indices.bytes() + p->bsr->row_indices.bytes() +
                                p->bsr->tiles.bytes());

@@ -28,9 +28,11 @@ struct btc_plan_s {
     std::unique_ptr<btc_bitbsr_s> bsr;
     btcx::DevBuf<uint32_t> dense_list;
     uint32_t n_dense = 0;
-    btcx::DevBuf<uint32_t> ustart, uend;
-    btcx::DevBuf<int32_t> urow;
-    btcx::DevBuf<uint4> items;  // 2 per item
+    btcx::DevBuf<unsigned char> lists;  // items (2 x uint4 each), then ustart, uend, urow
+    const uint4* items_p = nullptr;
+    const uint32_t* ustart_p = nullptr;
+    const uint32_t* uend_p = nullptr;
+    const int32_t* urow_p = nullptr;
     uint32_t nitems = 0;
     uint32_t n_block_items = 0;
     btcx::DevBuf<unsigned long long> scratch;  // [0] total, [1] work-item counter

@@ -26,6 +26,8 @@
 #include <thread>
 #include <unordered_map>
 
+#include <omp.h>
+
 #include "internal.h"
 
 namespace {
@@ -114,6 +116,8 @@ struct btc_server_s {
     }
 
     void prep_loop() {
+        // prep threads already run side by side: no nested OpenMP teams
+        omp_set_num_threads(1);
         (void)hipSetDevice(device);
         hipStream_t s = nullptr;
         (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);

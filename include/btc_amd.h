@@ -210,6 +210,30 @@ void btc_plan_free(btc_plan p);
 int btc_count(btc_graph g, const btc_count_opts* opts, void* stream, uint64_t* total,
               int32_t* per_vertex, float* kernel_ms);
 
+/* ------------------------------------------------------------ autotuner
+ * Times candidate plans (K in {32, 64, 128} selected by tile_ks_mask bits
+ * 1/2/4, tau in {default, 2 x default, 16K = no dense tiles}) and launch
+ * shapes (lanes per list, loads in flight, resident workgroups per CU) on
+ * the device within budget_ms and returns the fastest options.  The winner
+ * is recorded in a process-wide table keyed by log2 of n, m, wedges and the
+ * longest row; btc_plan_create fills options left on auto (tile_k 0, tau
+ * < 0, group_lanes 0, unroll 0, waves_per_eu 0) from that table.  Replaces
+ * the reference's fixed Lite rule and tau literals. */
+typedef struct {
+    int32_t budget_ms;    /* default 4000 */
+    int32_t reps;         /* timed counts per candidate, best kept; default 3 */
+    int32_t tile_ks_mask; /* default 7 */
+    int32_t reserved;
+} btc_tune_opts;
+
+void btc_tune_opts_default(btc_tune_opts* t);
+int btc_autotune(btc_graph g, const btc_tune_opts* t, void* stream, btc_count_opts* best, float* best_ms);
+/* BTC_OK and the tuned options when the table holds g's feature class. */
+int btc_tune_lookup(btc_graph g, btc_count_opts* opts);
+int btc_tune_table_save(const char* path);
+int btc_tune_table_load(const char* path);
+void btc_tune_table_clear(void);
+
 /* ------------------------------------------------------------- batching
  * Many graphs counted by one persistent launch: their L's are placed on the
  * diagonal of one block-diagonal L (each graph's vertices start at a
