@@ -72,15 +72,16 @@ def cpu_baseline(g, gpu_total):
 
 def load_traffic(workload):
     """HBM bytes per launch of the wedge kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration); None when no summary exists for this workload."""
-    path = os.path.join(ROOT, "profiles", "wedge_kernel_%s_traffic.json" % workload)
+    counter summary (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter
+    reads for a copy of known size); None when no summary exists for this
+    workload."""
+    path = os.path.join(ROOT, "profiles", "wedge_kernel_%s_traffic.md" % workload)
     if not os.path.exists(path):
         return None
-    try:
-        return json.load(open(path))["hbm_bytes_per_launch"]
-    except Exception:
-        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def copy_calibration(mb):
@@ -100,6 +101,92 @@ def copy_calibration(mb):
     hip.hipFree(a)
     hip.hipFree(b)
     log("copy calibration: %d MiB device-to-device" % mb)
+
+
+VALU_BITPAIRS_PEAK = 6.3e14   # 256 CUs x 2.4 GHz x 64 lanes: 2 v_and_b32 + 2 v_bcnt per 64 bit pairs
+MFMA_I8_MAC_PEAK = 2.5e15     # 5.0 POP/s dense int8 = 2.5e15 multiply-accumulates/s
+
+
+def dense_ops(b, tau):
+    """Algorithmic work of the dense path over Bit-BSR b: for every mask tile
+    with popcount > tau, every slab (row block Jo) of its 16 x K columns and
+    every tile column matched between its row block and Jo, 16 * 16 * K bit
+    pairs.  Returns (dense tiles, matched pairs, bit pairs)."""
+    indptr, indices, rows, tiles = b.to_host()
+    K = b.view["tile_k"]
+    wpt = K // 4
+    pop = np.unpackbits(tiles.view(np.uint8).reshape(-1, wpt * 8), axis=1).sum(axis=1)
+    dense = np.nonzero(pop > tau)[0]
+    n_rb = len(indptr) - 1
+    cols = {}
+
+    def tc(I):
+        if I not in cols:
+            cols[I] = set(indices[indptr[I]:indptr[I + 1]].tolist())
+        return cols[I]
+
+    pairs = 0
+    for t in dense.tolist():
+        A = tc(int(rows[t]))
+        for s in range(K // 16):
+            Jo = int(indices[t]) * (K // 16) + s
+            if Jo < n_rb:
+                pairs += len(A & tc(Jo))
+    return len(dense), pairs, pairs * 256 * K
+
+
+def dense_leg(btc, runs):
+    """Dense tiles, AND + popcount against i8 MFMA: the planted-community
+    graph (2^17 vertices, communities of 128 with p = 0.5, ER noise of
+    average degree 4) and the 27-point stencil 64^3, both in natural
+    (compacted input) order so that dense tiles survive."""
+    out = {}
+    work = [("planted-2^17-c128-p0.5-noise4", lambda: (btc.gen_planted(1 << 17, 128, 0.5, 4.0, seed=1), 1 << 17)),
+            ("stencil27-64x64x64", lambda: (btc.gen_stencil3d(64, 64, 64), 64 ** 3))]
+    for name, make in work:
+        (s, d), n = make()
+        g = btc.Graph.from_coo(s, d, n, btc.CANON_COMPACT)
+        want = None
+        res = {"m": g.m}
+        for K in (32, 64, 128):
+            b = btc.BitBSR(g, K)
+            tau = btc.default_tau(K)
+            nd, pairs, ops = dense_ops(b, tau)
+            row = {"tau": tau, "dense_tiles": nd, "matched_pairs": pairs, "bit_pairs": ops}
+            for mf in (False, True):
+                b.count(tau=tau, mfma=mf)
+                ms = []
+                for _ in range(runs):
+                    t, _, k = b.count(tau=tau, mfma=mf)
+                    want = t if want is None else want
+                    assert t == want, (name, K, mf, t, want)
+                    ms.append(k)
+                row["mfma_ms" if mf else "and_popc_ms"] = round(statistics.median(ms), 4)
+            # engine: dense-tile kernel alone (sparse edges go to the wedge kernel)
+            p = btc.Plan(g, tile_k=K, tau=tau)
+            for dk in (1, 2):
+                p.tune(dense_kernel=dk)
+                p.count(per_vertex=False)
+                dms = []
+                for _ in range(runs):
+                    t, _, _ = p.count()
+                    assert t == want, (name, K, dk)
+                    dms.append(p.last_timing()[0])
+                row["engine_dense_%s_ms" % ("mfma" if dk == 2 else "and_popc")] = round(statistics.median(dms), 4)
+            if ops and row["engine_dense_and_popc_ms"] > 0:
+                a = ops / (row["engine_dense_and_popc_ms"] * 1e-3)
+                f = ops / (row["engine_dense_mfma_ms"] * 1e-3)
+                row["and_popc_bitpairs_per_s"] = a
+                row["and_popc_frac_valu"] = round(a / VALU_BITPAIRS_PEAK, 4)
+                row["mfma_macs_per_s"] = f
+                row["mfma_frac_i8"] = round(f / MFMA_I8_MAC_PEAK, 4)
+            res["K%d" % K] = row
+            p.close()
+            b.close()
+        res["triangles"] = int(want)
+        log("dense %s: %s" % (name, res))
+        out[name] = res
+    return out
 
 
 def batched_leg(btc, count, steps, sample):
@@ -235,6 +322,7 @@ def main():
     ap.add_argument("--batch", type=int, default=4096, help="graphs in the batched leg (0 = skip)")
     ap.add_argument("--batch-sample", type=int, default=256, help="graphs timed one launch each")
     ap.add_argument("--serve", type=int, default=600, help="requests in the serving leg (0 = skip)")
+    ap.add_argument("--dense", type=int, default=5, help="timed runs per case in the dense-tile leg (0 = skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one D2D copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -293,8 +381,10 @@ def main():
     wedge_ms = statistics.mean(wms)
     b_alg = 4 * (gi["n"] + 1) + 4 * pi["sparse_edges"] + 4 * pi["sparse_merge_work"]
     achieved = b_alg / (wedge_ms * 1e-3) / 1e9 if wedge_ms > 0 else 0.0
+    traffic = load_traffic(config["workload"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(config["workload"]),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_GBps": round(traffic / (wedge_ms * 1e-3) / 1e9, 1) if traffic and wedge_ms > 0 else None,
                 "kernel": "wedge_kernel", "kernel_ms": round(wedge_ms, 4), "alg_bytes": b_alg,
                 "stream_bytes": pi["stream_bytes"],
                 "stream_GBps": round(pi["stream_bytes"] / (wedge_ms * 1e-3) / 1e9, 1) if wedge_ms > 0 else None}
@@ -326,6 +416,7 @@ def main():
 
     batched = batched_leg(btc, args.batch, max(3, args.steps // 2), args.batch_sample) if args.batch > 0 else None
     serving = serving_leg(btc, args.serve) if args.serve > 0 else None
+    dense = dense_leg(btc, args.dense) if args.dense > 0 else None
 
     config.update({"n": gi["n"], "m": gi["m"], "ordering": "degree-descending", "tile_k": pi["tile_k"],
                    "tau": pi["tau"], "dense_tiles": pi["dense_tiles"], "work_items": pi["work_items"]})
@@ -336,7 +427,7 @@ def main():
         "triangles": int(total0), "triangles_per_s": total0 / (wall / args.steps),
         "kernel_ms_median": statistics.median(kms), "canon_ms": round(canon_ms, 1), "plan_ms": round(build_ms, 1),
         "roofline": roofline, "cpu_baseline": cpu, "plain_hybrid": plain,
-        "batched": batched, "serving": serving,
+        "batched": batched, "serving": serving, "dense_tiles": dense,
     }
     print(json.dumps(out), flush=True)
 

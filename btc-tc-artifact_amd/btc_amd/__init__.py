@@ -30,6 +30,7 @@ CANON_DEFAULT = 3
 VARIANT_AUTO = 0
 VARIANT_BINSEARCH = 1
 VARIANT_DIRECT = 2
+DENSE_MFMA = 16
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 lib_path = os.path.join(_HERE, "lib", "libbtc_amd.so")
@@ -62,14 +63,14 @@ class BitBSRView(C.Structure):
 class CountOpts(C.Structure):
     _fields_ = [("tile_k", C.c_int32), ("tau", C.c_int32), ("chunk_work", C.c_int32),
                 ("waves_per_eu", C.c_int32), ("lookup", C.c_int32), ("group_lanes", C.c_int32),
-                ("unroll", C.c_int32), ("schedule", C.c_int32)]
+                ("unroll", C.c_int32), ("schedule", C.c_int32), ("dense_kernel", C.c_int32)]
 
 
 class PlanInfo(C.Structure):
     _fields_ = [("sparse_edges", C.c_int64), ("dense_edges", C.c_int64), ("dense_tiles", C.c_int64),
                 ("wedge_checks", C.c_int64), ("work_items", C.c_int64), ("device_bytes", C.c_int64),
-                ("sparse_merge_work", C.c_int64), ("stream_bytes", C.c_int64), ("tile_k", C.c_int32),
-                ("tau", C.c_int32)]
+                ("sparse_merge_work", C.c_int64), ("stream_bytes", C.c_int64), ("core_lists", C.c_int64),
+                ("core_rows", C.c_int64), ("tile_k", C.c_int32), ("tau", C.c_int32)]
 
 
 class ServerOpts(C.Structure):
@@ -317,9 +318,12 @@ class BitBSR:
         _check(lib().btc_bitbsr_copy_to_host(self._h, _ptr(indptr), _ptr(indices), _ptr(rows), _ptr(tiles)))
         return indptr, indices, rows, tiles
 
-    def count(self, variant=VARIANT_AUTO, threshold_blocks=0, tau=-1, per_vertex=False, stream=None):
-        """Returns (total, per_vertex or None, kernel_ms)."""
+    def count(self, variant=VARIANT_AUTO, threshold_blocks=0, tau=-1, per_vertex=False, stream=None, mfma=False):
+        """Returns (total, per_vertex or None, kernel_ms); mfma=True counts
+        dense tiles with the i8 MFMA form."""
         total, ms, pv = _count_out(self.view["n"], per_vertex)
+        if mfma:
+            variant |= DENSE_MFMA
         _check(lib().btc_bitbsr_count(self._h, variant, threshold_blocks, tau, stream, C.byref(total), _ptr(pv),
                                       C.byref(ms)))
         return total.value, pv, ms.value
@@ -337,12 +341,13 @@ class BitBSR:
 
 
 def make_opts(tile_k=None, tau=None, chunk_work=None, waves_per_eu=None, lookup=None, group_lanes=None,
-              unroll=None, schedule=None):
+              unroll=None, schedule=None, dense_kernel=None):
     """btc_count_opts with the library defaults, overridden where given."""
     o = CountOpts()
     lib().btc_count_opts_default(C.byref(o))
     for k, v in dict(tile_k=tile_k, tau=tau, chunk_work=chunk_work, waves_per_eu=waves_per_eu, lookup=lookup,
-                     group_lanes=group_lanes, unroll=unroll, schedule=schedule).items():
+                     group_lanes=group_lanes, unroll=unroll, schedule=schedule,
+                     dense_kernel=dense_kernel).items():
         if v is not None:
             setattr(o, k, v)
     return o
@@ -365,7 +370,7 @@ class Plan:
         return {k: getattr(pi, k) for k, _ in PlanInfo._fields_}
 
     def tune(self, **opts):
-        """Changes launch-only options (waves_per_eu, lookup, group_lanes, unroll, schedule)."""
+        """Changes launch-only options (waves_per_eu, lookup, group_lanes, unroll, schedule, dense_kernel)."""
         o = make_opts(**opts)
         _check(lib().btc_plan_tune(self._h, C.byref(o)))
 

@@ -1,6 +1,7 @@
-// Autotuner: picks the dense tile width K, the density threshold tau and
-// the wedge kernel's launch shape (lanes per list, loads in flight,
-// resident workgroups per CU) by timing candidates on the device, instead
+// Autotuner: picks the dense tile width K, the density threshold tau, the
+// dense-tile kernel form (AND + popcount or i8 MFMA) and the wedge kernel's
+// launch shape (lanes per list, loads in flight, resident workgroups per
+// CU) by timing candidates on the device, instead
 // of the reference's fixed rules (apps/btc_tc_lite.cu:45-65 for K,
 // tc_16x128_mma_v5.h:51 / tc_16x32_mma_v6.h:52 for tau).
 //
@@ -8,7 +9,7 @@
 // m, wedge count and longest row), which btc_plan_create consults when K,
 // tau or the launch shape are left on auto.  The table is saved and loaded
 // as text, one entry per line:
-//   btc-tune-1 <n_bits> <m_bits> <w_bits> <r_bits> <K> <tau> <G> <U> <wpe> <ms>
+//   btc-tune-2 <n_bits> <m_bits> <w_bits> <r_bits> <K> <tau> <G> <U> <wpe> <dense> <ms>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -50,6 +51,7 @@ bool tune_table_apply(const btc_graph_s* g, btc_count_opts* o) {
     if (o->group_lanes == 0) o->group_lanes = t.group_lanes;
     if (o->unroll == 0) o->unroll = t.unroll;
     if (o->waves_per_eu == 0) o->waves_per_eu = t.waves_per_eu;
+    if (o->dense_kernel == 0) o->dense_kernel = t.dense_kernel;
     return true;
 }
 
@@ -96,20 +98,11 @@ int btc_autotune(btc_graph g, const btc_tune_opts* topts, void* stream, btc_coun
                 btc_plan p = nullptr;
                 btcx::check_status(btc_plan_create(g, &o, stream, &p));
                 // launch-level candidates on the same plan
-                const int shapes[4][3] = {{8, 1, 0}, {8, 2, 0}, {16, 1, 0}, {8, 1, 6}};
-                for (const auto& sh : shapes) {
-                    btc_count_opts lo = o;
-                    lo.group_lanes = sh[0];
-                    lo.unroll = sh[1];
-                    lo.waves_per_eu = sh[2];
+                auto time_opts = [&](const btc_count_opts& lo) {
                     int rc = btc_plan_tune(p, &lo);
-                    if (rc != BTC_OK) {
-                        btc_plan_free(p);
-                        btcx::check_status(rc);
-                    }
                     uint64_t total = 0;
                     float ms = 0, mn = 1e30f;
-                    rc = btc_plan_count(p, stream, &total, nullptr, &ms);  // warm-up
+                    if (rc == BTC_OK) rc = btc_plan_count(p, stream, &total, nullptr, &ms);  // warm-up
                     for (int r = 0; r < t.reps && rc == BTC_OK; ++r) {
                         rc = btc_plan_count(p, stream, &total, nullptr, &ms);
                         mn = std::min(mn, ms);
@@ -123,6 +116,29 @@ int btc_autotune(btc_graph g, const btc_tune_opts* topts, void* stream, btc_coun
                         best_o = lo;
                         any = true;
                     }
+                    return mn;
+                };
+                const int shapes[4][3] = {{8, 1, 0}, {8, 2, 0}, {16, 1, 0}, {8, 1, 6}};
+                btc_count_opts plan_best = o;
+                float plan_ms = 1e30f;
+                for (const auto& sh : shapes) {
+                    btc_count_opts lo = o;
+                    lo.group_lanes = sh[0];
+                    lo.unroll = sh[1];
+                    lo.waves_per_eu = sh[2];
+                    lo.dense_kernel = 1;
+                    const float ms = time_opts(lo);
+                    if (ms < plan_ms) {
+                        plan_ms = ms;
+                        plan_best = lo;
+                    }
+                }
+                // the dense-tile kernel form, on the plan's best shape
+                btc_plan_info pi;
+                btcx::check_status(btc_plan_get_info(p, &pi));
+                if (pi.dense_tiles > 0) {
+                    plan_best.dense_kernel = 2;
+                    time_opts(plan_best);
                 }
                 btc_plan_free(p);
             }
@@ -151,9 +167,9 @@ int btc_tune_table_save(const char* path) {
         std::lock_guard<std::mutex> lk(btcx::g_mu);
         for (const auto& kv : btcx::g_table) {
             const auto& o = kv.second.opts;
-            std::fprintf(f, "btc-tune-1 %d %d %d %d %d %d %d %d %d %.4f\n", std::get<0>(kv.first),
+            std::fprintf(f, "btc-tune-2 %d %d %d %d %d %d %d %d %d %d %.4f\n", std::get<0>(kv.first),
                          std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), o.tile_k, o.tau,
-                         o.group_lanes, o.unroll, o.waves_per_eu, kv.second.ms);
+                         o.group_lanes, o.unroll, o.waves_per_eu, o.dense_kernel, kv.second.ms);
         }
         std::fclose(f);
     });
@@ -165,17 +181,17 @@ int btc_tune_table_load(const char* path) {
         FILE* f = std::fopen(path, "r");
         BTC_REQUIRE(f, BTC_ERR_ARG, "cannot read %s", path);
         char tag[32];
-        int a, b, c, d, K, tau, G, U, w;
+        int a, b, c, d, K, tau, G, U, w, D;
         float ms;
         int line = 0;
         std::map<btcx::Key, btcx::Tuned> tab;
         for (;;) {
-            const int r = std::fscanf(f, "%31s %d %d %d %d %d %d %d %d %d %f", tag, &a, &b, &c, &d, &K, &tau, &G, &U,
-                                      &w, &ms);
+            const int r = std::fscanf(f, "%31s %d %d %d %d %d %d %d %d %d %d %f", tag, &a, &b, &c, &d, &K, &tau, &G,
+                                      &U, &w, &D, &ms);
             if (r == EOF) break;
             ++line;
-            if (r != 11 || std::strcmp(tag, "btc-tune-1") != 0 || !(K == 32 || K == 64 || K == 128) ||
-                !(G == 8 || G == 16 || G == 32) || !(U == 1 || U == 2 || U == 4) || w < 0) {
+            if (r != 12 || std::strcmp(tag, "btc-tune-2") != 0 || !(K == 32 || K == 64 || K == 128) ||
+                !(G == 8 || G == 16 || G == 32) || !(U == 1 || U == 2 || U == 4) || w < 0 || D < 0 || D > 2) {
                 std::fclose(f);
                 btcx::fail(BTC_ERR_ARG, "%s: bad entry on line %d", path, line);
             }
@@ -186,6 +202,7 @@ int btc_tune_table_load(const char* path) {
             o.group_lanes = G;
             o.unroll = U;
             o.waves_per_eu = w;
+            o.dense_kernel = D;
             tab[btcx::Key{a, b, c, d}] = btcx::Tuned{o, ms};
         }
         std::fclose(f);

@@ -304,7 +304,54 @@ void build_impl(btc_graph_s* g, hipStream_t s, btc_bitbsr_s* b) {
 }
 
 // ------------------------------------------------------------------- count
-template <int K, bool BINSEARCH, bool PV>
+// i8 MFMA form of the dense intersection.  A matched tile pair (pa of row
+// block I, pb of row block Jo) contributes C = A B^T, A and B the 16 x K 0/1
+// matrices of the two tiles, to the 16 x 16 block of (row of I, row of Jo)
+// intersections.  v_mfma_i32_16x16x64_i8 takes A[i][k] with i = lane % 16
+// and B[k][j] with j = lane % 16, 16 bytes per lane; k is a sum index, so
+// any assignment of tile columns to a lane's bytes is exact as long as A and
+// B use the same one: lane group g = lane / 16 takes columns 16g..16g+15 of
+// each 64-column half (8g..8g+7 with the 16x16x32 form for K = 32).  D[r][c]
+// sits in register v of lane l with r = 4 (l / 16) + v, c = l % 16.
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// 4 bits -> 4 bytes of value 0/1 (byte b = bit b); x < 16, no carries.
+__device__ __forceinline__ int nib_bytes(uint32_t x) { return (int)((x * 0x00204081u) & 0x01010101u); }
+
+__device__ __forceinline__ v4i bytes16(uint32_t b) {
+    v4i v;
+    v.x = nib_bytes(b & 15u);
+    v.y = nib_bytes((b >> 4) & 15u);
+    v.z = nib_bytes((b >> 8) & 15u);
+    v.w = nib_bytes((b >> 12) & 15u);
+    return v;
+}
+
+__device__ __forceinline__ long bytes8(uint32_t b) {
+    return (long)((uint64_t)(uint32_t)nib_bytes(b & 15u) | ((uint64_t)(uint32_t)nib_bytes((b >> 4) & 15u) << 32));
+}
+
+template <int K>
+__device__ __forceinline__ v4i mfma_pair(const uint64_t* __restrict__ T, uint32_t pa, uint32_t pb, int i, int g,
+                                         v4i acc) {
+    if constexpr (K == 32) {
+        const uint32_t a = reinterpret_cast<const uint32_t*>(T + (size_t)pa * 8)[i];
+        const uint32_t b = reinterpret_cast<const uint32_t*>(T + (size_t)pb * 8)[i];
+        return __builtin_amdgcn_mfma_i32_16x16x32_i8(bytes8((a >> (8 * g)) & 0xFFu), bytes8((b >> (8 * g)) & 0xFFu),
+                                                     acc, 0, 0, 0);
+    } else {
+#pragma unroll
+        for (int h = 0; h < K / 64; ++h) {
+            const uint64_t a = T[(size_t)pa * (K / 4) + (K / 64) * i + h];
+            const uint64_t b = T[(size_t)pb * (K / 4) + (K / 64) * i + h];
+            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(bytes16((uint32_t)(a >> (16 * g)) & 0xFFFFu),
+                                                        bytes16((uint32_t)(b >> (16 * g)) & 0xFFFFu), acc, 0, 0, 0);
+        }
+        return acc;
+    }
+}
+
+template <int K, bool BINSEARCH, bool PV, bool MFMA>
 __global__ __launch_bounds__(256) void tile_count_kernel(
     int n, int n_rb, const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
     const uint32_t* __restrict__ row_indices, const uint64_t* __restrict__ tiles, uint32_t w0,
@@ -337,6 +384,52 @@ __global__ __launch_bounds__(256) void tile_count_kernel(
     const int r = lane >> 2, q = lane & 3;
     const RowBits<K> S = RowBits<K>::load(tiles, t, r);
     uint32_t sum = 0;
+    if (MFMA && dense) {
+        // lane owns rows 4g + v (v < 4) of the mask tile and column c of each slab
+        const int g = lane >> 4, c = lane & 15;
+        RowBits<K> M[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) M[v] = RowBits<K>::load(tiles, t, 4 * g + v);
+        uint32_t rs[4] = {0, 0, 0, 0};
+        for (int it = 0; it < SLABS; ++it) {
+            const uint32_t Jo = JL * SLABS + it;
+            if (Jo >= (uint32_t)n_rb) break;
+            const uint32_t b0 = uniform(indptr[Jo]), b1 = uniform(indptr[Jo + 1]);
+            v4i acc = {0, 0, 0, 0};
+            uint32_t pa = a0, pb = b0;
+            while (pa < a1 && pb < b1) {
+                const uint32_t ka = uniform(indices[pa]), kb = uniform(indices[pb]);
+                if (ka < kb) {
+                    ++pa;
+                } else if (ka > kb) {
+                    ++pb;
+                } else {
+                    acc = mfma_pair<K>(tiles, pa, pb, c, g, acc);
+                    ++pa;
+                    ++pb;
+                }
+            }
+            rs[0] += (M[0].bits(it * 16 + c) & 1u) ? (uint32_t)acc.x : 0u;
+            rs[1] += (M[1].bits(it * 16 + c) & 1u) ? (uint32_t)acc.y : 0u;
+            rs[2] += (M[2].bits(it * 16 + c) & 1u) ? (uint32_t)acc.z : 0u;
+            rs[3] += (M[3].bits(it * 16 + c) & 1u) ? (uint32_t)acc.w : 0u;
+        }
+        if (PV) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                uint32_t x = rs[v];
+                x += __shfl_xor(x, 1);
+                x += __shfl_xor(x, 2);
+                x += __shfl_xor(x, 4);
+                x += __shfl_xor(x, 8);
+                const int64_t vtx = (int64_t)I * 16 + 4 * g + v;
+                if (c == 0 && x && vtx < n) atomicAdd(pv + vtx, (int32_t)x);
+            }
+        }
+        const uint64_t tot = wave_sum_u64((uint64_t)(rs[0] + rs[1] + rs[2] + rs[3]));
+        if (lane == 0 && tot) atomicAdd(result, (unsigned long long)tot);
+        return;
+    }
     for (int it = 0; it < SLABS; ++it) {
         const uint32_t Jo = JL * SLABS + it;
         if (Jo >= (uint32_t)n_rb) break;
@@ -411,7 +504,7 @@ __global__ __launch_bounds__(256) void tile_popcount_kernel(const uint64_t* __re
 // into launches of kMaxTilesPerLaunch waves.
 constexpr uint32_t kMaxTilesPerLaunch = 1u << 24;
 
-template <int K, bool BS>
+template <int K, bool BS, bool MF>
 void launch_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t nt, int tau, unsigned long long* res,
               int32_t* pv, hipStream_t s) {
     for (uint32_t w0 = 0; w0 < nt; w0 += kMaxTilesPerLaunch) {
@@ -419,14 +512,25 @@ void launch_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t nt, int tau,
         const unsigned grid = (chunk + 3) / 4;
         const uint32_t end = w0 + chunk;
         if (pv)
-            hipLaunchKernelGGL((tile_count_kernel<K, BS, true>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
+            hipLaunchKernelGGL((tile_count_kernel<K, BS, true, MF>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
                                b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, end, list, tau,
                                res, pv);
         else
-            hipLaunchKernelGGL((tile_count_kernel<K, BS, false>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
+            hipLaunchKernelGGL((tile_count_kernel<K, BS, false, MF>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
                                b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, end, list, tau,
                                res, pv);
         BTC_CHECK_LAUNCH("tile_count_kernel");
+    }
+}
+
+template <int K>
+void launch_kk(const btc_bitbsr_s* b, const uint32_t* list, uint32_t nt, bool bs, bool mfma, int tau,
+               unsigned long long* res, int32_t* pv, hipStream_t s) {
+    if (bs) {
+        mfma ? launch_k<K, true, true>(b, list, nt, tau, res, pv, s) : launch_k<K, true, false>(b, list, nt, tau, res, pv, s);
+    } else {
+        mfma ? launch_k<K, false, true>(b, list, nt, tau, res, pv, s)
+             : launch_k<K, false, false>(b, list, nt, tau, res, pv, s);
     }
 }
 
@@ -443,23 +547,14 @@ void bitbsr_build(btc_graph_s* g, int K, hipStream_t s, btc_bitbsr_s* b) {
 }
 
 void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles, bool binsearch,
-                       int tau, unsigned long long* d_result, int32_t* d_pv, hipStream_t s) {
+                       bool mfma, int tau, unsigned long long* d_result, int32_t* d_pv, hipStream_t s) {
     if (ntiles == 0) return;
     BTC_REQUIRE(b->max_row <= 65536, BTC_ERR_OVERFLOW,
                 "longest row %d > 65536: per-lane 32-bit sums could overflow", b->max_row);
     switch (b->K) {
-        case 32:
-            binsearch ? launch_k<32, true>(b, tile_list, ntiles, tau, d_result, d_pv, s)
-                      : launch_k<32, false>(b, tile_list, ntiles, tau, d_result, d_pv, s);
-            break;
-        case 64:
-            binsearch ? launch_k<64, true>(b, tile_list, ntiles, tau, d_result, d_pv, s)
-                      : launch_k<64, false>(b, tile_list, ntiles, tau, d_result, d_pv, s);
-            break;
-        default:
-            binsearch ? launch_k<128, true>(b, tile_list, ntiles, tau, d_result, d_pv, s)
-                      : launch_k<128, false>(b, tile_list, ntiles, tau, d_result, d_pv, s);
-            break;
+        case 32: launch_kk<32>(b, tile_list, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
+        case 64: launch_kk<64>(b, tile_list, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
+        default: launch_kk<128>(b, tile_list, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
     }
 }
 
@@ -526,6 +621,8 @@ int btc_bitbsr_count(btc_bitbsr b, int32_t variant, int32_t threshold_blocks, in
                      uint64_t* total, int32_t* per_vertex, float* kernel_ms) {
     return btcx::guard([&] {
         BTC_REQUIRE(b && total, BTC_ERR_ARG, "NULL argument");
+        const bool mfma = (variant & BTC_DENSE_MFMA) != 0;
+        variant &= ~BTC_DENSE_MFMA;
         BTC_REQUIRE(variant >= BTC_VARIANT_AUTO && variant <= BTC_VARIANT_DIRECT, BTC_ERR_ARG,
                     "unknown variant %d", variant);
         hipStream_t s = btcx::as_stream(stream);
@@ -541,7 +638,7 @@ int btc_bitbsr_count(btc_bitbsr b, int32_t variant, int32_t threshold_blocks, in
         BTC_HIP(hipMemsetAsync(b->result.p, 0, 8, s));
         btcx::EventTimer tm(kernel_ms != nullptr);
         tm.start(s);
-        btcx::launch_tile_count(b, nullptr, b->nblk, bs, tau, b->result.p, pv.p, s);
+        btcx::launch_tile_count(b, nullptr, b->nblk, bs, mfma, tau, b->result.p, pv.p, s);
         tm.stop(s);
         unsigned long long h = 0;
         BTC_HIP(hipMemcpyAsync(&h, b->result.p, 8, hipMemcpyDeviceToHost, s));

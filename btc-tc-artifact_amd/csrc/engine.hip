@@ -39,13 +39,27 @@ constexpr int kWaveWords = 1024;  // per-wave slice of it for wave items
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 // Wave items: pivots with at most this many prefix lists.
 constexpr uint32_t kWaveItemLists = 64;
+// Hub core: ids below kCoreIds.  Rows with at least kCoreRowMin core ids
+// keep a kCoreIds-bit bitmap of them (kCoreWords 32-bit words, 512 bytes).
+constexpr uint32_t kCoreIds = 4096;
+constexpr uint32_t kCoreWords = kCoreIds / 32;
+constexpr uint32_t kCoreRowMin = 32;
+constexpr uint32_t kNoBitmap = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t hslot(uint32_t key, int shift) { return (key * 2654435761u) >> shift; }
+
+// 16-byte load that does not keep the line in the caches (read-once data).
+__device__ __forceinline__ uint4 load_nt(const uint4* p) {
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const v4u v = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
 
 // Membership tests for N_L(pivot).  Ids of a list's clipped prefix lie in
 // [min N_L(j), max N_L(j)]; ids outside the prefix (partial 16-byte loads)
 // are replaced by sentinel(), which every set reports absent.
-struct BitmapSet {  // bit (k - base) of an LDS window; base is a multiple of 32
+struct BitmapSet {  // bit (k - base) of an LDS window; base is a multiple of 128
+    static constexpr bool kBitmap = true;
     const uint32_t* words;  // LDS window minus base / 32 words: words[k >> 5] is k's word
     uint32_t last;          // base + window bits - 1: never set
     __device__ __forceinline__ uint32_t sentinel() const { return last; }
@@ -56,6 +70,7 @@ struct BitmapSet {  // bit (k - base) of an LDS window; base is a multiple of 32
 };
 
 struct HashSet {  // open addressing, linear probing, load <= 1/2
+    static constexpr bool kBitmap = false;
     const uint32_t* slots;
     int shift;
     uint32_t mask;
@@ -72,6 +87,7 @@ struct HashSet {  // open addressing, linear probing, load <= 1/2
 };
 
 struct SearchSet {  // binary search of the sorted pivot row in HBM
+    static constexpr bool kBitmap = false;
     const int32_t* col;
     uint32_t t0, t1;
     __device__ __forceinline__ uint32_t sentinel() const { return kEmpty; }  // above every id
@@ -89,8 +105,8 @@ struct SearchSet {  // binary search of the sorted pivot row in HBM
 // Device view of one plan's work lists.
 struct PlanDev {
     const int32_t* col;
-    const uint32_t* us;
-    const uint32_t* ue;
+    const uint4* lists;        // per list {s0, s_noncore, s1, row bitmap or kNoBitmap}
+    const uint4* rowbits;      // kCoreWords / 4 uint4 per row bitmap
     const int32_t* ur;
     const uint4* items;  // 2 x uint4 per item: {t0, t1, eb, ee}, {lo, hi, work, 0}
     uint32_t n_block;    // items [0, n_block) are workgroup items, the rest wave items
@@ -127,17 +143,19 @@ __device__ __forceinline__ void team_sync() {
     }
 }
 
-// Streams the prefixes [us[e], ue[e]) of lists e in [eb, ee) through `set`.
-// A group of G lanes owns lists eb + g, eb + g + NT/G, ...; it reads a list
-// with aligned 16-byte loads (4 ids per lane, U loads in flight) and
-// replaces ids outside the prefix by the set's sentinel.  The next list's
-// bounds are fetched before the current list is scanned; list bounds are
-// read once per launch, so they are loaded non-temporally and leave the
-// caches to the column ids, which are re-read for every pivot of a row.
-// (s0, s1) are the bounds of the group's first list, loaded by the caller.
+// Counts the lists e in [eb, ee) of one pivot against `set`.  A group of G
+// lanes owns lists eb + g, eb + g + NT/G, ...  A list whose row has a core
+// bitmap (and whose pivot is in an LDS bitmap) first ANDs the row's core
+// bitmap with the pivot's window word by word (popcount) and then looks up
+// only the prefix ids >= kCoreIds; any other list looks up its whole clipped
+// prefix.  Prefix ids are read with aligned 16-byte loads (4 per lane, U in
+// flight); ids outside the prefix become the set's sentinel.  The next
+// list's record is fetched before the current list is scanned; records are
+// read once per launch, so they are loaded non-temporally.  L is the
+// group's first record, loaded by the caller.
 template <int NT, int G, int U, bool PV, class Set>
 __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a, uint32_t tid, uint32_t eb,
-                                               uint32_t ee, uint32_t s0, uint32_t s1) {
+                                               uint32_t ee, uint4 L, const uint32_t* window, uint32_t bw) {
     constexpr uint32_t NG = NT / G;
     constexpr uint32_t STEP = 4 * G * U;
     const uint32_t g = tid / G, gl = tid % G;
@@ -147,13 +165,23 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
     uint64_t acc = 0;
     for (uint32_t e = eb + g; e < ee; e += NG) {
         const uint32_t en = e + NG;
-        uint32_t n0 = 0, n1 = 0;
-        if (en < ee) {
-            n0 = __builtin_nontemporal_load(a.us + en);
-            n1 = __builtin_nontemporal_load(a.ue + en);
-        }
-        const uint32_t len = s1 - s0;
+        uint4 nL = make_uint4(0, 0, 0, 0);
+        if (en < ee) nL = load_nt(a.lists + en);
         uint32_t hits = 0;
+        uint32_t s0 = L.x;
+        const uint32_t s1 = L.z;
+        if (Set::kBitmap && L.w != kNoBitmap) {
+            // core ids: row bitmap AND pivot window, 128 bits per lane step;
+            // window words below bw are empty (the pivot has no ids there)
+            const uint4* R = a.rowbits + (size_t)L.w * (kCoreWords / 4);
+            for (uint32_t w4 = (bw >> 2) + gl; w4 < kCoreWords / 4; w4 += G) {
+                const uint4 r = R[w4];
+                const uint4 pw = *reinterpret_cast<const uint4*>(window + 4 * w4 - bw);
+                hits += __popc(r.x & pw.x) + __popc(r.y & pw.y) + __popc(r.z & pw.z) + __popc(r.w & pw.w);
+            }
+            s0 = L.y;
+        }
+        const uint32_t len = s1 > s0 ? s1 - s0 : 0u;
         for (uint32_t q = (s0 & ~3u) + 4 * gl; q < s1; q += STEP) {
             uint4 v[U];
 #pragma unroll
@@ -178,8 +206,7 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
             for (int o = G / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, G);
             if (gl == 0 && c) atomicAdd(a.pv + a.ur[e], (int32_t)c);
         }
-        s0 = n0;
-        s1 = n1;
+        L = nL;
     }
     return acc;
 }
@@ -194,17 +221,14 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
                                                  uint32_t tid) {
     const uint32_t t0 = A.x, t1 = A.y, eb = A.z, ee = A.w, lo = B.x, hi = B.y;
     const uint32_t d = t1 - t0;  // >= 1 for every listed pivot
-    // bounds of this thread's first list, fetched while the set is built
-    uint32_t s0 = 0, s1 = 0;
+    // this thread's first list record, fetched while the set is built
+    uint4 L = make_uint4(0, 0, 0, 0);
     {
         const uint32_t e = eb + tid / G;
-        if (e < ee) {
-            s0 = __builtin_nontemporal_load(a.us + e);
-            s1 = __builtin_nontemporal_load(a.ue + e);
-        }
+        if (e < ee) L = load_nt(a.lists + e);
     }
     uint64_t acc = 0;
-    const uint32_t base = lo & ~31u;
+    const uint32_t base = lo & ~127u;  // 128-aligned: window words line up with core bitmap uint4s
     if (hi - base < 32u * W - 1u && lookup != 1) {  // the window's last bit stays 0 (sentinel)
         const uint32_t bw = base >> 5;
         for (uint32_t p = t0 + tid; p < t1; p += NT) {
@@ -212,7 +236,7 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
             atomicOr(&lds[(k >> 5) - bw], 1u << (k & 31u));
         }
         team_sync<NT>();
-        acc = scan_lists<NT, G, U, PV>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, s0, s1);
+        acc = scan_lists<NT, G, U, PV>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, L, lds, bw);
         team_sync<NT>();
         for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)a.col[p] >> 5) - bw] = 0;
     } else if (d <= (uint32_t)W / 2 && lookup != 2) {
@@ -228,11 +252,11 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
             while (atomicCAS(&lds[h], kEmpty, key) != kEmpty) h = (h + 1u) & mask;
         }
         team_sync<NT>();
-        acc = scan_lists<NT, G, U, PV>(HashSet{lds, shift, mask}, a, tid, eb, ee, s0, s1);
+        acc = scan_lists<NT, G, U, PV>(HashSet{lds, shift, mask}, a, tid, eb, ee, L, lds, 0u);
         team_sync<NT>();
         for (uint32_t x = tid; x <= mask; x += NT) lds[x] = 0;
     } else {
-        acc = scan_lists<NT, G, U, PV>(SearchSet{a.col, t0, t1}, a, tid, eb, ee, s0, s1);
+        acc = scan_lists<NT, G, U, PV>(SearchSet{a.col, t0, t1}, a, tid, eb, ee, L, lds, 0u);
     }
     return acc;
 }
@@ -285,7 +309,7 @@ __device__ __forceinline__ void flush(uint64_t& acc, unsigned long long* result)
 // atomic cursor.
 template <int G, int U, bool PV>
 __global__ __launch_bounds__(kThreads) void wedge_kernel(WedgeArgs a) {
-    __shared__ uint32_t lds[kLdsWords];
+    __shared__ __align__(16) uint32_t lds[kLdsWords];  // 16-byte reads of bitmap words
     __shared__ uint32_t s_item;
     const uint32_t tid = threadIdx.x;
     const int lane = tid & 63;
@@ -530,17 +554,46 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
     BTC_REQUIRE(ne < (int64_t)0xFFFFFFFFLL, BTC_ERR_OVERFLOW, "too many wedge lists");
 #pragma omp parallel for schedule(static)
     for (int32_t j = 0; j < n; ++j) cnt[j].store((uint32_t)off[j], std::memory_order_relaxed);
-    std::vector<uint32_t> us((size_t)ne), ue((size_t)ne);
+    // core bitmaps: rows with at least kCoreRowMin ids below kCoreIds
+    std::vector<uint32_t> cend((size_t)n), rbi((size_t)n, kNoBitmap);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int32_t i = 0; i < n; ++i)
+        cend[i] = (uint32_t)(std::lower_bound(cl + rp[i], cl + rp[i + 1], (int32_t)kCoreIds) - cl);
+    uint32_t nbm = 0;
+    for (int32_t i = 0; i < n; ++i)
+        if (cend[i] - rp[i] >= kCoreRowMin) rbi[i] = nbm++;
+    std::vector<uint32_t> rowbits((size_t)nbm * kCoreWords, 0u);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int32_t i = 0; i < n; ++i) {
+        if (rbi[i] == kNoBitmap) continue;
+        uint32_t* b = rowbits.data() + (size_t)rbi[i] * kCoreWords;
+        for (int64_t q = rp[i]; q < cend[i]; ++q) b[cl[q] >> 5] |= 1u << (cl[q] & 31);
+    }
+    std::vector<uint4> lst((size_t)ne);
     std::vector<int32_t> ur((size_t)ne);
+    std::atomic<int64_t> core_lists{0};
 #pragma omp parallel for schedule(dynamic, 1024)
     for (int32_t i = 0; i < n; ++i)
         for (int64_t q = rp[i]; q < rp[i + 1]; ++q) {
             if (dense[q]) continue;
             uint32_t s0, s1;
             if (!clip(i, q, s0, s1)) continue;
-            const uint32_t e = cnt[cl[q]].fetch_add(1, std::memory_order_relaxed);
-            us[e] = s0;
-            ue[e] = s1;
+            // the core AND costs (kCoreIds - base) / 32 words; per-id lookups
+            // cost about four times a word each: take the bitmap when cheaper
+            const int32_t j = cl[q];
+            const uint32_t base = (uint32_t)cl[rp[j]] & ~127u;
+            uint32_t sc = s0, rb = kNoBitmap;
+            if (rbi[i] != kNoBitmap && base < kCoreIds) {
+                const uint32_t core_ids = std::min(s1, cend[i]) > s0 ? std::min(s1, cend[i]) - s0 : 0u;
+                const uint32_t words = (kCoreIds - base) / 32;
+                if (core_ids * 4 > words + 64) {
+                    sc = std::max(s0, cend[i]);
+                    rb = rbi[i];
+                    core_lists.fetch_add(1, std::memory_order_relaxed);
+                }
+            }
+            const uint32_t e = cnt[j].fetch_add(1, std::memory_order_relaxed);
+            lst[e] = make_uint4(s0, sc, s1, rb);
             ur[e] = i;
         }
     // work items: consecutive lists of one pivot, about `chunk` checks each.
@@ -557,13 +610,18 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
         const uint32_t lo = (uint32_t)cl[rp[j]], hi = (uint32_t)cl[rp[j + 1] - 1];
         const bool wave_fit = (hi - lo < 32u * kWaveWords) || (t1 - t0 <= (uint32_t)kWaveWords / 2);
         const bool small = e1 - e <= (int64_t)kWaveItemLists && wave_fit;
+        // cost of a list: ids looked up, plus core words ANDed
+        auto cost = [&](int64_t f) -> int64_t {
+            const uint4& r = lst[f];
+            return (int64_t)(r.z > r.y ? r.z - r.y : 0) +
+                   (r.w != kNoBitmap ? (int64_t)(kCoreIds - (lo & ~127u)) / 32 : 0);
+        };
         while (e < e1) {
             int64_t w = 0, f = e;
-            while (f < e1 && (small || w == 0 || w + (ue[f] - us[f]) <= chunk)) {
-                w += ue[f] - us[f];
+            while (f < e1 && (small || w == 0 || w + cost(f) <= chunk)) {
+                w += cost(f);
                 ++f;
-            }
-            checks += w;
+            }            checks += w;
             pivot_words += t1 - t0;
             Item it{make_uint4(t0, t1, (uint32_t)e, (uint32_t)f),
                     make_uint4(lo, hi, (uint32_t)std::min<int64_t>(w, 0xFFFFFFFFLL), 0u)};
@@ -581,23 +639,27 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
     p->info.sparse_edges = sparse_edges;
     p->info.sparse_merge_work = sparse_work;
     p->info.wedge_checks = checks;
-    // prefix ids + pivot rows (4 B each), list entries (start, end, row: 12 B), items (32 B)
-    p->info.stream_bytes = 4 * (checks + pivot_words) + 12 * ne + 32 * (int64_t)items.size();
+    p->info.core_lists = core_lists.load();
+    p->info.core_rows = nbm;
+    // looked-up ids and pivot rows (4 B each), list records (16 B), items (32 B)
+    p->info.stream_bytes = 4 * (checks + pivot_words) + 16 * ne + 32 * (int64_t)items.size();
     p->info.work_items = (int64_t)items.size();
-    // one device allocation: items (16-byte aligned) first, then us, ue, ur
-    const size_t ib = items.size() * sizeof(Item), lb = (size_t)ne * 4;
-    std::vector<unsigned char> host(ib + 3 * lb);
+    // one device allocation: items, list records, row bitmaps (16-byte
+    // aligned), then list rows
+    const size_t ib = items.size() * sizeof(Item), lb = (size_t)ne * sizeof(uint4),
+                 bb = rowbits.size() * 4, rb = (size_t)ne * 4;
+    std::vector<unsigned char> host(ib + lb + bb + rb);
     std::memcpy(host.data(), items.data(), ib);
-    std::memcpy(host.data() + ib, us.data(), lb);
-    std::memcpy(host.data() + ib + lb, ue.data(), lb);
-    std::memcpy(host.data() + ib + 2 * lb, ur.data(), lb);
+    std::memcpy(host.data() + ib, lst.data(), lb);
+    std::memcpy(host.data() + ib + lb, rowbits.data(), bb);
+    std::memcpy(host.data() + ib + lb + bb, ur.data(), rb);
     p->lists.alloc(host.size());
     if (!host.empty())
         BTC_HIP(hipMemcpyAsync(p->lists.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
     p->items_p = reinterpret_cast<const uint4*>(p->lists.p);
-    p->ustart_p = reinterpret_cast<const uint32_t*>(p->lists.p + ib);
-    p->uend_p = reinterpret_cast<const uint32_t*>(p->lists.p + ib + lb);
-    p->urow_p = reinterpret_cast<const int32_t*>(p->lists.p + ib + 2 * lb);
+    p->list_p = reinterpret_cast<const uint4*>(p->lists.p + ib);
+    p->rowbits_p = reinterpret_cast<const uint4*>(p->lists.p + ib + lb);
+    p->urow_p = reinterpret_cast<const int32_t*>(p->lists.p + ib + lb + bb);
     BTC_HIP(hipStreamSynchronize(s));
 }
 
@@ -611,6 +673,8 @@ void check_launch_opts(const btc_count_opts& o) {
                 "unroll %d (expected 0, 1, 2 or 4)", o.unroll);
     BTC_REQUIRE(o.schedule == 0 || o.schedule == 1, BTC_ERR_ARG, "schedule %d (expected 0 or 1)", o.schedule);
     BTC_REQUIRE(o.waves_per_eu >= 0, BTC_ERR_ARG, "waves_per_eu %d < 0", o.waves_per_eu);
+    BTC_REQUIRE(o.dense_kernel >= 0 && o.dense_kernel <= 2, BTC_ERR_ARG, "dense_kernel %d (expected 0, 1 or 2)",
+                o.dense_kernel);
 }
 
 namespace {
@@ -628,7 +692,7 @@ WedgeFn wedge_fn(const btc_count_opts& o, bool pv, unsigned* grid) {
 }
 
 PlanDev plan_dev(btc_plan_s* p, unsigned long long* d_total, int32_t* d_pv) {
-    return PlanDev{p->g->d_col.p, p->ustart_p, p->uend_p, p->urow_p, p->items_p, p->n_block_items, p->nitems,
+    return PlanDev{p->g->d_col.p, p->list_p, p->rowbits_p, p->urow_p, p->items_p, p->n_block_items, p->nitems,
                    d_total, d_pv};
 }
 
@@ -658,7 +722,8 @@ void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, M
     }
     for (int j = 0; j < k; ++j)
         if (P[j]->n_dense)
-            launch_tile_count(P[j]->bsr.get(), P[j]->dense_list.p, P[j]->n_dense, false, -1, ws.totals.p + j,
+            launch_tile_count(P[j]->bsr.get(), P[j]->dense_list.p, P[j]->n_dense, false,
+                              P[j]->launch.dense_kernel == 2, -1, ws.totals.p + j,
                               nullptr, s);
     if (ni) {
         unsigned grid = 0;
@@ -698,7 +763,8 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
     unsigned long long* d_total = p->scratch.p;
     uint32_t* d_counters = reinterpret_cast<uint32_t*>(p->scratch.p + 1);
     if (tm) tm->start(s);
-    if (p->n_dense) launch_tile_count(p->bsr.get(), p->dense_list.p, p->n_dense, false, -1, d_total, d_pv, s);
+    if (p->n_dense) launch_tile_count(p->bsr.get(), p->dense_list.p, p->n_dense, false, p->launch.dense_kernel == 2, -1,
+                                        d_total, d_pv, s);
     if (tw) tw->start(s);
     if (p->nitems) launch_wedge(p, d_total, d_counters, d_pv, s);
     if (tm) tm->stop(s);

@@ -28,10 +28,10 @@ struct btc_plan_s {
     std::unique_ptr<btc_bitbsr_s> bsr;
     btcx::DevBuf<uint32_t> dense_list;
     uint32_t n_dense = 0;
-    btcx::DevBuf<unsigned char> lists;  // items (2 x uint4 each), then ustart, uend, urow
+    btcx::DevBuf<unsigned char> lists;  // items, list records, row core bitmaps, list rows
     const uint4* items_p = nullptr;
-    const uint32_t* ustart_p = nullptr;
-    const uint32_t* uend_p = nullptr;
+    const uint4* list_p = nullptr;
+    const uint4* rowbits_p = nullptr;
     const int32_t* urow_p = nullptr;
     uint32_t nitems = 0;
     uint32_t n_block_items = 0;
@@ -58,10 +58,11 @@ void scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, hipStream_t s)
 void bitbsr_build(btc_graph_s* g, int K, hipStream_t s, btc_bitbsr_s* b);
 
 // Count over mask tiles of b.  tile_list (device) selects ntiles mask tiles,
-// or all tiles when NULL.  tau < 0 sends every tile down the dense path.
+// or all tiles when NULL.  tau < 0 sends every tile down the dense path;
+// mfma selects the i8 MFMA form of the dense path over AND + popcount.
 // Adds the total to *d_result and per-row sums to d_pv (when not NULL).
 void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles,
-                       bool binsearch, int tau, unsigned long long* d_result, int32_t* d_pv,
+                       bool binsearch, bool mfma, int tau, unsigned long long* d_result, int32_t* d_pv,
                        hipStream_t s);
 
 // Per-tile popcounts (device, nblk entries).

@@ -153,6 +153,9 @@ void btc_bitbsr_free(btc_bitbsr b);
 #define BTC_VARIANT_AUTO 0
 #define BTC_VARIANT_BINSEARCH 1
 #define BTC_VARIANT_DIRECT 2
+/* OR'ed into variant: dense tiles take the i8 MFMA form (0/1-unpacked tile
+ * rows, v_mfma_i32_16x16x64_i8, exact in int32) instead of AND + popcount. */
+#define BTC_DENSE_MFMA 16
 int btc_bitbsr_count(btc_bitbsr b, int32_t variant, int32_t threshold_blocks, int32_t tau,
                      void* stream, uint64_t* total, int32_t* per_vertex, float* kernel_ms);
 
@@ -178,6 +181,8 @@ typedef struct {
     int32_t group_lanes;  /* lanes streaming one prefix list: 8, 16 or 32; 0 = auto */
     int32_t unroll;       /* 16-byte column loads in flight per lane: 1, 2 or 4; 0 = auto */
     int32_t schedule;     /* 0 = work stealing on one counter, 1 = static stride over sorted items */
+    int32_t dense_kernel; /* dense tiles: 0 = auto (AND + popcount), 1 = 64-bit AND + v_bcnt popcount,
+                             2 = i8 MFMA (v_mfma_i32_16x16x64_i8) on 0/1-unpacked tile rows */
 } btc_count_opts;
 
 typedef struct btc_plan_s* btc_plan;
@@ -191,6 +196,8 @@ typedef struct {
     int64_t device_bytes;      /* HBM held by the plan */
     int64_t sparse_merge_work; /* sum over sparse edges (i,j) of d_L(i) + d_L(j) */
     int64_t stream_bytes;      /* bytes the wedge kernel reads: prefixes, pivot rows, lists, items */
+    int64_t core_lists;        /* lists whose hub-core ids are counted by bitmap AND + popcount */
+    int64_t core_rows;         /* rows holding a hub-core bitmap */
     int32_t tile_k;
     int32_t tau;
 } btc_plan_info;
@@ -198,7 +205,7 @@ typedef struct {
 void btc_count_opts_default(btc_count_opts* opts);
 int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_plan* out);
 int btc_plan_get_info(btc_plan p, btc_plan_info* info);
-/* Changes the launch-only options of a plan (waves_per_eu, lookup,
+/* Changes the launch-only options of a plan (waves_per_eu, lookup, dense_kernel,
  * group_lanes, unroll, schedule); tile_k, tau and chunk_work need a new plan. */
 int btc_plan_tune(btc_plan p, const btc_count_opts* opts);
 int btc_plan_count(btc_plan p, void* stream, uint64_t* total, int32_t* per_vertex, float* kernel_ms);

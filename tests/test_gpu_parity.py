@@ -86,9 +86,10 @@ def test_bitbsr_count_matches_oracle(gpu, orc, K, name):
     b = gpu.BitBSR(g, K)
     for variant in (gpu.VARIANT_BINSEARCH, gpu.VARIANT_DIRECT, gpu.VARIANT_AUTO):
         for tau in (-1, 0, 16 * K):  # default, all tiles dense, all sparse
-            t, pv, _ = b.count(variant=variant, tau=tau, per_vertex=True)
-            assert t == want, (variant, tau)
-            np.testing.assert_array_equal(pv, want_pv)
+            for mfma in (False, True):  # dense tiles by AND + popcount / by i8 MFMA
+                t, pv, _ = b.count(variant=variant, tau=tau, per_vertex=True, mfma=mfma)
+                assert t == want, (variant, tau, mfma)
+                np.testing.assert_array_equal(pv, want_pv)
     t, pv, ms = b.count()
     assert t == want and pv is None and ms > 0
 
@@ -105,6 +106,10 @@ def test_engine_matches_oracle(gpu, orc, name, K, tau):
     np.testing.assert_array_equal(pv, want_pv)
     t2, pv2, _ = p.count()  # a plan is reusable; totals-only launch
     assert t2 == want and pv2 is None
+    p.tune(dense_kernel=2)  # dense tiles by i8 MFMA
+    t3, pv3, _ = p.count(per_vertex=True)
+    assert t3 == want
+    np.testing.assert_array_equal(pv3, want_pv)
     info = p.info
     assert info["sparse_edges"] + info["dense_edges"] == g.m
 
@@ -146,6 +151,24 @@ def test_plan_tune_rejects_bad_options(gpu):
         with pytest.raises(gpu.BtcError):
             p.tune(**bad)
     assert p.count()[0] == 1
+
+
+@pytest.mark.parametrize("name", ["rmat15", "planted", "complete", "messy"])
+def test_engine_core_bitmap_path(gpu, orc, name):
+    """Lists whose rows hold many hub-core ids (< 4096) are counted by a
+    bitmap AND + popcount for the core part; the counts match the oracle and
+    the path is really taken."""
+    s, d, n = case(name, gpu)
+    g = gpu.Graph.from_coo(s, d, n)
+    want, want_pv = oracle_counts(orc, g)
+    p = gpu.Plan(g, tau=100000)
+    assert p.info["core_lists"] > 0 and p.info["core_rows"] > 0
+    t, pv, _ = p.count(per_vertex=True)
+    assert t == want
+    np.testing.assert_array_equal(pv, want_pv)
+    for G in (8, 16, 32):
+        p.tune(group_lanes=G, unroll=2)
+        assert p.count()[0] == want
 
 
 @pytest.mark.parametrize("flags", [0, 1, 2, 3])
