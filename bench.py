@@ -323,6 +323,7 @@ def main():
     ap.add_argument("--batch-sample", type=int, default=256, help="graphs timed one launch each")
     ap.add_argument("--serve", type=int, default=600, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--dense", type=int, default=5, help="timed runs per case in the dense-tile leg (0 = skip)")
+    ap.add_argument("--host-path", type=int, default=1, help="also time host canonicalisation and planning")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one D2D copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -344,18 +345,41 @@ def main():
     t0 = time.perf_counter()
     src, dst, n_in, config = make_workload(btc, args.workload)
     log("generated %d pairs in %.2f s" % (src.size, time.perf_counter() - t0))
+    # end to end from host pairs: GPU canonicalisation, GPU planner, first count
     t0 = time.perf_counter()
-    g = btc.Graph.from_coo(src, dst, n_in)
+    g = btc.Graph.from_coo(src, dst, n_in, btc.CANON_DEFAULT | btc.CANON_DEVICE)
     canon_ms = (time.perf_counter() - t0) * 1e3
-    del src, dst
     gi = g.info
-    log("canonical L: n=%d m=%d max_row=%d wedges=%d (%.0f ms)" % (gi["n"], gi["m"], gi["max_row"], gi["wedges"],
-                                                                   canon_ms))
+    log("canonical L: n=%d m=%d max_row=%d wedges=%d (%.0f ms on the GPU)"
+        % (gi["n"], gi["m"], gi["max_row"], gi["wedges"], canon_ms))
     t0 = time.perf_counter()
     plan = btc.Plan(g, tile_k=args.tile_k, tau=args.tau, chunk_work=args.chunk)
     build_ms = (time.perf_counter() - t0) * 1e3
     pi = plan.info
     log("plan: %s (%.0f ms)" % (pi, build_ms))
+    t0 = time.perf_counter()
+    plan.count()
+    first_ms = (time.perf_counter() - t0) * 1e3
+    e2e = {"canon_ms": round(canon_ms, 1), "plan_ms": round(build_ms, 1), "first_count_ms": round(first_ms, 2),
+           "e2e_ms": round(canon_ms + build_ms + first_ms, 1),
+           "e2e_edges_per_s": gi["m"] / ((canon_ms + build_ms + first_ms) * 1e-3)}
+    if args.host_path:
+        # the same steps on host threads, for comparison
+        t0 = time.perf_counter()
+        gh = btc.Graph.from_coo(src, dst, n_in)
+        e2e["host_canon_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        t0 = time.perf_counter()
+        ph = btc.Plan(gh, tile_k=args.tile_k, tau=args.tau, chunk_work=args.chunk, planner=1)
+        e2e["host_plan_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        hi = ph.info
+        e2e["host_plan_same_stats"] = all(hi[k] == pi[k] for k in ("sparse_edges", "dense_edges", "dense_tiles",
+                                                                   "wedge_checks", "core_lists", "core_rows"))
+        e2e["host_count_equal"] = ph.count()[0] == plan.count()[0]
+        ph.close()
+        gh.close()
+        del ph, gh
+    log("end to end: %s" % e2e)
+    del src, dst
 
     total0 = None
     for _ in range(args.warmup):
@@ -426,7 +450,7 @@ def main():
         "vs_baseline": None, "dtype": "u32", "data": "synthetic", "config": config,
         "triangles": int(total0), "triangles_per_s": total0 / (wall / args.steps),
         "kernel_ms_median": statistics.median(kms), "canon_ms": round(canon_ms, 1), "plan_ms": round(build_ms, 1),
-        "roofline": roofline, "cpu_baseline": cpu, "plain_hybrid": plain,
+        "roofline": roofline, "cpu_baseline": cpu, "plain_hybrid": plain, "end_to_end": e2e,
         "batched": batched, "serving": serving, "dense_tiles": dense,
     }
     print(json.dumps(out), flush=True)

@@ -20,13 +20,14 @@ __all__ = [
     "batch_workload", "Server", "autotune", "tune_lookup", "tune_table_save", "tune_table_load",
     "tune_table_clear",
     "gen_rmat", "gen_er", "gen_stencil2d", "gen_stencil3d", "gen_planted", "lite_tile_k",
-    "default_tau", "version", "CANON_COMPACT", "CANON_DEGREE_ORDER", "CANON_DEFAULT",
+    "default_tau", "version", "CANON_COMPACT", "CANON_DEGREE_ORDER", "CANON_DEFAULT", "CANON_DEVICE",
     "VARIANT_AUTO", "VARIANT_BINSEARCH", "VARIANT_DIRECT",
 ]
 
 CANON_COMPACT = 1
 CANON_DEGREE_ORDER = 2
 CANON_DEFAULT = 3
+CANON_DEVICE = 4
 VARIANT_AUTO = 0
 VARIANT_BINSEARCH = 1
 VARIANT_DIRECT = 2
@@ -63,7 +64,8 @@ class BitBSRView(C.Structure):
 class CountOpts(C.Structure):
     _fields_ = [("tile_k", C.c_int32), ("tau", C.c_int32), ("chunk_work", C.c_int32),
                 ("waves_per_eu", C.c_int32), ("lookup", C.c_int32), ("group_lanes", C.c_int32),
-                ("unroll", C.c_int32), ("schedule", C.c_int32), ("dense_kernel", C.c_int32)]
+                ("unroll", C.c_int32), ("schedule", C.c_int32), ("dense_kernel", C.c_int32),
+                ("planner", C.c_int32)]
 
 
 class PlanInfo(C.Structure):
@@ -110,6 +112,7 @@ _SIGS = {
     "btc_gen_stencil3d": (C.c_int64, [C.c_int32, C.c_int32, C.c_int32, vp, vp]),
     "btc_gen_planted": (C.c_int64, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_uint64, vp, vp]),
     "btc_graph_from_coo": (C.c_int, [vp, vp, C.c_int64, C.c_int32, C.c_uint32, C.POINTER(vp)]),
+    "btc_graph_from_coo_device": (C.c_int, [vp, vp, C.c_int64, C.c_int32, C.c_uint32, vp, C.POINTER(vp)]),
     "btc_graph_from_csr": (C.c_int, [vp, vp, C.c_int32, C.POINTER(vp)]),
     "btc_graph_get_info": (C.c_int, [vp, C.POINTER(GraphInfo)]),
     "btc_graph_copy_csr": (C.c_int, [vp, vp, vp]),
@@ -235,6 +238,14 @@ class Graph:
         return cls(h.value)
 
     @classmethod
+    def from_coo_device(cls, src_ptr, dst_ptr, nnz, n=0, flags=CANON_DEFAULT, stream=None):
+        """From int32 pairs already in HBM (device addresses, e.g. a torch
+        tensor's data_ptr()), canonicalised on the GPU."""
+        h = vp()
+        _check(lib().btc_graph_from_coo_device(vp(src_ptr), vp(dst_ptr), nnz, n, flags, stream, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
     def from_csr(cls, rowptr, col):
         rowptr = np.ascontiguousarray(rowptr, np.int64)
         col = np.ascontiguousarray(col, np.int32)
@@ -341,13 +352,13 @@ class BitBSR:
 
 
 def make_opts(tile_k=None, tau=None, chunk_work=None, waves_per_eu=None, lookup=None, group_lanes=None,
-              unroll=None, schedule=None, dense_kernel=None):
+              unroll=None, schedule=None, dense_kernel=None, planner=None):
     """btc_count_opts with the library defaults, overridden where given."""
     o = CountOpts()
     lib().btc_count_opts_default(C.byref(o))
     for k, v in dict(tile_k=tile_k, tau=tau, chunk_work=chunk_work, waves_per_eu=waves_per_eu, lookup=lookup,
                      group_lanes=group_lanes, unroll=unroll, schedule=schedule,
-                     dense_kernel=dense_kernel).items():
+                     dense_kernel=dense_kernel, planner=planner).items():
         if v is not None:
             setattr(o, k, v)
     return o

@@ -66,6 +66,7 @@ int btc_batch_create(const btc_graph* graphs, int32_t count, const btc_count_opt
         int64_t m = 0;
         for (int32_t k = 0; k < count; ++k) {
             BTC_REQUIRE(graphs[k], BTC_ERR_ARG, "graph %d is NULL", k);
+            btcx::ensure_host(graphs[k]);
             const int64_t n = graphs[k]->n;
             b->voff[k + 1] = b->voff[k] + (n + btcx::kGraphAlign - 1) / btcx::kGraphAlign * btcx::kGraphAlign;
             m += graphs[k]->m;

@@ -101,7 +101,21 @@ int btc_graph_from_coo(const int32_t* src, const int32_t* dst, int64_t nnz, int3
         *out = nullptr;
         BTC_REQUIRE(nnz >= 0, BTC_ERR_ARG, "nnz %lld < 0", (long long)nnz);
         BTC_REQUIRE(nnz == 0 || (src && dst), BTC_ERR_ARG, "src/dst is NULL");
-        BTC_REQUIRE((flags & ~BTC_CANON_DEFAULT) == 0, BTC_ERR_ARG, "unknown flags 0x%x", flags);
+        BTC_REQUIRE((flags & ~(BTC_CANON_DEFAULT | BTC_CANON_DEVICE)) == 0, BTC_ERR_ARG, "unknown flags 0x%x", flags);
+        if (flags & BTC_CANON_DEVICE) {
+            // stage the pairs in HBM and canonicalise there
+            btcx::DevBuf<int32_t> ds, dd;
+            ds.alloc((size_t)nnz);
+            dd.alloc((size_t)nnz);
+            if (nnz) {
+                BTC_HIP(hipMemcpy(ds.p, src, (size_t)nnz * 4, hipMemcpyHostToDevice));
+                BTC_HIP(hipMemcpy(dd.p, dst, (size_t)nnz * 4, hipMemcpyHostToDevice));
+            }
+            auto g = std::make_unique<btc_graph_s>();
+            btcx::canonicalise_device(ds.p, dd.p, nnz, n, flags, nullptr, g.get());
+            *out = g.release();
+            return;
+        }
         int32_t lo_id = 0, hi_id = -1;
 #pragma omp parallel for reduction(min : lo_id) reduction(max : hi_id) schedule(static)
         for (int64_t e = 0; e < nnz; ++e) {
@@ -256,6 +270,7 @@ int btc_graph_get_info(btc_graph g, btc_graph_info* info) {
 int btc_graph_copy_csr(btc_graph g, int64_t* rowptr, int32_t* col) {
     return btcx::guard([&] {
         BTC_REQUIRE(g && rowptr && (col || g->m == 0), BTC_ERR_ARG, "NULL argument");
+        btcx::ensure_host(g);
         std::copy(g->rowptr.begin(), g->rowptr.end(), rowptr);
         std::copy(g->col.begin(), g->col.end(), col);
     });
@@ -264,6 +279,7 @@ int btc_graph_copy_csr(btc_graph g, int64_t* rowptr, int32_t* col) {
 int btc_graph_copy_labels(btc_graph g, int32_t* canon_to_input) {
     return btcx::guard([&] {
         BTC_REQUIRE(g && (canon_to_input || g->n == 0), BTC_ERR_ARG, "NULL argument");
+        btcx::ensure_host(g);
         std::copy(g->labels.begin(), g->labels.end(), canon_to_input);
     });
 }
@@ -275,6 +291,7 @@ void btc_graph_free(btc_graph g) { delete g; }
 int btc_count_cpu(btc_graph g, int32_t threads, uint64_t* total, int32_t* per_vertex) {
     return btcx::guard([&] {
         BTC_REQUIRE(g && total, BTC_ERR_ARG, "NULL argument");
+        btcx::ensure_host(g);
         const int32_t n = g->n;
         const int64_t* rp = g->rowptr.data();
         const int32_t* c = g->col.data();

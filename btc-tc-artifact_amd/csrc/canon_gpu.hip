@@ -1,0 +1,347 @@
+// Device canonicalisation: COO pairs in HBM -> strictly lower triangle L
+// (CSR) in HBM, equal bit for bit to the host path (canon.cpp).
+//
+// Same steps as btc/operations/convert.h: orient (u, v) to (max, min) and
+// drop self-loops (symmetrize_and_triangular_gpu, :281-324), sort and
+// deduplicate, compact the used ids in order (remap_node_ids_gpu,
+// :387-425); then the optional relabel by descending degree (ties by id).
+// A pair is one key row << b | col (b = bits of n), radix-sorted on 2b
+// bits; duplicates are dropped by a flag scan; CSR rows are found by a
+// binary search of each row's first key.  The host copies of the CSR are
+// made only when a host consumer asks (ensure_host).
+#include <climits>
+
+#include "device_util.h"
+#include "internal.h"
+#include "prim.h"
+
+namespace btcx {
+namespace {
+
+constexpr int kT = 256;
+
+int id_bits(int64_t n) {
+    int b = 1;
+    while (b < 31 && (1ll << b) < n) ++b;
+    return b;
+}
+
+unsigned grid_for(int64_t n) {
+    const int64_t g = (n + kT - 1) / kT;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+}
+
+__global__ void id_range_kernel(const int32_t* __restrict__ src, const int32_t* __restrict__ dst, int64_t nnz,
+                                int32_t* __restrict__ lohi) {
+    int32_t lo = INT_MAX, hi = -1;
+    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * kT) {
+        const int32_t u = src[e], v = dst[e];
+        lo = min(lo, min(u, v));
+        hi = max(hi, max(u, v));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&lohi[0], lo);
+        atomicMax(&lohi[1], hi);
+    }
+}
+
+// (u, v) -> (max << b) | min; self-loops become the sentinel (all 2b bits
+// set, which no oriented pair reaches since max > min).
+__global__ void orient_kernel(const int32_t* __restrict__ src, const int32_t* __restrict__ dst, int64_t nnz, int b,
+                              uint64_t sentinel, uint64_t* __restrict__ keys) {
+    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * kT) {
+        const uint32_t u = (uint32_t)src[e], v = (uint32_t)dst[e];
+        keys[e] = u == v ? sentinel : ((uint64_t)max(u, v) << b) | (uint64_t)min(u, v);
+    }
+}
+
+// flag[e] = key e is the first of its run and not the sentinel
+__global__ void first_flag_kernel(const uint64_t* __restrict__ keys, int64_t n, uint64_t sentinel,
+                                  uint32_t* __restrict__ flag) {
+    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < n; e += (int64_t)gridDim.x * kT) {
+        const uint64_t k = keys[e];
+        flag[e] = (k != sentinel && (e == 0 || keys[e - 1] != k)) ? 1u : 0u;
+    }
+}
+
+__global__ void compact_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flag,
+                               const uint32_t* __restrict__ pos, int64_t n, uint64_t* __restrict__ out) {
+    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < n; e += (int64_t)gridDim.x * kT)
+        if (flag[e]) out[pos[e]] = keys[e];
+}
+
+__global__ void mark_used_kernel(const uint64_t* __restrict__ keys, int64_t m, int b, uint32_t* __restrict__ used) {
+    const uint64_t mask = (1ull << b) - 1;
+    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT) {
+        const uint64_t k = keys[e];
+        used[k >> b] = 1u;
+        used[k & mask] = 1u;
+    }
+}
+
+// labels[newid[v]] = v for every used v (newid = exclusive scan of used)
+__global__ void labels_kernel(const uint32_t* __restrict__ used, const uint32_t* __restrict__ newid, int32_t n,
+                              int32_t* __restrict__ labels) {
+    for (int64_t v = blockIdx.x * (int64_t)kT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kT)
+        if (used[v]) labels[newid[v]] = (int32_t)v;
+}
+
+__global__ void iota_kernel(int32_t* __restrict__ x, int64_t n) {
+    for (int64_t v = blockIdx.x * (int64_t)kT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kT) x[v] = (int32_t)v;
+}
+
+// Rewrites keys (row << b | col) through map (NULL = identity) into
+// max(map row, map col) << bn | min(...).
+__global__ void relabel_kernel(uint64_t* __restrict__ keys, int64_t m, int b, const uint32_t* __restrict__ map,
+                               int bn) {
+    const uint64_t mask = (1ull << b) - 1;
+    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT) {
+        const uint64_t k = keys[e];
+        uint32_t r = (uint32_t)(k >> b), c = (uint32_t)(k & mask);
+        if (map) {
+            r = map[r];
+            c = map[c];
+        }
+        keys[e] = ((uint64_t)max(r, c) << bn) | (uint64_t)min(r, c);
+    }
+}
+
+__global__ void degree_kernel(const uint64_t* __restrict__ keys, int64_t m, int b, uint32_t* __restrict__ deg) {
+    const uint64_t mask = (1ull << b) - 1;
+    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT) {
+        const uint64_t k = keys[e];
+        atomicAdd(&deg[k >> b], 1u);
+        atomicAdd(&deg[k & mask], 1u);
+    }
+}
+
+// key = ~degree (ascending = descending degree), value = vertex
+__global__ void rank_keys_kernel(const uint32_t* __restrict__ deg, int32_t n, uint32_t* __restrict__ key,
+                                 uint32_t* __restrict__ val) {
+    for (int64_t v = blockIdx.x * (int64_t)kT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kT) {
+        key[v] = ~deg[v];
+        val[v] = (uint32_t)v;
+    }
+}
+
+// rank[order[r]] = r; labels_out[r] = labels_in[order[r]]
+__global__ void rank_scatter_kernel(const uint32_t* __restrict__ order, int32_t n, uint32_t* __restrict__ rank,
+                                    const int32_t* __restrict__ labels_in, int32_t* __restrict__ labels_out) {
+    for (int64_t r = blockIdx.x * (int64_t)kT + threadIdx.x; r < n; r += (int64_t)gridDim.x * kT) {
+        const uint32_t v = order[r];
+        rank[v] = (uint32_t)r;
+        labels_out[r] = labels_in[v];
+    }
+}
+
+// rowptr[r] = first e with keys[e] >= r << b (r = 0..n)
+__global__ void rowptr_kernel(const uint64_t* __restrict__ keys, int64_t m, int b, int32_t n,
+                              uint32_t* __restrict__ rowptr) {
+    for (int64_t r = blockIdx.x * (int64_t)kT + threadIdx.x; r <= n; r += (int64_t)gridDim.x * kT) {
+        const uint64_t t = (uint64_t)r << b;
+        int64_t lo = 0, hi = m;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < t) lo = mid + 1;
+            else hi = mid;
+        }
+        rowptr[r] = (uint32_t)lo;
+    }
+}
+
+__global__ void col_kernel(const uint64_t* __restrict__ keys, int64_t m, int b, int32_t* __restrict__ col) {
+    const uint64_t mask = (1ull << b) - 1;
+    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT)
+        col[e] = (int32_t)(keys[e] & mask);
+}
+
+// out: [0] longest row, [1] sum C(d, 2), [2] sum over edges of d(i) + d(j)
+__global__ void row_stats_kernel(const uint32_t* __restrict__ rowptr, int32_t n,
+                                 unsigned long long* __restrict__ out) {
+    uint32_t mx = 0;
+    uint64_t w = 0, work = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+        const uint64_t d = rowptr[i + 1] - rowptr[i];
+        mx = max(mx, (uint32_t)d);
+        w += d * (d - (d > 0)) / 2;
+        work += d * d;
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+    w = wave_sum_u64(w);
+    work = wave_sum_u64(work);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], (unsigned long long)mx);
+        atomicAdd(&out[1], (unsigned long long)w);
+        atomicAdd(&out[2], (unsigned long long)work);
+    }
+}
+
+__global__ void edge_stats_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int64_t m,
+                                  unsigned long long* __restrict__ out) {
+    uint64_t work = 0;
+    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT) {
+        const int32_t j = col[e];
+        work += rowptr[j + 1] - rowptr[j];
+    }
+    work = wave_sum_u64(work);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&out[2], (unsigned long long)work);
+}
+
+#define BTC_LAUNCH(kernel, n, s, ...)                                                   \
+    do {                                                                                \
+        hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(kT), 0, s, __VA_ARGS__);     \
+        BTC_CHECK_LAUNCH(#kernel);                                                      \
+    } while (0)
+
+}  // namespace
+
+void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n, uint32_t flags,
+                         hipStream_t s, btc_graph_s* g) {
+    BTC_REQUIRE(nnz < (int64_t)0xFFFFFFFFLL, BTC_ERR_OVERFLOW, "nnz %lld exceeds 2^32 - 1 pairs", (long long)nnz);
+    // id range (and n when not given)
+    int32_t h_lohi[2] = {INT_MAX, -1};
+    {
+        DevBuf<int32_t> lohi;
+        lohi.alloc(2);
+        BTC_HIP(hipMemcpyAsync(lohi.p, h_lohi, 8, hipMemcpyHostToDevice, s));
+        if (nnz) BTC_LAUNCH(id_range_kernel, nnz, s, src, dst, nnz, lohi.p);
+        BTC_HIP(hipMemcpyAsync(h_lohi, lohi.p, 8, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipStreamSynchronize(s));
+    }
+    if (nnz == 0) h_lohi[0] = 0;
+    BTC_REQUIRE(h_lohi[0] >= 0, BTC_ERR_ARG, "negative vertex id %d", h_lohi[0]);
+    if (n <= 0) n = h_lohi[1] + 1;
+    BTC_REQUIRE(h_lohi[1] < n, BTC_ERR_ARG, "vertex id %d >= n = %d", h_lohi[1], n);
+    g->n_input = n;
+    g->flags = flags;
+    const int b = id_bits(n);
+    const uint64_t sentinel = (1ull << (2 * b)) - 1;
+
+    // oriented keys, sorted, deduplicated
+    DevBuf<uint64_t> ka, kb;
+    ka.alloc((size_t)nnz);
+    kb.alloc((size_t)nnz);
+    if (nnz) BTC_LAUNCH(orient_kernel, nnz, s, src, dst, nnz, b, sentinel, ka.p);
+    uint64_t* sorted = sort_u64(ka.p, kb.p, (size_t)nnz, 2 * b, s);
+    uint64_t* other = sorted == ka.p ? kb.p : ka.p;
+    int64_t m = 0;
+    {
+        DevBuf<uint32_t> flag, pos;
+        flag.alloc((size_t)nnz);
+        pos.alloc((size_t)nnz + 1);
+        if (nnz) BTC_LAUNCH(first_flag_kernel, nnz, s, sorted, nnz, sentinel, flag.p);
+        exclusive_scan_u32(flag.p, pos.p, (size_t)nnz, s);
+        uint32_t hm = 0;
+        BTC_HIP(hipMemcpyAsync(&hm, pos.p + nnz, 4, hipMemcpyDeviceToHost, s));
+        if (nnz) BTC_LAUNCH(compact_kernel, nnz, s, sorted, flag.p, pos.p, nnz, other);
+        BTC_HIP(hipStreamSynchronize(s));
+        m = hm;
+    }
+    uint64_t* keys = other;      // m unique oriented pairs, ascending
+    uint64_t* spare = sorted;
+
+    // order-preserving compaction of the used ids (none when no edge)
+    int32_t nc = n;
+    DevBuf<uint32_t> newid;
+    DevBuf<int32_t> labels;
+    if ((flags & BTC_CANON_COMPACT) && m > 0) {
+        DevBuf<uint32_t> used;
+        used.alloc((size_t)n);
+        newid.alloc((size_t)n + 1);
+        BTC_HIP(hipMemsetAsync(used.p, 0, (size_t)n * 4, s));
+        BTC_LAUNCH(mark_used_kernel, m, s, keys, m, b, used.p);
+        exclusive_scan_u32(used.p, newid.p, (size_t)n, s);
+        uint32_t hn = 0;
+        BTC_HIP(hipMemcpyAsync(&hn, newid.p + n, 4, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipStreamSynchronize(s));
+        nc = (int32_t)hn;
+        labels.alloc((size_t)nc);
+        BTC_LAUNCH(labels_kernel, n, s, used.p, newid.p, n, labels.p);
+    } else {
+        labels.alloc((size_t)n);
+        BTC_LAUNCH(iota_kernel, n, s, labels.p, (int64_t)n);
+    }
+    const int bn = id_bits(nc);
+    if (m) BTC_LAUNCH(relabel_kernel, m, s, keys, m, b, newid.p, bn);  // monotone: order kept
+
+    if (flags & BTC_CANON_DEGREE_ORDER) {
+        DevBuf<uint32_t> deg, k0, v0, k1, v1, rank;
+        DevBuf<int32_t> lab2;
+        deg.alloc((size_t)nc);
+        k0.alloc((size_t)nc);
+        v0.alloc((size_t)nc);
+        k1.alloc((size_t)nc);
+        v1.alloc((size_t)nc);
+        rank.alloc((size_t)nc);
+        lab2.alloc((size_t)nc);
+        BTC_HIP(hipMemsetAsync(deg.p, 0, (size_t)nc * 4, s));
+        if (m) BTC_LAUNCH(degree_kernel, m, s, keys, m, bn, deg.p);
+        BTC_LAUNCH(rank_keys_kernel, nc, s, deg.p, nc, k0.p, v0.p);
+        const bool alt = sort_pairs_u32(k0.p, v0.p, k1.p, v1.p, (size_t)nc, 32, s);
+        BTC_LAUNCH(rank_scatter_kernel, nc, s, alt ? v1.p : v0.p, nc, rank.p, labels.p, lab2.p);
+        if (m) BTC_LAUNCH(relabel_kernel, m, s, keys, m, bn, rank.p, bn);
+        keys = sort_u64(keys, spare, (size_t)m, 2 * bn, s);
+        labels = std::move(lab2);
+    }
+
+    // CSR of L
+    g->n = nc;
+    g->m = m;
+    g->d_rowptr.alloc((size_t)nc + 1);
+    g->d_col.alloc((size_t)m + kColPad);
+    BTC_HIP(hipMemsetAsync(g->d_col.p + m, 0, kColPad * 4, s));
+    BTC_LAUNCH(rowptr_kernel, (int64_t)nc + 1, s, keys, m, bn, nc, g->d_rowptr.p);
+    if (m) BTC_LAUNCH(col_kernel, m, s, keys, m, bn, g->d_col.p);
+    g->d_labels = std::move(labels);
+
+    // derived statistics (graph_finalize on the device)
+    DevBuf<unsigned long long> st;
+    st.alloc(3);
+    BTC_HIP(hipMemsetAsync(st.p, 0, 24, s));
+    if (nc) BTC_LAUNCH(row_stats_kernel, nc, s, g->d_rowptr.p, nc, st.p);
+    if (m) BTC_LAUNCH(edge_stats_kernel, m, s, g->d_rowptr.p, g->d_col.p, m, st.p);
+    unsigned long long h[3] = {0, 0, 0};
+    BTC_HIP(hipMemcpyAsync(h, st.p, 24, hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    g->max_row = (int32_t)h[0];
+    g->wedges = (int64_t)h[1];
+    g->merge_work = (int64_t)h[2];
+    g->on_host = false;
+}
+
+void ensure_host(btc_graph_s* g) {
+    if (g->on_host) return;
+    std::vector<uint32_t> rp((size_t)g->n + 1);
+    std::vector<int32_t> col((size_t)g->m), labels((size_t)g->n);
+    BTC_HIP(hipMemcpy(rp.data(), g->d_rowptr.p, rp.size() * 4, hipMemcpyDeviceToHost));
+    if (g->m) BTC_HIP(hipMemcpy(col.data(), g->d_col.p, (size_t)g->m * 4, hipMemcpyDeviceToHost));
+    if (g->n) BTC_HIP(hipMemcpy(labels.data(), g->d_labels.p, (size_t)g->n * 4, hipMemcpyDeviceToHost));
+    g->rowptr.assign(rp.begin(), rp.end());
+    g->col.swap(col);
+    g->labels.swap(labels);
+    g->on_host = true;
+}
+
+}  // namespace btcx
+
+extern "C" {
+
+int btc_graph_from_coo_device(const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n, uint32_t flags,
+                              void* stream, btc_graph* out) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(out, BTC_ERR_ARG, "out is NULL");
+        *out = nullptr;
+        BTC_REQUIRE(nnz >= 0, BTC_ERR_ARG, "nnz %lld < 0", (long long)nnz);
+        BTC_REQUIRE(nnz == 0 || (src && dst), BTC_ERR_ARG, "src/dst is NULL");
+        BTC_REQUIRE((flags & ~BTC_CANON_DEFAULT) == 0, BTC_ERR_ARG, "unknown flags 0x%x", flags);
+        auto g = std::make_unique<btc_graph_s>();
+        btcx::canonicalise_device(src, dst, nnz, n, flags, btcx::as_stream(stream), g.get());
+        *out = g.release();
+    });
+}
+
+}  // extern "C"

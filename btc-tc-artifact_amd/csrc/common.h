@@ -123,16 +123,24 @@ struct btc_graph_s {
     uint32_t flags = 0;
     int64_t wedges = 0;
     int64_t merge_work = 0;
+    bool on_host = true;           // rowptr / col / labels below are valid
     std::vector<int64_t> rowptr;   // n + 1
     std::vector<int32_t> col;      // m
     std::vector<int32_t> labels;   // canonical -> input id
     btcx::DevBuf<uint32_t> d_rowptr;  // n + 1 (m < 2^32 is enforced)
-    btcx::DevBuf<int32_t> d_col;      // m
+    btcx::DevBuf<int32_t> d_col;      // m + padding
+    btcx::DevBuf<int32_t> d_labels;   // n, for graphs canonicalised on the device
 };
 
 namespace btcx {
 // Fills the derived statistics (rows, wedges, merge work) of g.
 void graph_finalize(btc_graph_s* g);
+// Canonicalises device-resident COO pairs into g (CSR and labels in HBM,
+// statistics filled, host copies left empty until ensure_host).
+void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n, uint32_t flags,
+                         hipStream_t s, btc_graph_s* g);
+// Makes the host CSR and labels of g valid (copies them from HBM once).
+void ensure_host(btc_graph_s* g);
 // Fills the auto fields of o from the autotuner's table when an entry
 // matches g's features; returns whether one did.
 bool tune_table_apply(const btc_graph_s* g, btc_count_opts* o);

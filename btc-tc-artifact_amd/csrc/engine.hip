@@ -10,16 +10,21 @@
 //    neighbour of i below j: k is in N_L(j) exactly when (i, j, k) is a
 //    triangle.  Because N_L(i) is sorted, those k are the prefix of row i
 //    in front of j, clipped to [min N_L(j), max N_L(j)].  Edges are grouped
-//    by pivot j; a workgroup loads N_L(j) once into an LDS hash table and
-//    streams the prefixes of all rows i that hold j through it, 64
-//    consecutive column ids per wave load.  With descending-degree labels
-//    this checks sum_i C(d_L(i), 2) wedges at most, far below the
-//    sum of d_L(i) + d_L(j) a merge would read.
+//    by pivot j; a workgroup (or one wave, for pivots with few prefixes)
+//    loads N_L(j) once into LDS as a bitmap window (else a hash table, else
+//    it stays in HBM for binary search) and streams the prefixes of all
+//    rows i that hold j through it with 16-byte loads.  Hub ids (below
+//    kCoreIds) of rows with many of them are counted by ANDing the row's
+//    core bitmap with the window.  With descending-degree labels this checks
+//    sum_i C(d_L(i), 2) wedges at most, far below the sum of d_L(i) +
+//    d_L(j) a merge would read.
 //
 // Work items are (pivot, range of prefixes) chunks of bounded size, taken
-// by a persistent grid through one atomic counter (work stealing), heaviest
-// first.  Per-vertex sums of row i collect the prefix counts of row i with
-// int32 atomics.
+// by a persistent grid in a static stride over work-sorted items (or from
+// one atomic counter), heaviest first.  Per-vertex sums of row i collect
+// the prefix counts of row i with int32 atomics.  The host planner below
+// builds the items for graphs held on the host; plan_gpu.hip builds the
+// same plan on the GPU.
 #include <omp.h>
 
 #include <algorithm>
@@ -34,17 +39,7 @@ namespace btcx {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kLdsWords = 4096;   // 16 KiB of LDS per workgroup
-constexpr int kWaveWords = 1024;  // per-wave slice of it for wave items
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-// Wave items: pivots with at most this many prefix lists.
-constexpr uint32_t kWaveItemLists = 64;
-// Hub core: ids below kCoreIds.  Rows with at least kCoreRowMin core ids
-// keep a kCoreIds-bit bitmap of them (kCoreWords 32-bit words, 512 bytes).
-constexpr uint32_t kCoreIds = 4096;
-constexpr uint32_t kCoreWords = kCoreIds / 32;
-constexpr uint32_t kCoreRowMin = 32;
-constexpr uint32_t kNoBitmap = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t hslot(uint32_t key, int shift) { return (key * 2654435761u) >> shift; }
 
@@ -155,7 +150,8 @@ __device__ __forceinline__ void team_sync() {
 // group's first record, loaded by the caller.
 template <int NT, int G, int U, bool PV, class Set>
 __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a, uint32_t tid, uint32_t eb,
-                                               uint32_t ee, uint4 L, const uint32_t* window, uint32_t bw) {
+                                               uint32_t ee, uint4 L, const uint32_t* window, uint32_t bw,
+                                               uint32_t we4) {
     constexpr uint32_t NG = NT / G;
     constexpr uint32_t STEP = 4 * G * U;
     const uint32_t g = tid / G, gl = tid % G;
@@ -171,10 +167,11 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
         uint32_t s0 = L.x;
         const uint32_t s1 = L.z;
         if (Set::kBitmap && L.w != kNoBitmap) {
-            // core ids: row bitmap AND pivot window, 128 bits per lane step;
-            // window words below bw are empty (the pivot has no ids there)
+            // core ids: row bitmap AND pivot window, 128 bits per lane step,
+            // over the window's words [bw, 4 we4): the pivot has no ids
+            // below bw or above them
             const uint4* R = a.rowbits + (size_t)L.w * (kCoreWords / 4);
-            for (uint32_t w4 = (bw >> 2) + gl; w4 < kCoreWords / 4; w4 += G) {
+            for (uint32_t w4 = (bw >> 2) + gl; w4 < we4; w4 += G) {
                 const uint4 r = R[w4];
                 const uint4 pw = *reinterpret_cast<const uint4*>(window + 4 * w4 - bw);
                 hits += __popc(r.x & pw.x) + __popc(r.y & pw.y) + __popc(r.z & pw.z) + __popc(r.w & pw.w);
@@ -236,7 +233,8 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
             atomicOr(&lds[(k >> 5) - bw], 1u << (k & 31u));
         }
         team_sync<NT>();
-        acc = scan_lists<NT, G, U, PV>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, L, lds, bw);
+        const uint32_t we4 = min(kCoreWords / 4, (hi >> 7) + 1u);  // uint4 words up to the pivot's last id
+        acc = scan_lists<NT, G, U, PV>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, L, lds, bw, we4);
         team_sync<NT>();
         for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)a.col[p] >> 5) - bw] = 0;
     } else if (d <= (uint32_t)W / 2 && lookup != 2) {
@@ -252,11 +250,11 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
             while (atomicCAS(&lds[h], kEmpty, key) != kEmpty) h = (h + 1u) & mask;
         }
         team_sync<NT>();
-        acc = scan_lists<NT, G, U, PV>(HashSet{lds, shift, mask}, a, tid, eb, ee, L, lds, 0u);
+        acc = scan_lists<NT, G, U, PV>(HashSet{lds, shift, mask}, a, tid, eb, ee, L, lds, 0u, 0u);
         team_sync<NT>();
         for (uint32_t x = tid; x <= mask; x += NT) lds[x] = 0;
     } else {
-        acc = scan_lists<NT, G, U, PV>(SearchSet{a.col, t0, t1}, a, tid, eb, ee, L, lds, 0u);
+        acc = scan_lists<NT, G, U, PV>(SearchSet{a.col, t0, t1}, a, tid, eb, ee, L, lds, 0u, 0u);
     }
     return acc;
 }
@@ -578,14 +576,14 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
             if (dense[q]) continue;
             uint32_t s0, s1;
             if (!clip(i, q, s0, s1)) continue;
-            // the core AND costs (kCoreIds - base) / 32 words; per-id lookups
+            // the core AND costs core_words(pivot) words; per-id lookups
             // cost about four times a word each: take the bitmap when cheaper
             const int32_t j = cl[q];
             const uint32_t base = (uint32_t)cl[rp[j]] & ~127u;
             uint32_t sc = s0, rb = kNoBitmap;
             if (rbi[i] != kNoBitmap && base < kCoreIds) {
                 const uint32_t core_ids = std::min(s1, cend[i]) > s0 ? std::min(s1, cend[i]) - s0 : 0u;
-                const uint32_t words = (kCoreIds - base) / 32;
+                const uint32_t words = core_words((uint32_t)cl[rp[j]], (uint32_t)cl[rp[j + 1] - 1]);
                 if (core_ids * 4 > words + 64) {
                     sc = std::max(s0, cend[i]);
                     rb = rbi[i];
@@ -608,13 +606,11 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
         if (e == e1) continue;
         const uint32_t t0 = (uint32_t)rp[j], t1 = (uint32_t)rp[j + 1];
         const uint32_t lo = (uint32_t)cl[rp[j]], hi = (uint32_t)cl[rp[j + 1] - 1];
-        const bool wave_fit = (hi - lo < 32u * kWaveWords) || (t1 - t0 <= (uint32_t)kWaveWords / 2);
-        const bool small = e1 - e <= (int64_t)kWaveItemLists && wave_fit;
+        const bool small = wave_pivot((uint32_t)std::min<int64_t>(e1 - e, 0xFFFFFFFFLL), lo, hi, t1 - t0);
         // cost of a list: ids looked up, plus core words ANDed
         auto cost = [&](int64_t f) -> int64_t {
             const uint4& r = lst[f];
-            return (int64_t)(r.z > r.y ? r.z - r.y : 0) +
-                   (r.w != kNoBitmap ? (int64_t)(kCoreIds - (lo & ~127u)) / 32 : 0);
+            return (int64_t)(r.z > r.y ? r.z - r.y : 0) + (r.w != kNoBitmap ? (int64_t)core_words(lo, hi) : 0);
         };
         while (e < e1) {
             int64_t w = 0, f = e;
@@ -675,6 +671,7 @@ void check_launch_opts(const btc_count_opts& o) {
     BTC_REQUIRE(o.waves_per_eu >= 0, BTC_ERR_ARG, "waves_per_eu %d < 0", o.waves_per_eu);
     BTC_REQUIRE(o.dense_kernel >= 0 && o.dense_kernel <= 2, BTC_ERR_ARG, "dense_kernel %d (expected 0, 1 or 2)",
                 o.dense_kernel);
+    BTC_REQUIRE(o.planner >= 0 && o.planner <= 2, BTC_ERR_ARG, "planner %d (expected 0, 1 or 2)", o.planner);
 }
 
 namespace {
@@ -829,18 +826,25 @@ int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_p
         BTC_REQUIRE((uint64_t)g->m + btcx::kColPad < (1ull << 30), BTC_ERR_OVERFLOW,
                     "nnz(L) = %lld: the wedge kernel addresses L with 32-bit byte offsets (limit 2^30 - %zu)",
                     (long long)g->m, btcx::kColPad);
-        std::vector<uint8_t> dense;
-        btcx::classify_dense(p.get(), s, dense);
         int64_t chunk = o.chunk_work > 0 ? o.chunk_work : 0;
         if (!chunk) {
             const int64_t cus = btcx::device_cu_count();
             chunk = std::max<int64_t>(8192, std::min<int64_t>(1 << 18, g->wedges / std::max<int64_t>(1, cus * 64)));
         }
-        btcx::build_wedge_lists(p.get(), dense, chunk, s);
+        const bool on_device = o.planner == 2 || (o.planner == 0 && (!g->on_host || g->m > btcx::kHostPlanEdges));
+        if (on_device) {
+            btcx::plan_device(p.get(), chunk, s);
+        } else {
+            btcx::ensure_host(g);
+            std::vector<uint8_t> dense;
+            btcx::classify_dense(p.get(), s, dense);
+            btcx::build_wedge_lists(p.get(), dense, chunk, s);
+        }
         p->scratch.alloc(2);
         p->info.tile_k = p->K;
         p->info.tau = p->tau;
-        int64_t bytes = (int64_t)(p->lists.bytes() + p->dense_list.bytes() + p->scratch.bytes());
+        int64_t bytes =
+            (int64_t)(p->lists.bytes() + p->items_dev.bytes() + p->dense_list.bytes() + p->scratch.bytes());
         if (p->bsr)
             bytes += (int64_t)(p->bsr->indptr.bytes() + p->bsr->indices.bytes() + p->bsr->row_indices.bytes() +
                                p->bsr->tiles.bytes());

@@ -29,6 +29,7 @@ struct btc_plan_s {
     btcx::DevBuf<uint32_t> dense_list;
     uint32_t n_dense = 0;
     btcx::DevBuf<unsigned char> lists;  // items, list records, row core bitmaps, list rows
+    btcx::DevBuf<uint4> items_dev;      // work items of a device-built plan (kept apart from lists)
     const uint4* items_p = nullptr;
     const uint4* list_p = nullptr;
     const uint4* rowbits_p = nullptr;
@@ -44,6 +45,42 @@ struct btc_plan_s {
 namespace btcx {
 
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;
+
+// Wedge-kernel plan layout shared by the host and device planners.
+constexpr int kLdsWords = 4096;   // 16 KiB of LDS per workgroup
+constexpr int kWaveWords = 1024;  // per-wave slice of it for wave items
+// Wave items: pivots with at most this many prefix lists.
+constexpr uint32_t kWaveItemLists = 64;
+// Hub core: ids below kCoreIds.  Rows with at least kCoreRowMin core ids
+// keep a kCoreIds-bit bitmap of them (kCoreWords 32-bit words, 512 bytes).
+constexpr uint32_t kCoreIds = 4096;
+constexpr uint32_t kCoreWords = kCoreIds / 32;
+constexpr uint32_t kCoreRowMin = 32;
+constexpr uint32_t kNoBitmap = 0xFFFFFFFFu;
+
+// Words of the core AND for a pivot whose ids span [lo, hi] (lo < kCoreIds):
+// from the 128-aligned window base to the 128-id block holding
+// min(hi, kCoreIds - 1), as the wedge kernel's loop runs.
+__host__ __device__ inline uint32_t core_words(uint32_t lo, uint32_t hi) {
+    const uint32_t h = hi < kCoreIds - 1 ? hi : kCoreIds - 1;
+    return ((h >> 7) + 1) * 4 - ((lo & ~127u) >> 5);
+}
+
+// A pivot's lists form one wave item when there are few of them and N_L(j)
+// fits a wave's LDS slice as a bitmap window or a hash table.
+__host__ __device__ inline bool wave_pivot(uint32_t lists, uint32_t lo, uint32_t hi, uint32_t d) {
+    return lists <= kWaveItemLists && (hi - lo < 32u * kWaveWords || d <= (uint32_t)kWaveWords / 2);
+}
+
+// Graphs with more edges than this are planned on the GPU unless the
+// options ask for the host planner.
+constexpr int64_t kHostPlanEdges = 1 << 20;
+
+// The edges of L a plan counts with the wedge kernel are cut into work
+// items of about chunk checks.  Device planner for a graph resident in HBM
+// (same plan contents as the host planner up to the order of lists within
+// a pivot and the cut points of items).
+void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s);
 // The column array of L is padded so that 16-byte loads that start inside
 // a row may run up to 3 ids past the last entry.
 constexpr size_t kColPad = 16;

@@ -1,0 +1,422 @@
+// Device planner: the wedge-kernel plan of a graph whose L is resident in
+// HBM, built by kernels, radix sorts and scans instead of host loops.
+//
+// The plan has the same contents as the host planner's (engine.hip,
+// classify_dense + build_wedge_lists): dense mask tiles (popcount > tau)
+// listed for the tile kernel; for every other edge (i, j) of L the prefix
+// of row i in front of j clipped to [min N_L(j), max N_L(j)], with its
+// hub-core bitmap choice; lists grouped by pivot j; work items of about
+// `chunk` checks, workgroup items then wave items, each class by work.  Two
+// details differ and neither changes a count: lists of one pivot are in
+// edge order (a stable radix sort by pivot), and a pivot's lists are cut
+// where the running cost crosses a multiple of `chunk` rather than
+// greedily.
+#include "device_util.h"
+#include "internal.h"
+#include "prim.h"
+
+namespace btcx {
+namespace {
+
+constexpr int kT = 256;
+
+unsigned grid_for(int64_t n) {
+    const int64_t g = (n + kT - 1) / kT;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+}
+
+int id_bits(int64_t n) {
+    int b = 1;
+    while (b < 32 && (1ll << b) < n) ++b;
+    return b;
+}
+
+#define GS_LOOP(i, n) for (int64_t i = blockIdx.x * (int64_t)kT + threadIdx.x; i < (n); i += (int64_t)gridDim.x * kT)
+
+#define BTC_LAUNCH(kernel, n, s, ...)                                               \
+    do {                                                                            \
+        hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(kT), 0, s, __VA_ARGS__); \
+        BTC_CHECK_LAUNCH(#kernel);                                                  \
+    } while (0)
+
+// First index in [lo, hi) of a[] with a[x] >= v (a ascending).
+template <class T>
+__device__ __forceinline__ uint32_t lower_bound_dev(const T* a, uint32_t lo, uint32_t hi, T v) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <class T>
+__device__ __forceinline__ uint32_t upper_bound_dev(const T* a, uint32_t lo, uint32_t hi, T v) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// erow[e] = row of entry e: one wave per row, lanes stride over its entries.
+__global__ void edge_rows_kernel(const uint32_t* __restrict__ rowptr, int32_t n, uint32_t* __restrict__ erow) {
+    const int64_t waves = ((int64_t)gridDim.x * kT) >> 6;
+    const int lane = threadIdx.x & 63;
+    for (int64_t i = (blockIdx.x * (int64_t)kT + threadIdx.x) >> 6; i < n; i += waves)
+        for (uint32_t p = rowptr[i] + lane; p < rowptr[i + 1]; p += 64) erow[p] = (uint32_t)i;
+}
+
+// Dense mask tiles: flag[t] = popcount > tau; dense edges summed.
+__global__ void dense_tiles_kernel(const uint32_t* __restrict__ pop, uint32_t nblk, int tau,
+                                   uint32_t* __restrict__ flag, unsigned long long* __restrict__ dense_edges) {
+    uint64_t sum = 0;
+    GS_LOOP(t, nblk) {
+        const bool d = (int)pop[t] > tau;
+        flag[t] = d;
+        sum += d ? pop[t] : 0u;
+    }
+    sum = wave_sum_u64(sum);
+    if ((threadIdx.x & 63) == 0 && sum) atomicAdd(dense_edges, (unsigned long long)sum);
+}
+
+__global__ void select_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int64_t n,
+                              uint32_t* __restrict__ out) {
+    GS_LOOP(t, n) if (flag[t]) out[pos[t]] = (uint32_t)t;
+}
+
+// dense[e] = the edge lies in a dense mask tile of the Bit-BSR.
+__global__ void dense_edges_kernel(const int32_t* __restrict__ col, const uint32_t* __restrict__ erow, int64_t m,
+                                   int K, const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
+                                   const uint32_t* __restrict__ pop, int tau, uint8_t* __restrict__ dense) {
+    GS_LOOP(e, m) {
+        const uint32_t I = erow[e] / 16, jc = (uint32_t)col[e] / (uint32_t)K;
+        const uint32_t t = lower_bound_dev(indices, indptr[I], indptr[I + 1], jc);
+        dense[e] = (int)pop[t] > tau;
+    }
+}
+
+// Clipped prefix of entry q (row i, pivot j = col[q]) as in the host
+// planner; false when nothing in it can match.
+__device__ __forceinline__ bool clip(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                     uint32_t i, uint32_t q, uint32_t& s0, uint32_t& s1) {
+    const uint32_t j = (uint32_t)col[q];
+    const uint32_t r0 = rowptr[i], t0 = rowptr[j], t1 = rowptr[j + 1];
+    if (t1 == t0 || q == r0) return false;
+    const int32_t lo = col[t0], hi = col[t1 - 1];
+    s0 = lower_bound_dev(col, r0, q, lo);
+    s1 = upper_bound_dev(col, s0, q, hi);
+    return s1 > s0;
+}
+
+// Pass over the edges: list flag per edge, sparse edge count and merge work.
+__global__ void clip_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                            const uint32_t* __restrict__ erow, const uint8_t* __restrict__ dense, int64_t m,
+                            uint32_t* __restrict__ flag, unsigned long long* __restrict__ stats) {
+    uint64_t se = 0, sw = 0;
+    GS_LOOP(q, m) {
+        uint32_t f = 0;
+        if (!dense || !dense[q]) {
+            const uint32_t i = erow[q], j = (uint32_t)col[q];
+            ++se;
+            sw += (rowptr[i + 1] - rowptr[i]) + (rowptr[j + 1] - rowptr[j]);
+            uint32_t s0, s1;
+            f = clip(rowptr, col, i, (uint32_t)q, s0, s1);
+        }
+        flag[q] = f;
+    }
+    se = wave_sum_u64(se);
+    sw = wave_sum_u64(sw);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], (unsigned long long)se);
+        atomicAdd(&stats[1], (unsigned long long)sw);
+    }
+}
+
+__global__ void list_keys_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                 const int32_t* __restrict__ col, int64_t m, uint32_t* __restrict__ key,
+                                 uint32_t* __restrict__ val) {
+    GS_LOOP(q, m) if (flag[q]) {
+        key[pos[q]] = (uint32_t)col[q];
+        val[pos[q]] = (uint32_t)q;
+    }
+}
+
+// off[j] = first list of pivot j (j = 0..n) in the pivot-sorted keys.
+__global__ void pivot_offsets_kernel(const uint32_t* __restrict__ key, uint32_t ne, int32_t n,
+                                     uint32_t* __restrict__ off) {
+    GS_LOOP(j, (int64_t)n + 1) off[j] = lower_bound_dev(key, 0u, ne, (uint32_t)j);
+}
+
+// Core ids (< kCoreIds) per row; rows with at least kCoreRowMin get a bitmap.
+__global__ void core_rows_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int32_t n,
+                                 uint32_t* __restrict__ cend, uint32_t* __restrict__ has) {
+    GS_LOOP(i, n) {
+        const uint32_t r0 = rowptr[i];
+        const uint32_t c = lower_bound_dev(col, r0, rowptr[i + 1], (int32_t)kCoreIds);
+        cend[i] = c;
+        has[i] = c - r0 >= kCoreRowMin;
+    }
+}
+
+__global__ void core_fill_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                 const uint32_t* __restrict__ cend, const uint32_t* __restrict__ has,
+                                 const uint32_t* __restrict__ rbi, int32_t n, uint32_t* __restrict__ rowbits) {
+    const int64_t waves = ((int64_t)gridDim.x * kT) >> 6;
+    const int lane = threadIdx.x & 63;
+    for (int64_t i = (blockIdx.x * (int64_t)kT + threadIdx.x) >> 6; i < n; i += waves) {
+        if (!has[i]) continue;
+        uint32_t* b = rowbits + (size_t)rbi[i] * kCoreWords;
+        for (uint32_t p = rowptr[i] + lane; p < cend[i]; p += 64) {
+            const uint32_t k = (uint32_t)col[p];
+            atomicOr(&b[k >> 5], 1u << (k & 31u));
+        }
+    }
+}
+
+// List record of every pivot-sorted list e (see the wedge kernel's
+// scan_lists), its row, and its cost (ids looked up + core words ANDed).
+__global__ void records_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                               const uint32_t* __restrict__ erow, const uint32_t* __restrict__ val, uint32_t ne,
+                               const uint32_t* __restrict__ cend, const uint32_t* __restrict__ has,
+                               const uint32_t* __restrict__ rbi, uint4* __restrict__ lst, int32_t* __restrict__ ur,
+                               uint64_t* __restrict__ cost, unsigned long long* __restrict__ ncore) {
+    uint64_t nc = 0;
+    GS_LOOP(e, ne) {
+        const uint32_t q = val[e], i = erow[q], j = (uint32_t)col[q];
+        uint32_t s0 = 0, s1 = 0;
+        clip(rowptr, col, i, q, s0, s1);
+        const uint32_t lo = (uint32_t)col[rowptr[j]], hi = (uint32_t)col[rowptr[j + 1] - 1];
+        const uint32_t base = lo & ~127u;
+        uint32_t sc = s0, rb = kNoBitmap, words = 0;
+        if (has[i] && base < kCoreIds) {
+            const uint32_t ce = min(s1, cend[i]);
+            const uint32_t core_ids = ce > s0 ? ce - s0 : 0u;
+            const uint32_t w = core_words(lo, hi);
+            if (core_ids * 4 > w + 64) {
+                sc = max(s0, cend[i]);
+                rb = rbi[i];
+                words = w;
+                ++nc;
+            }
+        }
+        lst[e] = make_uint4(s0, sc, s1, rb);
+        ur[e] = (int32_t)i;
+        cost[e] = (uint64_t)(s1 > sc ? s1 - sc : 0u) + words;
+    }
+    nc = wave_sum_u64(nc);
+    if ((threadIdx.x & 63) == 0 && nc) atomicAdd(ncore, (unsigned long long)nc);
+}
+
+__device__ __forceinline__ bool pivot_small(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                            const uint32_t* __restrict__ off, uint32_t j) {
+    const uint32_t t0 = rowptr[j], t1 = rowptr[j + 1];
+    return wave_pivot(off[j + 1] - off[j], (uint32_t)col[t0], (uint32_t)col[t1 - 1], t1 - t0);
+}
+
+// start[e] = list e opens a work item: the pivot's first list, or (large
+// pivots) the running cost crosses a multiple of chunk.
+__global__ void item_start_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                  const uint32_t* __restrict__ key, const uint32_t* __restrict__ off,
+                                  const uint64_t* __restrict__ P, uint32_t ne, uint64_t chunk,
+                                  uint32_t* __restrict__ start) {
+    GS_LOOP(e, ne) {
+        const uint32_t j = key[e], eb = off[j];
+        uint32_t st = e == eb;
+        if (!st && !pivot_small(rowptr, col, off, j)) st = (P[e] - P[eb]) / chunk != (P[e - 1] - P[eb]) / chunk;
+        start[e] = st;
+    }
+}
+
+// Item k = lists [first[k], first[k + 1]); sort key puts workgroup items
+// before wave items, each by descending work.
+__global__ void items_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                             const uint32_t* __restrict__ key, const uint32_t* __restrict__ off,
+                             const uint64_t* __restrict__ P, const uint32_t* __restrict__ first, uint32_t nitems,
+                             uint32_t ne, uint4* __restrict__ tmp_items, uint32_t* __restrict__ skey,
+                             uint32_t* __restrict__ sval, unsigned long long* __restrict__ stats) {
+    uint64_t words = 0, small_items = 0;
+    GS_LOOP(k, nitems) {
+        const uint32_t eb = first[k], ee = k + 1 < nitems ? first[k + 1] : ne;
+        const uint32_t j = key[eb];
+        const uint32_t t0 = rowptr[j], t1 = rowptr[j + 1];
+        const uint64_t w = P[ee] - P[eb];
+        const uint32_t w32 = (uint32_t)min(w, (uint64_t)0xFFFFFFFFu);
+        const bool small = pivot_small(rowptr, col, off, j);
+        tmp_items[2 * k] = make_uint4(t0, t1, eb, ee);
+        tmp_items[2 * k + 1] = make_uint4((uint32_t)col[t0], (uint32_t)col[t1 - 1], w32, 0u);
+        const uint32_t w31 = (uint32_t)min(w, (uint64_t)0x7FFFFFFFu);
+        skey[k] = (small ? 0x80000000u : 0u) | (0x7FFFFFFFu - w31);
+        sval[k] = (uint32_t)k;
+        words += t1 - t0;
+        small_items += small;
+    }
+    words = wave_sum_u64(words);
+    small_items = wave_sum_u64(small_items);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], (unsigned long long)words);
+        atomicAdd(&stats[1], (unsigned long long)small_items);
+    }
+}
+
+__global__ void place_items_kernel(const uint4* __restrict__ tmp_items, const uint32_t* __restrict__ order,
+                                   uint32_t nitems, uint4* __restrict__ items) {
+    GS_LOOP(r, nitems) {
+        const uint32_t k = order[r];
+        items[2 * r] = tmp_items[2 * k];
+        items[2 * r + 1] = tmp_items[2 * k + 1];
+    }
+}
+
+template <class T>
+T read_dev(const T* p, hipStream_t s) {
+    T h{};
+    BTC_HIP(hipMemcpyAsync(&h, p, sizeof(T), hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+}  // namespace
+
+void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
+    btc_graph_s* g = p->g;
+    const int32_t n = g->n;
+    const int64_t m = g->m;
+    const uint32_t* rowptr = g->d_rowptr.p;
+    const int32_t* col = g->d_col.p;
+    BTC_REQUIRE(m < (int64_t)0xFFFFFFFFLL, BTC_ERR_OVERFLOW, "nnz(L) %lld exceeds 2^32 - 1", (long long)m);
+
+    DevBuf<uint32_t> erow;
+    erow.alloc((size_t)m);
+    if (m) BTC_LAUNCH(edge_rows_kernel, (int64_t)n * 64, s, rowptr, n, erow.p);
+
+    // dense mask tiles
+    DevBuf<unsigned long long> st;
+    st.alloc(8);
+    BTC_HIP(hipMemsetAsync(st.p, 0, 64, s));
+    DevBuf<uint8_t> dense;
+    p->n_dense = 0;
+    if (p->tau < 16 * p->K && m > 0) {
+        p->bsr = std::make_unique<btc_bitbsr_s>();
+        bitbsr_build(g, p->K, s, p->bsr.get());
+        btc_bitbsr_s* b = p->bsr.get();
+        DevBuf<uint32_t> pop, flag, pos;
+        pop.alloc(b->nblk);
+        flag.alloc(b->nblk);
+        pos.alloc((size_t)b->nblk + 1);
+        launch_tile_popcount(b, pop.p, s);
+        BTC_LAUNCH(dense_tiles_kernel, b->nblk, s, pop.p, b->nblk, p->tau, flag.p, st.p + 0);
+        exclusive_scan_u32(flag.p, pos.p, b->nblk, s);
+        const uint32_t nd = read_dev(pos.p + b->nblk, s);
+        if (nd) {
+            p->dense_list.alloc(nd);
+            BTC_LAUNCH(select_kernel, b->nblk, s, flag.p, pos.p, (int64_t)b->nblk, p->dense_list.p);
+            dense.alloc((size_t)m);
+            BTC_LAUNCH(dense_edges_kernel, m, s, col, erow.p, m, p->K, b->indptr.p, b->indices.p, pop.p, p->tau,
+                       dense.p);
+            p->n_dense = nd;
+        }
+        BTC_HIP(hipStreamSynchronize(s));
+        if (!nd) p->bsr.reset();
+    }
+    p->info.dense_tiles = p->n_dense;
+    p->info.dense_edges = (int64_t)read_dev(st.p + 0, s);
+
+    // prefix lists: flag, compact (pivot, edge), stable sort by pivot
+    DevBuf<uint32_t> lflag, lpos;
+    lflag.alloc((size_t)m);
+    lpos.alloc((size_t)m + 1);
+    if (m) BTC_LAUNCH(clip_kernel, m, s, rowptr, col, erow.p, p->n_dense ? dense.p : nullptr, m, lflag.p, st.p + 1);
+    exclusive_scan_u32(lflag.p, lpos.p, (size_t)m, s);
+    const uint32_t ne = read_dev(lpos.p + m, s);
+    DevBuf<uint32_t> k0, v0, k1, v1;
+    k0.alloc(ne);
+    v0.alloc(ne);
+    k1.alloc(ne);
+    v1.alloc(ne);
+    if (m) BTC_LAUNCH(list_keys_kernel, m, s, lflag.p, lpos.p, col, m, k0.p, v0.p);
+    lflag.release();
+    lpos.release();
+    dense.release();
+    const bool alt = sort_pairs_u32(k0.p, v0.p, k1.p, v1.p, ne, id_bits(n), s);
+    const uint32_t* key = alt ? k1.p : k0.p;
+    const uint32_t* val = alt ? v1.p : v0.p;
+    DevBuf<uint32_t> off;
+    off.alloc((size_t)n + 1);
+    BTC_LAUNCH(pivot_offsets_kernel, (int64_t)n + 1, s, key, ne, n, off.p);
+
+    // hub-core bitmaps
+    DevBuf<uint32_t> cend, has, rbi;
+    cend.alloc((size_t)n);
+    has.alloc((size_t)n);
+    rbi.alloc((size_t)n + 1);
+    if (n) BTC_LAUNCH(core_rows_kernel, n, s, rowptr, col, n, cend.p, has.p);
+    exclusive_scan_u32(has.p, rbi.p, (size_t)n, s);
+    const uint32_t nbm = read_dev(rbi.p + n, s);
+
+    // one allocation: list records, row bitmaps, list rows; items apart
+    const size_t lb = (size_t)ne * sizeof(uint4), bb = (size_t)nbm * kCoreWords * 4, rb = (size_t)ne * 4;
+    p->lists.alloc(lb + bb + rb);
+    uint4* lst = reinterpret_cast<uint4*>(p->lists.p);
+    uint32_t* rowbits = reinterpret_cast<uint32_t*>(p->lists.p + lb);
+    int32_t* ur = reinterpret_cast<int32_t*>(p->lists.p + lb + bb);
+    if (bb) {
+        BTC_HIP(hipMemsetAsync(rowbits, 0, bb, s));
+        BTC_LAUNCH(core_fill_kernel, (int64_t)n * 64, s, rowptr, col, cend.p, has.p, rbi.p, n, rowbits);
+    }
+    DevBuf<uint64_t> cost, P;
+    cost.alloc(ne);
+    P.alloc((size_t)ne + 1);
+    if (ne)
+        BTC_LAUNCH(records_kernel, ne, s, rowptr, col, erow.p, val, ne, cend.p, has.p, rbi.p, lst, ur, cost.p,
+                   st.p + 3);
+    exclusive_scan_u64(cost.p, P.p, ne, s);
+    erow.release();
+
+    // work items
+    DevBuf<uint32_t> start, spos;
+    start.alloc(ne);
+    spos.alloc((size_t)ne + 1);
+    if (ne) BTC_LAUNCH(item_start_kernel, ne, s, rowptr, col, key, off.p, P.p, ne, (uint64_t)chunk, start.p);
+    exclusive_scan_u32(start.p, spos.p, ne, s);
+    const uint32_t nitems = read_dev(spos.p + ne, s);
+    DevBuf<uint32_t> first, sk0, sv0, sk1, sv1;
+    first.alloc(nitems);
+    if (ne) BTC_LAUNCH(select_kernel, ne, s, start.p, spos.p, (int64_t)ne, first.p);
+    DevBuf<uint4> tmp;
+    tmp.alloc((size_t)nitems * 2);
+    sk0.alloc(nitems);
+    sv0.alloc(nitems);
+    sk1.alloc(nitems);
+    sv1.alloc(nitems);
+    if (nitems)
+        BTC_LAUNCH(items_kernel, nitems, s, rowptr, col, key, off.p, P.p, first.p, nitems, ne, tmp.p, sk0.p, sv0.p,
+                   st.p + 4);
+    const bool ialt = sort_pairs_u32(sk0.p, sv0.p, sk1.p, sv1.p, nitems, 32, s);
+    p->items_dev.alloc((size_t)nitems * 2);
+    if (nitems)
+        BTC_LAUNCH(place_items_kernel, nitems, s, tmp.p, ialt ? sv1.p : sv0.p, nitems, p->items_dev.p);
+
+    // st: [0] dense edges, [1] sparse edges, [2] their merge work, [3] core
+    // lists, [4] pivot row words over items, [5] wave items
+    unsigned long long h[8];
+    BTC_HIP(hipMemcpyAsync(h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    const uint64_t checks = read_dev(P.p + ne, s);  // synchronises h as well
+    p->items_p = p->items_dev.p;
+    p->list_p = lst;
+    p->rowbits_p = reinterpret_cast<const uint4*>(rowbits);
+    p->urow_p = ur;
+    p->nitems = nitems;
+    p->n_block_items = nitems - (uint32_t)h[5];
+    p->info.sparse_edges = (int64_t)h[1];
+    p->info.sparse_merge_work = (int64_t)h[2];
+    p->info.core_lists = (int64_t)h[3];
+    p->info.core_rows = nbm;
+    p->info.wedge_checks = (int64_t)checks;
+    p->info.work_items = nitems;
+    // looked-up ids and pivot rows (4 B each), list records (16 B), items (32 B)
+    p->info.stream_bytes = 4 * ((int64_t)checks + (int64_t)h[4]) + 16 * (int64_t)ne + 32 * (int64_t)nitems;
+}
+
+}  // namespace btcx

@@ -1,0 +1,29 @@
+// Device-wide sort and scan used by the device canonicaliser and planner
+// (rocPRIM underneath; kept in one translation unit for compile time).
+// Every function runs on `s`, allocates its own temporary storage and
+// returns once the work is enqueued (temporaries are freed after a
+// stream synchronisation inside the call).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace btcx {
+
+// Sorts keys[0, n) ascending on bits [0, end_bit); tmp holds n keys.
+// Returns the buffer that holds the result (keys or tmp).
+uint64_t* sort_u64(uint64_t* keys, uint64_t* tmp, size_t n, int end_bit, hipStream_t s);
+
+// Stable sort of (key, value) pairs on key bits [0, end_bit); kt / vt hold n
+// entries each.  Returns whether the result is in (kt, vt) rather than
+// (keys, vals).
+bool sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* kt, uint32_t* vt, size_t n, int end_bit,
+                    hipStream_t s);
+
+// out[i] = sum of in[0, i) for i <= n (out has n + 1 entries; out[n] = total).
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, hipStream_t s);
+void exclusive_scan_u64(const uint64_t* in, uint64_t* out, size_t n, hipStream_t s);
+
+}  // namespace btcx

@@ -83,10 +83,14 @@ int64_t btc_gen_planted(int32_t n, int32_t community, double p_in, double noise_
  * by descending degree with ties broken by id (BTC_CANON_DEGREE_ORDER, not in
  * the reference; the triangle total is invariant under it).  The result is
  * the strictly lower triangle L in CSR with ascending columns, resident in
- * HBM.  Ids must lie in [0, n); n <= 0 means n = max id + 1. */
+ * HBM.  Ids must lie in [0, n); n <= 0 means n = max id + 1.
+ * BTC_CANON_DEVICE does the same on the GPU (pairs staged in HBM, radix
+ * sort of packed (max, min) keys); the result is identical bit for bit and
+ * the host copy of L is made only when a host consumer needs it. */
 #define BTC_CANON_COMPACT 1u
 #define BTC_CANON_DEGREE_ORDER 2u
 #define BTC_CANON_DEFAULT (BTC_CANON_COMPACT | BTC_CANON_DEGREE_ORDER)
+#define BTC_CANON_DEVICE 4u
 
 typedef struct btc_graph_s* btc_graph;
 
@@ -102,6 +106,11 @@ typedef struct {
 
 int btc_graph_from_coo(const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n,
                        uint32_t flags, btc_graph* out);
+/* The same from pairs already in HBM (device pointers), canonicalised on the
+ * GPU on `stream` — the reference's preprocess_for_triangle_counting on a
+ * device COO (convert.h:601-692). */
+int btc_graph_from_coo_device(const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n,
+                              uint32_t flags, void* stream, btc_graph* out);
 /* Adopt an already canonical L (strictly lower, ascending, duplicate-free). */
 int btc_graph_from_csr(const int64_t* rowptr, const int32_t* col, int32_t n, btc_graph* out);
 int btc_graph_get_info(btc_graph g, btc_graph_info* info);
@@ -183,6 +192,8 @@ typedef struct {
     int32_t schedule;     /* 0 = work stealing on one counter, 1 = static stride over sorted items */
     int32_t dense_kernel; /* dense tiles: 0 = auto (AND + popcount), 1 = 64-bit AND + v_bcnt popcount,
                              2 = i8 MFMA (v_mfma_i32_16x16x64_i8) on 0/1-unpacked tile rows */
+    int32_t planner;      /* work lists built by: 0 = auto (the GPU for graphs canonicalised there or with
+                             more than 2^20 edges), 1 = host threads, 2 = GPU kernels */
 } btc_count_opts;
 
 typedef struct btc_plan_s* btc_plan;
