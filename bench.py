@@ -230,7 +230,7 @@ def batched_leg(btc, count, steps, sample):
             "build_s": round(t_build, 2), "canon_s": round(t_canon, 2), "work_items": info["plan"]["work_items"]}
 
 
-def serving_leg(btc, requests, seed=1):
+def serving_leg(btc, requests, seed=1, canon_flags=None, prep_threads=None):
     """Poisson stream of exact-count queries: R-MAT graphs (edge factor 16)
     with n log-normal (median 2^13, sigma 1 in ln n, capped at 2^20), 30 %
     of requests repeat a graph submitted earlier.  A closed burst measures
@@ -257,7 +257,7 @@ def serving_leg(btc, requests, seed=1):
         return reqs
 
     def run(reqs, qps):
-        sv = btc.Server()
+        sv = btc.Server(canon_flags=canon_flags, prep_threads=prep_threads)
         gaps = rng.exponential(1.0 / qps, len(reqs)) if qps else np.zeros(len(reqs))
         t_start = time.perf_counter()
 
@@ -291,6 +291,8 @@ def serving_leg(btc, requests, seed=1):
     reqs = stream(requests, 2 << 40)
     res, st = run(reqs, qps)
     lat = np.array(sorted(r["t_done"] - r["t_submit"] for r in res))
+    wait = np.array([r["t_start"] - r["t_submit"] for r in res if not r["resident_hit"]])
+    run_ = np.array([r["t_done"] - r["t_start"] for r in res])
     span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
     graphs = {}
     for key, g in reqs:
@@ -303,8 +305,10 @@ def serving_leg(btc, requests, seed=1):
            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3), "exact_alone_vs_mix": exact,
            "resident_hits": st["resident_hits"], "launches": st["launches"],
            "mean_batch": round(st["launched_requests"] / max(1, st["launches"]), 2),
-           "distinct_graphs": len(graphs)}
-    log("serving: %s" % out)
+           "distinct_graphs": len(graphs),
+           "p50_prepare_ms": round(1e3 * float(np.percentile(wait, 50)), 3) if wait.size else None,
+           "p50_launch_ms": round(1e3 * float(np.percentile(run_, 50)), 3)}
+    log("serving (canon flags %s): %s" % (canon_flags, out))
     return out
 
 
@@ -324,6 +328,8 @@ def main():
     ap.add_argument("--serve", type=int, default=600, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--dense", type=int, default=5, help="timed runs per case in the dense-tile leg (0 = skip)")
     ap.add_argument("--host-path", type=int, default=1, help="also time host canonicalisation and planning")
+    ap.add_argument("--serve-host-prep", type=int, default=1,
+                    help="also run the serving leg with canonicalisation and planning on host threads")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one D2D copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -409,6 +415,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_GBps": round(traffic / (wedge_ms * 1e-3) / 1e9, 1) if traffic and wedge_ms > 0 else None,
+                "traffic_frac": round(traffic / (wedge_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                if traffic and wedge_ms > 0 else None,
                 "kernel": "wedge_kernel", "kernel_ms": round(wedge_ms, 4), "alg_bytes": b_alg,
                 "stream_bytes": pi["stream_bytes"],
                 "stream_GBps": round(pi["stream_bytes"] / (wedge_ms * 1e-3) / 1e9, 1) if wedge_ms > 0 else None}
@@ -440,6 +448,8 @@ def main():
 
     batched = batched_leg(btc, args.batch, max(3, args.steps // 2), args.batch_sample) if args.batch > 0 else None
     serving = serving_leg(btc, args.serve) if args.serve > 0 else None
+    serving_host = (serving_leg(btc, args.serve, canon_flags=btc.CANON_DEFAULT)
+                    if args.serve > 0 and args.serve_host_prep else None)
     dense = dense_leg(btc, args.dense) if args.dense > 0 else None
 
     config.update({"n": gi["n"], "m": gi["m"], "ordering": "degree-descending", "tile_k": pi["tile_k"],
@@ -451,7 +461,7 @@ def main():
         "triangles": int(total0), "triangles_per_s": total0 / (wall / args.steps),
         "kernel_ms_median": statistics.median(kms), "canon_ms": round(canon_ms, 1), "plan_ms": round(build_ms, 1),
         "roofline": roofline, "cpu_baseline": cpu, "plain_hybrid": plain, "end_to_end": e2e,
-        "batched": batched, "serving": serving, "dense_tiles": dense,
+        "batched": batched, "serving": serving, "serving_host_prep": serving_host, "dense_tiles": dense,
     }
     print(json.dumps(out), flush=True)
 

@@ -136,15 +136,15 @@ void scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, hipStream_t s)
         BTC_HIP(hipMemsetAsync(out, 0, sizeof(uint64_t), s));
         return;
     }
-    DevBuf<uint64_t> sums;
-    sums.alloc(nb);
+    TmpBuf<uint64_t> sums;
+    sums.alloc(nb, s);
     hipLaunchKernelGGL(scan_tile_sums, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, sums.p);
     BTC_CHECK_LAUNCH("scan_tile_sums");
     hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanThreads), 0, s, sums.p, nb, out + n);
     BTC_CHECK_LAUNCH("scan_sums");
     hipLaunchKernelGGL(scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, sums.p, out);
     BTC_CHECK_LAUNCH("scan_apply");
-    BTC_HIP(hipStreamSynchronize(s));  // sums is freed on return
+
 }
 
 // ---------------------------------------------------------------- builder
@@ -271,10 +271,10 @@ void build_impl(btc_graph_s* g, hipStream_t s, btc_bitbsr_s* b) {
     b->wpt = K / 4;
     b->max_row = g->max_row;
     const unsigned grid = (unsigned)((n_rb + 15) / 16);
-    DevBuf<uint32_t> counts;
-    DevBuf<uint64_t> offs;
-    counts.alloc((size_t)n_rb);
-    offs.alloc((size_t)n_rb + 1);
+    TmpBuf<uint32_t> counts;
+    TmpBuf<uint64_t> offs;
+    counts.alloc((size_t)n_rb, s);
+    offs.alloc((size_t)n_rb + 1, s);
     if (grid) {
         hipLaunchKernelGGL((build_kernel<K, false>), dim3(grid), dim3(256), 0, s, n, n_rb, g->d_rowptr.p,
                            g->d_col.p, counts.p, nullptr, nullptr, nullptr, nullptr);
@@ -300,7 +300,6 @@ void build_impl(btc_graph_s* g, hipStream_t s, btc_bitbsr_s* b) {
                            g->d_col.p, nullptr, offs.p, b->indices.p, b->row_indices.p, b->tiles.p);
         BTC_CHECK_LAUNCH("build_kernel<fill>");
     }
-    BTC_HIP(hipStreamSynchronize(s));
 }
 
 // ------------------------------------------------------------------- count
@@ -582,6 +581,7 @@ int btc_bitbsr_build(btc_graph g, int32_t tile_k, void* stream, btc_bitbsr* out)
                     "tile width K = %d (expected 32, 64 or 128)", tile_k);
         auto b = std::make_unique<btc_bitbsr_s>();
         btcx::bitbsr_build(g, tile_k, btcx::as_stream(stream), b.get());
+        BTC_HIP(hipStreamSynchronize(btcx::as_stream(stream)));
         *out = b.release();
     });
 }

@@ -205,8 +205,8 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
     // id range (and n when not given)
     int32_t h_lohi[2] = {INT_MAX, -1};
     {
-        DevBuf<int32_t> lohi;
-        lohi.alloc(2);
+        TmpBuf<int32_t> lohi;
+        lohi.alloc(2, s);
         BTC_HIP(hipMemcpyAsync(lohi.p, h_lohi, 8, hipMemcpyHostToDevice, s));
         if (nnz) BTC_LAUNCH(id_range_kernel, nnz, s, src, dst, nnz, lohi.p);
         BTC_HIP(hipMemcpyAsync(h_lohi, lohi.p, 8, hipMemcpyDeviceToHost, s));
@@ -222,17 +222,17 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
     const uint64_t sentinel = (1ull << (2 * b)) - 1;
 
     // oriented keys, sorted, deduplicated
-    DevBuf<uint64_t> ka, kb;
-    ka.alloc((size_t)nnz);
-    kb.alloc((size_t)nnz);
+    TmpBuf<uint64_t> ka, kb;
+    ka.alloc((size_t)nnz, s);
+    kb.alloc((size_t)nnz, s);
     if (nnz) BTC_LAUNCH(orient_kernel, nnz, s, src, dst, nnz, b, sentinel, ka.p);
     uint64_t* sorted = sort_u64(ka.p, kb.p, (size_t)nnz, 2 * b, s);
     uint64_t* other = sorted == ka.p ? kb.p : ka.p;
     int64_t m = 0;
     {
-        DevBuf<uint32_t> flag, pos;
-        flag.alloc((size_t)nnz);
-        pos.alloc((size_t)nnz + 1);
+        TmpBuf<uint32_t> flag, pos;
+        flag.alloc((size_t)nnz, s);
+        pos.alloc((size_t)nnz + 1, s);
         if (nnz) BTC_LAUNCH(first_flag_kernel, nnz, s, sorted, nnz, sentinel, flag.p);
         exclusive_scan_u32(flag.p, pos.p, (size_t)nnz, s);
         uint32_t hm = 0;
@@ -246,12 +246,12 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
 
     // order-preserving compaction of the used ids (none when no edge)
     int32_t nc = n;
-    DevBuf<uint32_t> newid;
+    TmpBuf<uint32_t> newid;
     DevBuf<int32_t> labels;
     if ((flags & BTC_CANON_COMPACT) && m > 0) {
-        DevBuf<uint32_t> used;
-        used.alloc((size_t)n);
-        newid.alloc((size_t)n + 1);
+        TmpBuf<uint32_t> used;
+        used.alloc((size_t)n, s);
+        newid.alloc((size_t)n + 1, s);
         BTC_HIP(hipMemsetAsync(used.p, 0, (size_t)n * 4, s));
         BTC_LAUNCH(mark_used_kernel, m, s, keys, m, b, used.p);
         exclusive_scan_u32(used.p, newid.p, (size_t)n, s);
@@ -269,14 +269,14 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
     if (m) BTC_LAUNCH(relabel_kernel, m, s, keys, m, b, newid.p, bn);  // monotone: order kept
 
     if (flags & BTC_CANON_DEGREE_ORDER) {
-        DevBuf<uint32_t> deg, k0, v0, k1, v1, rank;
+        TmpBuf<uint32_t> deg, k0, v0, k1, v1, rank;
         DevBuf<int32_t> lab2;
-        deg.alloc((size_t)nc);
-        k0.alloc((size_t)nc);
-        v0.alloc((size_t)nc);
-        k1.alloc((size_t)nc);
-        v1.alloc((size_t)nc);
-        rank.alloc((size_t)nc);
+        deg.alloc((size_t)nc, s);
+        k0.alloc((size_t)nc, s);
+        v0.alloc((size_t)nc, s);
+        k1.alloc((size_t)nc, s);
+        v1.alloc((size_t)nc, s);
+        rank.alloc((size_t)nc, s);
         lab2.alloc((size_t)nc);
         BTC_HIP(hipMemsetAsync(deg.p, 0, (size_t)nc * 4, s));
         if (m) BTC_LAUNCH(degree_kernel, m, s, keys, m, bn, deg.p);
@@ -299,8 +299,8 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
     g->d_labels = std::move(labels);
 
     // derived statistics (graph_finalize on the device)
-    DevBuf<unsigned long long> st;
-    st.alloc(3);
+    TmpBuf<unsigned long long> st;
+    st.alloc(3, s);
     BTC_HIP(hipMemsetAsync(st.p, 0, 24, s));
     if (nc) BTC_LAUNCH(row_stats_kernel, nc, s, g->d_rowptr.p, nc, st.p);
     if (m) BTC_LAUNCH(edge_stats_kernel, m, s, g->d_rowptr.p, g->d_col.p, m, st.p);

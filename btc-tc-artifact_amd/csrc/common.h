@@ -99,6 +99,42 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Keeps up to 1 GiB of freed stream-ordered memory cached in the current
+// device's default pool (once per device), so temporaries are reused.
+void mem_pool_init();
+
+// Stream-ordered temporary device buffer (hipMallocAsync / hipFreeAsync on
+// one stream): freeing it does not wait for the whole device, so threads
+// that build plans side by side do not serialise on each other's kernels.
+template <class T>
+struct TmpBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipStream_t s = nullptr;
+    TmpBuf() = default;
+    TmpBuf(const TmpBuf&) = delete;
+    TmpBuf& operator=(const TmpBuf&) = delete;
+    ~TmpBuf() { release(); }
+    void alloc(size_t count, hipStream_t stream) {
+        release();
+        if (count == 0) count = 1;
+        s = stream;
+        mem_pool_init();
+        hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&p), count * sizeof(T), s);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            fail(BTC_ERR_NOMEM, "hipMallocAsync of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFreeAsync(p, s);
+        p = nullptr;
+        n = 0;
+    }
+};
+
 // Kernel timing with HIP events recorded on the launch stream.
 struct EventTimer {
     hipEvent_t a = nullptr, b = nullptr;

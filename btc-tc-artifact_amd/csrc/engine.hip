@@ -787,7 +787,7 @@ void plan_count(btc_plan_s* p, hipStream_t s, uint64_t* total, int32_t* per_vert
     if (kernel_ms) {
         *kernel_ms = tm.ms();
         p->last_wedge_ms = tw.ms();
-        p->last_dense_ms = *kernel_ms - p->last_wedge_ms;
+        p->last_dense_ms = p->n_dense ? std::max(0.f, *kernel_ms - p->last_wedge_ms) : 0.f;
     }
     *total = h;
 }

@@ -287,24 +287,24 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     const int32_t* col = g->d_col.p;
     BTC_REQUIRE(m < (int64_t)0xFFFFFFFFLL, BTC_ERR_OVERFLOW, "nnz(L) %lld exceeds 2^32 - 1", (long long)m);
 
-    DevBuf<uint32_t> erow;
-    erow.alloc((size_t)m);
+    TmpBuf<uint32_t> erow;
+    erow.alloc((size_t)m, s);
     if (m) BTC_LAUNCH(edge_rows_kernel, (int64_t)n * 64, s, rowptr, n, erow.p);
 
     // dense mask tiles
-    DevBuf<unsigned long long> st;
-    st.alloc(8);
+    TmpBuf<unsigned long long> st;
+    st.alloc(8, s);
     BTC_HIP(hipMemsetAsync(st.p, 0, 64, s));
-    DevBuf<uint8_t> dense;
+    TmpBuf<uint8_t> dense;
     p->n_dense = 0;
     if (p->tau < 16 * p->K && m > 0) {
         p->bsr = std::make_unique<btc_bitbsr_s>();
         bitbsr_build(g, p->K, s, p->bsr.get());
         btc_bitbsr_s* b = p->bsr.get();
-        DevBuf<uint32_t> pop, flag, pos;
-        pop.alloc(b->nblk);
-        flag.alloc(b->nblk);
-        pos.alloc((size_t)b->nblk + 1);
+        TmpBuf<uint32_t> pop, flag, pos;
+        pop.alloc(b->nblk, s);
+        flag.alloc(b->nblk, s);
+        pos.alloc((size_t)b->nblk + 1, s);
         launch_tile_popcount(b, pop.p, s);
         BTC_LAUNCH(dense_tiles_kernel, b->nblk, s, pop.p, b->nblk, p->tau, flag.p, st.p + 0);
         exclusive_scan_u32(flag.p, pos.p, b->nblk, s);
@@ -312,7 +312,7 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
         if (nd) {
             p->dense_list.alloc(nd);
             BTC_LAUNCH(select_kernel, b->nblk, s, flag.p, pos.p, (int64_t)b->nblk, p->dense_list.p);
-            dense.alloc((size_t)m);
+            dense.alloc((size_t)m, s);
             BTC_LAUNCH(dense_edges_kernel, m, s, col, erow.p, m, p->K, b->indptr.p, b->indices.p, pop.p, p->tau,
                        dense.p);
             p->n_dense = nd;
@@ -324,17 +324,17 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     p->info.dense_edges = (int64_t)read_dev(st.p + 0, s);
 
     // prefix lists: flag, compact (pivot, edge), stable sort by pivot
-    DevBuf<uint32_t> lflag, lpos;
-    lflag.alloc((size_t)m);
-    lpos.alloc((size_t)m + 1);
+    TmpBuf<uint32_t> lflag, lpos;
+    lflag.alloc((size_t)m, s);
+    lpos.alloc((size_t)m + 1, s);
     if (m) BTC_LAUNCH(clip_kernel, m, s, rowptr, col, erow.p, p->n_dense ? dense.p : nullptr, m, lflag.p, st.p + 1);
     exclusive_scan_u32(lflag.p, lpos.p, (size_t)m, s);
     const uint32_t ne = read_dev(lpos.p + m, s);
-    DevBuf<uint32_t> k0, v0, k1, v1;
-    k0.alloc(ne);
-    v0.alloc(ne);
-    k1.alloc(ne);
-    v1.alloc(ne);
+    TmpBuf<uint32_t> k0, v0, k1, v1;
+    k0.alloc(ne, s);
+    v0.alloc(ne, s);
+    k1.alloc(ne, s);
+    v1.alloc(ne, s);
     if (m) BTC_LAUNCH(list_keys_kernel, m, s, lflag.p, lpos.p, col, m, k0.p, v0.p);
     lflag.release();
     lpos.release();
@@ -342,15 +342,15 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     const bool alt = sort_pairs_u32(k0.p, v0.p, k1.p, v1.p, ne, id_bits(n), s);
     const uint32_t* key = alt ? k1.p : k0.p;
     const uint32_t* val = alt ? v1.p : v0.p;
-    DevBuf<uint32_t> off;
-    off.alloc((size_t)n + 1);
+    TmpBuf<uint32_t> off;
+    off.alloc((size_t)n + 1, s);
     BTC_LAUNCH(pivot_offsets_kernel, (int64_t)n + 1, s, key, ne, n, off.p);
 
     // hub-core bitmaps
-    DevBuf<uint32_t> cend, has, rbi;
-    cend.alloc((size_t)n);
-    has.alloc((size_t)n);
-    rbi.alloc((size_t)n + 1);
+    TmpBuf<uint32_t> cend, has, rbi;
+    cend.alloc((size_t)n, s);
+    has.alloc((size_t)n, s);
+    rbi.alloc((size_t)n + 1, s);
     if (n) BTC_LAUNCH(core_rows_kernel, n, s, rowptr, col, n, cend.p, has.p);
     exclusive_scan_u32(has.p, rbi.p, (size_t)n, s);
     const uint32_t nbm = read_dev(rbi.p + n, s);
@@ -365,9 +365,9 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
         BTC_HIP(hipMemsetAsync(rowbits, 0, bb, s));
         BTC_LAUNCH(core_fill_kernel, (int64_t)n * 64, s, rowptr, col, cend.p, has.p, rbi.p, n, rowbits);
     }
-    DevBuf<uint64_t> cost, P;
-    cost.alloc(ne);
-    P.alloc((size_t)ne + 1);
+    TmpBuf<uint64_t> cost, P;
+    cost.alloc(ne, s);
+    P.alloc((size_t)ne + 1, s);
     if (ne)
         BTC_LAUNCH(records_kernel, ne, s, rowptr, col, erow.p, val, ne, cend.p, has.p, rbi.p, lst, ur, cost.p,
                    st.p + 3);
@@ -375,21 +375,21 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     erow.release();
 
     // work items
-    DevBuf<uint32_t> start, spos;
-    start.alloc(ne);
-    spos.alloc((size_t)ne + 1);
+    TmpBuf<uint32_t> start, spos;
+    start.alloc(ne, s);
+    spos.alloc((size_t)ne + 1, s);
     if (ne) BTC_LAUNCH(item_start_kernel, ne, s, rowptr, col, key, off.p, P.p, ne, (uint64_t)chunk, start.p);
     exclusive_scan_u32(start.p, spos.p, ne, s);
     const uint32_t nitems = read_dev(spos.p + ne, s);
-    DevBuf<uint32_t> first, sk0, sv0, sk1, sv1;
-    first.alloc(nitems);
+    TmpBuf<uint32_t> first, sk0, sv0, sk1, sv1;
+    first.alloc(nitems, s);
     if (ne) BTC_LAUNCH(select_kernel, ne, s, start.p, spos.p, (int64_t)ne, first.p);
-    DevBuf<uint4> tmp;
-    tmp.alloc((size_t)nitems * 2);
-    sk0.alloc(nitems);
-    sv0.alloc(nitems);
-    sk1.alloc(nitems);
-    sv1.alloc(nitems);
+    TmpBuf<uint4> tmp;
+    tmp.alloc((size_t)nitems * 2, s);
+    sk0.alloc(nitems, s);
+    sv0.alloc(nitems, s);
+    sk1.alloc(nitems, s);
+    sv1.alloc(nitems, s);
     if (nitems)
         BTC_LAUNCH(items_kernel, nitems, s, rowptr, col, key, off.p, P.p, first.p, nitems, ne, tmp.p, sk0.p, sv0.p,
                    st.p + 4);

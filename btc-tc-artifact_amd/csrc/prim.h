@@ -1,8 +1,7 @@
 // Device-wide sort and scan used by the device canonicaliser and planner
 // (rocPRIM underneath; kept in one translation unit for compile time).
-// Every function runs on `s`, allocates its own temporary storage and
-// returns once the work is enqueued (temporaries are freed after a
-// stream synchronisation inside the call).
+// Every function runs on `s` and returns once the work is enqueued; its
+// temporary storage is allocated and freed in stream order on `s`.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -21,12 +21,11 @@ void exclusive_scan_t(const T* in, T* out, size_t n, hipStream_t s) {
     if (n) {
         size_t bytes = 0;
         BTC_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, T(0), n, rocprim::plus<T>(), s));
-        DevBuf<unsigned char> tmp;
-        tmp.alloc(bytes);
+        TmpBuf<unsigned char> tmp;
+        tmp.alloc(bytes, s);
         BTC_HIP(rocprim::exclusive_scan(tmp.p, bytes, in, out, T(0), n, rocprim::plus<T>(), s));
         hipLaunchKernelGGL(scan_total<T>, dim3(1), dim3(64), 0, s, in, out, n);
         BTC_CHECK_LAUNCH("scan_total");
-        BTC_HIP(hipStreamSynchronize(s));
     } else {
         BTC_HIP(hipMemsetAsync(out, 0, sizeof(T), s));
     }
@@ -39,10 +38,9 @@ uint64_t* sort_u64(uint64_t* keys, uint64_t* tmp, size_t n, int end_bit, hipStre
     rocprim::double_buffer<uint64_t> db(keys, tmp);
     size_t bytes = 0;
     BTC_HIP(rocprim::radix_sort_keys(nullptr, bytes, db, n, 0u, (unsigned)end_bit, s));
-    DevBuf<unsigned char> t;
-    t.alloc(bytes);
+    TmpBuf<unsigned char> t;
+    t.alloc(bytes, s);
     BTC_HIP(rocprim::radix_sort_keys(t.p, bytes, db, n, 0u, (unsigned)end_bit, s));
-    BTC_HIP(hipStreamSynchronize(s));
     return db.current();
 }
 
@@ -52,10 +50,9 @@ bool sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* kt, uint32_t* vt, 
     rocprim::double_buffer<uint32_t> dk(keys, kt), dv(vals, vt);
     size_t bytes = 0;
     BTC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, dk, dv, n, 0u, (unsigned)end_bit, s));
-    DevBuf<unsigned char> t;
-    t.alloc(bytes);
+    TmpBuf<unsigned char> t;
+    t.alloc(bytes, s);
     BTC_HIP(rocprim::radix_sort_pairs(t.p, bytes, dk, dv, n, 0u, (unsigned)end_bit, s));
-    BTC_HIP(hipStreamSynchronize(s));
     return dk.current() == kt;
 }
 

@@ -2,6 +2,7 @@
 // dispatch rules shared by every path.
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 
 #include "common.h"
 
@@ -51,6 +52,26 @@ float EventTimer::ms() {
     float t = 0.f;
     BTC_HIP(hipEventElapsedTime(&t, a, b));
     return t;
+}
+
+}  // namespace btcx
+
+namespace btcx {
+
+void mem_pool_init() {
+    static std::once_flag once[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+    std::call_once(once[dev], [dev] {
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            return;
+        }
+        uint64_t keep = 1ull << 30;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        (void)hipGetLastError();
+    });
 }
 
 }  // namespace btcx

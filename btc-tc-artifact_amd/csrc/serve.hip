@@ -134,8 +134,25 @@ struct btc_server_s {
             }
             btc_graph g = nullptr;
             btc_plan p = nullptr;
-            int rc = btc_graph_from_coo(job.src.data(), job.dst.data(), (int64_t)job.src.size(), job.n,
-                                        opts.canon_flags ? (uint32_t)opts.canon_flags : BTC_CANON_DEFAULT, &g);
+            const uint32_t cf = opts.canon_flags ? (uint32_t)opts.canon_flags : BTC_CANON_DEFAULT | BTC_CANON_DEVICE;
+            int rc = BTC_OK;
+            if (cf & BTC_CANON_DEVICE) {
+                // canonicalise (and then plan) on the GPU, on this thread's stream
+                rc = btcx::guard([&] {
+                    const size_t nnz = job.src.size();
+                    btcx::DevBuf<int32_t> ds, dd;
+                    ds.alloc(nnz);
+                    dd.alloc(nnz);
+                    if (nnz) {
+                        BTC_HIP(hipMemcpyAsync(ds.p, job.src.data(), nnz * 4, hipMemcpyHostToDevice, s));
+                        BTC_HIP(hipMemcpyAsync(dd.p, job.dst.data(), nnz * 4, hipMemcpyHostToDevice, s));
+                    }
+                    btcx::check_status(btc_graph_from_coo_device(ds.p, dd.p, (int64_t)nnz, job.n, cf & ~BTC_CANON_DEVICE,
+                                                           s, &g));
+                });
+            } else {
+                rc = btc_graph_from_coo(job.src.data(), job.dst.data(), (int64_t)job.src.size(), job.n, cf, &g);
+            }
             std::vector<int32_t>().swap(job.src);
             std::vector<int32_t>().swap(job.dst);
             int64_t est = 0;
@@ -244,7 +261,7 @@ void btc_server_opts_default(btc_server_opts* o) {
     o->hbm_budget_bytes = (int64_t)64 << 30;
     o->max_batch = 512;
     o->prep_threads = 8;
-    o->canon_flags = BTC_CANON_DEFAULT;
+    o->canon_flags = BTC_CANON_DEFAULT | BTC_CANON_DEVICE;
 }
 
 int btc_server_create(const btc_server_opts* opts, btc_server* out) {

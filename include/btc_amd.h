@@ -293,7 +293,8 @@ typedef struct {
     int64_t hbm_budget_bytes; /* resident graphs + plans; default 64 GiB */
     int32_t max_batch;        /* requests per launch; default 512 */
     int32_t prep_threads;     /* host threads that canonicalise and plan; default 8 */
-    int32_t canon_flags;      /* default BTC_CANON_DEFAULT */
+    int32_t canon_flags;      /* default BTC_CANON_DEFAULT | BTC_CANON_DEVICE (canonicalise and plan on the
+                                 GPU, on each preparation thread's stream) */
     int32_t reserved;
 } btc_server_opts;
 

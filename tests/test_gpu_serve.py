@@ -41,10 +41,11 @@ def alone(gpu, s, d, n):
     return gpu.Plan(gpu.Graph.from_coo(s, d, n)).count()[0]
 
 
-def test_server_mix_equals_alone(gpu):
+@pytest.mark.parametrize("canon_flags", [3, 3 | 4])  # host / GPU canonicalisation and planning
+def test_server_mix_equals_alone(gpu, canon_flags):
     gs = make_graphs(gpu, 40, seed=1)
     want = [alone(gpu, *g) for g in gs]
-    sv = gpu.Server(max_batch=16, prep_threads=4)
+    sv = gpu.Server(max_batch=16, prep_threads=4, canon_flags=canon_flags)
     ids = {}
     for k, (s, d, n) in enumerate(gs):
         ids[sv.submit(k, s, d, n)] = k
