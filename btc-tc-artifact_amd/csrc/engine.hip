@@ -14,7 +14,7 @@
 //    loads N_L(j) once into LDS as a bitmap window (else a hash table, else
 //    it stays in HBM for binary search) and streams the prefixes of all
 //    rows i that hold j through it with 16-byte loads.  Hub ids (below
-//    kCoreIds) of rows with many of them are counted by ANDing the row's
+//    the plan's core size C) of rows with many of them are counted by ANDing the row's
 //    core bitmap with the window.  With descending-degree labels this checks
 //    sum_i C(d_L(i), 2) wedges at most, far below the sum of d_L(i) +
 //    d_L(j) a merge would read.
@@ -101,13 +101,14 @@ struct SearchSet {  // binary search of the sorted pivot row in HBM
 struct PlanDev {
     const int32_t* col;
     const uint4* lists;        // per list {s0, s_noncore, s1, row bitmap or kNoBitmap}
-    const uint4* rowbits;      // kCoreWords / 4 uint4 per row bitmap
+    const uint4* rowbits;      // core4 uint4 per row bitmap
     const int32_t* ur;
     const uint4* items;  // 2 x uint4 per item: {t0, t1, eb, ee}, {lo, hi, work, 0}
     uint32_t n_block;    // items [0, n_block) are workgroup items, the rest wave items
     uint32_t n_items;
     unsigned long long* result;
     int32_t* pv;
+    uint32_t core4;  // hub-core ids / 128 (uint4 words of a row bitmap)
 };
 
 // One launch counts one plan (nplans == 1: p0) or several (plans[], with map
@@ -142,7 +143,7 @@ __device__ __forceinline__ void team_sync() {
 // lanes owns lists eb + g, eb + g + NT/G, ...  A list whose row has a core
 // bitmap (and whose pivot is in an LDS bitmap) first ANDs the row's core
 // bitmap with the pivot's window word by word (popcount) and then looks up
-// only the prefix ids >= kCoreIds; any other list looks up its whole clipped
+// only the prefix ids >= C; any other list looks up its whole clipped
 // prefix.  Prefix ids are read with aligned 16-byte loads (4 per lane, U in
 // flight); ids outside the prefix become the set's sentinel.  The next
 // list's record is fetched before the current list is scanned; records are
@@ -170,7 +171,7 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
             // core ids: row bitmap AND pivot window, 128 bits per lane step,
             // over the window's words [bw, 4 we4): the pivot has no ids
             // below bw or above them
-            const uint4* R = a.rowbits + (size_t)L.w * (kCoreWords / 4);
+            const uint4* R = a.rowbits + (size_t)L.w * a.core4;
             for (uint32_t w4 = (bw >> 2) + gl; w4 < we4; w4 += G) {
                 const uint4 r = R[w4];
                 const uint4 pw = *reinterpret_cast<const uint4*>(window + 4 * w4 - bw);
@@ -233,7 +234,7 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
             atomicOr(&lds[(k >> 5) - bw], 1u << (k & 31u));
         }
         team_sync<NT>();
-        const uint32_t we4 = min(kCoreWords / 4, (hi >> 7) + 1u);  // uint4 words up to the pivot's last id
+        const uint32_t we4 = min(a.core4, (hi >> 7) + 1u);  // uint4 words up to the pivot's last id
         acc = scan_lists<NT, G, U, PV>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, L, lds, bw, we4);
         team_sync<NT>();
         for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)a.col[p] >> 5) - bw] = 0;
@@ -305,8 +306,15 @@ __device__ __forceinline__ void flush(uint64_t& acc, unsigned long long* result)
 // with its own 4 KiB slice.  schedule 1 strides statically over the sorted
 // items and prefetches the next item; schedule 0 takes items from an
 // atomic cursor.
+// The default shape (G = 8, U = 1, totals only) fits 64 VGPRs without
+// spilling, which keeps 8 waves per SIMD resident; the others are left to
+// the compiler (forcing them spills to scratch).
 template <int G, int U, bool PV>
-__global__ __launch_bounds__(kThreads) void wedge_kernel(WedgeArgs a) {
+constexpr int kMinWaves = (G == 8 && U == 1 && !PV) ? 8 : 1;
+
+template <int G, int U, bool PV>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<G, U, PV>))) void wedge_kernel(
+    WedgeArgs a) {
     __shared__ __align__(16) uint32_t lds[kLdsWords];  // 16-byte reads of bitmap words
     __shared__ uint32_t s_item;
     const uint32_t tid = threadIdx.x;
@@ -552,19 +560,20 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
     BTC_REQUIRE(ne < (int64_t)0xFFFFFFFFLL, BTC_ERR_OVERFLOW, "too many wedge lists");
 #pragma omp parallel for schedule(static)
     for (int32_t j = 0; j < n; ++j) cnt[j].store((uint32_t)off[j], std::memory_order_relaxed);
-    // core bitmaps: rows with at least kCoreRowMin ids below kCoreIds
+    // core bitmaps: rows with at least kCoreRowMin ids below C
+    const uint32_t C = p->core_ids, CW = C / 32;
     std::vector<uint32_t> cend((size_t)n), rbi((size_t)n, kNoBitmap);
 #pragma omp parallel for schedule(dynamic, 4096)
     for (int32_t i = 0; i < n; ++i)
-        cend[i] = (uint32_t)(std::lower_bound(cl + rp[i], cl + rp[i + 1], (int32_t)kCoreIds) - cl);
+        cend[i] = (uint32_t)(std::lower_bound(cl + rp[i], cl + rp[i + 1], (int32_t)C) - cl);
     uint32_t nbm = 0;
     for (int32_t i = 0; i < n; ++i)
         if (cend[i] - rp[i] >= kCoreRowMin) rbi[i] = nbm++;
-    std::vector<uint32_t> rowbits((size_t)nbm * kCoreWords, 0u);
+    std::vector<uint32_t> rowbits((size_t)nbm * CW, 0u);
 #pragma omp parallel for schedule(dynamic, 1024)
     for (int32_t i = 0; i < n; ++i) {
         if (rbi[i] == kNoBitmap) continue;
-        uint32_t* b = rowbits.data() + (size_t)rbi[i] * kCoreWords;
+        uint32_t* b = rowbits.data() + (size_t)rbi[i] * CW;
         for (int64_t q = rp[i]; q < cend[i]; ++q) b[cl[q] >> 5] |= 1u << (cl[q] & 31);
     }
     std::vector<uint4> lst((size_t)ne);
@@ -581,9 +590,9 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
             const int32_t j = cl[q];
             const uint32_t base = (uint32_t)cl[rp[j]] & ~127u;
             uint32_t sc = s0, rb = kNoBitmap;
-            if (rbi[i] != kNoBitmap && base < kCoreIds) {
+            if (rbi[i] != kNoBitmap && base < C) {
                 const uint32_t core_ids = std::min(s1, cend[i]) > s0 ? std::min(s1, cend[i]) - s0 : 0u;
-                const uint32_t words = core_words((uint32_t)cl[rp[j]], (uint32_t)cl[rp[j + 1] - 1]);
+                const uint32_t words = core_words((uint32_t)cl[rp[j]], (uint32_t)cl[rp[j + 1] - 1], C);
                 if (core_ids * 4 > words + 64) {
                     sc = std::max(s0, cend[i]);
                     rb = rbi[i];
@@ -610,7 +619,7 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
         // cost of a list: ids looked up, plus core words ANDed
         auto cost = [&](int64_t f) -> int64_t {
             const uint4& r = lst[f];
-            return (int64_t)(r.z > r.y ? r.z - r.y : 0) + (r.w != kNoBitmap ? (int64_t)core_words(lo, hi) : 0);
+            return (int64_t)(r.z > r.y ? r.z - r.y : 0) + (r.w != kNoBitmap ? (int64_t)core_words(lo, hi, C) : 0);
         };
         while (e < e1) {
             int64_t w = 0, f = e;
@@ -690,7 +699,7 @@ WedgeFn wedge_fn(const btc_count_opts& o, bool pv, unsigned* grid) {
 
 PlanDev plan_dev(btc_plan_s* p, unsigned long long* d_total, int32_t* d_pv) {
     return PlanDev{p->g->d_col.p, p->list_p, p->rowbits_p, p->urow_p, p->items_p, p->n_block_items, p->nitems,
-                   d_total, d_pv};
+                   d_total, d_pv, p->core_ids / 128};
 }
 
 void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counters, int32_t* d_pv, hipStream_t s) {
@@ -820,6 +829,10 @@ int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_p
         p->g = g;
         p->K = o.tile_k ? o.tile_k : 64;
         p->tau = o.tau >= 0 ? o.tau : btc_default_tau(p->K);
+        BTC_REQUIRE(o.core_ids <= 0 || (o.core_ids % 128 == 0 && (uint32_t)o.core_ids <= btcx::kCoreIdsMax),
+                    BTC_ERR_ARG, "core_ids %d (expected a multiple of 128 up to %u, 0 = default, < 0 = none)",
+                    o.core_ids, btcx::kCoreIdsMax);
+        p->core_ids = o.core_ids == 0 ? btcx::kCoreIdsDefault : o.core_ids < 0 ? 0u : (uint32_t)o.core_ids;
         btcx::check_launch_opts(o);
         p->launch = o;
         btcx::ensure_device(g, s);
@@ -843,6 +856,7 @@ int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_p
         p->scratch.alloc(2);
         p->info.tile_k = p->K;
         p->info.tau = p->tau;
+        p->info.core_ids = p->core_ids;
         int64_t bytes =
             (int64_t)(p->lists.bytes() + p->items_dev.bytes() + p->dense_list.bytes() + p->scratch.bytes());
         if (p->bsr)

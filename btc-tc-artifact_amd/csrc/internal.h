@@ -24,6 +24,7 @@ struct btc_plan_s {
     btc_graph_s* g = nullptr;
     int K = 0;
     int tau = 0;
+    uint32_t core_ids = 0;  // hub-core id range C of the row bitmaps (0 = none)
     btc_count_opts launch{};  // launch-only options (btc_plan_tune)
     std::unique_ptr<btc_bitbsr_s> bsr;
     btcx::DevBuf<uint32_t> dense_list;
@@ -51,18 +52,20 @@ constexpr int kLdsWords = 4096;   // 16 KiB of LDS per workgroup
 constexpr int kWaveWords = 1024;  // per-wave slice of it for wave items
 // Wave items: pivots with at most this many prefix lists.
 constexpr uint32_t kWaveItemLists = 64;
-// Hub core: ids below kCoreIds.  Rows with at least kCoreRowMin core ids
-// keep a kCoreIds-bit bitmap of them (kCoreWords 32-bit words, 512 bytes).
-constexpr uint32_t kCoreIds = 4096;
-constexpr uint32_t kCoreWords = kCoreIds / 32;
+// Hub core: ids below C (a plan's core_ids, a multiple of 128, default
+// kCoreIdsDefault).  Rows with at least kCoreRowMin core ids keep a C-bit
+// bitmap of them (C / 32 words; 256 bytes for C = 2048).  2048 measured
+// fastest on R-MAT s22 among 512..32768.
+constexpr uint32_t kCoreIdsDefault = 2048;
+constexpr uint32_t kCoreIdsMax = 32768;  // = the id span of a wave's LDS bitmap window
 constexpr uint32_t kCoreRowMin = 32;
 constexpr uint32_t kNoBitmap = 0xFFFFFFFFu;
 
-// Words of the core AND for a pivot whose ids span [lo, hi] (lo < kCoreIds):
-// from the 128-aligned window base to the 128-id block holding
-// min(hi, kCoreIds - 1), as the wedge kernel's loop runs.
-__host__ __device__ inline uint32_t core_words(uint32_t lo, uint32_t hi) {
-    const uint32_t h = hi < kCoreIds - 1 ? hi : kCoreIds - 1;
+// Words of the core AND for a pivot whose ids span [lo, hi] (lo < C): from
+// the 128-aligned window base to the 128-id block holding min(hi, C - 1),
+// as the wedge kernel's loop runs.
+__host__ __device__ inline uint32_t core_words(uint32_t lo, uint32_t hi, uint32_t C) {
+    const uint32_t h = hi < C - 1 ? hi : C - 1;
     return ((h >> 7) + 1) * 4 - ((lo & ~127u) >> 5);
 }
 

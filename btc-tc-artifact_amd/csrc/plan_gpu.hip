@@ -149,12 +149,12 @@ __global__ void pivot_offsets_kernel(const uint32_t* __restrict__ key, uint32_t 
     GS_LOOP(j, (int64_t)n + 1) off[j] = lower_bound_dev(key, 0u, ne, (uint32_t)j);
 }
 
-// Core ids (< kCoreIds) per row; rows with at least kCoreRowMin get a bitmap.
+// Core ids (< C) per row; rows with at least kCoreRowMin get a bitmap.
 __global__ void core_rows_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int32_t n,
-                                 uint32_t* __restrict__ cend, uint32_t* __restrict__ has) {
+                                 uint32_t C, uint32_t* __restrict__ cend, uint32_t* __restrict__ has) {
     GS_LOOP(i, n) {
         const uint32_t r0 = rowptr[i];
-        const uint32_t c = lower_bound_dev(col, r0, rowptr[i + 1], (int32_t)kCoreIds);
+        const uint32_t c = lower_bound_dev(col, r0, rowptr[i + 1], (int32_t)C);
         cend[i] = c;
         has[i] = c - r0 >= kCoreRowMin;
     }
@@ -162,12 +162,13 @@ __global__ void core_rows_kernel(const uint32_t* __restrict__ rowptr, const int3
 
 __global__ void core_fill_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                  const uint32_t* __restrict__ cend, const uint32_t* __restrict__ has,
-                                 const uint32_t* __restrict__ rbi, int32_t n, uint32_t* __restrict__ rowbits) {
+                                 const uint32_t* __restrict__ rbi, int32_t n, uint32_t CW,
+                                 uint32_t* __restrict__ rowbits) {
     const int64_t waves = ((int64_t)gridDim.x * kT) >> 6;
     const int lane = threadIdx.x & 63;
     for (int64_t i = (blockIdx.x * (int64_t)kT + threadIdx.x) >> 6; i < n; i += waves) {
         if (!has[i]) continue;
-        uint32_t* b = rowbits + (size_t)rbi[i] * kCoreWords;
+        uint32_t* b = rowbits + (size_t)rbi[i] * CW;
         for (uint32_t p = rowptr[i] + lane; p < cend[i]; p += 64) {
             const uint32_t k = (uint32_t)col[p];
             atomicOr(&b[k >> 5], 1u << (k & 31u));
@@ -180,7 +181,8 @@ __global__ void core_fill_kernel(const uint32_t* __restrict__ rowptr, const int3
 __global__ void records_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                const uint32_t* __restrict__ erow, const uint32_t* __restrict__ val, uint32_t ne,
                                const uint32_t* __restrict__ cend, const uint32_t* __restrict__ has,
-                               const uint32_t* __restrict__ rbi, uint4* __restrict__ lst, int32_t* __restrict__ ur,
+                               const uint32_t* __restrict__ rbi, uint32_t C, uint4* __restrict__ lst,
+                               int32_t* __restrict__ ur,
                                uint64_t* __restrict__ cost, unsigned long long* __restrict__ ncore) {
     uint64_t nc = 0;
     GS_LOOP(e, ne) {
@@ -190,10 +192,10 @@ __global__ void records_kernel(const uint32_t* __restrict__ rowptr, const int32_
         const uint32_t lo = (uint32_t)col[rowptr[j]], hi = (uint32_t)col[rowptr[j + 1] - 1];
         const uint32_t base = lo & ~127u;
         uint32_t sc = s0, rb = kNoBitmap, words = 0;
-        if (has[i] && base < kCoreIds) {
+        if (has[i] && base < C) {
             const uint32_t ce = min(s1, cend[i]);
             const uint32_t core_ids = ce > s0 ? ce - s0 : 0u;
-            const uint32_t w = core_words(lo, hi);
+            const uint32_t w = core_words(lo, hi, C);
             if (core_ids * 4 > w + 64) {
                 sc = max(s0, cend[i]);
                 rb = rbi[i];
@@ -351,25 +353,26 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     cend.alloc((size_t)n, s);
     has.alloc((size_t)n, s);
     rbi.alloc((size_t)n + 1, s);
-    if (n) BTC_LAUNCH(core_rows_kernel, n, s, rowptr, col, n, cend.p, has.p);
+    const uint32_t C = p->core_ids;
+    if (n) BTC_LAUNCH(core_rows_kernel, n, s, rowptr, col, n, C, cend.p, has.p);
     exclusive_scan_u32(has.p, rbi.p, (size_t)n, s);
     const uint32_t nbm = read_dev(rbi.p + n, s);
 
     // one allocation: list records, row bitmaps, list rows; items apart
-    const size_t lb = (size_t)ne * sizeof(uint4), bb = (size_t)nbm * kCoreWords * 4, rb = (size_t)ne * 4;
+    const size_t lb = (size_t)ne * sizeof(uint4), bb = (size_t)nbm * (C / 32) * 4, rb = (size_t)ne * 4;
     p->lists.alloc(lb + bb + rb);
     uint4* lst = reinterpret_cast<uint4*>(p->lists.p);
     uint32_t* rowbits = reinterpret_cast<uint32_t*>(p->lists.p + lb);
     int32_t* ur = reinterpret_cast<int32_t*>(p->lists.p + lb + bb);
     if (bb) {
         BTC_HIP(hipMemsetAsync(rowbits, 0, bb, s));
-        BTC_LAUNCH(core_fill_kernel, (int64_t)n * 64, s, rowptr, col, cend.p, has.p, rbi.p, n, rowbits);
+        BTC_LAUNCH(core_fill_kernel, (int64_t)n * 64, s, rowptr, col, cend.p, has.p, rbi.p, n, C / 32, rowbits);
     }
     TmpBuf<uint64_t> cost, P;
     cost.alloc(ne, s);
     P.alloc((size_t)ne + 1, s);
     if (ne)
-        BTC_LAUNCH(records_kernel, ne, s, rowptr, col, erow.p, val, ne, cend.p, has.p, rbi.p, lst, ur, cost.p,
+        BTC_LAUNCH(records_kernel, ne, s, rowptr, col, erow.p, val, ne, cend.p, has.p, rbi.p, C, lst, ur, cost.p,
                    st.p + 3);
     exclusive_scan_u64(cost.p, P.p, ne, s);
     erow.release();

@@ -194,6 +194,8 @@ typedef struct {
                              2 = i8 MFMA (v_mfma_i32_16x16x64_i8) on 0/1-unpacked tile rows */
     int32_t planner;      /* work lists built by: 0 = auto (the GPU for graphs canonicalised there or with
                              more than 2^20 edges), 1 = host threads, 2 = GPU kernels */
+    int32_t core_ids;     /* hub-core id range C of the rows' core bitmaps: a multiple of 128 up to
+                             32768; 0 = 2048, < 0 = no core bitmaps */
 } btc_count_opts;
 
 typedef struct btc_plan_s* btc_plan;
@@ -209,6 +211,7 @@ typedef struct {
     int64_t stream_bytes;      /* bytes the wedge kernel reads: prefixes, pivot rows, lists, items */
     int64_t core_lists;        /* lists whose hub-core ids are counted by bitmap AND + popcount */
     int64_t core_rows;         /* rows holding a hub-core bitmap */
+    int64_t core_ids;          /* hub-core id range C of the plan (0 = no core bitmaps) */
     int32_t tile_k;
     int32_t tau;
 } btc_plan_info;

@@ -155,7 +155,7 @@ def test_plan_tune_rejects_bad_options(gpu):
 
 @pytest.mark.parametrize("name", ["rmat15", "planted", "complete", "messy"])
 def test_engine_core_bitmap_path(gpu, orc, name):
-    """Lists whose rows hold many hub-core ids (< 4096) are counted by a
+    """Lists whose rows hold many hub-core ids (< the core size) are counted by a
     bitmap AND + popcount for the core part; the counts match the oracle and
     the path is really taken."""
     s, d, n = case(name, gpu)
@@ -169,6 +169,33 @@ def test_engine_core_bitmap_path(gpu, orc, name):
     for G in (8, 16, 32):
         p.tune(group_lanes=G, unroll=2)
         assert p.count()[0] == want
+
+
+@pytest.mark.parametrize("planner", [1, 2])
+@pytest.mark.parametrize("core_ids", [-1, 128, 1024, 8192, 32768])
+def test_engine_core_size(gpu, orc, core_ids, planner):
+    """Any hub-core size (none, one 128-id block, up to a wave's window span)
+    gives the same counts, with both planners."""
+    s, d, n = case("rmat15", gpu)
+    g = gpu.Graph.from_coo(s, d, n)
+    want, want_pv = oracle_counts(orc, g)
+    p = gpu.Plan(g, tau=100000, core_ids=core_ids, planner=planner)
+    info = p.info
+    assert info["core_ids"] == max(core_ids, 0)
+    assert (info["core_lists"] > 0) == (core_ids > 0)
+    t, pv, _ = p.count(per_vertex=True)
+    assert t == want
+    np.testing.assert_array_equal(pv, want_pv)
+    for lookup in (1, 2):
+        p.tune(lookup=lookup)
+        assert p.count()[0] == want
+
+
+def test_engine_core_size_rejects(gpu):
+    g = gpu.Graph.from_coo(*case("karate", gpu)[:2])
+    for bad in (100, 65536, 32768 + 128):
+        with pytest.raises(gpu.BtcError):
+            gpu.Plan(g, core_ids=bad)
 
 
 @pytest.mark.parametrize("flags", [0, 1, 2, 3])
