@@ -223,10 +223,27 @@ def batched_leg(btc, count, steps, sample):
     exact = all(int(per[k]) == single[j] for j, k in enumerate(idx))
     log("one launch per graph: %.3f ms per graph (%.0f graphs/s) on %d graphs; exact %s"
         % (t_single * 1e3, 1 / t_single, len(plans), exact))
+    # the same with every count replayed from a captured HIP graph
+    for p in plans:
+        p.capture()
+    t1 = time.perf_counter()
+    replay = [p.count()[0] for p in plans]
+    t_replay = (time.perf_counter() - t1) / len(plans)
+    exact = exact and replay == single
+    batch.capture()
+    wall_r = []
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        total_r, per_r, _ = batch.count()
+        wall_r.append(time.perf_counter() - t1)
+    exact = exact and total_r == total and bool((per_r == per).all())
+    log("replayed: %.3f ms per single graph, %.3f ms per batch launch" % (t_replay * 1e3,
+                                                                         statistics.median(wall_r) * 1e3))
     return {"graphs": count, "edges": info["m"], "ms_per_launch": round(t_batch * 1e3, 3),
             "graphs_per_s": count / t_batch, "edges_per_s": info["m"] / t_batch,
             "single_launch_graphs_per_s": 1.0 / t_single, "speedup_vs_single": round(t_single * count / t_batch, 2),
             "single_sample": len(plans), "exact_vs_single": exact, "triangles": int(total),
+            "single_replay_graphs_per_s": 1.0 / t_replay, "ms_per_launch_replay": round(statistics.median(wall_r) * 1e3, 3),
             "build_s": round(t_build, 2), "canon_s": round(t_canon, 2), "work_items": info["plan"]["work_items"]}
 
 

@@ -133,6 +133,8 @@ _SIGS = {
     "btc_plan_count": (C.c_int, [vp, vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
     "btc_plan_last_timing": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "btc_plan_free": (None, [vp]),
+    "btc_plan_capture": (C.c_int, [vp, vp]),
+    "btc_batch_capture": (C.c_int, [vp, vp]),
     "btc_count": (C.c_int, [vp, C.POINTER(CountOpts), vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
     "btc_count_cpu": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_uint64), vp]),
     "btc_batch_create": (C.c_int, [vp, C.c_int32, C.POINTER(CountOpts), vp, C.POINTER(vp)]),
@@ -390,8 +392,13 @@ class Plan:
         _check(lib().btc_plan_count(self._h, stream, C.byref(total), _ptr(pv), C.byref(ms)))
         return total.value, pv, ms.value
 
+    def capture(self, stream=None):
+        """Captures the totals-only count into a HIP graph that later
+        count() calls without per_vertex replay."""
+        _check(lib().btc_plan_capture(self._h, stream))
+
     def last_timing(self):
-        """(dense-tile kernel ms, wedge kernel ms) of the last count."""
+        """(dense-tile kernel ms, wedge kernel ms) of the last count (-1 after a replay)."""
         dm, wm = C.c_float(0), C.c_float(0)
         _check(lib().btc_plan_last_timing(self._h, C.byref(dm), C.byref(wm)))
         return dm.value, wm.value
@@ -436,6 +443,10 @@ class Batch:
     def tune(self, **opts):
         o = make_opts(**opts)
         _check(lib().btc_batch_tune(self._h, C.byref(o)))
+
+    def capture(self, stream=None):
+        """Captures the batch count into a HIP graph that later count() calls replay."""
+        _check(lib().btc_batch_capture(self._h, stream))
 
     def count(self, stream=None):
         """Returns (total, per-graph uint64 array, kernel_ms)."""

@@ -27,9 +27,12 @@ struct btc_batch_s {
     btcx::DevBuf<int32_t> d_pv;
     btcx::DevBuf<unsigned long long> d_pg;
     std::vector<uint64_t> h_pg;
+    btcx::CapturedGraph count_graph;  // zero + count + per-graph sums, replayed when captured
     ~btc_batch_s() {
         if (plan) btc_plan_free(plan);
     }
+    // dense-tile + wedge kernels with per-vertex sums, then per-graph sums
+    void enqueue_count(hipStream_t s);
 };
 
 namespace btcx {
@@ -51,6 +54,12 @@ __global__ __launch_bounds__(256) void graph_sums_kernel(const int32_t* __restri
 
 }  // namespace
 }  // namespace btcx
+
+void btc_batch_s::enqueue_count(hipStream_t s) {
+    btcx::plan_enqueue(plan, s, d_pv.p, nullptr, nullptr);
+    hipLaunchKernelGGL(btcx::graph_sums_kernel, dim3((unsigned)count), dim3(256), 0, s, d_pv.p, d_voff.p, d_pg.p);
+    BTC_CHECK_LAUNCH("graph_sums_kernel");
+}
 
 extern "C" {
 
@@ -122,6 +131,18 @@ int btc_batch_tune(btc_batch b, const btc_count_opts* opts) {
     return btcx::guard([&] {
         BTC_REQUIRE(b, BTC_ERR_ARG, "NULL handle");
         btcx::check_status(btc_plan_tune(b->plan, opts));
+        b->count_graph.reset();
+    });
+}
+
+int btc_batch_capture(btc_batch b, void* stream) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(b, BTC_ERR_ARG, "NULL handle");
+        (void)stream;
+        b->count_graph.capture([&](hipStream_t cs) {
+            btcx::launch_zero_u32(reinterpret_cast<uint32_t*>(b->d_pv.p), b->d_pv.n, cs);
+            b->enqueue_count(cs);
+        });
     });
 }
 
@@ -129,12 +150,15 @@ int btc_batch_count(btc_batch b, void* stream, uint64_t* per_graph, uint64_t* to
     return btcx::guard([&] {
         BTC_REQUIRE(b && total, BTC_ERR_ARG, "NULL argument");
         hipStream_t s = btcx::as_stream(stream);
-        BTC_HIP(hipMemsetAsync(b->d_pv.p, 0, b->d_pv.bytes(), s));
-        btcx::EventTimer tm(kernel_ms != nullptr), tw(kernel_ms != nullptr);
-        btcx::plan_enqueue(b->plan, s, b->d_pv.p, &tm, &tw);
-        hipLaunchKernelGGL(btcx::graph_sums_kernel, dim3((unsigned)b->count), dim3(256), 0, s, b->d_pv.p,
-                           b->d_voff.p, b->d_pg.p);
-        BTC_CHECK_LAUNCH("graph_sums_kernel");
+        btcx::EventTimer tm(kernel_ms != nullptr);
+        if (b->count_graph.exec) {
+            tm.start(s);
+            b->count_graph.launch(s);
+        } else {
+            btcx::launch_zero_u32(reinterpret_cast<uint32_t*>(b->d_pv.p), b->d_pv.n, s);
+            tm.start(s);
+            b->enqueue_count(s);
+        }
         if (kernel_ms) tm.stop(s);  // both kernels and the per-graph sums
         unsigned long long t = 0;
         BTC_HIP(hipMemcpyAsync(&t, b->plan->scratch.p, 8, hipMemcpyDeviceToHost, s));

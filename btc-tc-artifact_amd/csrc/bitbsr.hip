@@ -557,6 +557,19 @@ void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_
     }
 }
 
+namespace {
+__global__ void zero_u32_kernel(uint32_t* __restrict__ p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+}  // namespace
+
+void launch_zero_u32(uint32_t* p, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(grid), dim3(256), 0, s, p, n);
+    BTC_CHECK_LAUNCH("zero_u32_kernel");
+}
+
 void launch_tile_popcount(const btc_bitbsr_s* b, uint32_t* d_pop, hipStream_t s) {
     if (b->nblk == 0) return;
     const unsigned grid = (b->nblk + 3) / 4;

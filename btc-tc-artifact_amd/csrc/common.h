@@ -135,6 +135,52 @@ struct TmpBuf {
     }
 };
 
+// A HIP graph captured from a sequence of kernel launches, replayed with
+// one hipGraphLaunch.  Captured work contains kernels only (buffers are
+// cleared by fill kernels, not memset nodes).
+struct CapturedGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    CapturedGraph() = default;
+    CapturedGraph(const CapturedGraph&) = delete;
+    CapturedGraph& operator=(const CapturedGraph&) = delete;
+    ~CapturedGraph() { reset(); }
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    // Captures what enqueue(stream) launches on a private stream.
+    template <class F>
+    void capture(F&& enqueue) {
+        reset();
+        hipStream_t cs = nullptr;
+        BTC_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            try {
+                enqueue(cs);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(cs, &g);
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipStreamDestroy(cs);
+                throw;
+            }
+            e = hipStreamEndCapture(cs, &graph);
+        }
+        (void)hipStreamDestroy(cs);
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            reset();
+            (void)hipGetLastError();
+            fail(BTC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        }
+    }
+    void launch(hipStream_t s) { BTC_HIP(hipGraphLaunch(exec, s)); }
+};
+
 // Kernel timing with HIP events recorded on the launch stream.
 struct EventTimer {
     hipEvent_t a = nullptr, b = nullptr;

@@ -765,7 +765,7 @@ void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, M
 void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, EventTimer* tw) {
     BTC_REQUIRE(!d_pv || p->g->max_row <= 65536, BTC_ERR_OVERFLOW,
                 "longest row %d > 65536: per-vertex counts could overflow int32", p->g->max_row);
-    BTC_HIP(hipMemsetAsync(p->scratch.p, 0, 2 * sizeof(unsigned long long), s));
+    launch_zero_u32(reinterpret_cast<uint32_t*>(p->scratch.p), 4, s);
     unsigned long long* d_total = p->scratch.p;
     uint32_t* d_counters = reinterpret_cast<uint32_t*>(p->scratch.p + 1);
     if (tm) tm->start(s);
@@ -781,6 +781,22 @@ namespace {
 
 void plan_count(btc_plan_s* p, hipStream_t s, uint64_t* total, int32_t* per_vertex, float* kernel_ms) {
     btc_graph_s* g = p->g;
+    if (!per_vertex && p->count_graph.exec) {
+        // replay of the captured totals-only count
+        EventTimer tm(kernel_ms != nullptr);
+        tm.start(s);
+        p->count_graph.launch(s);
+        tm.stop(s);
+        unsigned long long h = 0;
+        BTC_HIP(hipMemcpyAsync(&h, p->scratch.p, 8, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipStreamSynchronize(s));
+        if (kernel_ms) {
+            *kernel_ms = tm.ms();
+            p->last_dense_ms = p->last_wedge_ms = -1.f;  // not split inside a replay
+        }
+        *total = h;
+        return;
+    }
     DevBuf<int32_t> pv;
     if (per_vertex) {
         pv.alloc((size_t)g->n);
@@ -872,6 +888,15 @@ int btc_plan_tune(btc_plan p, const btc_count_opts* opts) {
         BTC_REQUIRE(p && opts, BTC_ERR_ARG, "NULL argument");
         btcx::check_launch_opts(*opts);
         p->launch = *opts;
+        p->count_graph.reset();  // a captured count used the old launch shape
+    });
+}
+
+int btc_plan_capture(btc_plan p, void* stream) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(p, BTC_ERR_ARG, "NULL handle");
+        (void)stream;  // captured on a private stream; replays run on the count's stream
+        p->count_graph.capture([&](hipStream_t cs) { btcx::plan_enqueue(p, cs, nullptr, nullptr, nullptr); });
     });
 }
 

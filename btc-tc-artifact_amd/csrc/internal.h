@@ -41,6 +41,7 @@ struct btc_plan_s {
     btc_plan_info info{};
     float last_dense_ms = 0.f;
     float last_wedge_ms = 0.f;
+    btcx::CapturedGraph count_graph;  // totals-only count, replayed when captured
 };
 
 namespace btcx {
@@ -104,6 +105,9 @@ void bitbsr_build(btc_graph_s* g, int K, hipStream_t s, btc_bitbsr_s* b);
 void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles,
                        bool binsearch, bool mfma, int tau, unsigned long long* d_result, int32_t* d_pv,
                        hipStream_t s);
+
+// p[0, n) = 0 by a kernel (safe inside captured graphs).
+void launch_zero_u32(uint32_t* p, size_t n, hipStream_t s);
 
 // Per-tile popcounts (device, nblk entries).
 void launch_tile_popcount(const btc_bitbsr_s* b, uint32_t* d_pop, hipStream_t s);

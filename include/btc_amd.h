@@ -226,6 +226,13 @@ int btc_plan_count(btc_plan p, void* stream, uint64_t* total, int32_t* per_verte
 /* Event times of the last btc_plan_count with kernel_ms != NULL, split by
  * kernel: dense-tile kernel and wedge kernel (0 when not launched). */
 int btc_plan_last_timing(btc_plan p, float* dense_ms, float* wedge_ms);
+/* Captures the totals-only count of p (zero fill, dense-tile kernel, wedge
+ * kernel) into a HIP graph; later btc_plan_count calls without per_vertex
+ * replay it with one hipGraphLaunch on their stream (kernel_ms then times
+ * the replay and btc_plan_last_timing reports -1 for the split).
+ * btc_plan_tune drops the graph.  No reference counterpart (the reference
+ * launches every kernel from the host each time). */
+int btc_plan_capture(btc_plan p, void* stream);
 void btc_plan_free(btc_plan p);
 /* One-shot: plan + count + free. */
 int btc_count(btc_graph g, const btc_count_opts* opts, void* stream, uint64_t* total,
@@ -277,6 +284,9 @@ int btc_batch_get_info(btc_batch b, btc_batch_info* info);
 int btc_batch_tune(btc_batch b, const btc_count_opts* opts);
 /* per_graph: host uint64[count] or NULL; total: sum over the batch. */
 int btc_batch_count(btc_batch b, void* stream, uint64_t* per_graph, uint64_t* total, float* kernel_ms);
+/* Captures a batch count (zero fill, kernels, per-graph sums) into a HIP
+ * graph that later btc_batch_count calls replay; btc_batch_tune drops it. */
+int btc_batch_capture(btc_batch b, void* stream);
 void btc_batch_free(btc_batch b);
 
 /* -------------------------------------------------------------- serving
