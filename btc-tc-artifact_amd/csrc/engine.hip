@@ -40,6 +40,9 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+// Plans with at least this many wedge checks (a wedge kernel of roughly
+// 20 us or more) run the dense-tile kernel beside it on a second stream.
+constexpr int64_t kOverlapChecks = 1 << 24;
 
 __device__ __forceinline__ uint32_t hslot(uint32_t key, int shift) { return (key * 2654435761u) >> shift; }
 
@@ -768,6 +771,28 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
     launch_zero_u32(reinterpret_cast<uint32_t*>(p->scratch.p), 4, s);
     unsigned long long* d_total = p->scratch.p;
     uint32_t* d_counters = reinterpret_cast<uint32_t*>(p->scratch.p + 1);
+    if (p->n_dense && p->nitems && p->info.wedge_checks >= kOverlapChecks) {
+        // the latency-bound dense-tile kernel runs on a second stream while
+        // the HBM-bound wedge kernel runs on s; both add into d_total (for
+        // small plans the fork and join would cost more than they hide)
+        if (!p->side) {
+            BTC_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+            BTC_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+            BTC_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+        }
+        if (tm) tm->start(s);
+        BTC_HIP(hipEventRecord(p->ev_fork, s));
+        BTC_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+        if (tw) tw->start(s);
+        launch_wedge(p, d_total, d_counters, d_pv, s);
+        if (tw) tw->stop(s);
+        launch_tile_count(p->bsr.get(), p->dense_list.p, p->n_dense, false, p->launch.dense_kernel == 2, -1,
+                          d_total, d_pv, p->side);
+        BTC_HIP(hipEventRecord(p->ev_join, p->side));
+        BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
+        if (tm) tm->stop(s);
+        return;
+    }
     if (tm) tm->start(s);
     if (p->n_dense) launch_tile_count(p->bsr.get(), p->dense_list.p, p->n_dense, false, p->launch.dense_kernel == 2, -1,
                                         d_total, d_pv, s);

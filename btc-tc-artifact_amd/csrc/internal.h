@@ -42,6 +42,18 @@ struct btc_plan_s {
     float last_dense_ms = 0.f;
     float last_wedge_ms = 0.f;
     btcx::CapturedGraph count_graph;  // totals-only count, replayed when captured
+    // second stream on which the dense-tile kernel runs beside the wedge kernel
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    btc_plan_s() = default;
+    btc_plan_s(const btc_plan_s&) = delete;
+    btc_plan_s& operator=(const btc_plan_s&) = delete;
+    ~btc_plan_s() {
+        count_graph.reset();
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
+    }
 };
 
 namespace btcx {

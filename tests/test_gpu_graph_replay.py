@@ -50,3 +50,24 @@ def test_batch_replay_equals_eager(gpu):
     t, per, _ = b.count()
     assert t == t0
     np.testing.assert_array_equal(per, per0)
+
+
+def test_overlapped_dense_and_wedge_kernels(gpu, orc):
+    """A plan large enough to run the dense-tile kernel on a second stream
+    beside the wedge kernel: eager, per-vertex and replayed counts agree
+    with the oracle."""
+    s, d = gpu.gen_rmat(18, 16, seed=18)
+    g = gpu.Graph.from_coo(s, d, 1 << 18)
+    want, want_pv = oracle_counts(orc, g)
+    p = gpu.Plan(g)
+    info = p.info
+    assert info["dense_tiles"] > 0 and info["wedge_checks"] >= 1 << 24
+    for _ in range(3):
+        assert p.count()[0] == want
+        t, pv, _ = p.count(per_vertex=True)
+        assert t == want
+        np.testing.assert_array_equal(pv, want_pv)
+    p.capture()
+    for _ in range(3):
+        assert p.count()[0] == want
+        p.count(per_vertex=True)
