@@ -189,10 +189,58 @@ def dense_leg(btc, runs):
     return out
 
 
-def batched_leg(btc, count, steps, sample):
+def configs_leg(btc, runs):
+    """The other single-graph configurations: Erdos-Renyi n = 16384 with
+    average degree 16, the 27-point stencil 64^3 and the planted-community
+    graph 2^17.  Each: the engine (degree-ordered L, plan resident), the
+    reference-semantics kernels at the Lite tile width on the natural
+    (compacted input) order, and the reference CPU verifier; all three
+    totals must agree (and the stencil's must equal its closed form)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    out = {}
+    for name in ("er", "stencil", "planted"):
+        src, dst, n_in, cfg = make_workload(btc, name)
+        g = btc.Graph.from_coo(src, dst, n_in, btc.CANON_DEFAULT | btc.CANON_DEVICE)
+        p = btc.Plan(g)
+        p.count()
+        ks = []
+        for _ in range(runs):
+            t, _, ms = p.count()
+            ks.append(ms)
+        gn = btc.Graph.from_coo(src, dst, n_in, btc.CANON_COMPACT)
+        K = btc.lite_tile_k(gn.n, gn.m)
+        b = btc.BitBSR(gn, K)
+        b.count()
+        pk = []
+        for _ in range(runs):
+            tp, _, ms = b.count()
+            pk.append(ms)
+        rp, col = g.csr()
+        threads = int(os.environ.get("BTC_CPU_THREADS", min(16, os.cpu_count() or 1)))
+        t0 = time.perf_counter()
+        tc = oracle.ref_cpu_tc(rp, col, threads) if oracle.ref_available() else oracle.tc_merge(rp, col, threads)
+        cpu_s = time.perf_counter() - t0
+        exact = t == tp == int(tc) and (name != "stencil" or t == 11049696)
+        km, pm = statistics.median(ks), statistics.median(pk)
+        out[cfg["workload"]] = {"n": g.n, "m": g.m, "triangles": int(t), "exact": exact,
+                                "engine_ms": round(km, 4), "engine_edges_per_s": g.m / (km * 1e-3),
+                                "plain_tile_k": K, "plain_ms": round(pm, 4), "plain_edges_per_s": gn.m / (pm * 1e-3),
+                                "engine_speedup": round(pm / km, 2), "cpu_s": round(cpu_s, 4),
+                                "cpu_edges_per_s": g.m / cpu_s, "cpu_threads": threads}
+        log("config %s: %s" % (cfg["workload"], out[cfg["workload"]]))
+        assert exact, ("counts differ", name, t, tp, tc)
+        p.close()
+        b.close()
+    return out
+
+
+def batched_leg(btc, count, steps, sample, tune_ms):
     """4096 graphs (40 % ER, 40 % R-MAT, 20 % 9-point stencils, n log-uniform
-    in [256, 16384]) counted by one persistent launch, against one launch per
-    graph on a sample of them (each with its plan resident)."""
+    in [256, 16384]) counted by one persistent launch with the options the
+    autotuner picks for the batch, against one launch per graph on a sample
+    of them (each with its plan resident)."""
     t0 = time.perf_counter()
     work = btc.batch_workload(count, seed=1)
     graphs = [btc.Graph.from_coo(s, d, n) for s, d, n, _ in work]
@@ -209,9 +257,24 @@ def batched_leg(btc, count, steps, sample):
         t1 = time.perf_counter()
         total, per, ms = batch.count()
         wall.append(time.perf_counter() - t1)
+    t_default = statistics.median(wall)
+    log("batched, default options: %d graphs, %d edges, %.3f ms per launch (%.0f graphs/s); build %.1f s, canon %.1f s"
+        % (count, info["m"], t_default * 1e3, count / t_default, t_build, t_canon))
+    tuned = None
+    if tune_ms > 0:
+        t0 = time.perf_counter()
+        tuned, tms = batch.autotune(budget_ms=tune_ms)
+        t_tune = time.perf_counter() - t0
+        wall = []
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            total_t, per_t, _ = batch.count()
+            wall.append(time.perf_counter() - t1)
+            assert total_t == total and bool((per_t == per).all()), "tuned batch count differs"
+        log("batched, tuned %s in %.1f s: %.3f ms per launch" % (tuned, t_tune, statistics.median(wall) * 1e3))
+        tuned = {k: tuned[k] for k in ("tile_k", "tau", "group_lanes", "unroll", "waves_per_eu", "dense_kernel")}
+        tuned["tune_s"] = round(t_tune, 1)
     t_batch = statistics.median(wall)
-    log("batched: %d graphs, %d edges, %.3f ms per launch (%.0f graphs/s); build %.1f s, canon %.1f s"
-        % (count, info["m"], t_batch * 1e3, count / t_batch, t_build, t_canon))
     # one launch per graph, plans resident, on an evenly spaced sample
     idx = list(range(0, count, max(1, count // sample)))[:sample]
     plans = [btc.Plan(graphs[k]) for k in idx]
@@ -239,7 +302,8 @@ def batched_leg(btc, count, steps, sample):
     exact = exact and total_r == total and bool((per_r == per).all())
     log("replayed: %.3f ms per single graph, %.3f ms per batch launch" % (t_replay * 1e3,
                                                                          statistics.median(wall_r) * 1e3))
-    return {"graphs": count, "edges": info["m"], "ms_per_launch": round(t_batch * 1e3, 3),
+    return {"graphs": count, "edges": info["m"], "ms_per_launch": round(t_batch * 1e3, 3), "tuned_options": tuned,
+            "default_ms_per_launch": round(t_default * 1e3, 3),
             "graphs_per_s": count / t_batch, "edges_per_s": info["m"] / t_batch,
             "single_launch_graphs_per_s": 1.0 / t_single, "speedup_vs_single": round(t_single * count / t_batch, 2),
             "single_sample": len(plans), "exact_vs_single": exact, "triangles": int(total),
@@ -342,9 +406,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=4096, help="graphs in the batched leg (0 = skip)")
     ap.add_argument("--batch-sample", type=int, default=256, help="graphs timed one launch each")
+    ap.add_argument("--batch-tune-ms", type=int, default=3000, help="autotuner budget for the batch (0 = defaults)")
     ap.add_argument("--serve", type=int, default=600, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--dense", type=int, default=5, help="timed runs per case in the dense-tile leg (0 = skip)")
     ap.add_argument("--host-path", type=int, default=1, help="also time host canonicalisation and planning")
+    ap.add_argument("--configs", type=int, default=5, help="timed runs per case of the ER / stencil / planted leg")
     ap.add_argument("--serve-host-prep", type=int, default=1,
                     help="also run the serving leg with canonicalisation and planning on host threads")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -463,11 +529,13 @@ def main():
         log("cpu baseline: %s" % cpu)
         assert cpu["matches_gpu"], "GPU count differs from the CPU baseline"
 
-    batched = batched_leg(btc, args.batch, max(3, args.steps // 2), args.batch_sample) if args.batch > 0 else None
+    batched = (batched_leg(btc, args.batch, max(3, args.steps // 2), args.batch_sample, args.batch_tune_ms)
+               if args.batch > 0 else None)
     serving = serving_leg(btc, args.serve) if args.serve > 0 else None
     serving_host = (serving_leg(btc, args.serve, canon_flags=btc.CANON_DEFAULT)
                     if args.serve > 0 and args.serve_host_prep else None)
     dense = dense_leg(btc, args.dense) if args.dense > 0 else None
+    configs = configs_leg(btc, args.configs) if args.configs > 0 else None
 
     config.update({"n": gi["n"], "m": gi["m"], "ordering": "degree-descending", "tile_k": pi["tile_k"],
                    "tau": pi["tau"], "dense_tiles": pi["dense_tiles"], "work_items": pi["work_items"]})
@@ -479,6 +547,7 @@ def main():
         "kernel_ms_median": statistics.median(kms), "canon_ms": round(canon_ms, 1), "plan_ms": round(build_ms, 1),
         "roofline": roofline, "cpu_baseline": cpu, "plain_hybrid": plain, "end_to_end": e2e,
         "batched": batched, "serving": serving, "serving_host_prep": serving_host, "dense_tiles": dense,
+        "other_configs": configs,
     }
     print(json.dumps(out), flush=True)
 

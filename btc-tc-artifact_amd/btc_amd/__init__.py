@@ -135,6 +135,7 @@ _SIGS = {
     "btc_plan_free": (None, [vp]),
     "btc_plan_capture": (C.c_int, [vp, vp]),
     "btc_batch_capture": (C.c_int, [vp, vp]),
+    "btc_batch_autotune": (C.c_int, [vp, C.POINTER(TuneOpts), vp, C.POINTER(CountOpts), C.POINTER(C.c_float)]),
     "btc_count": (C.c_int, [vp, C.POINTER(CountOpts), vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
     "btc_count_cpu": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_uint64), vp]),
     "btc_batch_create": (C.c_int, [vp, C.c_int32, C.POINTER(CountOpts), vp, C.POINTER(vp)]),
@@ -447,6 +448,19 @@ class Batch:
     def capture(self, stream=None):
         """Captures the batch count into a HIP graph that later count() calls replay."""
         _check(lib().btc_batch_capture(self._h, stream))
+
+    def autotune(self, budget_ms=None, reps=None, tile_ks_mask=None, stream=None):
+        """Tunes the batch's options on the device and re-plans it; returns
+        (best options dict, ms)."""
+        t = TuneOpts()
+        lib().btc_tune_opts_default(C.byref(t))
+        for k, v in dict(budget_ms=budget_ms, reps=reps, tile_ks_mask=tile_ks_mask).items():
+            if v is not None:
+                setattr(t, k, v)
+        best = CountOpts()
+        ms = C.c_float(0)
+        _check(lib().btc_batch_autotune(self._h, C.byref(t), stream, C.byref(best), C.byref(ms)))
+        return _opts_dict(best), ms.value
 
     def count(self, stream=None):
         """Returns (total, per-graph uint64 array, kernel_ms)."""

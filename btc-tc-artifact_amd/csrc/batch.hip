@@ -135,6 +135,20 @@ int btc_batch_tune(btc_batch b, const btc_count_opts* opts) {
     });
 }
 
+int btc_batch_autotune(btc_batch b, const btc_tune_opts* topts, void* stream, btc_count_opts* best,
+                       float* best_ms) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(b && best, BTC_ERR_ARG, "NULL argument");
+        btcx::check_status(btc_autotune(&b->graph, topts, stream, best, best_ms));
+        // re-plan the batch with the winner (tile width and tau need a new plan)
+        btc_plan p = nullptr;
+        btcx::check_status(btc_plan_create(&b->graph, best, stream, &p));
+        btc_plan_free(b->plan);
+        b->plan = p;
+        b->count_graph.reset();
+    });
+}
+
 int btc_batch_capture(btc_batch b, void* stream) {
     return btcx::guard([&] {
         BTC_REQUIRE(b, BTC_ERR_ARG, "NULL handle");
