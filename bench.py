@@ -311,7 +311,7 @@ def batched_leg(btc, count, steps, sample, tune_ms):
             "build_s": round(t_build, 2), "canon_s": round(t_canon, 2), "work_items": info["plan"]["work_items"]}
 
 
-def serving_leg(btc, requests, seed=1, canon_flags=None, prep_threads=None):
+def serving_leg(btc, requests, seed=1, canon_flags=None, prep_threads=None, group_graphs=None):
     """Poisson stream of exact-count queries: R-MAT graphs (edge factor 16)
     with n log-normal (median 2^13, sigma 1 in ln n, capped at 2^20), 30 %
     of requests repeat a graph submitted earlier.  A closed burst measures
@@ -338,7 +338,7 @@ def serving_leg(btc, requests, seed=1, canon_flags=None, prep_threads=None):
         return reqs
 
     def run(reqs, qps):
-        sv = btc.Server(canon_flags=canon_flags, prep_threads=prep_threads)
+        sv = btc.Server(canon_flags=canon_flags, prep_threads=prep_threads, group_graphs=group_graphs)
         gaps = rng.exponential(1.0 / qps, len(reqs)) if qps else np.zeros(len(reqs))
         t_start = time.perf_counter()
 

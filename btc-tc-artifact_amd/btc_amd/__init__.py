@@ -77,7 +77,7 @@ class PlanInfo(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_batch", C.c_int32), ("prep_threads", C.c_int32),
-                ("canon_flags", C.c_int32), ("reserved", C.c_int32)]
+                ("canon_flags", C.c_int32), ("group_graphs", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -520,11 +520,12 @@ def batch_workload(count=4096, seed=1, n_lo=256, n_hi=16384):
 class Server:
     """Request loop with continuous batching (btc_server_*)."""
 
-    def __init__(self, hbm_budget_bytes=None, max_batch=None, prep_threads=None, canon_flags=None):
+    def __init__(self, hbm_budget_bytes=None, max_batch=None, prep_threads=None, canon_flags=None,
+                 group_graphs=None):
         o = ServerOpts()
         lib().btc_server_opts_default(C.byref(o))
         for k, v in dict(hbm_budget_bytes=hbm_budget_bytes, max_batch=max_batch, prep_threads=prep_threads,
-                         canon_flags=canon_flags).items():
+                         canon_flags=canon_flags, group_graphs=group_graphs).items():
             if v is not None:
                 setattr(o, k, v)
         h = vp()

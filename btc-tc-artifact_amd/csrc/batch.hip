@@ -55,6 +55,33 @@ __global__ __launch_bounds__(256) void graph_sums_kernel(const int32_t* __restri
 }  // namespace
 }  // namespace btcx
 
+namespace btcx {
+
+// Plans the batch's block-diagonal graph and allocates its sums.
+void batch_finish(btc_batch_s* b, const btc_count_opts* opts, hipStream_t s) {
+    check_status(btc_plan_create(&b->graph, opts, s, &b->plan));
+    std::vector<uint32_t> vo32(b->voff.begin(), b->voff.end());
+    b->d_voff.alloc(vo32.size());
+    b->d_pv.alloc((size_t)b->graph.n);
+    b->d_pg.alloc((size_t)b->count);
+    b->h_pg.assign((size_t)b->count, 0);
+    BTC_HIP(hipMemcpyAsync(b->d_voff.p, vo32.data(), vo32.size() * 4, hipMemcpyHostToDevice, s));
+    BTC_HIP(hipStreamSynchronize(s));
+}
+
+btc_batch_s* batch_from_graph(btc_graph_s&& g, const std::vector<int64_t>& voff, const btc_count_opts* opts,
+                              hipStream_t s) {
+    BTC_REQUIRE(voff.size() >= 2 && voff.front() == 0 && voff.back() == g.n, BTC_ERR_ARG, "bad graph ranges");
+    auto b = std::make_unique<btc_batch_s>();
+    b->count = (int32_t)voff.size() - 1;
+    b->voff = voff;
+    b->graph = std::move(g);
+    batch_finish(b.get(), opts, s);
+    return b.release();
+}
+
+}  // namespace btcx
+
 void btc_batch_s::enqueue_count(hipStream_t s) {
     btcx::plan_enqueue(plan, s, d_pv.p, nullptr, nullptr);
     hipLaunchKernelGGL(btcx::graph_sums_kernel, dim3((unsigned)count), dim3(256), 0, s, d_pv.p, d_voff.p, d_pg.p);
@@ -104,15 +131,7 @@ int btc_batch_create(const btc_graph* graphs, int32_t count, const btc_count_opt
         G.labels.resize((size_t)N);
         for (int64_t v = 0; v < N; ++v) G.labels[v] = (int32_t)v;
         btcx::graph_finalize(&G);
-        hipStream_t s = btcx::as_stream(stream);
-        btcx::check_status(btc_plan_create(&G, opts, stream, &b->plan));
-        std::vector<uint32_t> vo32(b->voff.begin(), b->voff.end());
-        b->d_voff.alloc(vo32.size());
-        b->d_pv.alloc((size_t)N);
-        b->d_pg.alloc((size_t)count);
-        b->h_pg.assign((size_t)count, 0);
-        BTC_HIP(hipMemcpyAsync(b->d_voff.p, vo32.data(), vo32.size() * 4, hipMemcpyHostToDevice, s));
-        BTC_HIP(hipStreamSynchronize(s));
+        btcx::batch_finish(b.get(), opts, btcx::as_stream(stream));
         *out = b.release();
     });
 }

@@ -112,7 +112,7 @@ int btc_graph_from_coo(const int32_t* src, const int32_t* dst, int64_t nnz, int3
                 BTC_HIP(hipMemcpy(dd.p, dst, (size_t)nnz * 4, hipMemcpyHostToDevice));
             }
             auto g = std::make_unique<btc_graph_s>();
-            btcx::canonicalise_device(ds.p, dd.p, nnz, n, flags, nullptr, g.get());
+            btcx::canonicalise_device(ds.p, dd.p, nnz, n, flags, nullptr, g.get(), nullptr, nullptr);
             *out = g.release();
             return;
         }

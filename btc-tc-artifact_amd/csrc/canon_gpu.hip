@@ -119,13 +119,34 @@ __global__ void degree_kernel(const uint64_t* __restrict__ keys, int64_t m, int 
     }
 }
 
-// key = ~degree (ascending = descending degree), value = vertex
-__global__ void rank_keys_kernel(const uint32_t* __restrict__ deg, int32_t n, uint32_t* __restrict__ key,
-                                 uint32_t* __restrict__ val) {
+// key = ~degree (ascending = descending degree), value = vertex.  With
+// segments (seg = nseg + 1 ascending vertex offsets), key = segment << 26 |
+// (2^26 - 1 - degree), so that a stable sort orders by segment, then
+// descending degree, then id.
+constexpr int kSegShift = 26;
+__global__ void rank_keys_kernel(const uint32_t* __restrict__ deg, int32_t n, const uint32_t* __restrict__ seg,
+                                 int nseg, uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
     for (int64_t v = blockIdx.x * (int64_t)kT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kT) {
-        key[v] = ~deg[v];
+        if (seg) {
+            int lo = 0, hi = nseg;  // last segment with seg[s] <= v
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (seg[mid] <= (uint32_t)v) lo = mid;
+                else hi = mid;
+            }
+            key[v] = ((uint32_t)lo << kSegShift) | (((1u << kSegShift) - 1) - deg[v]);
+        } else {
+            key[v] = ~deg[v];
+        }
         val[v] = (uint32_t)v;
     }
+}
+
+// out[k] = map[at[k]] for k <= nseg (compacted segment starts)
+__global__ void gather_kernel(const uint32_t* __restrict__ map, const uint32_t* __restrict__ at, int cnt,
+                              uint32_t* __restrict__ out) {
+    const int k = blockIdx.x * kT + threadIdx.x;
+    if (k < cnt) out[k] = map[at[k]];
 }
 
 // rank[order[r]] = r; labels_out[r] = labels_in[order[r]]
@@ -200,8 +221,15 @@ __global__ void edge_stats_kernel(const uint32_t* __restrict__ rowptr, const int
 }  // namespace
 
 void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n, uint32_t flags,
-                         hipStream_t s, btc_graph_s* g) {
+                         hipStream_t s, btc_graph_s* g, const std::vector<int64_t>* segs,
+                         std::vector<int64_t>* csegs) {
     BTC_REQUIRE(nnz < (int64_t)0xFFFFFFFFLL, BTC_ERR_OVERFLOW, "nnz %lld exceeds 2^32 - 1 pairs", (long long)nnz);
+    const int nseg = segs ? (int)segs->size() - 1 : 0;
+    if (segs) {
+        BTC_REQUIRE(nseg >= 1 && nseg <= 64 && csegs && (*segs)[0] == 0 && segs->back() == n && n < (1 << kSegShift),
+                    BTC_ERR_ARG, "bad segments (%d segments over %d vertices)", nseg, n);
+        for (int k = 0; k < nseg; ++k) BTC_REQUIRE((*segs)[k] <= (*segs)[k + 1], BTC_ERR_ARG, "segments not ascending");
+    }
     // id range (and n when not given)
     int32_t h_lohi[2] = {INT_MAX, -1};
     {
@@ -267,6 +295,24 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
     }
     const int bn = id_bits(nc);
     if (m) BTC_LAUNCH(relabel_kernel, m, s, keys, m, b, newid.p, bn);  // monotone: order kept
+    // segment starts after compaction (the relabel is monotone, so every
+    // segment stays one contiguous range)
+    TmpBuf<uint32_t> dseg;
+    if (segs) {
+        std::vector<uint32_t> at(segs->begin(), segs->end()), cs(at.size());
+        dseg.alloc(at.size(), s);
+        BTC_HIP(hipMemcpyAsync(dseg.p, at.data(), at.size() * 4, hipMemcpyHostToDevice, s));
+        if (newid.p) {
+            TmpBuf<uint32_t> tmp;
+            tmp.alloc(at.size(), s);
+            hipLaunchKernelGGL(gather_kernel, dim3(1), dim3(kT), 0, s, newid.p, dseg.p, (int)at.size(), tmp.p);
+            BTC_CHECK_LAUNCH("gather_kernel");
+            BTC_HIP(hipMemcpyAsync(dseg.p, tmp.p, at.size() * 4, hipMemcpyDeviceToDevice, s));
+        }
+        BTC_HIP(hipMemcpyAsync(cs.data(), dseg.p, cs.size() * 4, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipStreamSynchronize(s));
+        csegs->assign(cs.begin(), cs.end());
+    }
 
     if (flags & BTC_CANON_DEGREE_ORDER) {
         TmpBuf<uint32_t> deg, k0, v0, k1, v1, rank;
@@ -280,7 +326,7 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
         lab2.alloc((size_t)nc);
         BTC_HIP(hipMemsetAsync(deg.p, 0, (size_t)nc * 4, s));
         if (m) BTC_LAUNCH(degree_kernel, m, s, keys, m, bn, deg.p);
-        BTC_LAUNCH(rank_keys_kernel, nc, s, deg.p, nc, k0.p, v0.p);
+        BTC_LAUNCH(rank_keys_kernel, nc, s, deg.p, nc, segs ? dseg.p : nullptr, nseg, k0.p, v0.p);
         const bool alt = sort_pairs_u32(k0.p, v0.p, k1.p, v1.p, (size_t)nc, 32, s);
         BTC_LAUNCH(rank_scatter_kernel, nc, s, alt ? v1.p : v0.p, nc, rank.p, labels.p, lab2.p);
         if (m) BTC_LAUNCH(relabel_kernel, m, s, keys, m, bn, rank.p, bn);
@@ -339,7 +385,7 @@ int btc_graph_from_coo_device(const int32_t* src, const int32_t* dst, int64_t nn
         BTC_REQUIRE(nnz == 0 || (src && dst), BTC_ERR_ARG, "src/dst is NULL");
         BTC_REQUIRE((flags & ~BTC_CANON_DEFAULT) == 0, BTC_ERR_ARG, "unknown flags 0x%x", flags);
         auto g = std::make_unique<btc_graph_s>();
-        btcx::canonicalise_device(src, dst, nnz, n, flags, btcx::as_stream(stream), g.get());
+        btcx::canonicalise_device(src, dst, nnz, n, flags, btcx::as_stream(stream), g.get(), nullptr, nullptr);
         *out = g.release();
     });
 }

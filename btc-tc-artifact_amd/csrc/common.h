@@ -218,9 +218,13 @@ namespace btcx {
 // Fills the derived statistics (rows, wedges, merge work) of g.
 void graph_finalize(btc_graph_s* g);
 // Canonicalises device-resident COO pairs into g (CSR and labels in HBM,
-// statistics filled, host copies left empty until ensure_host).
+// statistics filled, host copies left empty until ensure_host).  With
+// segs (ascending vertex offsets of up to 64 independent graphs, segs[0] =
+// 0, back = n), each graph stays one contiguous vertex range, relabelled by
+// degree within itself; csegs receives the ranges after compaction.
 void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n, uint32_t flags,
-                         hipStream_t s, btc_graph_s* g);
+                         hipStream_t s, btc_graph_s* g, const std::vector<int64_t>* segs,
+                         std::vector<int64_t>* csegs);
 // Makes the host CSR and labels of g valid (copies them from HBM once).
 void ensure_host(btc_graph_s* g);
 // Fills the auto fields of o from the autotuner's table when an entry

@@ -88,6 +88,11 @@ __host__ __device__ inline bool wave_pivot(uint32_t lists, uint32_t lo, uint32_t
     return lists <= kWaveItemLists && (hi - lo < 32u * kWaveWords || d <= (uint32_t)kWaveWords / 2);
 }
 
+// A batch over an already block-diagonal canonical graph whose graphs
+// occupy the vertex ranges [voff[k], voff[k + 1]) (takes g over).
+btc_batch_s* batch_from_graph(btc_graph_s&& g, const std::vector<int64_t>& voff, const btc_count_opts* opts,
+                              hipStream_t s);
+
 // Graphs with more edges than this are planned on the GPU unless the
 // options ask for the host planner.
 constexpr int64_t kHostPlanEdges = 1 << 20;

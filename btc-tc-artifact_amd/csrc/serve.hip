@@ -1,19 +1,25 @@
 // Serving: a request loop with continuous batching.
 //
-//   submit -> [prep threads] canonicalise + plan on their own HIP streams
-//             (admission: the resident set stays under the HBM budget; idle
-//             graphs are evicted least recently used first)
-//          -> [GPU worker] every request whose plan is resident joins the
-//             next launch: one persistent wedge-kernel launch over all of
-//             their plans (plus dense-tile launches) on the worker's stream
+//   submit -> [prep threads] each takes the queued new graphs (up to
+//             opts.group_graphs graphs and kGroupPairs pairs), canonicalises
+//             them on the GPU in one pass as separate vertex ranges and plans
+//             them as one block-diagonal unit (a single graph is planned on
+//             its own); admission keeps the resident set under the HBM
+//             budget, evicting idle units least recently used first
+//          -> [GPU worker] every ready request joins the next launch: the
+//             single-graph units in one persistent wedge-kernel launch over
+//             all of their plans, each multi-graph unit in one launch that
+//             yields all of its graphs' counts (per-vertex sums reduced per
+//             graph)
 //          -> completion queue (btc_server_poll)
 //
 // Requests for a graph key that is resident (or being prepared) reuse it.
 // While a launch is on the GPU, new arrivals are prepared concurrently on
-// the prep streams and join the next launch.  Every count is exact and
-// independent of what it is batched with: plans are per graph and each plan
-// has its own total.  No reference counterpart (the reference is a one-shot
-// CLI).
+// the prep streams.  Under light load units hold one graph; under heavy
+// load the queue is drained in groups, so preparation keeps up.  Every count
+// is exact and independent of what it is batched or grouped with (a graph's
+// count is the sum over its own vertex range).  No reference counterpart
+// (the reference is a one-shot CLI).
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -34,15 +40,36 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-struct Entry {  // one graph known to the server
-    btc_graph graph = nullptr;
+// A prep thread groups at most opts.group_graphs (<= kGroupGraphsMax) new
+// graphs and kGroupPairs input pairs.
+constexpr int kGroupGraphsMax = 64;
+constexpr size_t kGroupPairs = size_t(1) << 22;
+constexpr int64_t kGroupVertices = int64_t(1) << 25;  // below the segmented relabel's 2^26 limit
+
+struct Unit {  // what is prepared, resident and evicted together
+    btc_graph graph = nullptr;  // a single graph: its L and plan
     btc_plan plan = nullptr;
+    btc_batch batch = nullptr;  // several graphs: block-diagonal L, plan and per-graph sums
+    std::vector<uint64_t> keys; // member graph keys, in vertex-range order
     int64_t bytes = 0;
+    std::list<Unit*>::iterator lru;
+    bool in_lru = false;
+    Unit() = default;
+    Unit(const Unit&) = delete;
+    Unit& operator=(const Unit&) = delete;
+    ~Unit() {
+        if (plan) btc_plan_free(plan);
+        if (graph) btc_graph_free(graph);
+        if (batch) btc_batch_free(batch);
+    }
+};
+
+struct Entry {  // one graph known to the server
+    std::shared_ptr<Unit> unit;
+    int idx = 0;    // its range in a multi-graph unit
     int state = 0;  // 0 preparing, 1 resident, 2 failed
     int status = BTC_OK;
-    int pins = 0;   // requests holding the entry (waiting, ready or launched)
-    std::list<uint64_t>::iterator lru;
-    bool in_lru = false;
+    int pins = 0;   // requests holding the graph (waiting, ready or launched)
 };
 
 struct Request {
@@ -58,6 +85,18 @@ struct PrepJob {
     int32_t n = 0;
 };
 
+// Vertex count of a job (n, or max id + 1 when n <= 0); -1 when an id is
+// negative or not below n.
+int64_t job_vertices(const PrepJob& j) {
+    int32_t lo = 0, hi = -1;
+    for (size_t e = 0; e < j.src.size(); ++e) {
+        lo = std::min(lo, std::min(j.src[e], j.dst[e]));
+        hi = std::max(hi, std::max(j.src[e], j.dst[e]));
+    }
+    const int64_t n = j.n > 0 ? j.n : (int64_t)hi + 1;
+    return (lo < 0 || hi >= n) ? -1 : n;
+}
+
 }  // namespace
 
 struct btc_server_s {
@@ -69,7 +108,7 @@ struct btc_server_s {
     bool stop = false;
     uint64_t next_id = 1;
     std::unordered_map<uint64_t, Entry> cache;
-    std::list<uint64_t> lru;  // front = most recently used resident graph
+    std::list<Unit*> lru;  // front = most recently used resident unit
     int64_t resident_bytes = 0;
     std::deque<PrepJob> prep_queue;
     std::unordered_map<uint64_t, std::vector<Request>> waiting;  // key -> requests waiting for its plan
@@ -80,21 +119,25 @@ struct btc_server_s {
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
-    // Caller holds mu.  Evicts idle resident graphs (LRU first) until `need`
+    bool unit_idle(const Unit* u) {
+        for (uint64_t k : u->keys)
+            if (cache[k].pins > 0) return false;
+        return true;
+    }
+
+    // Caller holds mu.  Evicts idle resident units (LRU first) until `need`
     // more bytes fit; returns whether they fit.
     bool make_room(int64_t need) {
         auto it = lru.end();
         while (resident_bytes + need > opts.hbm_budget_bytes && it != lru.begin()) {
             --it;
-            Entry& e = cache[*it];
-            if (e.pins > 0) continue;
-            resident_bytes -= e.bytes;
-            btc_plan_free(e.plan);
-            btc_graph_free(e.graph);
-            const uint64_t key = *it;
+            Unit* u = *it;
+            if (!unit_idle(u)) continue;
+            resident_bytes -= u->bytes;
             it = lru.erase(it);
-            cache.erase(key);
-            ++stats.evictions;
+            const std::vector<uint64_t> keys = u->keys;
+            for (uint64_t k : keys) cache.erase(k);  // the last reference frees the unit
+            stats.evictions += (int64_t)keys.size();
         }
         return resident_bytes + need <= opts.hbm_budget_bytes;
     }
@@ -115,6 +158,151 @@ struct btc_server_s {
         if (status != BTC_OK) ++stats.failed;
     }
 
+    // Caller holds mu: the graph `key` failed to prepare.
+    void fail_key(uint64_t key, int rc) {
+        Entry& e = cache[key];
+        e.state = 2;
+        e.status = rc;
+        for (Request& r : waiting[key]) {
+            --e.pins;
+            finish(r, rc, 0, 0);
+        }
+        waiting.erase(key);
+        cv_done.notify_all();
+    }
+
+    // Caller holds mu: unit u (prepared for `keys`) becomes resident.
+    void publish(std::shared_ptr<Unit> u) {
+        resident_bytes += u->bytes;
+        lru.push_front(u.get());
+        u->lru = lru.begin();
+        u->in_lru = true;
+        for (size_t k = 0; k < u->keys.size(); ++k) {
+            const uint64_t key = u->keys[k];
+            Entry& e = cache[key];
+            e.unit = u;
+            e.idx = (int)k;
+            e.state = 1;
+            ++stats.prepared;
+            for (Request& r : waiting[key]) ready.push_back(r);
+            waiting.erase(key);
+        }
+        cv_gpu.notify_one();
+    }
+
+    // Canonicalises (host or GPU, per the server's flags) and plans one graph.
+    int prepare_single(PrepJob& job, uint32_t cf, const btc_count_opts& co, hipStream_t s,
+                       std::shared_ptr<Unit>& out) {
+        btc_graph g = nullptr;
+        int rc = BTC_OK;
+        if (cf & BTC_CANON_DEVICE) {
+            rc = btcx::guard([&] {
+                const size_t nnz = job.src.size();
+                btcx::DevBuf<int32_t> ds, dd;
+                ds.alloc(nnz);
+                dd.alloc(nnz);
+                if (nnz) {
+                    BTC_HIP(hipMemcpyAsync(ds.p, job.src.data(), nnz * 4, hipMemcpyHostToDevice, s));
+                    BTC_HIP(hipMemcpyAsync(dd.p, job.dst.data(), nnz * 4, hipMemcpyHostToDevice, s));
+                }
+                btcx::check_status(btc_graph_from_coo_device(ds.p, dd.p, (int64_t)nnz, job.n,
+                                                             cf & ~BTC_CANON_DEVICE, s, &g));
+            });
+        } else {
+            rc = btc_graph_from_coo(job.src.data(), job.dst.data(), (int64_t)job.src.size(), job.n, cf, &g);
+        }
+        std::vector<int32_t>().swap(job.src);
+        std::vector<int32_t>().swap(job.dst);
+        if (rc != BTC_OK) return rc;
+        btc_graph_info gi;
+        btc_graph_get_info(g, &gi);
+        const int64_t est = 24 * gi.m + 8 * (int64_t)gi.n + (1 << 16);  // lists + items, before the plan exists
+        if (!admit(est)) {
+            btc_graph_free(g);
+            return BTC_ERR_STATE;
+        }
+        btc_plan p = nullptr;
+        rc = btc_plan_create(g, &co, s, &p);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            resident_bytes -= est;
+        }
+        if (rc != BTC_OK) {
+            btc_graph_free(g);
+            return rc;
+        }
+        btc_plan_info pi;
+        btc_plan_get_info(p, &pi);
+        out = std::make_shared<Unit>();
+        out->graph = g;
+        out->plan = p;
+        out->bytes = pi.device_bytes + 4 * (gi.m + gi.n + 1);
+        out->keys = {job.key};
+        return BTC_OK;
+    }
+
+    // One GPU canonicalisation and one block-diagonal plan for several graphs.
+    int prepare_group(std::vector<PrepJob>& jobs, const std::vector<int64_t>& nv, uint32_t cf,
+                      const btc_count_opts& co, hipStream_t s, std::shared_ptr<Unit>& out) {
+        std::vector<int64_t> segs(jobs.size() + 1, 0), csegs;
+        size_t pairs = 0;
+        for (size_t k = 0; k < jobs.size(); ++k) {
+            segs[k + 1] = segs[k] + nv[k];
+            pairs += jobs[k].src.size();
+        }
+        btc_graph_s G;
+        int rc = btcx::guard([&] {
+            BTC_REQUIRE(segs.back() < INT32_MAX, BTC_ERR_OVERFLOW, "group too large");
+            std::vector<int32_t> src(pairs), dst(pairs);
+            size_t o = 0;
+            for (size_t k = 0; k < jobs.size(); ++k) {
+                const int32_t v0 = (int32_t)segs[k];
+                for (size_t e = 0; e < jobs[k].src.size(); ++e, ++o) {
+                    src[o] = jobs[k].src[e] + v0;
+                    dst[o] = jobs[k].dst[e] + v0;
+                }
+                std::vector<int32_t>().swap(jobs[k].src);
+                std::vector<int32_t>().swap(jobs[k].dst);
+            }
+            btcx::TmpBuf<int32_t> ds, dd;
+            ds.alloc(pairs, s);
+            dd.alloc(pairs, s);
+            if (pairs) {
+                BTC_HIP(hipMemcpyAsync(ds.p, src.data(), pairs * 4, hipMemcpyHostToDevice, s));
+                BTC_HIP(hipMemcpyAsync(dd.p, dst.data(), pairs * 4, hipMemcpyHostToDevice, s));
+            }
+            btcx::canonicalise_device(ds.p, dd.p, (int64_t)pairs, (int32_t)segs.back(), cf & ~BTC_CANON_DEVICE, s,
+                                      &G, &segs, &csegs);
+        });
+        if (rc != BTC_OK) return rc;
+        const int64_t est = 24 * G.m + 8 * (int64_t)G.n + (1 << 16);
+        if (!admit(est)) return BTC_ERR_STATE;
+        btc_batch b = nullptr;
+        rc = btcx::guard([&] { b = btcx::batch_from_graph(std::move(G), csegs, &co, s); });
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            resident_bytes -= est;
+        }
+        if (rc != BTC_OK) return rc;
+        btc_batch_info bi;
+        btc_batch_get_info(b, &bi);
+        out = std::make_shared<Unit>();
+        out->batch = b;
+        out->bytes = bi.plan.device_bytes + 4 * (bi.m + 2 * (int64_t)bi.n + 1) + 8 * bi.graphs;
+        for (const PrepJob& j : jobs) out->keys.push_back(j.key);
+        return BTC_OK;
+    }
+
+    // Waits until `est` more bytes fit next to the pinned units and reserves
+    // them; false when the server stops meanwhile.
+    bool admit(int64_t est) {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_prep.wait(lk, [&] { return stop || make_room(est) || resident_bytes == 0; });
+        if (stop) return false;
+        resident_bytes += est;  // reserved until the unit's real size is known
+        return true;
+    }
+
     void prep_loop() {
         // prep threads already run side by side: no nested OpenMP teams
         omp_set_num_threads(1);
@@ -123,84 +311,66 @@ struct btc_server_s {
         (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
         btc_count_opts co;
         btc_count_opts_default(&co);
+        const uint32_t cf = opts.canon_flags ? (uint32_t)opts.canon_flags : BTC_CANON_DEFAULT | BTC_CANON_DEVICE;
         for (;;) {
-            PrepJob job;
+            std::vector<PrepJob> jobs;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_prep.wait(lk, [&] { return stop || !prep_queue.empty(); });
                 if (stop) break;
-                job = std::move(prep_queue.front());
-                prep_queue.pop_front();
+                size_t pairs = 0;
+                // graphs are grouped only when canonicalised on the GPU
+                const size_t cap = (cf & BTC_CANON_DEVICE) ? (size_t)opts.group_graphs : 1;
+                while (!prep_queue.empty() && jobs.size() < cap &&
+                       (jobs.empty() || pairs + prep_queue.front().src.size() <= kGroupPairs)) {
+                    pairs += prep_queue.front().src.size();
+                    jobs.push_back(std::move(prep_queue.front()));
+                    prep_queue.pop_front();
+                }
             }
-            btc_graph g = nullptr;
-            btc_plan p = nullptr;
-            const uint32_t cf = opts.canon_flags ? (uint32_t)opts.canon_flags : BTC_CANON_DEFAULT | BTC_CANON_DEVICE;
-            int rc = BTC_OK;
-            if (cf & BTC_CANON_DEVICE) {
-                // canonicalise (and then plan) on the GPU, on this thread's stream
-                rc = btcx::guard([&] {
-                    const size_t nnz = job.src.size();
-                    btcx::DevBuf<int32_t> ds, dd;
-                    ds.alloc(nnz);
-                    dd.alloc(nnz);
-                    if (nnz) {
-                        BTC_HIP(hipMemcpyAsync(ds.p, job.src.data(), nnz * 4, hipMemcpyHostToDevice, s));
-                        BTC_HIP(hipMemcpyAsync(dd.p, job.dst.data(), nnz * 4, hipMemcpyHostToDevice, s));
+            std::vector<std::shared_ptr<Unit>> units;
+            std::vector<std::pair<uint64_t, int>> failed;
+            if (jobs.size() == 1) {
+                std::shared_ptr<Unit> u;
+                const int rc = prepare_single(jobs[0], cf, co, s, u);
+                if (rc == BTC_OK) units.push_back(u);
+                else failed.emplace_back(jobs[0].key, rc);
+            } else {
+                // bad graphs fail alone; the rest form one unit
+                std::vector<PrepJob> good;
+                std::vector<int64_t> nv;
+                for (PrepJob& j : jobs) {
+                    const int64_t n = job_vertices(j);
+                    if (n < 0) {
+                        failed.emplace_back(j.key, BTC_ERR_ARG);
+                        continue;
                     }
-                    btcx::check_status(btc_graph_from_coo_device(ds.p, dd.p, (int64_t)nnz, job.n, cf & ~BTC_CANON_DEVICE,
-                                                           s, &g));
-                });
-            } else {
-                rc = btc_graph_from_coo(job.src.data(), job.dst.data(), (int64_t)job.src.size(), job.n, cf, &g);
-            }
-            std::vector<int32_t>().swap(job.src);
-            std::vector<int32_t>().swap(job.dst);
-            int64_t est = 0;
-            if (rc == BTC_OK) {
-                btc_graph_info gi;
-                btc_graph_get_info(g, &gi);
-                est = 24 * gi.m + 8 * (int64_t)gi.n + (1 << 16);  // CSR + lists + items, before the plan exists
-                std::unique_lock<std::mutex> lk(mu);
-                // admission: wait until the graph fits next to the pinned ones
-                cv_prep.wait(lk, [&] { return stop || make_room(est) || resident_bytes == 0; });
-                if (stop) {
-                    btc_graph_free(g);
-                    break;
+                    nv.push_back(n);
+                    good.push_back(std::move(j));
                 }
-                resident_bytes += est;  // reserved until the plan's real size is known
+                int64_t nsum = 0;
+                for (int64_t n : nv) nsum += n;
+                if (good.size() == 1 || nsum >= kGroupVertices) {
+                    // too few or too many vertices for one unit: one graph at a time
+                    for (PrepJob& j : good) {
+                        std::shared_ptr<Unit> u;
+                        const int rc = prepare_single(j, cf, co, s, u);
+                        if (rc == BTC_OK) units.push_back(u);
+                        else failed.emplace_back(j.key, rc);
+                    }
+                } else if (!good.empty()) {
+                    std::vector<uint64_t> keys;
+                    for (const PrepJob& j : good) keys.push_back(j.key);
+                    std::shared_ptr<Unit> u;
+                    const int rc = prepare_group(good, nv, cf, co, s, u);
+                    if (rc == BTC_OK) units.push_back(u);
+                    else
+                        for (uint64_t k : keys) failed.emplace_back(k, rc);
+                }
             }
-            if (rc == BTC_OK) rc = btc_plan_create(g, &co, s, &p);
             std::lock_guard<std::mutex> lk(mu);
-            Entry& e = cache[job.key];
-            if (est) resident_bytes -= est;
-            if (rc == BTC_OK) {
-                btc_plan_info pi;
-                btc_plan_get_info(p, &pi);
-                btc_graph_info gi;
-                btc_graph_get_info(g, &gi);
-                e.graph = g;
-                e.plan = p;
-                e.bytes = pi.device_bytes + 4 * (gi.m + gi.n + 1);
-                e.state = 1;
-                resident_bytes += e.bytes;
-                lru.push_front(job.key);
-                e.lru = lru.begin();
-                e.in_lru = true;
-                ++stats.prepared;
-                for (Request& r : waiting[job.key]) ready.push_back(r);
-                waiting.erase(job.key);
-                cv_gpu.notify_one();
-            } else {
-                if (g) btc_graph_free(g);
-                e.state = 2;
-                e.status = rc;
-                for (Request& r : waiting[job.key]) {
-                    --e.pins;
-                    finish(r, rc, 0, 0);
-                }
-                waiting.erase(job.key);
-                cv_done.notify_all();
-            }
+            for (auto& u : units) publish(u);
+            for (auto& f : failed) fail_key(f.first, f.second);
         }
         if (s) (void)hipStreamDestroy(s);
     }
@@ -211,10 +381,15 @@ struct btc_server_s {
         (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
         btcx::MultiWorkspace ws;
         std::vector<Request> batch;
+        std::vector<std::shared_ptr<Unit>> units;   // distinct units of the launch
+        std::vector<int> unit_of;                   // request -> units index
         std::vector<btc_plan_s*> plans;
-        std::vector<uint64_t> keys, totals;
+        std::vector<uint64_t> totals;
+        std::vector<std::vector<uint64_t>> per_graph;
         for (;;) {
             batch.clear();
+            units.clear();
+            unit_of.clear();
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_gpu.wait(lk, [&] { return stop || !ready.empty(); });
@@ -225,29 +400,55 @@ struct btc_server_s {
                     ready.pop_front();
                     batch.back().t_start = t;
                 }
-                // one plan per distinct graph in the launch
-                plans.clear();
-                keys.clear();
                 for (const Request& r : batch) {
-                    if (std::find(keys.begin(), keys.end(), r.key) != keys.end()) continue;
-                    keys.push_back(r.key);
-                    plans.push_back(cache[r.key].plan);
+                    const std::shared_ptr<Unit>& u = cache[r.key].unit;
+                    size_t k = 0;
+                    while (k < units.size() && units[k] != u) ++k;
+                    if (k == units.size()) units.push_back(u);
+                    unit_of.push_back((int)k);
                 }
             }
+            // single-graph units: one multi-plan launch; groups: one launch each
+            plans.clear();
+            std::vector<int> plan_slot(units.size(), -1);
+            for (size_t k = 0; k < units.size(); ++k)
+                if (units[k]->plan) {
+                    plan_slot[k] = (int)plans.size();
+                    plans.push_back(units[k]->plan);
+                }
             totals.assign(plans.size(), 0);
-            int rc = btcx::guard([&] { btcx::plans_count(plans.data(), (int)plans.size(), s, totals.data(), ws); });
+            per_graph.assign(units.size(), {});
+            int rc = BTC_OK;
+            int64_t launches = 0;
+            if (!plans.empty()) {
+                rc = btcx::guard([&] { btcx::plans_count(plans.data(), (int)plans.size(), s, totals.data(), ws); });
+                ++launches;
+            }
+            std::vector<int> unit_rc(units.size(), rc);
+            for (size_t k = 0; k < units.size(); ++k) {
+                if (!units[k]->batch) continue;
+                per_graph[k].assign(units[k]->keys.size(), 0);
+                uint64_t t = 0;
+                unit_rc[k] = btc_batch_count(units[k]->batch, s, per_graph[k].data(), &t, nullptr);
+                ++launches;
+            }
             std::lock_guard<std::mutex> lk(mu);
-            ++stats.launches;
+            stats.launches += launches;
             stats.launched_requests += (int64_t)batch.size();
-            for (const Request& r : batch) {
-                const size_t k = (size_t)(std::find(keys.begin(), keys.end(), r.key) - keys.begin());
+            for (size_t i = 0; i < batch.size(); ++i) {
+                const Request& r = batch[i];
+                const int k = unit_of[i];
+                Unit* u = units[(size_t)k].get();
                 Entry& e = cache[r.key];
                 --e.pins;
-                if (e.in_lru) lru.splice(lru.begin(), lru, e.lru);
-                finish(r, rc, rc == BTC_OK ? totals[k] : 0, (int)batch.size());
+                if (u->in_lru) lru.splice(lru.begin(), lru, u->lru);
+                const int st = unit_rc[(size_t)k];
+                uint64_t tri = 0;
+                if (st == BTC_OK) tri = u->plan ? totals[(size_t)plan_slot[(size_t)k]] : per_graph[(size_t)k][(size_t)e.idx];
+                finish(r, st, tri, (int)batch.size());
             }
             cv_done.notify_all();
-            cv_prep.notify_all();  // unpinned graphs may now be evicted for waiting admissions
+            cv_prep.notify_all();  // unpinned units may now be evicted for waiting admissions
         }
         if (s) (void)hipStreamDestroy(s);
     }
@@ -262,6 +463,7 @@ void btc_server_opts_default(btc_server_opts* o) {
     o->max_batch = 512;
     o->prep_threads = 8;
     o->canon_flags = BTC_CANON_DEFAULT | BTC_CANON_DEVICE;
+    o->group_graphs = 16;
 }
 
 int btc_server_create(const btc_server_opts* opts, btc_server* out) {
@@ -273,6 +475,8 @@ int btc_server_create(const btc_server_opts* opts, btc_server* out) {
         if (opts) sv->opts = *opts;
         BTC_REQUIRE(sv->opts.hbm_budget_bytes > 0 && sv->opts.max_batch > 0 && sv->opts.prep_threads > 0,
                     BTC_ERR_ARG, "server options must be positive");
+        BTC_REQUIRE(sv->opts.group_graphs >= 1 && sv->opts.group_graphs <= kGroupGraphsMax, BTC_ERR_ARG,
+                    "group_graphs %d (expected 1..%d)", sv->opts.group_graphs, kGroupGraphsMax);
         BTC_HIP(hipGetDevice(&sv->device));
         btc_server_s* raw = sv.get();
         for (int i = 0; i < sv->opts.prep_threads; ++i) sv->workers.emplace_back([raw] { raw->prep_loop(); });
@@ -349,7 +553,8 @@ int btc_server_get_stats(btc_server sv, btc_server_stats* st) {
         std::lock_guard<std::mutex> lk(sv->mu);
         *st = sv->stats;
         st->resident_bytes = sv->resident_bytes;
-        st->resident_graphs = (int64_t)sv->lru.size();
+        st->resident_graphs = 0;
+        for (const Unit* u : sv->lru) st->resident_graphs += (int64_t)u->keys.size();
         st->clock = sv->now();
     });
 }
@@ -363,11 +568,7 @@ void btc_server_free(btc_server sv) {
     sv->cv_prep.notify_all();
     sv->cv_gpu.notify_all();
     for (auto& t : sv->workers) t.join();
-    for (auto& kv : sv->cache) {
-        if (kv.second.plan) btc_plan_free(kv.second.plan);
-        if (kv.second.graph) btc_graph_free(kv.second.graph);
-    }
-    delete sv;
+    delete sv;  // cache entries release their units
 }
 
 }  // extern "C"

@@ -66,13 +66,31 @@ def test_server_mix_equals_alone(gpu, canon_flags):
 def test_server_eviction_keeps_answers(gpu):
     gs = make_graphs(gpu, 12, seed=2)
     want = [alone(gpu, *g) for g in gs]
-    sv = gpu.Server(hbm_budget_bytes=3 << 20, max_batch=4, prep_threads=2)
+    # host preparation: one graph per resident unit
+    sv = gpu.Server(hbm_budget_bytes=3 << 20, max_batch=4, prep_threads=2, canon_flags=3)
     for rnd in range(2):
         ids = {sv.submit(k, *g): k for k, g in enumerate(gs)}
         for r in drain(sv, len(ids)):
             assert r["status"] == 0 and r["triangles"] == want[ids[r["request_id"]]]
     st = sv.stats()
     assert st["evictions"] > 0 and st["resident_bytes"] <= (3 << 20) + (1 << 22)
+    sv.close()
+
+
+def test_server_grouped_units_evict_and_stay_exact(gpu):
+    """GPU preparation groups queued graphs into one unit; a tight budget
+    evicts whole idle units, and every answer stays exact."""
+    gs = make_graphs(gpu, 24, seed=5)
+    want = [alone(gpu, *g) for g in gs]
+    sv = gpu.Server(hbm_budget_bytes=4 << 20, max_batch=64, prep_threads=2)
+    for rnd in range(3):
+        base = 1000 * rnd  # fresh keys every round: earlier units must make room
+        ids = {sv.submit(base + k, *g): k for k, g in enumerate(gs)}
+        ids.update({sv.submit(base + k): k for k in range(0, 24, 5)})  # repeats
+        for r in drain(sv, len(ids)):
+            assert r["status"] == 0 and r["triangles"] == want[ids[r["request_id"]]]
+    st = sv.stats()
+    assert st["evictions"] > 0 and st["prepared"] == 72 and st["failed"] == 0
     sv.close()
 
 
