@@ -461,9 +461,9 @@ void btc_server_opts_default(btc_server_opts* o) {
     std::memset(o, 0, sizeof(*o));
     o->hbm_budget_bytes = (int64_t)64 << 30;
     o->max_batch = 512;
-    o->prep_threads = 8;
+    o->prep_threads = 3;
     o->canon_flags = BTC_CANON_DEFAULT | BTC_CANON_DEVICE;
-    o->group_graphs = 16;
+    o->group_graphs = 8;
 }
 
 int btc_server_create(const btc_server_opts* opts, btc_server* out) {
