@@ -419,7 +419,8 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the engine shards by row-block ranges by design, see DESIGN.md); run with --gpus 1",
+              "(a k-way split would give each GPU a 1/k share of the work items and all-reduce one u64); "
+              "run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
