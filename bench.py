@@ -418,7 +418,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the engine has no multi-GPU path "
               "(a k-way split would give each GPU a 1/k share of the work items and all-reduce one u64); "
               "run with --gpus 1",
               file=sys.stderr)
