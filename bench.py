@@ -411,6 +411,7 @@ def main():
     ap.add_argument("--dense", type=int, default=5, help="timed runs per case in the dense-tile leg (0 = skip)")
     ap.add_argument("--host-path", type=int, default=1, help="also time host canonicalisation and planning")
     ap.add_argument("--configs", type=int, default=5, help="timed runs per case of the ER / stencil / planted leg")
+    ap.add_argument("--shares", type=int, default=1, help="time one GPU's share of 2-, 4- and 8-way splits")
     ap.add_argument("--serve-host-prep", type=int, default=1,
                     help="also run the serving leg with canonicalisation and planning on host threads")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -505,6 +506,31 @@ def main():
                 "stream_bytes": pi["stream_bytes"],
                 "stream_GBps": round(pi["stream_bytes"] / (wedge_ms * 1e-3) / 1e9, 1) if wedge_ms > 0 else None}
 
+    # one GPU's share of a k-way split (the work each GPU would run), each
+    # share on the whole chip, from a plan whose work items are cut for a
+    # 1/k share (item size = the single-GPU rule divided by k); the shares'
+    # totals must add up
+    sharding = None
+    if args.shares:
+        sharding = {}
+        full_ms = statistics.median(kms)
+        cus = 256
+        for k in (2, 4, 8):
+            chunk = int(max(1024, min(1 << 18, gi["wedges"] // (cus * 64 * k))))
+            pk = btc.Plan(g, tile_k=args.tile_k, tau=args.tau, chunk_work=chunk)
+            ms_k, tot = [], 0
+            for j in range(k):
+                t_j, ms = pk.count_share(k, j)
+                ms_j = [pk.count_share(k, j)[1] for _ in range(3)]
+                ms_k.append(statistics.median(ms_j))
+                tot += t_j
+            pk.close()
+            sharding[str(k)] = {"chunk_work": chunk, "max_share_ms": round(max(ms_k), 4),
+                                "min_share_ms": round(min(ms_k), 4),
+                                "projected_speedup": round(full_ms / max(ms_k), 2), "shares_add_up": tot == total0}
+            assert tot == total0, ("shares do not add up", k, tot, total0)
+        log("sharding (one GPU's share of a k-way split): %s" % sharding)
+
     plain = None
     if args.plain_runs > 0:
         K = 128 if gi["n"] >= 50000 else btc.lite_tile_k(gi["n"], gi["m"])
@@ -546,7 +572,7 @@ def main():
         "vs_baseline": None, "dtype": "u32", "data": "synthetic", "config": config,
         "triangles": int(total0), "triangles_per_s": total0 / (wall / args.steps),
         "kernel_ms_median": statistics.median(kms), "canon_ms": round(canon_ms, 1), "plan_ms": round(build_ms, 1),
-        "roofline": roofline, "cpu_baseline": cpu, "plain_hybrid": plain, "end_to_end": e2e,
+        "roofline": roofline, "cpu_baseline": cpu, "plain_hybrid": plain, "end_to_end": e2e, "sharding": sharding,
         "batched": batched, "serving": serving, "serving_host_prep": serving_host, "dense_tiles": dense,
         "other_configs": configs,
     }

@@ -134,6 +134,7 @@ _SIGS = {
     "btc_plan_last_timing": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "btc_plan_free": (None, [vp]),
     "btc_plan_capture": (C.c_int, [vp, vp]),
+    "btc_plan_count_share": (C.c_int, [vp, C.c_int32, C.c_int32, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
     "btc_batch_capture": (C.c_int, [vp, vp]),
     "btc_batch_autotune": (C.c_int, [vp, C.POINTER(TuneOpts), vp, C.POINTER(CountOpts), C.POINTER(C.c_float)]),
     "btc_count": (C.c_int, [vp, C.POINTER(CountOpts), vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
@@ -392,6 +393,12 @@ class Plan:
         total, ms, pv = _count_out(self.graph.n, per_vertex)
         _check(lib().btc_plan_count(self._h, stream, C.byref(total), _ptr(pv), C.byref(ms)))
         return total.value, pv, ms.value
+
+    def count_share(self, shares, share, stream=None):
+        """(total, kernel_ms) of share `share` of a `shares`-way split."""
+        total, ms = C.c_uint64(0), C.c_float(0)
+        _check(lib().btc_plan_count_share(self._h, shares, share, stream, C.byref(total), C.byref(ms)))
+        return total.value, ms.value
 
     def capture(self, stream=None):
         """Captures the totals-only count into a HIP graph that later

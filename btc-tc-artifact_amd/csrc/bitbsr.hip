@@ -354,14 +354,14 @@ template <int K, bool BINSEARCH, bool PV, bool MFMA>
 __global__ __launch_bounds__(256) void tile_count_kernel(
     int n, int n_rb, const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
     const uint32_t* __restrict__ row_indices, const uint64_t* __restrict__ tiles, uint32_t w0,
-    uint32_t w_end, const uint32_t* __restrict__ tile_list, int tau,
+    uint32_t w_end, const uint32_t* __restrict__ tile_list, uint32_t lstride, int tau,
     unsigned long long* __restrict__ result, int32_t* __restrict__ pv) {
     constexpr int WPT = K / 4;
     constexpr int SLABS = K / 16;
     const int lane = threadIdx.x & 63;
     const uint32_t w = uniform(w0 + blockIdx.x * 4 + (threadIdx.x >> 6));
     if (w >= w_end) return;
-    const uint32_t t = tile_list ? uniform(tile_list[w]) : w;
+    const uint32_t t = tile_list ? uniform(tile_list[(size_t)w * lstride]) : w;
     uint32_t I;
     if (BINSEARCH) {  // row block of tile t by binary search over indptr (tc_16x128_mma_v4.h:29-38)
         uint32_t lo = 0, hi = (uint32_t)n_rb;
@@ -504,32 +504,33 @@ __global__ __launch_bounds__(256) void tile_popcount_kernel(const uint64_t* __re
 constexpr uint32_t kMaxTilesPerLaunch = 1u << 24;
 
 template <int K, bool BS, bool MF>
-void launch_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t nt, int tau, unsigned long long* res,
-              int32_t* pv, hipStream_t s) {
+void launch_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstride, uint32_t nt, int tau,
+              unsigned long long* res, int32_t* pv, hipStream_t s) {
     for (uint32_t w0 = 0; w0 < nt; w0 += kMaxTilesPerLaunch) {
         const uint32_t chunk = std::min(kMaxTilesPerLaunch, nt - w0);
         const unsigned grid = (chunk + 3) / 4;
         const uint32_t end = w0 + chunk;
         if (pv)
             hipLaunchKernelGGL((tile_count_kernel<K, BS, true, MF>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
-                               b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, end, list, tau,
+                               b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, end, list, lstride, tau,
                                res, pv);
         else
             hipLaunchKernelGGL((tile_count_kernel<K, BS, false, MF>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
-                               b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, end, list, tau,
+                               b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, end, list, lstride, tau,
                                res, pv);
         BTC_CHECK_LAUNCH("tile_count_kernel");
     }
 }
 
 template <int K>
-void launch_kk(const btc_bitbsr_s* b, const uint32_t* list, uint32_t nt, bool bs, bool mfma, int tau,
-               unsigned long long* res, int32_t* pv, hipStream_t s) {
+void launch_kk(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstride, uint32_t nt, bool bs, bool mfma,
+               int tau, unsigned long long* res, int32_t* pv, hipStream_t s) {
     if (bs) {
-        mfma ? launch_k<K, true, true>(b, list, nt, tau, res, pv, s) : launch_k<K, true, false>(b, list, nt, tau, res, pv, s);
+        mfma ? launch_k<K, true, true>(b, list, lstride, nt, tau, res, pv, s)
+             : launch_k<K, true, false>(b, list, lstride, nt, tau, res, pv, s);
     } else {
-        mfma ? launch_k<K, false, true>(b, list, nt, tau, res, pv, s)
-             : launch_k<K, false, false>(b, list, nt, tau, res, pv, s);
+        mfma ? launch_k<K, false, true>(b, list, lstride, nt, tau, res, pv, s)
+             : launch_k<K, false, false>(b, list, lstride, nt, tau, res, pv, s);
     }
 }
 
@@ -546,14 +547,15 @@ void bitbsr_build(btc_graph_s* g, int K, hipStream_t s, btc_bitbsr_s* b) {
 }
 
 void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles, bool binsearch,
-                       bool mfma, int tau, unsigned long long* d_result, int32_t* d_pv, hipStream_t s) {
+                       bool mfma, int tau, unsigned long long* d_result, int32_t* d_pv, hipStream_t s,
+                       uint32_t lstride) {
     if (ntiles == 0) return;
     BTC_REQUIRE(b->max_row <= 65536, BTC_ERR_OVERFLOW,
                 "longest row %d > 65536: per-lane 32-bit sums could overflow", b->max_row);
     switch (b->K) {
-        case 32: launch_kk<32>(b, tile_list, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
-        case 64: launch_kk<64>(b, tile_list, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
-        default: launch_kk<128>(b, tile_list, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
+        case 32: launch_kk<32>(b, tile_list, lstride, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
+        case 64: launch_kk<64>(b, tile_list, lstride, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
+        default: launch_kk<128>(b, tile_list, lstride, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
     }
 }
 

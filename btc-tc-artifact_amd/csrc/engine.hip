@@ -127,6 +127,7 @@ struct WedgeArgs {
     uint32_t* counters;  // [0] workgroup-item cursor, [1] wave-item cursor (schedule 0)
     int lookup;          // 0 = auto, 1 = never bitmap, 2 = never hash
     int schedule;        // 0 = work stealing, 1 = static stride
+    uint32_t share = 0, shares = 1;  // single plan: launch item t is plan item share + t * shares of its class
 };
 
 // Barrier of the threads that share one LDS region: the workgroup (NT = 256)
@@ -285,7 +286,10 @@ __device__ __forceinline__ Item fetch_item(const WedgeArgs& a, uint32_t t) {
     Item r;
     uint32_t li = t;
     if (a.nplans == 1) {
-        r.P = a.p0;  // single plan: the launch numbering is the plan's own
+        // single plan: every shares-th item of each class from share on
+        // (the plan's own numbering when shares = 1)
+        r.P = a.p0;
+        li = t < a.n_block ? a.share + t * a.shares : r.P.n_block + a.share + (t - a.n_block) * a.shares;
     } else {
         const uint2 m = a.map[t];
         r.P = uniform_plan(a.plans + uniform(m.x));
@@ -705,11 +709,19 @@ PlanDev plan_dev(btc_plan_s* p, unsigned long long* d_total, int32_t* d_pv) {
                    d_total, d_pv, p->core_ids / 128};
 }
 
-void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counters, int32_t* d_pv, hipStream_t s) {
+// Items of one class (n of them) that fall to `share` of `shares`.
+uint32_t share_count(uint32_t n, uint32_t share, uint32_t shares) {
+    return n > share ? (n - share + shares - 1) / shares : 0u;
+}
+
+void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counters, int32_t* d_pv, hipStream_t s,
+                  uint32_t share = 0, uint32_t shares = 1) {
     unsigned grid = 0;
     WedgeFn fn = wedge_fn(p->launch, d_pv != nullptr, &grid);
-    WedgeArgs a{plan_dev(p, d_total, d_pv), nullptr, nullptr, 1u, p->n_block_items, p->nitems, d_counters,
-                (int)p->launch.lookup, (int)p->launch.schedule};
+    const uint32_t nb = share_count(p->n_block_items, share, shares);
+    const uint32_t nw = share_count(p->nitems - p->n_block_items, share, shares);
+    WedgeArgs a{plan_dev(p, d_total, d_pv), nullptr, nullptr, 1u, nb, nb + nw, d_counters, (int)p->launch.lookup,
+                (int)p->launch.schedule, share, shares};
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
     BTC_CHECK_LAUNCH("wedge_kernel");
 }
@@ -914,6 +926,31 @@ int btc_plan_tune(btc_plan p, const btc_count_opts* opts) {
         btcx::check_launch_opts(*opts);
         p->launch = *opts;
         p->count_graph.reset();  // a captured count used the old launch shape
+    });
+}
+
+int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream, uint64_t* total,
+                         float* kernel_ms) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(p && total, BTC_ERR_ARG, "NULL argument");
+        BTC_REQUIRE(shares >= 1 && share >= 0 && share < shares, BTC_ERR_ARG, "share %d of %d", share, shares);
+        hipStream_t s = btcx::as_stream(stream);
+        const uint32_t k = (uint32_t)shares, j = (uint32_t)share;
+        btcx::launch_zero_u32(reinterpret_cast<uint32_t*>(p->scratch.p), 4, s);
+        btcx::EventTimer tm(kernel_ms != nullptr);
+        tm.start(s);
+        const uint32_t nd = btcx::share_count(p->n_dense, j, k);
+        if (nd)
+            btcx::launch_tile_count(p->bsr.get(), p->dense_list.p + j, nd, false, p->launch.dense_kernel == 2, -1,
+                                    p->scratch.p, nullptr, s, k);
+        if (btcx::share_count(p->nitems, 0, 1))
+            btcx::launch_wedge(p, p->scratch.p, reinterpret_cast<uint32_t*>(p->scratch.p + 1), nullptr, s, j, k);
+        tm.stop(s);
+        unsigned long long h = 0;
+        BTC_HIP(hipMemcpyAsync(&h, p->scratch.p, 8, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipStreamSynchronize(s));
+        if (kernel_ms) *kernel_ms = tm.ms();
+        *total = h;
     });
 }
 

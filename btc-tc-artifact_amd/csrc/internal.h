@@ -116,12 +116,13 @@ void scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, hipStream_t s)
 void bitbsr_build(btc_graph_s* g, int K, hipStream_t s, btc_bitbsr_s* b);
 
 // Count over mask tiles of b.  tile_list (device) selects ntiles mask tiles,
-// or all tiles when NULL.  tau < 0 sends every tile down the dense path;
-// mfma selects the i8 MFMA form of the dense path over AND + popcount.
-// Adds the total to *d_result and per-row sums to d_pv (when not NULL).
+// or all tiles when NULL (list entries lstride apart: tile_list[w * lstride]).
+// tau < 0 sends every tile down the dense path; mfma selects the i8 MFMA
+// form of the dense path over AND + popcount.  Adds the total to *d_result
+// and per-row sums to d_pv (when not NULL).
 void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles,
                        bool binsearch, bool mfma, int tau, unsigned long long* d_result, int32_t* d_pv,
-                       hipStream_t s);
+                       hipStream_t s, uint32_t lstride = 1);
 
 // p[0, n) = 0 by a kernel (safe inside captured graphs).
 void launch_zero_u32(uint32_t* p, size_t n, hipStream_t s);

@@ -226,6 +226,13 @@ int btc_plan_count(btc_plan p, void* stream, uint64_t* total, int32_t* per_verte
 /* Event times of the last btc_plan_count with kernel_ms != NULL, split by
  * kernel: dense-tile kernel and wedge kernel (0 when not launched). */
 int btc_plan_last_timing(btc_plan p, float* dense_ms, float* wedge_ms);
+/* Counts share `share` of a `shares`-way split of the plan's work: the dense
+ * tiles and the work items (of each class, both sorted by work) whose index
+ * is share modulo shares — what one GPU of a split would run.  The totals of
+ * all shares add up to the plan's total.  No reference counterpart (the
+ * reference counts on one GPU). */
+int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream, uint64_t* total,
+                         float* kernel_ms);
 /* Captures the totals-only count of p (zero fill, dense-tile kernel, wedge
  * kernel) into a HIP graph; later btc_plan_count calls without per_vertex
  * replay it with one hipGraphLaunch on their stream (kernel_ms then times

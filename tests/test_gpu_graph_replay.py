@@ -71,3 +71,17 @@ def test_overlapped_dense_and_wedge_kernels(gpu, orc):
     for _ in range(3):
         assert p.count()[0] == want
         p.count(per_vertex=True)
+
+
+@pytest.mark.parametrize("name", ["rmat15", "planted", "star"])
+def test_shares_add_up(gpu, orc, name):
+    """The parts of a k-way split of a plan's work add up to its count."""
+    s, d, n = case(name, gpu)
+    g = gpu.Graph.from_coo(s, d, n)
+    want, _ = oracle_counts(orc, g)
+    p = gpu.Plan(g, tile_k=64, tau=0 if name == "planted" else -1)
+    for k in (1, 2, 3, 8):
+        assert sum(p.count_share(k, j)[0] for j in range(k)) == want, k
+    with pytest.raises(gpu.BtcError):
+        p.count_share(4, 4)
+    assert p.count()[0] == want
