@@ -622,18 +622,19 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
         if (e == e1) continue;
         const uint32_t t0 = (uint32_t)rp[j], t1 = (uint32_t)rp[j + 1];
         const uint32_t lo = (uint32_t)cl[rp[j]], hi = (uint32_t)cl[rp[j + 1] - 1];
-        const bool small = wave_pivot((uint32_t)std::min<int64_t>(e1 - e, 0xFFFFFFFFLL), lo, hi, t1 - t0);
         // cost of a list: ids looked up, plus core words ANDed
         auto cost = [&](int64_t f) -> int64_t {
             const uint4& r = lst[f];
             return (int64_t)(r.z > r.y ? r.z - r.y : 0) + (r.w != kNoBitmap ? (int64_t)core_words(lo, hi, C) : 0);
         };
+        const bool small = wave_pivot((uint32_t)std::min<int64_t>(e1 - e, 0xFFFFFFFFLL), lo, hi, t1 - t0);
         while (e < e1) {
             int64_t w = 0, f = e;
             while (f < e1 && (small || w == 0 || w + cost(f) <= chunk)) {
                 w += cost(f);
                 ++f;
-            }            checks += w;
+            }
+            checks += w;
             pivot_words += t1 - t0;
             Item it{make_uint4(t0, t1, (uint32_t)e, (uint32_t)f),
                     make_uint4(lo, hi, (uint32_t)std::min<int64_t>(w, 0xFFFFFFFFLL), 0u)};
@@ -777,6 +778,13 @@ void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, M
     for (int j = 0; j < k; ++j) totals[j] = h[(size_t)j];
 }
 
+void ensure_side_stream(btc_plan_s* p) {
+    if (p->side) return;
+    BTC_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+    BTC_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+    BTC_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+}
+
 void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, EventTimer* tw) {
     BTC_REQUIRE(!d_pv || p->g->max_row <= 65536, BTC_ERR_OVERFLOW,
                 "longest row %d > 65536: per-vertex counts could overflow int32", p->g->max_row);
@@ -787,11 +795,7 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
         // the latency-bound dense-tile kernel runs on a second stream while
         // the HBM-bound wedge kernel runs on s; both add into d_total (for
         // small plans the fork and join would cost more than they hide)
-        if (!p->side) {
-            BTC_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
-            BTC_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
-            BTC_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
-        }
+        ensure_side_stream(p);
         if (tm) tm->start(s);
         BTC_HIP(hipEventRecord(p->ev_fork, s));
         BTC_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
@@ -940,11 +944,23 @@ int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream
         btcx::EventTimer tm(kernel_ms != nullptr);
         tm.start(s);
         const uint32_t nd = btcx::share_count(p->n_dense, j, k);
+        // the dense-tile part beside the wedge part, as in a full count
+        hipStream_t ds = s;
+        if (nd && p->nitems) {
+            btcx::ensure_side_stream(p);
+            BTC_HIP(hipEventRecord(p->ev_fork, s));
+            BTC_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+            ds = p->side;
+        }
+        if (p->nitems)
+            btcx::launch_wedge(p, p->scratch.p, reinterpret_cast<uint32_t*>(p->scratch.p + 1), nullptr, s, j, k);
         if (nd)
             btcx::launch_tile_count(p->bsr.get(), p->dense_list.p + j, nd, false, p->launch.dense_kernel == 2, -1,
-                                    p->scratch.p, nullptr, s, k);
-        if (btcx::share_count(p->nitems, 0, 1))
-            btcx::launch_wedge(p, p->scratch.p, reinterpret_cast<uint32_t*>(p->scratch.p + 1), nullptr, s, j, k);
+                                    p->scratch.p, nullptr, ds, k);
+        if (ds != s) {
+            BTC_HIP(hipEventRecord(p->ev_join, ds));
+            BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
+        }
         tm.stop(s);
         unsigned long long h = 0;
         BTC_HIP(hipMemcpyAsync(&h, p->scratch.p, 8, hipMemcpyDeviceToHost, s));
