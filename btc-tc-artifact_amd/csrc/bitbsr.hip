@@ -487,6 +487,120 @@ __global__ __launch_bounds__(256) void tile_count_kernel(
     if (lane == 0 && tot) atomicAdd(result, (unsigned long long)tot);
 }
 
+// Calls f(pa, pb) for every tile column present in both sorted lists
+// indices[a0, a1) and indices[b0, b1), in ascending order; wave-uniform.
+// Each lane takes one element of the shorter list per round of 64 and
+// binary-searches the longer one, so the matching costs log-depth load
+// chains instead of a two-pointer walk of both lists.
+template <class F>
+__device__ __forceinline__ void for_each_match(const uint32_t* __restrict__ indices, uint32_t a0, uint32_t a1,
+                                               uint32_t b0, uint32_t b1, int lane, F&& f) {
+    const bool swap = (b1 - b0) < (a1 - a0);
+    const uint32_t s0 = swap ? b0 : a0, s1 = swap ? b1 : a1;  // iterated (shorter)
+    const uint32_t l0 = swap ? a0 : b0, l1 = swap ? a1 : b1;  // searched (longer)
+    for (uint32_t cbase = s0; cbase < s1; cbase += 64) {
+        const uint32_t ps = cbase + (uint32_t)lane;
+        uint32_t hit = 0xFFFFFFFFu;
+        if (ps < s1) {
+            const uint32_t k = indices[ps];
+            uint32_t lo = l0, hi = l1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (indices[mid] < k) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < l1 && indices[lo] == k) hit = lo;
+        }
+        uint64_t m = __ballot(hit != 0xFFFFFFFFu);
+        while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const uint32_t x = uniform(cbase + (uint32_t)l), y = uniform((uint32_t)__shfl((int)hit, l));
+            if (swap) f(y, x);
+            else f(x, y);
+        }
+    }
+}
+
+// The engine's dense mask tiles (every listed tile on the dense path): one
+// wave per tile as in tile_count_kernel, with the tile columns of the two
+// row blocks matched by for_each_match.
+template <int K, bool PV, bool MFMA>
+__global__ __launch_bounds__(256) void dense_list_kernel(
+    int n, int n_rb, const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
+    const uint32_t* __restrict__ row_indices, const uint64_t* __restrict__ tiles, uint32_t w0, uint32_t w_end,
+    const uint32_t* __restrict__ tile_list, uint32_t lstride, unsigned long long* __restrict__ result,
+    int32_t* __restrict__ pv) {
+    constexpr int SLABS = K / 16;
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = uniform(w0 + blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (w >= w_end) return;
+    const uint32_t t = uniform(tile_list[(size_t)w * lstride]);
+    const uint32_t I = uniform(row_indices[t]);
+    const uint32_t JL = uniform(indices[t]);
+    const uint32_t a0 = uniform(indptr[I]), a1 = uniform(indptr[I + 1]);
+    if (MFMA) {
+        const int g = lane >> 4, c = lane & 15;
+        RowBits<K> M[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) M[v] = RowBits<K>::load(tiles, t, 4 * g + v);
+        uint32_t rs[4] = {0, 0, 0, 0};
+        for (int it = 0; it < SLABS; ++it) {
+            const uint32_t Jo = JL * SLABS + it;
+            if (Jo >= (uint32_t)n_rb) break;
+            const uint32_t b0 = uniform(indptr[Jo]), b1 = uniform(indptr[Jo + 1]);
+            v4i acc = {0, 0, 0, 0};
+            for_each_match(indices, a0, a1, b0, b1, lane,
+                           [&](uint32_t pa, uint32_t pb) { acc = mfma_pair<K>(tiles, pa, pb, c, g, acc); });
+            rs[0] += (M[0].bits(it * 16 + c) & 1u) ? (uint32_t)acc.x : 0u;
+            rs[1] += (M[1].bits(it * 16 + c) & 1u) ? (uint32_t)acc.y : 0u;
+            rs[2] += (M[2].bits(it * 16 + c) & 1u) ? (uint32_t)acc.z : 0u;
+            rs[3] += (M[3].bits(it * 16 + c) & 1u) ? (uint32_t)acc.w : 0u;
+        }
+        if (PV) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                uint32_t x = rs[v];
+                x += __shfl_xor(x, 1);
+                x += __shfl_xor(x, 2);
+                x += __shfl_xor(x, 4);
+                x += __shfl_xor(x, 8);
+                const int64_t vtx = (int64_t)I * 16 + 4 * g + v;
+                if (c == 0 && x && vtx < n) atomicAdd(pv + vtx, (int32_t)x);
+            }
+        }
+        const uint64_t tot = wave_sum_u64((uint64_t)(rs[0] + rs[1] + rs[2] + rs[3]));
+        if (lane == 0 && tot) atomicAdd(result, (unsigned long long)tot);
+        return;
+    }
+    const int r = lane >> 2, q = lane & 3;
+    const RowBits<K> S = RowBits<K>::load(tiles, t, r);
+    uint32_t sum = 0;
+    for (int it = 0; it < SLABS; ++it) {
+        const uint32_t Jo = JL * SLABS + it;
+        if (Jo >= (uint32_t)n_rb) break;
+        const uint32_t b0 = uniform(indptr[Jo]), b1 = uniform(indptr[Jo + 1]);
+        const uint32_t m4 = S.bits(it * 16 + 4 * q) & 0xFu;
+        uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+        for_each_match(indices, a0, a1, b0, b1, lane, [&](uint32_t pa, uint32_t pb) {
+            const RowBits<K> A = RowBits<K>::load(tiles, pa, r);
+            acc0 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 0));
+            acc1 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 1));
+            acc2 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 2));
+            acc3 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 3));
+        });
+        sum += ((m4 & 1u) ? acc0 : 0u) + ((m4 & 2u) ? acc1 : 0u) + ((m4 & 4u) ? acc2 : 0u) + ((m4 & 8u) ? acc3 : 0u);
+    }
+    if (PV) {
+        uint32_t rsum = sum + __shfl_xor(sum, 1);
+        rsum += __shfl_xor(rsum, 2);
+        const int64_t v = (int64_t)I * 16 + r;
+        if (q == 0 && rsum && v < n) atomicAdd(pv + v, (int32_t)rsum);
+    }
+    const uint64_t tot = wave_sum_u64((uint64_t)sum);
+    if (lane == 0 && tot) atomicAdd(result, (unsigned long long)tot);
+}
+
 template <int K>
 __global__ __launch_bounds__(256) void tile_popcount_kernel(const uint64_t* __restrict__ tiles, uint32_t nblk,
                                                              uint32_t* __restrict__ pop) {
@@ -556,6 +670,41 @@ void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_
         case 32: launch_kk<32>(b, tile_list, lstride, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
         case 64: launch_kk<64>(b, tile_list, lstride, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
         default: launch_kk<128>(b, tile_list, lstride, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
+    }
+}
+
+namespace {
+template <int K, bool MF>
+void launch_dense_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstride, uint32_t nt,
+                    unsigned long long* res, int32_t* pv, hipStream_t s) {
+    for (uint32_t w0 = 0; w0 < nt; w0 += kMaxTilesPerLaunch) {
+        const uint32_t chunk = std::min(kMaxTilesPerLaunch, nt - w0);
+        const unsigned grid = (chunk + 3) / 4;
+        if (pv)
+            hipLaunchKernelGGL((dense_list_kernel<K, true, MF>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
+                               b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, w0 + chunk, list, lstride,
+                               res, pv);
+        else
+            hipLaunchKernelGGL((dense_list_kernel<K, false, MF>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
+                               b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, w0 + chunk, list, lstride,
+                               res, pv);
+        BTC_CHECK_LAUNCH("dense_list_kernel");
+    }
+}
+}  // namespace
+
+void launch_dense_list(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles, bool mfma,
+                       unsigned long long* d_result, int32_t* d_pv, hipStream_t s, uint32_t lstride) {
+    if (ntiles == 0) return;
+    BTC_REQUIRE(b->max_row <= 65536, BTC_ERR_OVERFLOW,
+                "longest row %d > 65536: per-lane 32-bit sums could overflow", b->max_row);
+    switch (b->K) {
+        case 32: mfma ? launch_dense_k<32, true>(b, tile_list, lstride, ntiles, d_result, d_pv, s)
+                      : launch_dense_k<32, false>(b, tile_list, lstride, ntiles, d_result, d_pv, s); break;
+        case 64: mfma ? launch_dense_k<64, true>(b, tile_list, lstride, ntiles, d_result, d_pv, s)
+                      : launch_dense_k<64, false>(b, tile_list, lstride, ntiles, d_result, d_pv, s); break;
+        default: mfma ? launch_dense_k<128, true>(b, tile_list, lstride, ntiles, d_result, d_pv, s)
+                      : launch_dense_k<128, false>(b, tile_list, lstride, ntiles, d_result, d_pv, s); break;
     }
 }
 

@@ -744,9 +744,8 @@ void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, M
     }
     for (int j = 0; j < k; ++j)
         if (P[j]->n_dense)
-            launch_tile_count(P[j]->bsr.get(), P[j]->dense_list.p, P[j]->n_dense, false,
-                              P[j]->launch.dense_kernel == 2, -1, ws.totals.p + j,
-                              nullptr, s);
+            launch_dense_list(P[j]->bsr.get(), P[j]->dense_list.p, P[j]->n_dense, P[j]->launch.dense_kernel == 2,
+                              ws.totals.p + j, nullptr, s);
     if (ni) {
         unsigned grid = 0;
         WedgeFn fn = wedge_fn(P[0]->launch, false, &grid);
@@ -802,16 +801,16 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
         if (tw) tw->start(s);
         launch_wedge(p, d_total, d_counters, d_pv, s);
         if (tw) tw->stop(s);
-        launch_tile_count(p->bsr.get(), p->dense_list.p, p->n_dense, false, p->launch.dense_kernel == 2, -1,
-                          d_total, d_pv, p->side);
+        launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, p->launch.dense_kernel == 2, d_total, d_pv,
+                          p->side);
         BTC_HIP(hipEventRecord(p->ev_join, p->side));
         BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
         if (tm) tm->stop(s);
         return;
     }
     if (tm) tm->start(s);
-    if (p->n_dense) launch_tile_count(p->bsr.get(), p->dense_list.p, p->n_dense, false, p->launch.dense_kernel == 2, -1,
-                                        d_total, d_pv, s);
+    if (p->n_dense)
+        launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, p->launch.dense_kernel == 2, d_total, d_pv, s);
     if (tw) tw->start(s);
     if (p->nitems) launch_wedge(p, d_total, d_counters, d_pv, s);
     if (tm) tm->stop(s);
@@ -955,8 +954,8 @@ int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream
         if (p->nitems)
             btcx::launch_wedge(p, p->scratch.p, reinterpret_cast<uint32_t*>(p->scratch.p + 1), nullptr, s, j, k);
         if (nd)
-            btcx::launch_tile_count(p->bsr.get(), p->dense_list.p + j, nd, false, p->launch.dense_kernel == 2, -1,
-                                    p->scratch.p, nullptr, ds, k);
+            btcx::launch_dense_list(p->bsr.get(), p->dense_list.p + j, nd, p->launch.dense_kernel == 2, p->scratch.p,
+                                    nullptr, ds, k);
         if (ds != s) {
             BTC_HIP(hipEventRecord(p->ev_join, ds));
             BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));

@@ -124,6 +124,12 @@ void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_
                        bool binsearch, bool mfma, int tau, unsigned long long* d_result, int32_t* d_pv,
                        hipStream_t s, uint32_t lstride = 1);
 
+// Dense-path count over a list of mask tiles (the engine's dense tiles):
+// like launch_tile_count with tau < 0, the two row blocks' tile columns
+// matched by a wave-parallel binary search instead of a two-pointer walk.
+void launch_dense_list(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles, bool mfma,
+                       unsigned long long* d_result, int32_t* d_pv, hipStream_t s, uint32_t lstride = 1);
+
 // p[0, n) = 0 by a kernel (safe inside captured graphs).
 void launch_zero_u32(uint32_t* p, size_t n, hipStream_t s);
 
