@@ -194,11 +194,9 @@ def configs_leg(btc, runs):
     average degree 16, the 27-point stencil 64^3 and the planted-community
     graph 2^17.  Each: the engine (degree-ordered L, plan resident), the
     reference-semantics kernels at the Lite tile width on the natural
-    (compacted input) order, and the reference CPU verifier; all three
-    totals must agree (and the stencil's must equal its closed form)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-
+    (compacted input) order, and the library's host merge count
+    (btc_count_cpu); all three totals must agree (and the stencil's must
+    equal its closed form)."""
     out = {}
     for name in ("er", "stencil", "planted"):
         src, dst, n_in, cfg = make_workload(btc, name)
@@ -217,10 +215,10 @@ def configs_leg(btc, runs):
         for _ in range(runs):
             tp, _, ms = b.count()
             pk.append(ms)
-        rp, col = g.csr()
         threads = int(os.environ.get("BTC_CPU_THREADS", min(16, os.cpu_count() or 1)))
+        btc.count_cpu(g, threads)  # the host copy of L is made on first use
         t0 = time.perf_counter()
-        tc = oracle.ref_cpu_tc(rp, col, threads) if oracle.ref_available() else oracle.tc_merge(rp, col, threads)
+        tc, _ = btc.count_cpu(g, threads)
         cpu_s = time.perf_counter() - t0
         exact = t == tp == int(tc) and (name != "stencil" or t == 11049696)
         km, pm = statistics.median(ks), statistics.median(pk)
