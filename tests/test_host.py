@@ -141,3 +141,15 @@ def test_lite_policy_matches_oracle(btc, orc):
         for mode in (0, 1, 2):
             assert btc.lite_tile_k(n, m, mode) == orc.lite_tile_k(n, m, mode)
     assert [btc.default_tau(k) for k in (32, 64, 128)] == [64, 128, 512]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says why."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
