@@ -1,7 +1,7 @@
 // Autotuner: picks the dense tile width K, the density threshold tau, the
-// dense-tile kernel form (AND + popcount or i8 MFMA) and the wedge kernel's
-// launch shape (lanes per list, loads in flight, resident workgroups per
-// CU) by timing candidates on the device, instead
+// dense-tile kernel form (AND + popcount or i8 MFMA), the hub-core size and
+// the wedge kernel's launch shape (lanes per list, loads in flight, resident
+// workgroups per CU) by timing candidates on the device, instead
 // of the reference's fixed rules (apps/btc_tc_lite.cu:45-65 for K,
 // tc_16x128_mma_v5.h:51 / tc_16x32_mma_v6.h:52 for tau).
 //
@@ -9,7 +9,7 @@
 // m, wedge count and longest row), which btc_plan_create consults when K,
 // tau or the launch shape are left on auto.  The table is saved and loaded
 // as text, one entry per line:
-//   btc-tune-2 <n_bits> <m_bits> <w_bits> <r_bits> <K> <tau> <G> <U> <wpe> <dense> <ms>
+//   btc-tune-3 <n_bits> <m_bits> <w_bits> <r_bits> <K> <tau> <G> <U> <wpe> <dense> <core> <ms>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -52,6 +52,7 @@ bool tune_table_apply(const btc_graph_s* g, btc_count_opts* o) {
     if (o->unroll == 0) o->unroll = t.unroll;
     if (o->waves_per_eu == 0) o->waves_per_eu = t.waves_per_eu;
     if (o->dense_kernel == 0) o->dense_kernel = t.dense_kernel;
+    if (o->core_ids == 0) o->core_ids = t.core_ids;
     return true;
 }
 
@@ -144,6 +145,27 @@ int btc_autotune(btc_graph g, const btc_tune_opts* topts, void* stream, btc_coun
             }
         }
         BTC_REQUIRE(any, BTC_ERR_ARG, "no tile width selected by tile_ks_mask");
+        // hub-core size on the best plan's tile width, tau and launch shape
+        for (int core : {1024, 4096, -1}) {
+            if (elapsed_ms() > t.budget_ms) break;
+            btc_count_opts o = best_o;
+            o.core_ids = core;
+            btc_plan p = nullptr;
+            btcx::check_status(btc_plan_create(g, &o, stream, &p));
+            uint64_t total = 0;
+            float ms = 0, mn = 1e30f;
+            int rc = btc_plan_count(p, stream, &total, nullptr, &ms);  // warm-up
+            for (int r = 0; r < t.reps && rc == BTC_OK; ++r) {
+                rc = btc_plan_count(p, stream, &total, nullptr, &ms);
+                mn = std::min(mn, ms);
+            }
+            btc_plan_free(p);
+            btcx::check_status(rc);
+            if (mn < best_t) {
+                best_t = mn;
+                best_o = o;
+            }
+        }
         *best = best_o;
         if (best_ms) *best_ms = best_t;
         std::lock_guard<std::mutex> lk(btcx::g_mu);
@@ -167,9 +189,9 @@ int btc_tune_table_save(const char* path) {
         std::lock_guard<std::mutex> lk(btcx::g_mu);
         for (const auto& kv : btcx::g_table) {
             const auto& o = kv.second.opts;
-            std::fprintf(f, "btc-tune-2 %d %d %d %d %d %d %d %d %d %d %.4f\n", std::get<0>(kv.first),
+            std::fprintf(f, "btc-tune-3 %d %d %d %d %d %d %d %d %d %d %d %.4f\n", std::get<0>(kv.first),
                          std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), o.tile_k, o.tau,
-                         o.group_lanes, o.unroll, o.waves_per_eu, o.dense_kernel, kv.second.ms);
+                         o.group_lanes, o.unroll, o.waves_per_eu, o.dense_kernel, o.core_ids, kv.second.ms);
         }
         std::fclose(f);
     });
@@ -181,17 +203,18 @@ int btc_tune_table_load(const char* path) {
         FILE* f = std::fopen(path, "r");
         BTC_REQUIRE(f, BTC_ERR_ARG, "cannot read %s", path);
         char tag[32];
-        int a, b, c, d, K, tau, G, U, w, D;
+        int a, b, c, d, K, tau, G, U, w, D, core;
         float ms;
         int line = 0;
         std::map<btcx::Key, btcx::Tuned> tab;
         for (;;) {
-            const int r = std::fscanf(f, "%31s %d %d %d %d %d %d %d %d %d %d %f", tag, &a, &b, &c, &d, &K, &tau, &G,
-                                      &U, &w, &D, &ms);
+            const int r = std::fscanf(f, "%31s %d %d %d %d %d %d %d %d %d %d %d %f", tag, &a, &b, &c, &d, &K, &tau,
+                                      &G, &U, &w, &D, &core, &ms);
             if (r == EOF) break;
             ++line;
-            if (r != 12 || std::strcmp(tag, "btc-tune-2") != 0 || !(K == 32 || K == 64 || K == 128) ||
-                !(G == 8 || G == 16 || G == 32) || !(U == 1 || U == 2 || U == 4) || w < 0 || D < 0 || D > 2) {
+            if (r != 13 || std::strcmp(tag, "btc-tune-3") != 0 || !(K == 32 || K == 64 || K == 128) ||
+                !(G == 8 || G == 16 || G == 32) || !(U == 1 || U == 2 || U == 4) || w < 0 || D < 0 || D > 2 ||
+                !(core <= 0 || (core % 128 == 0 && core <= 32768))) {
                 std::fclose(f);
                 btcx::fail(BTC_ERR_ARG, "%s: bad entry on line %d", path, line);
             }
@@ -203,6 +226,7 @@ int btc_tune_table_load(const char* path) {
             o.unroll = U;
             o.waves_per_eu = w;
             o.dense_kernel = D;
+            o.core_ids = core;
             tab[btcx::Key{a, b, c, d}] = btcx::Tuned{o, ms};
         }
         std::fclose(f);

@@ -15,16 +15,16 @@ def test_table_roundtrip_and_lookup(btc, tmp_path):
     key = (bits(i["n"]), bits(i["m"]), bits(i["wedges"]), bits(i["max_row"]))
     assert btc.tune_lookup(g) is None
     p = tmp_path / "tune.txt"
-    p.write_text("btc-tune-2 %d %d %d %d 128 700 16 2 6 2 1.2345\n" % key)
+    p.write_text("btc-tune-3 %d %d %d %d 128 700 16 2 6 2 1024 1.2345\n" % key)
     btc.tune_table_load(p)
     o = btc.tune_lookup(g)
-    assert (o["tile_k"], o["tau"], o["group_lanes"], o["unroll"], o["waves_per_eu"], o["dense_kernel"]) == \
-        (128, 700, 16, 2, 6, 2)
+    assert (o["tile_k"], o["tau"], o["group_lanes"], o["unroll"], o["waves_per_eu"], o["dense_kernel"],
+            o["core_ids"]) == (128, 700, 16, 2, 6, 2, 1024)
     q = tmp_path / "tune2.txt"
     btc.tune_table_save(q)
-    assert q.read_text().split()[:11] == p.read_text().split()[:11]
+    assert q.read_text().split()[:12] == p.read_text().split()[:12]
     bad = tmp_path / "bad.txt"
-    bad.write_text("btc-tune-2 1 2 3 4 96 7 8 1 0 1 1.0\n")  # K = 96 is not a tile width
+    bad.write_text("btc-tune-3 1 2 3 4 96 7 8 1 0 1 0 1.0\n")  # K = 96 is not a tile width
     with pytest.raises(btc.BtcError):
         btc.tune_table_load(bad)
     btc.tune_table_clear()
