@@ -216,7 +216,7 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
 // One work item (pivot row [t0, t1) of col, lists [eb, ee)) processed by NT
 // threads that share W words of zeroed LDS.  N_L(j) goes into LDS as a
 // bitmap window when max - min < 32 W, else into a hash table when
-// d_L(j) <= W / 2, else stays in HBM for binary search.  The LDS words are
+// d_L(j) <= hash_capacity(W), else stays in HBM for binary search.  The LDS words are
 // zero again on return.
 template <int NT, int W, int G, int U, bool PV>
 __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, uint4 A, uint4 B, uint32_t* lds,
@@ -242,8 +242,8 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
         acc = scan_lists<NT, G, U, PV>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, L, lds, bw, we4);
         team_sync<NT>();
         for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)a.col[p] >> 5) - bw] = 0;
-    } else if (d <= (uint32_t)W / 2 && lookup != 2) {
-        constexpr int kMaxLog = W == 4096 ? 12 : 10;
+    } else if (d <= hash_capacity(W) && lookup != 2) {
+        constexpr int kMaxLog = hash_log_max(W);
         const int logS = min(kMaxLog, max(6, 32 - __clz(4 * d - 1)));  // S >= 4d while it fits
         const int shift = 32 - logS;
         const uint32_t mask = (1u << logS) - 1u;

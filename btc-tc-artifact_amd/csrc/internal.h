@@ -84,8 +84,13 @@ __host__ __device__ inline uint32_t core_words(uint32_t lo, uint32_t hi, uint32_
 
 // A pivot's lists form one wave item when there are few of them and N_L(j)
 // fits a wave's LDS slice as a bitmap window or a hash table.
+// An LDS hash table in W words has 2^hash_log_max(W) slots and takes sets of
+// up to half that many ids (load <= 1/2).
+__host__ __device__ constexpr int hash_log_max(int W) { return W >= 4096 ? 12 : W >= 2048 ? 11 : 10; }
+__host__ __device__ constexpr uint32_t hash_capacity(int W) { return (1u << hash_log_max(W)) / 2; }
+
 __host__ __device__ inline bool wave_pivot(uint32_t lists, uint32_t lo, uint32_t hi, uint32_t d) {
-    return lists <= kWaveItemLists && (hi - lo < 32u * kWaveWords || d <= (uint32_t)kWaveWords / 2);
+    return lists <= kWaveItemLists && (hi - lo < 32u * kWaveWords || d <= hash_capacity(kWaveWords));
 }
 
 // A batch over an already block-diagonal canonical graph whose graphs
