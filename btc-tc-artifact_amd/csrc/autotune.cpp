@@ -1,7 +1,8 @@
 // Autotuner: picks the dense tile width K, the density threshold tau, the
 // dense-tile kernel form (AND + popcount or i8 MFMA), the hub-core size and
 // the wedge kernel's launch shape (lanes per list, loads in flight, resident
-// workgroups per CU) by timing candidates on the device, instead
+// workgroups per CU, static stride or work stealing from one counter) by
+// timing candidates on the device, instead
 // of the reference's fixed rules (apps/btc_tc_lite.cu:45-65 for K,
 // tc_16x128_mma_v5.h:51 / tc_16x32_mma_v6.h:52 for tau).
 //
@@ -119,7 +120,8 @@ int btc_autotune(btc_graph g, const btc_tune_opts* topts, void* stream, btc_coun
                     }
                     return mn;
                 };
-                const int shapes[4][3] = {{8, 1, 0}, {8, 2, 0}, {16, 1, 0}, {8, 1, 6}};
+                // {lanes per list, loads in flight, workgroups per CU, schedule}
+                const int shapes[5][4] = {{8, 1, 0, 1}, {8, 2, 0, 1}, {16, 1, 0, 1}, {8, 1, 6, 1}, {8, 1, 0, 0}};
                 btc_count_opts plan_best = o;
                 float plan_ms = 1e30f;
                 for (const auto& sh : shapes) {
@@ -127,6 +129,7 @@ int btc_autotune(btc_graph g, const btc_tune_opts* topts, void* stream, btc_coun
                     lo.group_lanes = sh[0];
                     lo.unroll = sh[1];
                     lo.waves_per_eu = sh[2];
+                    lo.schedule = sh[3];
                     lo.dense_kernel = 1;
                     const float ms = time_opts(lo);
                     if (ms < plan_ms) {

@@ -248,14 +248,15 @@ int btc_count(btc_graph g, const btc_count_opts* opts, void* stream, uint64_t* t
 /* ------------------------------------------------------------ autotuner
  * Times candidate plans (K in {32, 64, 128} selected by tile_ks_mask bits
  * 1/2/4, tau in {default, 2 x default, 16K = no dense tiles}), launch shapes
- * (lanes per list, loads in flight, resident workgroups per CU), the
- * dense-tile kernel form and the hub-core size (1024, 2048, 4096, none) on
+ * (lanes per list, loads in flight, resident workgroups per CU, static
+ * stride or work stealing), the dense-tile kernel form and the hub-core
+ * size (1024, 2048, 4096, none) on
  * the device within budget_ms and returns the fastest options.  The winner
  * is recorded in a process-wide table keyed by log2 of n, m, wedges and the
  * longest row; btc_plan_create fills options left on auto (tile_k 0, tau
  * < 0, group_lanes 0, unroll 0, waves_per_eu 0, dense_kernel 0, core_ids 0)
- * from that table.  Replaces the reference's fixed Lite rule and tau
- * literals. */
+ * from that table (the schedule is returned in `best` but not tabled).
+ * Replaces the reference's fixed Lite rule and tau literals. */
 typedef struct {
     int32_t budget_ms;    /* default 4000 */
     int32_t reps;         /* timed counts per candidate, best kept; default 3 */
