@@ -241,6 +241,15 @@ def batched_leg(btc, count, steps, sample, tune_ms):
     of them (each with its plan resident)."""
     t0 = time.perf_counter()
     work = btc.batch_workload(count, seed=1)
+    t_gen = time.perf_counter() - t0
+    # the batch built on the GPU straight from the pairs (one segmented
+    # canonicalisation + one plan), against per-graph host canonicalisation
+    t0 = time.perf_counter()
+    gb = btc.Batch.from_coo([(s, d, n) for s, d, n, _ in work])
+    t_gpu_build = time.perf_counter() - t0
+    _, per_gpu, _ = gb.count()
+    gb.close()
+    t0 = time.perf_counter()
     graphs = [btc.Graph.from_coo(s, d, n) for s, d, n, _ in work]
     del work
     t_canon = time.perf_counter() - t0
@@ -249,6 +258,9 @@ def batched_leg(btc, count, steps, sample, tune_ms):
     t_build = time.perf_counter() - t0
     info = batch.info
     batch.count()
+    gpu_build_exact = bool((batch.count()[1] == per_gpu).all())
+    log("batch built on the GPU from pairs in %.2f s (generation %.1f s); per-graph counts equal: %s"
+        % (t_gpu_build, t_gen, gpu_build_exact))
     wall = []
     per = None
     for _ in range(steps):
@@ -306,7 +318,8 @@ def batched_leg(btc, count, steps, sample, tune_ms):
             "single_launch_graphs_per_s": 1.0 / t_single, "speedup_vs_single": round(t_single * count / t_batch, 2),
             "single_sample": len(plans), "exact_vs_single": exact, "triangles": int(total),
             "single_replay_graphs_per_s": 1.0 / t_replay, "ms_per_launch_replay": round(statistics.median(wall_r) * 1e3, 3),
-            "build_s": round(t_build, 2), "canon_s": round(t_canon, 2), "work_items": info["plan"]["work_items"]}
+            "build_s": round(t_build, 2), "canon_s": round(t_canon, 2), "work_items": info["plan"]["work_items"],
+            "gpu_build_from_pairs_s": round(t_gpu_build, 3), "gpu_build_exact": gpu_build_exact}
 
 
 def serving_leg(btc, requests, seed=1, canon_flags=None, prep_threads=None, group_graphs=None):

@@ -137,6 +137,8 @@ _SIGS = {
     "btc_plan_count_share": (C.c_int, [vp, C.c_int32, C.c_int32, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
     "btc_batch_capture": (C.c_int, [vp, vp]),
     "btc_batch_autotune": (C.c_int, [vp, C.POINTER(TuneOpts), vp, C.POINTER(CountOpts), C.POINTER(C.c_float)]),
+    "btc_batch_from_coo": (C.c_int, [vp, vp, vp, vp, C.c_int32, C.c_uint32, C.POINTER(CountOpts), vp,
+                                     C.POINTER(vp)]),
     "btc_count": (C.c_int, [vp, C.POINTER(CountOpts), vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
     "btc_count_cpu": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_uint64), vp]),
     "btc_batch_create": (C.c_int, [vp, C.c_int32, C.POINTER(CountOpts), vp, C.POINTER(vp)]),
@@ -439,6 +441,24 @@ class Batch:
         h = vp()
         _check(lib().btc_batch_create(arr, self.n_graphs, C.byref(self.opts), stream, C.byref(h)))
         self._h = h
+
+    @classmethod
+    def from_coo(cls, graphs, flags=CANON_DEFAULT, stream=None, **opts):
+        """A batch built on the GPU straight from [(src, dst, n), ...]
+        (btc_batch_from_coo)."""
+        self = cls.__new__(cls)
+        self.n_graphs = len(graphs)
+        self.opts = make_opts(**opts)
+        off = np.zeros(self.n_graphs + 1, np.int64)
+        off[1:] = np.cumsum([len(g[0]) for g in graphs])
+        src = np.ascontiguousarray(np.concatenate([np.asarray(g[0], np.int32) for g in graphs]), np.int32)
+        dst = np.ascontiguousarray(np.concatenate([np.asarray(g[1], np.int32) for g in graphs]), np.int32)
+        nv = np.ascontiguousarray([g[2] for g in graphs], np.int32)
+        h = vp()
+        _check(lib().btc_batch_from_coo(_ptr(src), _ptr(dst), _ptr(off), _ptr(nv), self.n_graphs, flags,
+                                        C.byref(self.opts), stream, C.byref(h)))
+        self._h = h
+        return self
 
     @property
     def info(self):

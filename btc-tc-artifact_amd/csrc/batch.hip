@@ -136,6 +136,59 @@ int btc_batch_create(const btc_graph* graphs, int32_t count, const btc_count_opt
     });
 }
 
+int btc_batch_from_coo(const int32_t* src, const int32_t* dst, const int64_t* edge_offsets,
+                       const int32_t* n_per_graph, int32_t count, uint32_t flags, const btc_count_opts* opts,
+                       void* stream, btc_batch* out) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(out && edge_offsets && n_per_graph && count > 0, BTC_ERR_ARG, "bad argument");
+        *out = nullptr;
+        BTC_REQUIRE((flags & ~(BTC_CANON_DEFAULT | BTC_CANON_DEVICE)) == 0, BTC_ERR_ARG, "unknown flags 0x%x", flags);
+        BTC_REQUIRE(edge_offsets[0] == 0, BTC_ERR_ARG, "edge_offsets[0] != 0");
+        const int64_t pairs = edge_offsets[count];
+        BTC_REQUIRE(pairs == 0 || (src && dst), BTC_ERR_ARG, "src/dst is NULL");
+        // vertex ranges of the graphs (n, or max id + 1 when n <= 0); ids checked per graph
+        std::vector<int64_t> segs((size_t)count + 1, 0);
+        for (int32_t k = 0; k < count; ++k) {
+            const int64_t e0 = edge_offsets[k], e1 = edge_offsets[k + 1];
+            BTC_REQUIRE(e0 <= e1, BTC_ERR_ARG, "edge_offsets not ascending at graph %d", k);
+            int32_t lo = 0, hi = -1;
+            for (int64_t e = e0; e < e1; ++e) {
+                lo = std::min(lo, std::min(src[e], dst[e]));
+                hi = std::max(hi, std::max(src[e], dst[e]));
+            }
+            const int64_t n = n_per_graph[k] > 0 ? n_per_graph[k] : (int64_t)hi + 1;
+            BTC_REQUIRE(lo >= 0 && hi < n, BTC_ERR_ARG, "graph %d: vertex id out of [0, %lld)", k, (long long)n);
+            segs[(size_t)k + 1] = segs[(size_t)k] + n;
+        }
+        BTC_REQUIRE(segs.back() < INT32_MAX, BTC_ERR_OVERFLOW, "batch has %lld vertices (limit 2^31 - 1)",
+                    (long long)segs.back());
+        hipStream_t s = btcx::as_stream(stream);
+        std::vector<int32_t> hs((size_t)pairs), hd((size_t)pairs);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int32_t k = 0; k < count; ++k) {
+            const int32_t v0 = (int32_t)segs[(size_t)k];
+            for (int64_t e = edge_offsets[k]; e < edge_offsets[k + 1]; ++e) {
+                hs[(size_t)e] = src[e] + v0;
+                hd[(size_t)e] = dst[e] + v0;
+            }
+        }
+        btc_graph_s G;
+        std::vector<int64_t> csegs;
+        {
+            btcx::TmpBuf<int32_t> ds, dd;
+            ds.alloc((size_t)pairs, s);
+            dd.alloc((size_t)pairs, s);
+            if (pairs) {
+                BTC_HIP(hipMemcpyAsync(ds.p, hs.data(), (size_t)pairs * 4, hipMemcpyHostToDevice, s));
+                BTC_HIP(hipMemcpyAsync(dd.p, hd.data(), (size_t)pairs * 4, hipMemcpyHostToDevice, s));
+            }
+            btcx::canonicalise_device(ds.p, dd.p, pairs, (int32_t)segs.back(), flags & ~BTC_CANON_DEVICE, s, &G,
+                                      &segs, &csegs);
+        }
+        *out = btcx::batch_from_graph(std::move(G), csegs, opts, s);
+    });
+}
+
 int btc_batch_get_info(btc_batch b, btc_batch_info* info) {
     return btcx::guard([&] {
         BTC_REQUIRE(b && info, BTC_ERR_ARG, "NULL argument");

@@ -119,26 +119,29 @@ __global__ void degree_kernel(const uint64_t* __restrict__ keys, int64_t m, int 
     }
 }
 
-// key = ~degree (ascending = descending degree), value = vertex.  With
-// segments (seg = nseg + 1 ascending vertex offsets), key = segment << 26 |
-// (2^26 - 1 - degree), so that a stable sort orders by segment, then
-// descending degree, then id.
-constexpr int kSegShift = 26;
-__global__ void rank_keys_kernel(const uint32_t* __restrict__ deg, int32_t n, const uint32_t* __restrict__ seg,
-                                 int nseg, uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+// key = ~degree (ascending = descending degree), value = vertex
+__global__ void rank_keys_kernel(const uint32_t* __restrict__ deg, int32_t n, uint32_t* __restrict__ key,
+                                 uint32_t* __restrict__ val) {
     for (int64_t v = blockIdx.x * (int64_t)kT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kT) {
-        if (seg) {
-            int lo = 0, hi = nseg;  // last segment with seg[s] <= v
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (seg[mid] <= (uint32_t)v) lo = mid;
-                else hi = mid;
-            }
-            key[v] = ((uint32_t)lo << kSegShift) | (((1u << kSegShift) - 1) - deg[v]);
-        } else {
-            key[v] = ~deg[v];
-        }
+        key[v] = ~deg[v];
         val[v] = (uint32_t)v;
+    }
+}
+
+// key[r] = segment of vertex order[r] (seg = nseg + 1 ascending vertex
+// offsets; the last segment whose start is <= the vertex), val[r] = order[r]
+__global__ void segment_keys_kernel(const uint32_t* __restrict__ order, int32_t n, const uint32_t* __restrict__ seg,
+                                    int nseg, uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+    for (int64_t r = blockIdx.x * (int64_t)kT + threadIdx.x; r < n; r += (int64_t)gridDim.x * kT) {
+        const uint32_t v = order[r];
+        int lo = 0, hi = nseg;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (seg[mid] <= v) lo = mid;
+            else hi = mid;
+        }
+        key[r] = (uint32_t)lo;
+        val[r] = v;
     }
 }
 
@@ -226,8 +229,8 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
     BTC_REQUIRE(nnz < (int64_t)0xFFFFFFFFLL, BTC_ERR_OVERFLOW, "nnz %lld exceeds 2^32 - 1 pairs", (long long)nnz);
     const int nseg = segs ? (int)segs->size() - 1 : 0;
     if (segs) {
-        BTC_REQUIRE(nseg >= 1 && nseg <= 64 && csegs && (*segs)[0] == 0 && segs->back() == n && n < (1 << kSegShift),
-                    BTC_ERR_ARG, "bad segments (%d segments over %d vertices)", nseg, n);
+        BTC_REQUIRE(nseg >= 1 && csegs && (*segs)[0] == 0 && segs->back() == n, BTC_ERR_ARG,
+                    "bad segments (%d segments over %d vertices)", nseg, n);
         for (int k = 0; k < nseg; ++k) BTC_REQUIRE((*segs)[k] <= (*segs)[k + 1], BTC_ERR_ARG, "segments not ascending");
     }
     // id range (and n when not given)
@@ -305,7 +308,8 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
         if (newid.p) {
             TmpBuf<uint32_t> tmp;
             tmp.alloc(at.size(), s);
-            hipLaunchKernelGGL(gather_kernel, dim3(1), dim3(kT), 0, s, newid.p, dseg.p, (int)at.size(), tmp.p);
+            hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((at.size() + kT - 1) / kT)), dim3(kT), 0, s, newid.p,
+                               dseg.p, (int)at.size(), tmp.p);
             BTC_CHECK_LAUNCH("gather_kernel");
             BTC_HIP(hipMemcpyAsync(dseg.p, tmp.p, at.size() * 4, hipMemcpyDeviceToDevice, s));
         }
@@ -326,8 +330,21 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
         lab2.alloc((size_t)nc);
         BTC_HIP(hipMemsetAsync(deg.p, 0, (size_t)nc * 4, s));
         if (m) BTC_LAUNCH(degree_kernel, m, s, keys, m, bn, deg.p);
-        BTC_LAUNCH(rank_keys_kernel, nc, s, deg.p, nc, segs ? dseg.p : nullptr, nseg, k0.p, v0.p);
-        const bool alt = sort_pairs_u32(k0.p, v0.p, k1.p, v1.p, (size_t)nc, 32, s);
+        BTC_LAUNCH(rank_keys_kernel, nc, s, deg.p, nc, k0.p, v0.p);
+        bool alt = sort_pairs_u32(k0.p, v0.p, k1.p, v1.p, (size_t)nc, 32, s);
+        if (segs) {
+            // then a stable sort by segment: (segment, -degree, id)
+            uint32_t* ord = alt ? v1.p : v0.p;
+            uint32_t* kk = alt ? k0.p : k1.p;  // free key buffer
+            uint32_t* vv = alt ? v0.p : v1.p;  // free value buffer
+            BTC_LAUNCH(segment_keys_kernel, nc, s, ord, nc, dseg.p, nseg, kk, vv);
+            int sb = 1;
+            while (sb < 32 && (1ll << sb) < (int64_t)nseg) ++sb;
+            // sort (kk, vv) using (other key buffer, ord) as the spare pair
+            uint32_t* kk2 = alt ? k1.p : k0.p;
+            const bool alt2 = sort_pairs_u32(kk, vv, kk2, ord, (size_t)nc, sb, s);
+            alt = alt2 ? (ord == v1.p) : (vv == v1.p);
+        }
         BTC_LAUNCH(rank_scatter_kernel, nc, s, alt ? v1.p : v0.p, nc, rank.p, labels.p, lab2.p);
         if (m) BTC_LAUNCH(relabel_kernel, m, s, keys, m, bn, rank.p, bn);
         keys = sort_u64(keys, spare, (size_t)m, 2 * bn, s);

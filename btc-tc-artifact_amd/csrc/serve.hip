@@ -44,7 +44,7 @@ using Clock = std::chrono::steady_clock;
 // graphs and kGroupPairs input pairs.
 constexpr int kGroupGraphsMax = 64;
 constexpr size_t kGroupPairs = size_t(1) << 22;
-constexpr int64_t kGroupVertices = int64_t(1) << 25;  // below the segmented relabel's 2^26 limit
+constexpr int64_t kGroupVertices = int64_t(1) << 25;  // keeps a group's vertex ids well inside int32
 
 struct Unit {  // what is prepared, resident and evicted together
     btc_graph graph = nullptr;  // a single graph: its L and plan

@@ -290,6 +290,14 @@ typedef struct {
 
 int btc_batch_create(const btc_graph* graphs, int32_t count, const btc_count_opts* opts, void* stream,
                      btc_batch* out);
+/* A batch straight from the graphs' pairs (host buffers): graph k's pairs are
+ * src/dst[edge_offsets[k], edge_offsets[k+1]) with ids in [0, n_per_graph[k])
+ * (n <= 0: max id + 1).  All graphs are canonicalised in one GPU pass as
+ * separate vertex ranges (each compacted and relabelled by degree within
+ * itself, per `flags`) and planned as one block-diagonal graph. */
+int btc_batch_from_coo(const int32_t* src, const int32_t* dst, const int64_t* edge_offsets,
+                       const int32_t* n_per_graph, int32_t count, uint32_t flags, const btc_count_opts* opts,
+                       void* stream, btc_batch* out);
 int btc_batch_get_info(btc_batch b, btc_batch_info* info);
 int btc_batch_tune(btc_batch b, const btc_count_opts* opts);
 /* per_graph: host uint64[count] or NULL; total: sum over the batch. */

@@ -58,3 +58,40 @@ def test_batch_graphs_may_be_freed(gpu, orc):
     for g in gs:
         g.close()
     assert b.count()[1].tolist() == want
+
+
+@pytest.mark.parametrize("flags", [0, 1, 3])
+def test_batch_from_coo_matches_graphs(gpu, orc, flags):
+    """One GPU canonicalisation of many graphs as separate vertex ranges
+    (with empty, loop-only and padded graphs among them) counts every graph
+    exactly as it counts alone."""
+    work = gpu.batch_workload(40, seed=11, n_lo=64, n_hi=3000)
+    graphs = [(s, d, n) for s, d, n, _ in work]
+    graphs.insert(5, (np.zeros(0, np.int32), np.zeros(0, np.int32), 17))     # no edges
+    graphs.insert(9, (np.arange(6, dtype=np.int32), np.arange(6, dtype=np.int32), 6))  # loops only
+    s, d, n = graphs[12]
+    graphs[12] = (s, d, n + 500)  # isolated vertices after the used ids
+    want = []
+    for s, d, n in graphs:
+        g = gpu.Graph.from_coo(s, d, n, flags)
+        rp, c = g.csr()
+        want.append(orc.tc_merge(rp, c, threads=2))
+    b = gpu.Batch.from_coo(graphs, flags)
+    total, per, _ = b.count()
+    assert [int(x) for x in per] == want and total == sum(want)
+    assert b.info["graphs"] == len(graphs)
+    with pytest.raises(gpu.BtcError):
+        gpu.Batch.from_coo([(np.array([0, 3], np.int32), np.array([1, 2], np.int32), 3)])  # id 3 >= n
+
+
+def test_batch_from_coo_many_graphs(gpu, orc):
+    """Hundreds of graphs (more than one block of segment starts)."""
+    work = gpu.batch_workload(700, seed=13, n_lo=16, n_hi=200)
+    graphs = [(s, d, n) for s, d, n, _ in work]
+    b = gpu.Batch.from_coo(graphs)
+    total, per, _ = b.count()
+    want = []
+    for s, d, n in graphs:
+        rp, c = gpu.Graph.from_coo(s, d, n).csr()
+        want.append(orc.tc_merge(rp, c))
+    assert [int(x) for x in per] == want and total == sum(want)
