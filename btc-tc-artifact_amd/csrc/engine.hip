@@ -103,6 +103,7 @@ struct SearchSet {  // binary search of the sorted pivot row in HBM
 // Device view of one plan's work lists.
 struct PlanDev {
     const int32_t* col;
+    const uint16_t* col16;     // col's low 16 bits, read for pivots whose ids are below kNarrowIds
     const uint4* lists;        // per list {s0, s_noncore, s1, row bitmap or kNoBitmap}
     const uint4* rowbits;      // core4 uint4 per row bitmap
     const int32_t* ur;
@@ -148,12 +149,14 @@ __device__ __forceinline__ void team_sync() {
 // bitmap (and whose pivot is in an LDS bitmap) first ANDs the row's core
 // bitmap with the pivot's window word by word (popcount) and then looks up
 // only the prefix ids >= C; any other list looks up its whole clipped
-// prefix.  Prefix ids are read with aligned 16-byte loads (4 per lane, U in
-// flight); ids outside the prefix become the set's sentinel.  The next
+// prefix.  Prefix ids are read 4 per lane (U loads in flight) with aligned
+// loads: 16 bytes of 32-bit ids, or, when N16 (every id of the pivot below
+// kNarrowIds, so every prefix id too), 8 bytes of the 16-bit copy;
+// ids outside the prefix become the set's sentinel.  The next
 // list's record is fetched before the current list is scanned; records are
 // read once per launch, so they are loaded non-temporally.  L is the
 // group's first record, loaded by the caller.
-template <int NT, int G, int U, bool PV, class Set>
+template <int NT, int G, int U, bool PV, bool N16, class Set>
 __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a, uint32_t tid, uint32_t eb,
                                                uint32_t ee, uint4 L, const uint32_t* window, uint32_t bw,
                                                uint32_t we4) {
@@ -161,7 +164,7 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
     constexpr uint32_t STEP = 4 * G * U;
     const uint32_t g = tid / G, gl = tid % G;
     // 32-bit byte offsets (nnz(L) + padding < 2^30 is enforced by the planner)
-    const char* colb = reinterpret_cast<const char*>(a.col);
+    const char* colb = N16 ? reinterpret_cast<const char*>(a.col16) : reinterpret_cast<const char*>(a.col);
     const uint32_t sent = set.sentinel();
     uint64_t acc = 0;
     for (uint32_t e = eb + g; e < ee; e += NG) {
@@ -189,8 +192,14 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t qq = q + 4 * G * u;
-                v[u] = qq < s1 ? *reinterpret_cast<const uint4*>(colb + (qq << 2))
-                               : make_uint4(sent, sent, sent, sent);
+                if constexpr (N16) {
+                    uint2 h = make_uint2(0u, 0u);  // ids at or past s1 are masked below
+                    if (qq < s1) h = *reinterpret_cast<const uint2*>(colb + (qq << 1));
+                    v[u] = make_uint4(h.x & 0xFFFFu, h.x >> 16, h.y & 0xFFFFu, h.y >> 16);
+                } else {
+                    v[u] = qq < s1 ? *reinterpret_cast<const uint4*>(colb + (qq << 2))
+                                   : make_uint4(sent, sent, sent, sent);
+                }
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -230,6 +239,7 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
         if (e < ee) L = load_nt(a.lists + e);
     }
     uint64_t acc = 0;
+    const bool narrow = hi < kNarrowIds;
     const uint32_t base = lo & ~127u;  // 128-aligned: window words line up with core bitmap uint4s
     if (hi - base < 32u * W - 1u && lookup != 1) {  // the window's last bit stays 0 (sentinel)
         const uint32_t bw = base >> 5;
@@ -239,7 +249,9 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
         }
         team_sync<NT>();
         const uint32_t we4 = min(a.core4, (hi >> 7) + 1u);  // uint4 words up to the pivot's last id
-        acc = scan_lists<NT, G, U, PV>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, L, lds, bw, we4);
+        const BitmapSet set{lds - bw, base + 32u * W - 1u};
+        acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, lds, bw, we4)
+                     : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, lds, bw, we4);
         team_sync<NT>();
         for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)a.col[p] >> 5) - bw] = 0;
     } else if (d <= hash_capacity(W) && lookup != 2) {
@@ -255,11 +267,15 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
             while (atomicCAS(&lds[h], kEmpty, key) != kEmpty) h = (h + 1u) & mask;
         }
         team_sync<NT>();
-        acc = scan_lists<NT, G, U, PV>(HashSet{lds, shift, mask}, a, tid, eb, ee, L, lds, 0u, 0u);
+        const HashSet set{lds, shift, mask};
+        acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, lds, 0u, 0u)
+                     : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, lds, 0u, 0u);
         team_sync<NT>();
         for (uint32_t x = tid; x <= mask; x += NT) lds[x] = 0;
     } else {
-        acc = scan_lists<NT, G, U, PV>(SearchSet{a.col, t0, t1}, a, tid, eb, ee, L, lds, 0u, 0u);
+        const SearchSet set{a.col, t0, t1};
+        acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, lds, 0u, 0u)
+                     : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, lds, 0u, 0u);
     }
     return acc;
 }
@@ -275,29 +291,37 @@ __device__ __forceinline__ PlanDev uniform_plan(const PlanDev* p) {
     return q;
 }
 
-// Item t of the launch: its plan and its two descriptor words.  Wave items
-// are numbered after all workgroup items.
+// Item t of the launch: its plan's index and its two descriptor words
+// (prefetched one item ahead; the plan descriptor itself is read when the
+// item runs, which keeps the prefetch to a few registers).  Wave items are
+// numbered after all workgroup items.
 struct Item {
-    PlanDev P;
+    uint32_t plan;
     uint4 A, B;
 };
 
 __device__ __forceinline__ Item fetch_item(const WedgeArgs& a, uint32_t t) {
     Item r;
+    const uint4* items = a.p0.items;
     uint32_t li = t;
+    r.plan = 0;
     if (a.nplans == 1) {
         // single plan: every shares-th item of each class from share on
         // (the plan's own numbering when shares = 1)
-        r.P = a.p0;
-        li = t < a.n_block ? a.share + t * a.shares : r.P.n_block + a.share + (t - a.n_block) * a.shares;
+        li = t < a.n_block ? a.share + t * a.shares : a.p0.n_block + a.share + (t - a.n_block) * a.shares;
     } else {
         const uint2 m = a.map[t];
-        r.P = uniform_plan(a.plans + uniform(m.x));
+        r.plan = uniform(m.x);
         li = uniform(m.y);
+        items = a.plans[r.plan].items;
     }
-    r.A = uniform4(r.P.items[2 * li]);
-    r.B = uniform4(r.P.items[2 * li + 1]);
+    r.A = uniform4(items[2 * li]);
+    r.B = uniform4(items[2 * li + 1]);
     return r;
+}
+
+__device__ __forceinline__ PlanDev item_plan(const WedgeArgs& a, const Item& it) {
+    return a.nplans == 1 ? a.p0 : uniform_plan(a.plans + it.plan);
 }
 
 // Adds a team's partial counts into a plan's total (multi-plan launches
@@ -341,8 +365,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             const uint32_t nx = it + gridDim.x;
             Item nxt;
             if (nx < a.n_block) nxt = fetch_item(a, nx);
-            acc += process_item<kThreads, kLdsWords, G, U, PV>(cur.P, a.lookup, cur.A, cur.B, lds, tid);
-            if (multi) flush(acc, cur.P.result);
+            const PlanDev P = item_plan(a, cur);
+            acc += process_item<kThreads, kLdsWords, G, U, PV>(P, a.lookup, cur.A, cur.B, lds, tid);
+            if (multi) flush(acc, P.result);
             __syncthreads();
             it = nx;
             cur = nxt;
@@ -354,8 +379,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             const uint32_t it = uniform(s_item);
             if (it >= a.n_block) break;
             const Item cur = fetch_item(a, it);
-            acc += process_item<kThreads, kLdsWords, G, U, PV>(cur.P, a.lookup, cur.A, cur.B, lds, tid);
-            if (multi) flush(acc, cur.P.result);
+            const PlanDev P = item_plan(a, cur);
+            acc += process_item<kThreads, kLdsWords, G, U, PV>(P, a.lookup, cur.A, cur.B, lds, tid);
+            if (multi) flush(acc, P.result);
             __syncthreads();
         }
     }
@@ -372,8 +398,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             const uint32_t nx = it + stride;
             Item nxt;
             if (nx < nwave) nxt = fetch_item(a, a.n_block + nx);
-            acc += process_item<64, kWaveWords, G, U, PV>(cur.P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
-            if (multi) flush(acc, cur.P.result);
+            const PlanDev P = item_plan(a, cur);
+            acc += process_item<64, kWaveWords, G, U, PV>(P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
+            if (multi) flush(acc, P.result);
             team_sync<64>();
             it = nx;
             cur = nxt;
@@ -385,8 +412,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             it = uniform(__shfl(it, 0));
             if (it >= nwave) break;
             const Item cur = fetch_item(a, a.n_block + it);
-            acc += process_item<64, kWaveWords, G, U, PV>(cur.P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
-            if (multi) flush(acc, cur.P.result);
+            const PlanDev P = item_plan(a, cur);
+            acc += process_item<64, kWaveWords, G, U, PV>(P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
+            if (multi) flush(acc, P.result);
             team_sync<64>();
         }
     }
@@ -422,6 +450,20 @@ WedgeFn pick_wedge(int g, int u, bool pv) {
 
 namespace btcx {
 namespace {
+
+// col16[q] = low 16 bits of col[q], over L and its padding.
+__global__ void narrow_col_kernel(const int32_t* __restrict__ col, uint32_t n, uint16_t* __restrict__ col16) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+        col16[q] = (uint16_t)col[q];
+}
+
+void build_col16(btc_plan_s* p, hipStream_t s) {
+    const uint32_t n = (uint32_t)p->g->m + (uint32_t)kColPad;
+    p->col16.alloc(n);
+    const unsigned grid = std::min<unsigned>((n + kThreads - 1) / kThreads, 1u << 16);
+    hipLaunchKernelGGL(narrow_col_kernel, dim3(grid), dim3(kThreads), 0, s, p->g->d_col.p, n, p->col16.p);
+    BTC_CHECK_LAUNCH("narrow_col_kernel");
+}
 
 // Marks the L entries that fall into dense tiles and uploads the dense tile
 // list.  Leaves p->bsr empty when no tile is dense.
@@ -616,7 +658,7 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
         uint4 a, b;
     };
     std::vector<Item> block_items, wave_items;
-    int64_t checks = 0, pivot_words = 0;
+    int64_t checks = 0, pivot_words = 0, narrow_ids = 0;
     for (int32_t j = 0; j < n; ++j) {
         int64_t e = off[j], e1 = off[j + 1];
         if (e == e1) continue;
@@ -636,6 +678,8 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
             }
             checks += w;
             pivot_words += t1 - t0;
+            if (hi < kNarrowIds)
+                for (int64_t x = e; x < f; ++x) narrow_ids += lst[x].z > lst[x].y ? lst[x].z - lst[x].y : 0;
             Item it{make_uint4(t0, t1, (uint32_t)e, (uint32_t)f),
                     make_uint4(lo, hi, (uint32_t)std::min<int64_t>(w, 0xFFFFFFFFLL), 0u)};
             (small ? wave_items : block_items).push_back(it);
@@ -654,8 +698,9 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
     p->info.wedge_checks = checks;
     p->info.core_lists = core_lists.load();
     p->info.core_rows = nbm;
-    // looked-up ids and pivot rows (4 B each), list records (16 B), items (32 B)
-    p->info.stream_bytes = 4 * (checks + pivot_words) + 16 * ne + 32 * (int64_t)items.size();
+    // looked-up ids (4 B, or 2 B below kNarrowIds), core words and pivot
+    // rows (4 B), list records (16 B), items (32 B)
+    p->info.stream_bytes = 4 * (checks + pivot_words) - 2 * narrow_ids + 16 * ne + 32 * (int64_t)items.size();
     p->info.work_items = (int64_t)items.size();
     // one device allocation: items, list records, row bitmaps (16-byte
     // aligned), then list rows
@@ -706,7 +751,7 @@ WedgeFn wedge_fn(const btc_count_opts& o, bool pv, unsigned* grid) {
 }
 
 PlanDev plan_dev(btc_plan_s* p, unsigned long long* d_total, int32_t* d_pv) {
-    return PlanDev{p->g->d_col.p, p->list_p, p->rowbits_p, p->urow_p, p->items_p, p->n_block_items, p->nitems,
+    return PlanDev{p->g->d_col.p, p->col16.p, p->list_p, p->rowbits_p, p->urow_p, p->items_p, p->n_block_items, p->nitems,
                    d_total, d_pv, p->core_ids / 128};
 }
 
@@ -895,6 +940,7 @@ int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_p
         BTC_REQUIRE((uint64_t)g->m + btcx::kColPad < (1ull << 30), BTC_ERR_OVERFLOW,
                     "nnz(L) = %lld: the wedge kernel addresses L with 32-bit byte offsets (limit 2^30 - %zu)",
                     (long long)g->m, btcx::kColPad);
+        btcx::build_col16(p.get(), s);  // ordered before the planners' closing synchronisation
         int64_t chunk = o.chunk_work > 0 ? o.chunk_work : 0;
         if (!chunk) {
             const int64_t cus = btcx::device_cu_count();
@@ -914,7 +960,8 @@ int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_p
         p->info.tau = p->tau;
         p->info.core_ids = p->core_ids;
         int64_t bytes =
-            (int64_t)(p->lists.bytes() + p->items_dev.bytes() + p->dense_list.bytes() + p->scratch.bytes());
+            (int64_t)(p->lists.bytes() + p->items_dev.bytes() + p->dense_list.bytes() + p->scratch.bytes() +
+                      p->col16.bytes());
         if (p->bsr)
             bytes += (int64_t)(p->bsr->indptr.bytes() + p->bsr->indices.bytes() + p->bsr->row_indices.bytes() +
                                p->bsr->tiles.bytes());

@@ -35,6 +35,7 @@ struct btc_plan_s {
     const uint4* list_p = nullptr;
     const uint4* rowbits_p = nullptr;
     const int32_t* urow_p = nullptr;
+    btcx::DevBuf<uint16_t> col16;  // low 16 bits of every id of L (m + padding)
     uint32_t nitems = 0;
     uint32_t n_block_items = 0;
     btcx::DevBuf<unsigned long long> scratch;  // [0] total, [1] work-item counter
@@ -73,6 +74,10 @@ constexpr uint32_t kCoreIdsDefault = 2048;
 constexpr uint32_t kCoreIdsMax = 32768;  // = the id span of a wave's LDS bitmap window
 constexpr uint32_t kCoreRowMin = 32;
 constexpr uint32_t kNoBitmap = 0xFFFFFFFFu;
+// Pivots whose ids are all below this read their lists' prefixes from the
+// 16-bit copy of L's columns (every prefix id is at most the pivot's last
+// id), at half the bytes of the 32-bit ids.
+constexpr uint32_t kNarrowIds = 1u << 16;
 
 // Words of the core AND for a pivot whose ids span [lo, hi] (lo < C): from
 // the 128-aligned window base to the 128-id block holding min(hi, C - 1),

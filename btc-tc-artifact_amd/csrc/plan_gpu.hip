@@ -182,9 +182,9 @@ __global__ void records_kernel(const uint32_t* __restrict__ rowptr, const int32_
                                const uint32_t* __restrict__ erow, const uint32_t* __restrict__ val, uint32_t ne,
                                const uint32_t* __restrict__ cend, const uint32_t* __restrict__ has,
                                const uint32_t* __restrict__ rbi, uint32_t C, uint4* __restrict__ lst,
-                               int32_t* __restrict__ ur,
-                               uint64_t* __restrict__ cost, unsigned long long* __restrict__ ncore) {
-    uint64_t nc = 0;
+                               int32_t* __restrict__ ur, uint64_t* __restrict__ cost,
+                               unsigned long long* __restrict__ ncore, unsigned long long* __restrict__ nnarrow) {
+    uint64_t nc = 0, nn = 0;
     GS_LOOP(e, ne) {
         const uint32_t q = val[e], i = erow[q], j = (uint32_t)col[q];
         uint32_t s0 = 0, s1 = 0;
@@ -206,9 +206,12 @@ __global__ void records_kernel(const uint32_t* __restrict__ rowptr, const int32_
         lst[e] = make_uint4(s0, sc, s1, rb);
         ur[e] = (int32_t)i;
         cost[e] = (uint64_t)(s1 > sc ? s1 - sc : 0u) + words;
+        if (hi < kNarrowIds) nn += s1 > sc ? s1 - sc : 0u;
     }
     nc = wave_sum_u64(nc);
+    nn = wave_sum_u64(nn);
     if ((threadIdx.x & 63) == 0 && nc) atomicAdd(ncore, (unsigned long long)nc);
+    if ((threadIdx.x & 63) == 0 && nn) atomicAdd(nnarrow, (unsigned long long)nn);
 }
 
 __device__ __forceinline__ bool pivot_small(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
@@ -373,7 +376,7 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     P.alloc((size_t)ne + 1, s);
     if (ne)
         BTC_LAUNCH(records_kernel, ne, s, rowptr, col, erow.p, val, ne, cend.p, has.p, rbi.p, C, lst, ur, cost.p,
-                   st.p + 3);
+                   st.p + 3, st.p + 6);
     exclusive_scan_u64(cost.p, P.p, ne, s);
     erow.release();
 
@@ -402,7 +405,8 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
         BTC_LAUNCH(place_items_kernel, nitems, s, tmp.p, ialt ? sv1.p : sv0.p, nitems, p->items_dev.p);
 
     // st: [0] dense edges, [1] sparse edges, [2] their merge work, [3] core
-    // lists, [4] pivot row words over items, [5] wave items
+    // lists, [4] pivot row words over items, [5] wave items, [6] ids looked
+    // up in the 16-bit copy
     unsigned long long h[8];
     BTC_HIP(hipMemcpyAsync(h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
     const uint64_t checks = read_dev(P.p + ne, s);  // synchronises h as well
@@ -418,8 +422,10 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     p->info.core_rows = nbm;
     p->info.wedge_checks = (int64_t)checks;
     p->info.work_items = nitems;
-    // looked-up ids and pivot rows (4 B each), list records (16 B), items (32 B)
-    p->info.stream_bytes = 4 * ((int64_t)checks + (int64_t)h[4]) + 16 * (int64_t)ne + 32 * (int64_t)nitems;
+    // looked-up ids (4 B, or 2 B below kNarrowIds), core words and pivot
+    // rows (4 B), list records (16 B), items (32 B)
+    p->info.stream_bytes = 4 * ((int64_t)checks + (int64_t)h[4]) - 2 * (int64_t)h[6] + 16 * (int64_t)ne +
+                           32 * (int64_t)nitems;
 }
 
 }  // namespace btcx
