@@ -53,22 +53,62 @@ __device__ __forceinline__ uint4 load_nt(const uint4* p) {
     return make_uint4(v.x, v.y, v.z, v.w);
 }
 
+// Pointers into the global address space.  Every pointer of a plan points
+// to device memory, but the compiler cannot tell for pointers read from a
+// plan descriptor and emits flat loads, which also count against the LDS
+// wait counter, so each LDS wait would wait for them too.  Base pointers
+// are cast once and indexed in the global space (a cast applied after the
+// address arithmetic does not survive optimisation).
+// (Builtin vector types: HIP's uint2 / uint4 classes cannot be copied out
+// of an address-space qualified object.)
+typedef unsigned int V2U __attribute__((ext_vector_type(2)));
+typedef unsigned int V4U __attribute__((ext_vector_type(4)));
+typedef const char __attribute__((address_space(1)))* GlobalChar;
+typedef const int32_t __attribute__((address_space(1)))* GlobalI32;
+typedef const V2U __attribute__((address_space(1)))* GlobalU2;
+typedef const V4U __attribute__((address_space(1)))* GlobalU4;
+__device__ __forceinline__ uint2 to_u2(V2U v) { return make_uint2(v.x, v.y); }
+__device__ __forceinline__ uint4 to_u4(V4U v) { return make_uint4(v.x, v.y, v.z, v.w); }
+
 // Membership tests for N_L(pivot).  Ids of a list's clipped prefix lie in
 // [min N_L(j), max N_L(j)]; ids outside the prefix (partial 16-byte loads)
 // are replaced by sentinel(), which every set reports absent.
 struct BitmapSet {  // bit (k - base) of an LDS window; base is a multiple of 128
     static constexpr bool kBitmap = true;
+    static constexpr bool kFull16 = false;
     const uint32_t* words;  // LDS window minus base / 32 words: words[k >> 5] is k's word
     uint32_t last;          // base + window bits - 1: never set
     __device__ __forceinline__ uint32_t sentinel() const { return last; }
     __device__ __forceinline__ uint32_t has(uint32_t k) const {
-        // v_bfe_u32 reads the bit offset from k's low 5 bits
-        return __builtin_amdgcn_ubfe(words[k >> 5], k, 1);
+        // word index by a bit-field extract (one op with the scaled add
+        // after it); v_bfe_u32 reads the bit offset from k's low 5 bits
+        return __builtin_amdgcn_ubfe(words[__builtin_amdgcn_ubfe(k, 5, 27)], k, 1);
+    }
+};
+
+// An LDS window that starts at id 0 and spans every 16-bit id, for pivots
+// whose ids are all below kNarrowIds: any id read from the 16-bit copy of L
+// indexes inside it, so ids outside a prefix need no clamping and are
+// dropped afterwards by a mask.
+struct FullWindow16 {
+    static constexpr bool kBitmap = true;
+    static constexpr bool kFull16 = true;
+    const uint32_t* words;
+    // bits of the four 16-bit ids packed in h (bit i = id i); v_bfe_u32
+    // reads the bit offset from the low 5 bits of its operand, so the low
+    // id of a dword needs no unpacking
+    __device__ __forceinline__ uint32_t bits4(uint2 h) const {
+        const uint32_t b0 = __builtin_amdgcn_ubfe(words[__builtin_amdgcn_ubfe(h.x, 5, 11)], h.x, 1);
+        const uint32_t b1 = __builtin_amdgcn_ubfe(words[h.x >> 21], h.x >> 16, 1);
+        const uint32_t b2 = __builtin_amdgcn_ubfe(words[__builtin_amdgcn_ubfe(h.y, 5, 11)], h.y, 1);
+        const uint32_t b3 = __builtin_amdgcn_ubfe(words[h.y >> 21], h.y >> 16, 1);
+        return b0 | (b1 << 1) | (b2 << 2) | (b3 << 3);
     }
 };
 
 struct HashSet {  // open addressing, linear probing, load <= 1/2
     static constexpr bool kBitmap = false;
+    static constexpr bool kFull16 = false;
     const uint32_t* slots;
     int shift;
     uint32_t mask;
@@ -86,6 +126,7 @@ struct HashSet {  // open addressing, linear probing, load <= 1/2
 
 struct SearchSet {  // binary search of the sorted pivot row in HBM
     static constexpr bool kBitmap = false;
+    static constexpr bool kFull16 = false;
     const int32_t* col;
     uint32_t t0, t1;
     __device__ __forceinline__ uint32_t sentinel() const { return kEmpty; }  // above every id
@@ -152,20 +193,23 @@ __device__ __forceinline__ void team_sync() {
 // prefix.  Prefix ids are read 4 per lane (U loads in flight) with aligned
 // loads: 16 bytes of 32-bit ids, or, when N16 (every id of the pivot below
 // kNarrowIds, so every prefix id too), 8 bytes of the 16-bit copy;
-// ids outside the prefix become the set's sentinel.  The next
+// ids outside the prefix become the set's sentinel, or, in a FullWindow16,
+// are masked out of the four bits a lane reads.  The next
 // list's record is fetched before the current list is scanned; records are
 // read once per launch, so they are loaded non-temporally.  L is the
-// group's first record, loaded by the caller.
+// group's first record, loaded by the caller.  For bitmap sets, wabs[w] is
+// the window's word w of ids (absolute), and the core AND runs over the
+// uint4 words [c4, we4).
 template <int NT, int G, int U, bool PV, bool N16, class Set>
 __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a, uint32_t tid, uint32_t eb,
-                                               uint32_t ee, uint4 L, const uint32_t* window, uint32_t bw,
+                                               uint32_t ee, uint4 L, const uint32_t* wabs, uint32_t c4,
                                                uint32_t we4) {
+    static_assert(N16 || !Set::kFull16, "a full 16-bit window reads 16-bit ids");
     constexpr uint32_t NG = NT / G;
     constexpr uint32_t STEP = 4 * G * U;
     const uint32_t g = tid / G, gl = tid % G;
     // 32-bit byte offsets (nnz(L) + padding < 2^30 is enforced by the planner)
-    const char* colb = N16 ? reinterpret_cast<const char*>(a.col16) : reinterpret_cast<const char*>(a.col);
-    const uint32_t sent = set.sentinel();
+    const GlobalChar colb = N16 ? (GlobalChar)a.col16 : (GlobalChar)a.col;
     uint64_t acc = 0;
     for (uint32_t e = eb + g; e < ee; e += NG) {
         const uint32_t en = e + NG;
@@ -176,16 +220,33 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
         const uint32_t s1 = L.z;
         if (Set::kBitmap && L.w != kNoBitmap) {
             // core ids: row bitmap AND pivot window, 128 bits per lane step,
-            // over the window's words [bw, 4 we4): the pivot has no ids
-            // below bw or above them
-            const uint4* R = a.rowbits + (size_t)L.w * a.core4;
-            for (uint32_t w4 = (bw >> 2) + gl; w4 < we4; w4 += G) {
-                const uint4 r = R[w4];
-                const uint4 pw = *reinterpret_cast<const uint4*>(window + 4 * w4 - bw);
+            // over the uint4 words [c4, we4): the pivot has no ids below
+            // them or above them
+            const GlobalU4 R = (GlobalU4)a.rowbits + (size_t)L.w * a.core4;
+            for (uint32_t w4 = c4 + gl; w4 < we4; w4 += G) {
+                const uint4 r = to_u4(R[w4]);
+                const uint4 pw = *reinterpret_cast<const uint4*>(wabs + 4 * w4);
                 hits += __popc(r.x & pw.x) + __popc(r.y & pw.y) + __popc(r.z & pw.z) + __popc(r.w & pw.w);
             }
             s0 = L.y;
         }
+        if constexpr (Set::kFull16) {
+            for (uint32_t q = (s0 & ~3u) + 4 * gl; q < s1; q += STEP) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t qq = q + 4 * G * u;
+                    if (qq < s1) {
+                        const uint32_t b = set.bits4(to_u2(*(GlobalU2)(colb + (qq << 1))));
+                        // the prefix starts d0 (0..3) ids into this load and
+                        // ends e (1..4) ids into it (empty when e <= d0)
+                        const uint32_t d0 = s0 > qq ? s0 - qq : 0u;
+                        const uint32_t e = min(s1 - qq, 4u);
+                        hits += __popc(b & (0xFu << d0) & ((1u << e) - 1u));
+                    }
+                }
+            }
+        } else {
+        const uint32_t sent = set.sentinel();
         const uint32_t len = s1 > s0 ? s1 - s0 : 0u;
         for (uint32_t q = (s0 & ~3u) + 4 * gl; q < s1; q += STEP) {
             uint4 v[U];
@@ -194,11 +255,11 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
                 const uint32_t qq = q + 4 * G * u;
                 if constexpr (N16) {
                     uint2 h = make_uint2(0u, 0u);  // ids at or past s1 are masked below
-                    if (qq < s1) h = *reinterpret_cast<const uint2*>(colb + (qq << 1));
+                    if (qq < s1) h = to_u2(*(GlobalU2)(colb + (qq << 1)));
                     v[u] = make_uint4(h.x & 0xFFFFu, h.x >> 16, h.y & 0xFFFFu, h.y >> 16);
                 } else {
-                    v[u] = qq < s1 ? *reinterpret_cast<const uint4*>(colb + (qq << 2))
-                                   : make_uint4(sent, sent, sent, sent);
+                    v[u] = make_uint4(sent, sent, sent, sent);
+                    if (qq < s1) v[u] = to_u4(*(GlobalU4)(colb + (qq << 2)));
                 }
             }
 #pragma unroll
@@ -209,6 +270,7 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
                 hits += set.has(o + 2 < len ? v[u].z : sent);
                 hits += set.has(o + 3 < len ? v[u].w : sent);
             }
+        }
         }
         acc += hits;
         if (PV) {
@@ -224,9 +286,10 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
 
 // One work item (pivot row [t0, t1) of col, lists [eb, ee)) processed by NT
 // threads that share W words of zeroed LDS.  N_L(j) goes into LDS as a
-// bitmap window when max - min < 32 W, else into a hash table when
-// d_L(j) <= hash_capacity(W), else stays in HBM for binary search.  The LDS words are
-// zero again on return.
+// bitmap window when max - min < 32 W (from id 0 for a workgroup whose
+// pivot's ids are all below kNarrowIds), else into a hash table when
+// d_L(j) <= hash_capacity(W), else stays in HBM for binary search.  The LDS
+// words are zero again on return.
 template <int NT, int W, int G, int U, bool PV>
 __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, uint4 A, uint4 B, uint32_t* lds,
                                                  uint32_t tid) {
@@ -240,20 +303,35 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
     }
     uint64_t acc = 0;
     const bool narrow = hi < kNarrowIds;
-    const uint32_t base = lo & ~127u;  // 128-aligned: window words line up with core bitmap uint4s
+    // a workgroup's window holds 32 W >= 2 kNarrowIds ids: from id 0 it
+    // spans every 16-bit id
+    static_assert(NT != kThreads || 32u * W >= 2u * kNarrowIds, "workgroup window below 2^17 ids");
+    const bool full16 = NT == kThreads && narrow;
+    const uint32_t base = full16 ? 0u : lo & ~127u;  // 128-aligned: window words line up with core bitmap uint4s
     if (hi - base < 32u * W - 1u && lookup != 1) {  // the window's last bit stays 0 (sentinel)
         const uint32_t bw = base >> 5;
         for (uint32_t p = t0 + tid; p < t1; p += NT) {
-            const uint32_t k = (uint32_t)a.col[p];
+            const uint32_t k = (uint32_t)((GlobalI32)a.col)[p];
             atomicOr(&lds[(k >> 5) - bw], 1u << (k & 31u));
         }
         team_sync<NT>();
-        const uint32_t we4 = min(a.core4, (hi >> 7) + 1u);  // uint4 words up to the pivot's last id
-        const BitmapSet set{lds - bw, base + 32u * W - 1u};
-        acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, lds, bw, we4)
-                     : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, lds, bw, we4);
+        // core AND over the uint4 words from the pivot's first 128-id
+        // block up to the one holding its last id (or the core's end)
+        const uint32_t c4 = lo >> 7, we4 = min(a.core4, (hi >> 7) + 1u);
+        if constexpr (NT == kThreads) {
+            if (full16) {
+                acc = scan_lists<NT, G, U, PV, true>(FullWindow16{lds}, a, tid, eb, ee, L, lds, c4, we4);
+            } else {
+                acc = scan_lists<NT, G, U, PV, false>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, L,
+                                                      lds - bw, c4, we4);
+            }
+        } else {
+            const BitmapSet set{lds - bw, base + 32u * W - 1u};
+            acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, lds - bw, c4, we4)
+                         : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, lds - bw, c4, we4);
+        }
         team_sync<NT>();
-        for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)a.col[p] >> 5) - bw] = 0;
+        for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)((GlobalI32)a.col)[p] >> 5) - bw] = 0;
     } else if (d <= hash_capacity(W) && lookup != 2) {
         constexpr int kMaxLog = hash_log_max(W);
         const int logS = min(kMaxLog, max(6, 32 - __clz(4 * d - 1)));  // S >= 4d while it fits
@@ -262,20 +340,20 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
         for (uint32_t x = tid; x <= mask; x += NT) lds[x] = kEmpty;
         team_sync<NT>();
         for (uint32_t p = t0 + tid; p < t1; p += NT) {
-            const uint32_t key = (uint32_t)a.col[p];
+            const uint32_t key = (uint32_t)((GlobalI32)a.col)[p];
             uint32_t h = hslot(key, shift);
             while (atomicCAS(&lds[h], kEmpty, key) != kEmpty) h = (h + 1u) & mask;
         }
         team_sync<NT>();
         const HashSet set{lds, shift, mask};
-        acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, lds, 0u, 0u)
-                     : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, lds, 0u, 0u);
+        acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, nullptr, 0u, 0u)
+                     : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, nullptr, 0u, 0u);
         team_sync<NT>();
         for (uint32_t x = tid; x <= mask; x += NT) lds[x] = 0;
     } else {
         const SearchSet set{a.col, t0, t1};
-        acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, lds, 0u, 0u)
-                     : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, lds, 0u, 0u);
+        acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, nullptr, 0u, 0u)
+                     : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, nullptr, 0u, 0u);
     }
     return acc;
 }
@@ -341,7 +419,7 @@ __device__ __forceinline__ void flush(uint64_t& acc, unsigned long long* result)
 // spilling, which keeps 8 waves per SIMD resident; the others are left to
 // the compiler (forcing them spills to scratch).
 template <int G, int U, bool PV>
-constexpr int kMinWaves = (G == 8 && U == 1 && !PV) ? 8 : 1;
+constexpr int kMinWaves = (G <= 8 && U == 1 && !PV) ? 8 : 1;
 
 template <int G, int U, bool PV>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<G, U, PV>))) void wedge_kernel(
@@ -439,6 +517,7 @@ WedgeFn pick_u(int u, bool pv) {
 
 WedgeFn pick_wedge(int g, int u, bool pv) {
     switch (g) {
+        case 4: return pick_u<4>(u, pv);
         case 16: return pick_u<16>(u, pv);
         case 32: return pick_u<32>(u, pv);
         default: return pick_u<8>(u, pv);
@@ -634,15 +713,15 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
             if (dense[q]) continue;
             uint32_t s0, s1;
             if (!clip(i, q, s0, s1)) continue;
-            // the core AND costs core_words(pivot) words; per-id lookups
-            // cost about four times a word each: take the bitmap when cheaper
+            // the core AND costs core_words(pivot) words against the
+            // lookups of the core ids
             const int32_t j = cl[q];
             const uint32_t base = (uint32_t)cl[rp[j]] & ~127u;
             uint32_t sc = s0, rb = kNoBitmap;
             if (rbi[i] != kNoBitmap && base < C) {
                 const uint32_t core_ids = std::min(s1, cend[i]) > s0 ? std::min(s1, cend[i]) - s0 : 0u;
                 const uint32_t words = core_words((uint32_t)cl[rp[j]], (uint32_t)cl[rp[j + 1] - 1], C);
-                if (core_ids * 4 > words + 64) {
+                if (take_core_bitmap(core_ids, words)) {
                     sc = std::max(s0, cend[i]);
                     rb = rbi[i];
                     core_lists.fetch_add(1, std::memory_order_relaxed);
@@ -725,8 +804,9 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
 
 void check_launch_opts(const btc_count_opts& o) {
     BTC_REQUIRE(o.lookup >= 0 && o.lookup <= 2, BTC_ERR_ARG, "lookup %d (expected 0, 1 or 2)", o.lookup);
-    BTC_REQUIRE(o.group_lanes == 0 || o.group_lanes == 8 || o.group_lanes == 16 || o.group_lanes == 32,
-                BTC_ERR_ARG, "group_lanes %d (expected 0, 8, 16 or 32)", o.group_lanes);
+    BTC_REQUIRE(o.group_lanes == 0 || o.group_lanes == 4 || o.group_lanes == 8 || o.group_lanes == 16 ||
+                    o.group_lanes == 32,
+                BTC_ERR_ARG, "group_lanes %d (expected 0, 4, 8, 16 or 32)", o.group_lanes);
     BTC_REQUIRE(o.unroll == 0 || o.unroll == 1 || o.unroll == 2 || o.unroll == 4, BTC_ERR_ARG,
                 "unroll %d (expected 0, 1, 2 or 4)", o.unroll);
     BTC_REQUIRE(o.schedule == 0 || o.schedule == 1, BTC_ERR_ARG, "schedule %d (expected 0 or 1)", o.schedule);

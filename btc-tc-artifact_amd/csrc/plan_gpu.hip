@@ -196,7 +196,7 @@ __global__ void records_kernel(const uint32_t* __restrict__ rowptr, const int32_
             const uint32_t ce = min(s1, cend[i]);
             const uint32_t core_ids = ce > s0 ? ce - s0 : 0u;
             const uint32_t w = core_words(lo, hi, C);
-            if (core_ids * 4 > w + 64) {
+            if (take_core_bitmap(core_ids, w)) {
                 sc = max(s0, cend[i]);
                 rb = rbi[i];
                 words = w;

@@ -43,6 +43,9 @@ def case(name, btc):
     elif name == "planted":
         s, d = btc.gen_planted(4096, 128, 0.5, 4.0, seed=7)
         n = 4096
+    elif name == "planted_dense":  # 256-vertex communities at p = 0.9: rows full of hub-core ids
+        s, d = btc.gen_planted(2048, 256, 0.9, 2.0, seed=7)
+        n = 2048
     elif name == "complete":
         s, d = graphs.complete(150)
         n = 150
@@ -153,7 +156,7 @@ def test_plan_tune_rejects_bad_options(gpu):
     assert p.count()[0] == 1
 
 
-@pytest.mark.parametrize("name", ["rmat15", "planted", "complete", "messy"])
+@pytest.mark.parametrize("name", ["rmat15", "rmat12", "planted_dense", "complete"])
 def test_engine_core_bitmap_path(gpu, orc, name):
     """Lists whose rows hold many hub-core ids (< the core size) are counted by a
     bitmap AND + popcount for the core part; the counts match the oracle and
@@ -169,6 +172,25 @@ def test_engine_core_bitmap_path(gpu, orc, name):
     for G in (8, 16, 32):
         p.tune(group_lanes=G, unroll=2)
         assert p.count()[0] == want
+
+
+@pytest.mark.parametrize("planner", [1, 2])
+def test_engine_16_and_32_bit_prefix_reads(gpu, orc, planner):
+    """Pivots whose ids are all below 2^16 read prefixes from the 16-bit copy
+    of L's columns, the others from the 32-bit ids: on a graph with
+    communities spread over 2^17 ids both kinds count, and the totals and
+    per-vertex counts match the oracle."""
+    s, d = gpu.gen_planted(1 << 17, 128, 0.5, 4.0, seed=11)
+    g = gpu.Graph.from_coo(s, d, 1 << 17)
+    want, want_pv = oracle_counts(orc, g)
+    rowptr, col = g.csr()
+    last = col[rowptr[1:][np.diff(rowptr) > 0] - 1]  # each pivot's largest id
+    assert (last < 1 << 16).any() and (last >= 1 << 16).any()
+    for tau in (-1, 100000):
+        p = gpu.Plan(g, tau=tau, planner=planner)
+        t, pv, _ = p.count(per_vertex=True)
+        assert t == want
+        np.testing.assert_array_equal(pv, want_pv)
 
 
 @pytest.mark.parametrize("planner", [1, 2])
