@@ -331,7 +331,12 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
                          : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, lds - bw, c4, we4);
         }
         team_sync<NT>();
-        for (uint32_t p = t0 + tid; p < t1; p += NT) lds[((uint32_t)((GlobalI32)a.col)[p] >> 5) - bw] = 0;
+        // zero the words that can hold bits, [0, (hi - base) / 32], with
+        // 16-byte stores rather than reading the pivot row again (W is a
+        // multiple of 4, so every store stays inside the window)
+        static_assert(W % 4 == 0, "window of whole uint4 words");
+        const uint32_t nw = ((hi - base) >> 5) + 1u;
+        for (uint32_t w = 4 * tid; w < nw; w += 4 * NT) *reinterpret_cast<uint4*>(lds + w) = make_uint4(0, 0, 0, 0);
     } else if (d <= hash_capacity(W) && lookup != 2) {
         constexpr int kMaxLog = hash_log_max(W);
         const int logS = min(kMaxLog, max(6, 32 - __clz(4 * d - 1)));  // S >= 4d while it fits
