@@ -121,7 +121,8 @@ int btc_autotune(btc_graph g, const btc_tune_opts* topts, void* stream, btc_coun
                     return mn;
                 };
                 // {lanes per list, loads in flight, workgroups per CU, schedule}
-                const int shapes[5][4] = {{8, 1, 0, 1}, {8, 2, 0, 1}, {16, 1, 0, 1}, {8, 1, 6, 1}, {8, 1, 0, 0}};
+                const int shapes[6][4] = {{8, 1, 0, 1}, {4, 1, 0, 1}, {8, 2, 0, 1},
+                                          {16, 1, 0, 1}, {8, 1, 6, 1}, {8, 1, 0, 0}};
                 btc_count_opts plan_best = o;
                 float plan_ms = 1e30f;
                 for (const auto& sh : shapes) {

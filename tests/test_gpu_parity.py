@@ -137,11 +137,11 @@ def test_engine_launch_shapes(gpu, orc):
         g = gpu.Graph.from_coo(s, d, n)
         want, want_pv = oracle_counts(orc, g)
         p = gpu.Plan(g)
-        for G in (8, 16, 32):
+        for G in (4, 8, 16, 32):
             for U in (1, 2, 4):
                 for sched in (0, 1):
                     p.tune(group_lanes=G, unroll=U, schedule=sched, waves_per_eu=(0 if U != 2 else 1))
-                    t, pv, _ = p.count(per_vertex=(G == 16))
+                    t, pv, _ = p.count(per_vertex=(G in (4, 16)))
                     assert t == want, (name, G, U, sched)
                     if pv is not None:
                         np.testing.assert_array_equal(pv, want_pv)
