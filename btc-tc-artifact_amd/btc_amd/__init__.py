@@ -72,7 +72,8 @@ class PlanInfo(C.Structure):
     _fields_ = [("sparse_edges", C.c_int64), ("dense_edges", C.c_int64), ("dense_tiles", C.c_int64),
                 ("wedge_checks", C.c_int64), ("work_items", C.c_int64), ("device_bytes", C.c_int64),
                 ("sparse_merge_work", C.c_int64), ("stream_bytes", C.c_int64), ("core_lists", C.c_int64),
-                ("core_rows", C.c_int64), ("core_ids", C.c_int64), ("tile_k", C.c_int32), ("tau", C.c_int32)]
+                ("core_rows", C.c_int64), ("core_ids", C.c_int64), ("tile_k", C.c_int32), ("tau", C.c_int32),
+                ("tiny_items", C.c_int64)]
 
 
 class ServerOpts(C.Structure):

@@ -504,6 +504,30 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
     if (!multi) flush(acc, a.p0.result);
 }
 
+// Tiny-pivot items [first, first + share + t * shares) of one plan (t < n):
+// each group of kTinyLanes lanes takes one item with a kTinyWords-slot LDS
+// hash table of its own (static stride over the work-sorted items), so a
+// wave has 8 pivots' HBM round trips in flight where a wave item has one.
+// The groups of a wave run different items; every step of process_item
+// for a team of kTinyLanes lanes stays within the group.
+template <bool PV>
+__global__ __launch_bounds__(kThreads) void tiny_kernel(PlanDev P, int lookup, uint32_t first, uint32_t n,
+                                                         uint32_t share, uint32_t shares) {
+    constexpr uint32_t kGroups = kThreads / kTinyLanes;
+    __shared__ __align__(16) uint32_t lds[kGroups * kTinyWords];
+    const uint32_t tid = threadIdx.x, grp = tid / kTinyLanes, gl = tid % kTinyLanes;
+    for (uint32_t i = tid; i < kGroups * kTinyWords; i += kThreads) lds[i] = 0;
+    __syncthreads();
+    uint32_t* table = lds + grp * kTinyWords;
+    uint64_t acc = 0;
+    for (uint32_t t = blockIdx.x * kGroups + grp; t < n; t += gridDim.x * kGroups) {
+        const uint32_t li = first + share + t * shares;
+        const uint4 A = P.items[2 * li], B = P.items[2 * li + 1];
+        acc += process_item<kTinyLanes, kTinyWords, kTinyLanes, 1, PV>(P, lookup, A, B, table, gl);
+    }
+    flush(acc, P.result);
+}
+
 using WedgeFn = void (*)(WedgeArgs);
 
 template <int G, int U>
@@ -737,11 +761,12 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
             ur[e] = i;
         }
     // work items: consecutive lists of one pivot, about `chunk` checks each.
-    // Pivots with few lists become wave items; the rest workgroup items.
+    // Pivots with few lists become wave items (tiny items when tiny_pivot);
+    // the rest workgroup items.
     struct Item {
         uint4 a, b;
     };
-    std::vector<Item> block_items, wave_items;
+    std::vector<Item> block_items, wave_items, tiny_items;
     int64_t checks = 0, pivot_words = 0, narrow_ids = 0;
     for (int32_t j = 0; j < n; ++j) {
         int64_t e = off[j], e1 = off[j + 1];
@@ -753,7 +778,9 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
             const uint4& r = lst[f];
             return (int64_t)(r.z > r.y ? r.z - r.y : 0) + (r.w != kNoBitmap ? (int64_t)core_words(lo, hi, C) : 0);
         };
-        const bool small = wave_pivot((uint32_t)std::min<int64_t>(e1 - e, 0xFFFFFFFFLL), lo, hi, t1 - t0);
+        const uint32_t nl = (uint32_t)std::min<int64_t>(e1 - e, 0xFFFFFFFFLL);
+        const bool small = wave_pivot(nl, lo, hi, t1 - t0);
+        const bool tiny = small && tiny_pivot(nl, lo, hi, t1 - t0);
         while (e < e1) {
             int64_t w = 0, f = e;
             while (f < e1 && (small || w == 0 || w + cost(f) <= chunk)) {
@@ -766,17 +793,21 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
                 for (int64_t x = e; x < f; ++x) narrow_ids += lst[x].z > lst[x].y ? lst[x].z - lst[x].y : 0;
             Item it{make_uint4(t0, t1, (uint32_t)e, (uint32_t)f),
                     make_uint4(lo, hi, (uint32_t)std::min<int64_t>(w, 0xFFFFFFFFLL), 0u)};
-            (small ? wave_items : block_items).push_back(it);
+            (tiny ? tiny_items : small ? wave_items : block_items).push_back(it);
             e = f;
         }
     }
     auto by_work = [](const Item& x, const Item& y) { return x.b.z > y.b.z; };
     std::stable_sort(block_items.begin(), block_items.end(), by_work);
     std::stable_sort(wave_items.begin(), wave_items.end(), by_work);
+    std::stable_sort(tiny_items.begin(), tiny_items.end(), by_work);
     std::vector<Item> items(block_items);
     items.insert(items.end(), wave_items.begin(), wave_items.end());
+    items.insert(items.end(), tiny_items.begin(), tiny_items.end());
     p->nitems = (uint32_t)items.size();
     p->n_block_items = (uint32_t)block_items.size();
+    p->n_tiny_items = (uint32_t)tiny_items.size();
+    p->info.tiny_items = p->n_tiny_items;
     p->info.sparse_edges = sparse_edges;
     p->info.sparse_merge_work = sparse_work;
     p->info.wedge_checks = checks;
@@ -845,16 +876,33 @@ uint32_t share_count(uint32_t n, uint32_t share, uint32_t shares) {
     return n > share ? (n - share + shares - 1) / shares : 0u;
 }
 
+// Workgroup and wave items of a single plan (tiny items go to launch_tiny).
 void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counters, int32_t* d_pv, hipStream_t s,
                   uint32_t share = 0, uint32_t shares = 1) {
+    const uint32_t nb = share_count(p->n_block_items, share, shares);
+    const uint32_t nw = share_count(p->nitems - p->n_tiny_items - p->n_block_items, share, shares);
+    if (nb + nw == 0) return;
     unsigned grid = 0;
     WedgeFn fn = wedge_fn(p->launch, d_pv != nullptr, &grid);
-    const uint32_t nb = share_count(p->n_block_items, share, shares);
-    const uint32_t nw = share_count(p->nitems - p->n_block_items, share, shares);
     WedgeArgs a{plan_dev(p, d_total, d_pv), nullptr, nullptr, 1u, nb, nb + nw, d_counters, (int)p->launch.lookup,
                 (int)p->launch.schedule, share, shares};
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
     BTC_CHECK_LAUNCH("wedge_kernel");
+}
+
+void launch_tiny(btc_plan_s* p, unsigned long long* d_total, int32_t* d_pv, hipStream_t s, uint32_t share = 0,
+                 uint32_t shares = 1) {
+    const uint32_t n = share_count(p->n_tiny_items, share, shares);
+    if (!n) return;
+    auto fn = d_pv ? tiny_kernel<true> : tiny_kernel<false>;
+    int per_cu = 0;
+    BTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kThreads, 0));
+    constexpr uint32_t kGroups = kThreads / kTinyLanes;
+    const unsigned grid =
+        (unsigned)std::min<int64_t>((int64_t)device_cu_count() * std::max(1, per_cu), (n + kGroups - 1) / kGroups);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, plan_dev(p, d_total, d_pv), (int)p->launch.lookup,
+                       p->nitems - p->n_tiny_items, n, share, shares);
+    BTC_CHECK_LAUNCH("tiny_kernel");
 }
 
 }  // namespace
@@ -910,8 +958,10 @@ void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, M
 void ensure_side_stream(btc_plan_s* p) {
     if (p->side) return;
     BTC_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+    BTC_HIP(hipStreamCreateWithFlags(&p->side2, hipStreamNonBlocking));
     BTC_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
     BTC_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+    BTC_HIP(hipEventCreateWithFlags(&p->ev_join2, hipEventDisableTiming));
 }
 
 void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, EventTimer* tw) {
@@ -920,21 +970,33 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
     launch_zero_u32(reinterpret_cast<uint32_t*>(p->scratch.p), 4, s);
     unsigned long long* d_total = p->scratch.p;
     uint32_t* d_counters = reinterpret_cast<uint32_t*>(p->scratch.p + 1);
-    if (p->n_dense && p->nitems && p->info.wedge_checks >= kOverlapChecks) {
-        // the latency-bound dense-tile kernel runs on a second stream while
-        // the HBM-bound wedge kernel runs on s; both add into d_total (for
-        // small plans the fork and join would cost more than they hide)
+    if ((p->n_dense || p->n_tiny_items) && p->nitems && p->info.wedge_checks >= kOverlapChecks) {
+        // the latency-bound tiny-pivot and dense-tile kernels run on two
+        // more streams while the HBM-bound wedge kernel runs on s; all add
+        // into d_total (for small plans the fork and join would cost more
+        // than they hide).  Both are launched after the wedge kernel: they
+        // then take the slots its persistent workgroups leave in its tail
+        // (launched first, the tiny kernel slowed the wedge kernel by a
+        // third at R-MAT s22).
         ensure_side_stream(p);
         if (tm) tm->start(s);
         BTC_HIP(hipEventRecord(p->ev_fork, s));
-        BTC_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
         if (tw) tw->start(s);
         launch_wedge(p, d_total, d_counters, d_pv, s);
         if (tw) tw->stop(s);
-        launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, p->launch.dense_kernel == 2, d_total, d_pv,
-                          p->side);
-        BTC_HIP(hipEventRecord(p->ev_join, p->side));
-        BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
+        if (p->n_tiny_items) {
+            BTC_HIP(hipStreamWaitEvent(p->side2, p->ev_fork, 0));
+            launch_tiny(p, d_total, d_pv, p->side2);
+            BTC_HIP(hipEventRecord(p->ev_join2, p->side2));
+        }
+        if (p->n_dense) {
+            BTC_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+            launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, p->launch.dense_kernel == 2, d_total, d_pv,
+                              p->side);
+            BTC_HIP(hipEventRecord(p->ev_join, p->side));
+            BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
+        }
+        if (p->n_tiny_items) BTC_HIP(hipStreamWaitEvent(s, p->ev_join2, 0));
         if (tm) tm->stop(s);
         return;
     }
@@ -943,8 +1005,9 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
         launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, p->launch.dense_kernel == 2, d_total, d_pv, s);
     if (tw) tw->start(s);
     if (p->nitems) launch_wedge(p, d_total, d_counters, d_pv, s);
-    if (tm) tm->stop(s);
     if (tw) tw->stop(s);
+    launch_tiny(p, d_total, d_pv, s);
+    if (tm) tm->stop(s);
 }
 
 namespace {
@@ -982,7 +1045,7 @@ void plan_count(btc_plan_s* p, hipStream_t s, uint64_t* total, int32_t* per_vert
     if (kernel_ms) {
         *kernel_ms = tm.ms();
         p->last_wedge_ms = tw.ms();
-        p->last_dense_ms = p->n_dense ? std::max(0.f, *kernel_ms - p->last_wedge_ms) : 0.f;
+        p->last_dense_ms = p->n_dense || p->n_tiny_items ? std::max(0.f, *kernel_ms - p->last_wedge_ms) : 0.f;
     }
     *total = h;
 }
@@ -1075,22 +1138,35 @@ int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream
         btcx::EventTimer tm(kernel_ms != nullptr);
         tm.start(s);
         const uint32_t nd = btcx::share_count(p->n_dense, j, k);
-        // the dense-tile part beside the wedge part, as in a full count
-        hipStream_t ds = s;
-        if (nd && p->nitems) {
+        const uint32_t nt = btcx::share_count(p->n_tiny_items, j, k);
+        // the dense-tile and tiny-pivot parts beside the wedge part, as in
+        // a full count
+        hipStream_t ds = s, ts = s;
+        if ((nd || nt) && p->nitems) {
             btcx::ensure_side_stream(p);
             BTC_HIP(hipEventRecord(p->ev_fork, s));
-            BTC_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
-            ds = p->side;
+            if (nd) {
+                BTC_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+                ds = p->side;
+            }
+            if (nt) {
+                BTC_HIP(hipStreamWaitEvent(p->side2, p->ev_fork, 0));
+                ts = p->side2;
+            }
         }
         if (p->nitems)
             btcx::launch_wedge(p, p->scratch.p, reinterpret_cast<uint32_t*>(p->scratch.p + 1), nullptr, s, j, k);
+        btcx::launch_tiny(p, p->scratch.p, nullptr, ts, j, k);
         if (nd)
             btcx::launch_dense_list(p->bsr.get(), p->dense_list.p + j, nd, p->launch.dense_kernel == 2, p->scratch.p,
                                     nullptr, ds, k);
         if (ds != s) {
             BTC_HIP(hipEventRecord(p->ev_join, ds));
             BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
+        }
+        if (ts != s) {
+            BTC_HIP(hipEventRecord(p->ev_join2, ts));
+            BTC_HIP(hipStreamWaitEvent(s, p->ev_join2, 0));
         }
         tm.stop(s);
         unsigned long long h = 0;

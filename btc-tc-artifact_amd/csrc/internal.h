@@ -38,14 +38,16 @@ struct btc_plan_s {
     btcx::DevBuf<uint16_t> col16;  // low 16 bits of every id of L (m + padding)
     uint32_t nitems = 0;
     uint32_t n_block_items = 0;
+    uint32_t n_tiny_items = 0;  // the last items: tiny pivots (tiny_pivot)
     btcx::DevBuf<unsigned long long> scratch;  // [0] total, [1] work-item counter
     btc_plan_info info{};
     float last_dense_ms = 0.f;
     float last_wedge_ms = 0.f;
     btcx::CapturedGraph count_graph;  // totals-only count, replayed when captured
-    // second stream on which the dense-tile kernel runs beside the wedge kernel
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // second and third streams on which the dense-tile and tiny-pivot
+    // kernels run beside the wedge kernel
+    hipStream_t side = nullptr, side2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
     btc_plan_s() = default;
     btc_plan_s(const btc_plan_s&) = delete;
     btc_plan_s& operator=(const btc_plan_s&) = delete;
@@ -53,7 +55,9 @@ struct btc_plan_s {
         count_graph.reset();
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_join2) (void)hipEventDestroy(ev_join2);
         if (side) (void)hipStreamDestroy(side);
+        if (side2) (void)hipStreamDestroy(side2);
     }
 };
 
@@ -97,11 +101,25 @@ __host__ __device__ inline bool take_core_bitmap(uint32_t ids, uint32_t words) {
 // fits a wave's LDS slice as a bitmap window or a hash table.
 // An LDS hash table in W words has 2^hash_log_max(W) slots and takes sets of
 // up to half that many ids (load <= 1/2).
-__host__ __device__ constexpr int hash_log_max(int W) { return W >= 4096 ? 12 : W >= 2048 ? 11 : 10; }
+__host__ __device__ constexpr int hash_log_max(int W) {
+    return W >= 4096 ? 12 : W >= 2048 ? 11 : W >= 1024 ? 10 : W >= 512 ? 9 : W >= 256 ? 8 : 7;
+}
 __host__ __device__ constexpr uint32_t hash_capacity(int W) { return (1u << hash_log_max(W)) / 2; }
 
 __host__ __device__ inline bool wave_pivot(uint32_t lists, uint32_t lo, uint32_t hi, uint32_t d) {
     return lists <= kWaveItemLists && (hi - lo < 32u * kWaveWords || d <= hash_capacity(kWaveWords));
+}
+
+// Tiny pivots: wave pivots whose ids span too wide a range for a wave's
+// bitmap window and that fit an LDS hash table of kTinyWords slots.  Their
+// items are the plan's last; a single-plan count runs them in tiny_kernel,
+// where each group of kTinyLanes lanes takes one pivot (8 per wave) so that
+// the HBM round trips of many pivots overlap.  Multi-plan launches count
+// them as wave items.
+constexpr int kTinyWords = 128;
+constexpr int kTinyLanes = 8;
+__host__ __device__ inline bool tiny_pivot(uint32_t lists, uint32_t lo, uint32_t hi, uint32_t d) {
+    return lists <= kWaveItemLists && hi - (lo & ~127u) >= 32u * kWaveWords - 1u && d <= hash_capacity(kTinyWords);
 }
 
 // A batch over an already block-diagonal canonical graph whose graphs

@@ -235,33 +235,38 @@ __global__ void item_start_kernel(const uint32_t* __restrict__ rowptr, const int
 }
 
 // Item k = lists [first[k], first[k + 1]); sort key puts workgroup items
-// before wave items, each by descending work.
+// before wave items before tiny items, each by descending work.
 __global__ void items_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                              const uint32_t* __restrict__ key, const uint32_t* __restrict__ off,
                              const uint64_t* __restrict__ P, const uint32_t* __restrict__ first, uint32_t nitems,
                              uint32_t ne, uint4* __restrict__ tmp_items, uint32_t* __restrict__ skey,
                              uint32_t* __restrict__ sval, unsigned long long* __restrict__ stats) {
-    uint64_t words = 0, small_items = 0;
+    uint64_t words = 0, small_items = 0, tiny_items = 0;
     GS_LOOP(k, nitems) {
         const uint32_t eb = first[k], ee = k + 1 < nitems ? first[k + 1] : ne;
         const uint32_t j = key[eb];
         const uint32_t t0 = rowptr[j], t1 = rowptr[j + 1];
         const uint64_t w = P[ee] - P[eb];
         const uint32_t w32 = (uint32_t)min(w, (uint64_t)0xFFFFFFFFu);
+        const uint32_t lo = (uint32_t)col[t0], hi = (uint32_t)col[t1 - 1];
         const bool small = pivot_small(rowptr, col, off, j);
+        const bool tiny = small && tiny_pivot(off[j + 1] - off[j], lo, hi, t1 - t0);
         tmp_items[2 * k] = make_uint4(t0, t1, eb, ee);
-        tmp_items[2 * k + 1] = make_uint4((uint32_t)col[t0], (uint32_t)col[t1 - 1], w32, 0u);
-        const uint32_t w31 = (uint32_t)min(w, (uint64_t)0x7FFFFFFFu);
-        skey[k] = (small ? 0x80000000u : 0u) | (0x7FFFFFFFu - w31);
+        tmp_items[2 * k + 1] = make_uint4(lo, hi, w32, 0u);
+        const uint32_t w30 = (uint32_t)min(w, (uint64_t)0x3FFFFFFFu);
+        skey[k] = ((tiny ? 2u : small ? 1u : 0u) << 30) | (0x3FFFFFFFu - w30);
         sval[k] = (uint32_t)k;
         words += t1 - t0;
         small_items += small;
+        tiny_items += tiny;
     }
     words = wave_sum_u64(words);
     small_items = wave_sum_u64(small_items);
+    tiny_items = wave_sum_u64(tiny_items);
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&stats[0], (unsigned long long)words);
         atomicAdd(&stats[1], (unsigned long long)small_items);
+        atomicAdd(&stats[3], (unsigned long long)tiny_items);
     }
 }
 
@@ -405,8 +410,8 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
         BTC_LAUNCH(place_items_kernel, nitems, s, tmp.p, ialt ? sv1.p : sv0.p, nitems, p->items_dev.p);
 
     // st: [0] dense edges, [1] sparse edges, [2] their merge work, [3] core
-    // lists, [4] pivot row words over items, [5] wave items, [6] ids looked
-    // up in the 16-bit copy
+    // lists, [4] pivot row words over items, [5] wave items (tiny ones
+    // included), [6] ids looked up in the 16-bit copy, [7] tiny items
     unsigned long long h[8];
     BTC_HIP(hipMemcpyAsync(h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
     const uint64_t checks = read_dev(P.p + ne, s);  // synchronises h as well
@@ -416,6 +421,8 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     p->urow_p = ur;
     p->nitems = nitems;
     p->n_block_items = nitems - (uint32_t)h[5];
+    p->n_tiny_items = (uint32_t)h[7];
+    p->info.tiny_items = p->n_tiny_items;
     p->info.sparse_edges = (int64_t)h[1];
     p->info.sparse_merge_work = (int64_t)h[2];
     p->info.core_lists = (int64_t)h[3];

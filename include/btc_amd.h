@@ -214,6 +214,8 @@ typedef struct {
     int64_t core_ids;          /* hub-core id range C of the plan (0 = no core bitmaps) */
     int32_t tile_k;
     int32_t tau;
+    int64_t tiny_items;        /* work items of tiny pivots: few lists, a small row and ids spread wider
+                                  than a wave's bitmap window; one group of 8 lanes counts each */
 } btc_plan_info;
 
 void btc_count_opts_default(btc_count_opts* opts);
@@ -223,8 +225,10 @@ int btc_plan_get_info(btc_plan p, btc_plan_info* info);
  * group_lanes, unroll, schedule); tile_k, tau and chunk_work need a new plan. */
 int btc_plan_tune(btc_plan p, const btc_count_opts* opts);
 int btc_plan_count(btc_plan p, void* stream, uint64_t* total, int32_t* per_vertex, float* kernel_ms);
-/* Event times of the last btc_plan_count with kernel_ms != NULL, split by
- * kernel: dense-tile kernel and wedge kernel (0 when not launched). */
+/* Event times of the last btc_plan_count with kernel_ms != NULL: wedge_ms
+ * = the wedge kernel, dense_ms = the rest of the count (dense-tile and
+ * tiny-pivot kernels, beside or around the wedge kernel; 0 when neither is
+ * launched; both -1 after a replayed capture). */
 int btc_plan_last_timing(btc_plan p, float* dense_ms, float* wedge_ms);
 /* Counts share `share` of a `shares`-way split of the plan's work: the dense
  * tiles and the work items (of each class, both sorted by work) whose index
@@ -233,8 +237,8 @@ int btc_plan_last_timing(btc_plan p, float* dense_ms, float* wedge_ms);
  * reference counts on one GPU). */
 int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream, uint64_t* total,
                          float* kernel_ms);
-/* Captures the totals-only count of p (zero fill, dense-tile kernel, wedge
- * kernel) into a HIP graph; later btc_plan_count calls without per_vertex
+/* Captures the totals-only count of p (zero fill, dense-tile, wedge and
+ * tiny-pivot kernels) into a HIP graph; later btc_plan_count calls without per_vertex
  * replay it with one hipGraphLaunch on their stream (kernel_ms then times
  * the replay and btc_plan_last_timing reports -1 for the split).
  * btc_plan_tune drops the graph.  No reference counterpart (the reference

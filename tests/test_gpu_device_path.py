@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 INFO_KEYS = ("n", "n_input", "m", "max_row", "wedges", "merge_work")
 PLAN_KEYS = ("sparse_edges", "dense_edges", "dense_tiles", "wedge_checks", "sparse_merge_work", "core_lists",
-             "core_rows")
+             "core_rows", "tiny_items")
 
 
 def same_graph(a, b):

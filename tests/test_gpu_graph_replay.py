@@ -61,7 +61,7 @@ def test_overlapped_dense_and_wedge_kernels(gpu, orc):
     want, want_pv = oracle_counts(orc, g)
     p = gpu.Plan(g)
     info = p.info
-    assert info["dense_tiles"] > 0 and info["wedge_checks"] >= 1 << 24
+    assert info["dense_tiles"] > 0 and info["wedge_checks"] >= 1 << 24 and info["tiny_items"] > 0
     for _ in range(3):
         assert p.count()[0] == want
         t, pv, _ = p.count(per_vertex=True)

@@ -194,6 +194,43 @@ def test_engine_16_and_32_bit_prefix_reads(gpu, orc, planner):
 
 
 @pytest.mark.parametrize("planner", [1, 2])
+def test_engine_tiny_pivot_items(gpu, orc, planner):
+    """Pivots with few lists, at most 64 ids and ids spread over more than a
+    wave's bitmap window are counted by groups of 8 lanes in their own
+    kernel (beside the wedge kernel for large plans); totals, per-vertex
+    counts, shares, every lookup mode and replays match the oracle."""
+    s, d = gpu.gen_planted(1 << 17, 128, 0.5, 4.0, seed=11)
+    g = gpu.Graph.from_coo(s, d, 1 << 17)
+    want, want_pv = oracle_counts(orc, g)
+    p = gpu.Plan(g, planner=planner)
+    info = p.info
+    assert 0 < info["tiny_items"] < info["work_items"]
+    t, pv, _ = p.count(per_vertex=True)
+    assert t == want
+    np.testing.assert_array_equal(pv, want_pv)
+    for k in (2, 3):
+        assert sum(p.count_share(k, j)[0] for j in range(k)) == want, k
+    for lookup in (1, 2):
+        p.tune(lookup=lookup)
+        assert p.count()[0] == want, lookup
+    p.tune(lookup=0)
+    p.capture()
+    for _ in range(3):
+        assert p.count()[0] == want
+        p.count(per_vertex=True)
+    # no dense tiles; and a smaller graph (plans below 2^24 checks run their
+    # kernels one after another on one stream)
+    q = gpu.Plan(g, planner=planner, tau=100000)
+    assert q.count()[0] == want
+    s2, d2 = gpu.gen_rmat(16, 8, seed=16)
+    g2 = gpu.Graph.from_coo(s2, d2, 1 << 16)
+    want2, want2_pv = oracle_counts(orc, g2)
+    t2, pv2, _ = gpu.Plan(g2, planner=planner).count(per_vertex=True)
+    assert t2 == want2
+    np.testing.assert_array_equal(pv2, want2_pv)
+
+
+@pytest.mark.parametrize("planner", [1, 2])
 @pytest.mark.parametrize("core_ids", [-1, 128, 1024, 8192, 32768])
 def test_engine_core_size(gpu, orc, core_ids, planner):
     """Any hub-core size (none, one 128-id block, up to a wave's window span)
