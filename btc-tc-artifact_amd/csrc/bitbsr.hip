@@ -522,24 +522,114 @@ __device__ __forceinline__ void for_each_match(const uint32_t* __restrict__ indi
     }
 }
 
-// The engine's dense mask tiles (every listed tile on the dense path): one
-// wave per tile as in tile_count_kernel, with the tile columns of the two
-// row blocks matched by for_each_match.
+// Tile columns of one row block as an LDS bitmap over column blocks with
+// the number of set bits before each word: find(c) is the position of
+// column c in the row block's sorted list (a0 + its rank), or kNoTile.
+// Graphs with at most 32 kColSetWords column blocks use it.
+constexpr uint32_t kColSetWords = 1344;
+constexpr uint32_t kDenseTilesPerWave = 4;
+
+struct ColumnSet {
+    uint32_t* bits;
+    uint16_t* rank;  // < 2^16: a row block has fewer tiles than 32 kColSetWords
+    uint32_t a0;
+    uint32_t I = kNoTile;  // row block whose columns the set holds (kNoTile: empty)
+    __device__ __forceinline__ uint32_t find(uint32_t c) const {
+        const uint32_t w = bits[c >> 5];
+        if (!((w >> (c & 31u)) & 1u)) return kNoTile;
+        return a0 + rank[c >> 5] + (uint32_t)__popc(w & ((1u << (c & 31u)) - 1u));
+    }
+    // Sets the bits of indices[a0, a1) (the words [0, nw) are zero on
+    // entry) and the ranks; one wave.
+    __device__ void build(const uint32_t* __restrict__ indices, uint32_t a1, uint32_t nw, int lane) {
+        for (uint32_t p = a0 + (uint32_t)lane; p < a1; p += 64) {
+            const uint32_t c = indices[p];
+            atomicOr(&bits[c >> 5], 1u << (c & 31u));
+        }
+        wave_lds_sync();
+        const uint32_t per = (nw + 63) / 64, wb = min(nw, (uint32_t)lane * per), we = min(nw, wb + per);
+        uint32_t s = 0;
+        for (uint32_t x = wb; x < we; ++x) s += (uint32_t)__popc(bits[x]);
+        uint32_t inc = s;  // inclusive scan over the lanes
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)inc, o);
+            if (lane >= o) inc += y;
+        }
+        uint32_t run = inc - s;
+        for (uint32_t x = wb; x < we; ++x) {
+            rank[x] = (uint16_t)run;
+            run += (uint32_t)__popc(bits[x]);
+        }
+        wave_lds_sync();
+    }
+    __device__ void clear(uint32_t nw, int lane) {
+        wave_lds_sync();
+        for (uint32_t x = (uint32_t)lane; x < nw; x += 64) bits[x] = 0;
+        wave_lds_sync();
+    }
+    __device__ static void wave_lds_sync() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
+// Streaming a list of lb tile columns through a column set takes about
+// lb / 64 dependent rounds; for_each_match takes min / 64 rounds of
+// log2(max)-deep searches.
+__device__ __forceinline__ bool stream_cheaper(uint32_t la, uint32_t lb) {
+    const uint32_t mn = min(la, lb), mx = max(la, lb);
+    return (lb + 63) / 64 <= ((mn + 63) / 64) * (uint32_t)(32 - __clz(mx | 1u));
+}
+
+// Calls f(pa, pb) for every tile column of indices[b0, b1) that the column
+// set holds (pa its position in the set's row block, pb = its position
+// here); wave-uniform.  The list is read 64 entries at a time, coalesced.
+template <class F>
+__device__ __forceinline__ void for_each_match_set(const ColumnSet& cs, const uint32_t* __restrict__ indices,
+                                                   uint32_t b0, uint32_t b1, int lane, F&& f) {
+    for (uint32_t base = b0; base < b1; base += 64) {
+        const uint32_t pb = base + (uint32_t)lane;
+        uint32_t hit = kNoTile;
+        if (pb < b1) hit = cs.find(indices[pb]);
+        uint64_t m = __ballot(hit != kNoTile);
+        while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            f(uniform((uint32_t)__shfl((int)hit, l)), uniform(base + (uint32_t)l));
+        }
+    }
+}
+
+// One dense mask tile t of row block I (tile columns [a0, a1)): returns this
+// lane's part of the tile's count and adds per-vertex sums into pv.  The
+// tile columns of I and of each row block J under the tile are matched
+// through the column set cs (use_set) or by for_each_match.
 template <int K, bool PV, bool MFMA>
-__global__ __launch_bounds__(256) void dense_list_kernel(
-    int n, int n_rb, const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
-    const uint32_t* __restrict__ row_indices, const uint64_t* __restrict__ tiles, uint32_t w0, uint32_t w_end,
-    const uint32_t* __restrict__ tile_list, uint32_t lstride, unsigned long long* __restrict__ result,
-    int32_t* __restrict__ pv) {
+__device__ __forceinline__ uint64_t dense_tile(int n, int n_rb, const uint32_t* __restrict__ indptr,
+                                               const uint32_t* __restrict__ indices,
+                                               const uint64_t* __restrict__ tiles, uint32_t t, uint32_t I, uint32_t a0,
+                                               uint32_t a1, uint32_t set_words, ColumnSet& cs, int lane,
+                                               int32_t* __restrict__ pv) {
     constexpr int SLABS = K / 16;
-    const int lane = threadIdx.x & 63;
-    const uint32_t w = uniform(w0 + blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (w >= w_end) return;
-    const uint32_t t = uniform(tile_list[(size_t)w * lstride]);
-    const uint32_t I = uniform(row_indices[t]);
     const uint32_t JL = uniform(indices[t]);
-    const uint32_t a0 = uniform(indptr[I]), a1 = uniform(indptr[I + 1]);
-    if (MFMA) {
+    // matches of I's and J's tile columns: through the column set (built
+    // for I on first use) when streaming J's list is cheaper, else by search
+    auto match = [&](uint32_t b0, uint32_t b1, auto&& f) {
+        if (set_words && stream_cheaper(a1 - a0, b1 - b0)) {
+            if (cs.I != I) {
+                if (cs.I != kNoTile) cs.clear(set_words, lane);
+                cs.a0 = a0;
+                cs.build(indices, a1, set_words, lane);
+                cs.I = I;
+            }
+            for_each_match_set(cs, indices, b0, b1, lane, f);
+        } else {
+            for_each_match(indices, a0, a1, b0, b1, lane, f);
+        }
+    };
+    if constexpr (MFMA) {
         const int g = lane >> 4, c = lane & 15;
         RowBits<K> M[4];
 #pragma unroll
@@ -550,8 +640,8 @@ __global__ __launch_bounds__(256) void dense_list_kernel(
             if (Jo >= (uint32_t)n_rb) break;
             const uint32_t b0 = uniform(indptr[Jo]), b1 = uniform(indptr[Jo + 1]);
             v4i acc = {0, 0, 0, 0};
-            for_each_match(indices, a0, a1, b0, b1, lane,
-                           [&](uint32_t pa, uint32_t pb) { acc = mfma_pair<K>(tiles, pa, pb, c, g, acc); });
+            auto f = [&](uint32_t pa, uint32_t pb) { acc = mfma_pair<K>(tiles, pa, pb, c, g, acc); };
+            match(b0, b1, f);
             rs[0] += (M[0].bits(it * 16 + c) & 1u) ? (uint32_t)acc.x : 0u;
             rs[1] += (M[1].bits(it * 16 + c) & 1u) ? (uint32_t)acc.y : 0u;
             rs[2] += (M[2].bits(it * 16 + c) & 1u) ? (uint32_t)acc.z : 0u;
@@ -569,35 +659,70 @@ __global__ __launch_bounds__(256) void dense_list_kernel(
                 if (c == 0 && x && vtx < n) atomicAdd(pv + vtx, (int32_t)x);
             }
         }
-        const uint64_t tot = wave_sum_u64((uint64_t)(rs[0] + rs[1] + rs[2] + rs[3]));
-        if (lane == 0 && tot) atomicAdd(result, (unsigned long long)tot);
-        return;
+        return (uint64_t)(rs[0] + rs[1] + rs[2] + rs[3]);
+    } else {
+        const int r = lane >> 2, q = lane & 3;
+        const RowBits<K> S = RowBits<K>::load(tiles, t, r);
+        uint32_t sum = 0;
+        for (int it = 0; it < SLABS; ++it) {
+            const uint32_t Jo = JL * SLABS + it;
+            if (Jo >= (uint32_t)n_rb) break;
+            const uint32_t b0 = uniform(indptr[Jo]), b1 = uniform(indptr[Jo + 1]);
+            const uint32_t m4 = S.bits(it * 16 + 4 * q) & 0xFu;
+            uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+            auto f = [&](uint32_t pa, uint32_t pb) {
+                const RowBits<K> A = RowBits<K>::load(tiles, pa, r);
+                acc0 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 0));
+                acc1 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 1));
+                acc2 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 2));
+                acc3 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 3));
+            };
+            match(b0, b1, f);
+            sum += ((m4 & 1u) ? acc0 : 0u) + ((m4 & 2u) ? acc1 : 0u) + ((m4 & 4u) ? acc2 : 0u) +
+                   ((m4 & 8u) ? acc3 : 0u);
+        }
+        if (PV) {
+            uint32_t rsum = sum + __shfl_xor(sum, 1);
+            rsum += __shfl_xor(rsum, 2);
+            const int64_t v = (int64_t)I * 16 + r;
+            if (q == 0 && rsum && v < n) atomicAdd(pv + v, (int32_t)rsum);
+        }
+        return (uint64_t)sum;
     }
-    const int r = lane >> 2, q = lane & 3;
-    const RowBits<K> S = RowBits<K>::load(tiles, t, r);
-    uint32_t sum = 0;
-    for (int it = 0; it < SLABS; ++it) {
-        const uint32_t Jo = JL * SLABS + it;
-        if (Jo >= (uint32_t)n_rb) break;
-        const uint32_t b0 = uniform(indptr[Jo]), b1 = uniform(indptr[Jo + 1]);
-        const uint32_t m4 = S.bits(it * 16 + 4 * q) & 0xFu;
-        uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
-        for_each_match(indices, a0, a1, b0, b1, lane, [&](uint32_t pa, uint32_t pb) {
-            const RowBits<K> A = RowBits<K>::load(tiles, pa, r);
-            acc0 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 0));
-            acc1 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 1));
-            acc2 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 2));
-            acc3 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 3));
-        });
-        sum += ((m4 & 1u) ? acc0 : 0u) + ((m4 & 2u) ? acc1 : 0u) + ((m4 & 4u) ? acc2 : 0u) + ((m4 & 8u) ? acc3 : 0u);
+}
+
+// The engine's dense mask tiles (every listed tile on the dense path), as
+// in tile_count_kernel's dense path.  Each wave takes kDenseTilesPerWave
+// consecutive list entries (tile order, so mostly one row block I).  With
+// set_words != 0 (column blocks <= 32 kColSetWords) I's tile columns go into
+// an LDS ColumnSet, built when first needed for I, and the tile columns of
+// a row block J stream through it where that is cheaper than searching;
+// otherwise the two lists are matched by for_each_match.
+template <int K, bool PV, bool MFMA>
+__global__ __launch_bounds__(256) void dense_list_kernel(
+    int n, int n_rb, const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
+    const uint32_t* __restrict__ row_indices, const uint64_t* __restrict__ tiles, uint32_t w0, uint32_t w_end,
+    const uint32_t* __restrict__ tile_list, uint32_t lstride, uint32_t set_words,
+    unsigned long long* __restrict__ result, int32_t* __restrict__ pv) {
+    // dynamic LDS sized to the graph (dense_set_bytes): 4 waves' bits, then
+    // their ranks
+    extern __shared__ uint32_t dyn_lds[];
+    const int lane = threadIdx.x & 63;
+    const uint32_t wv = uniform(threadIdx.x >> 6);
+    const uint32_t first = w0 + (blockIdx.x * 4 + wv) * kDenseTilesPerWave;
+    if (first >= w_end) return;
+    const uint32_t last = min(w_end, first + kDenseTilesPerWave);
+    ColumnSet cs{dyn_lds + wv * set_words, reinterpret_cast<uint16_t*>(dyn_lds + 4 * set_words) + wv * set_words, 0};
+    if (set_words)
+        for (uint32_t x = (uint32_t)lane; x < set_words; x += 64) cs.bits[x] = 0;
+    uint64_t tot = 0;
+    for (uint32_t w = first; w < last; ++w) {
+        const uint32_t t = uniform(tile_list[(size_t)w * lstride]);
+        const uint32_t I = uniform(row_indices[t]);
+        const uint32_t a0 = uniform(indptr[I]), a1 = uniform(indptr[I + 1]);
+        tot += dense_tile<K, PV, MFMA>(n, n_rb, indptr, indices, tiles, t, I, a0, a1, set_words, cs, lane, pv);
     }
-    if (PV) {
-        uint32_t rsum = sum + __shfl_xor(sum, 1);
-        rsum += __shfl_xor(rsum, 2);
-        const int64_t v = (int64_t)I * 16 + r;
-        if (q == 0 && rsum && v < n) atomicAdd(pv + v, (int32_t)rsum);
-    }
-    const uint64_t tot = wave_sum_u64((uint64_t)sum);
+    tot = wave_sum_u64(tot);
     if (lane == 0 && tot) atomicAdd(result, (unsigned long long)tot);
 }
 
@@ -677,17 +802,22 @@ namespace {
 template <int K, bool MF>
 void launch_dense_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstride, uint32_t nt,
                     unsigned long long* res, int32_t* pv, hipStream_t s) {
+    // column blocks of the graph: ids of tile columns are below this
+    const uint64_t ncb = ((uint64_t)b->n + K - 1) / K;
+    const uint32_t set_words = ncb <= 32ull * kColSetWords ? (uint32_t)((ncb + 31) / 32) : 0u;
+    // 4 waves x (set_words bit words + set_words 16-bit ranks), whole words
+    const size_t lds = 4 * (size_t)set_words * 4 + (4 * (size_t)set_words * 2 + 3) / 4 * 4;
     for (uint32_t w0 = 0; w0 < nt; w0 += kMaxTilesPerLaunch) {
         const uint32_t chunk = std::min(kMaxTilesPerLaunch, nt - w0);
-        const unsigned grid = (chunk + 3) / 4;
+        const unsigned grid = (chunk + 4 * kDenseTilesPerWave - 1) / (4 * kDenseTilesPerWave);
         if (pv)
-            hipLaunchKernelGGL((dense_list_kernel<K, true, MF>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
+            hipLaunchKernelGGL((dense_list_kernel<K, true, MF>), dim3(grid), dim3(256), lds, s, b->n, b->n_rb,
                                b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, w0 + chunk, list, lstride,
-                               res, pv);
+                               set_words, res, pv);
         else
-            hipLaunchKernelGGL((dense_list_kernel<K, false, MF>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
+            hipLaunchKernelGGL((dense_list_kernel<K, false, MF>), dim3(grid), dim3(256), lds, s, b->n, b->n_rb,
                                b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, w0 + chunk, list, lstride,
-                               res, pv);
+                               set_words, res, pv);
         BTC_CHECK_LAUNCH("dense_list_kernel");
     }
 }
