@@ -527,7 +527,13 @@ __device__ __forceinline__ void for_each_match(const uint32_t* __restrict__ indi
 // column c in the row block's sorted list (a0 + its rank), or kNoTile.
 // Graphs with at most 32 kColSetWords column blocks use it.
 constexpr uint32_t kColSetWords = 1344;
-constexpr uint32_t kDenseTilesPerWave = 4;
+// Lists of at least this many dense tiles give each wave 4 consecutive
+// tiles (the column set is reused while I stays; fewer waves leave more of
+// the chip to the wedge kernel running beside): 2.86 vs 3.06 ms per count
+// at R-MAT s22 (24 K tiles).  Shorter lists take one tile per wave, which
+// keeps the chip busy (planted 2^17, 4 K tiles at K = 128: 4 per wave took
+// 0.32 ms, one 0.09 ms).
+constexpr uint32_t kDenseRunTiles = 16384;
 
 struct ColumnSet {
     uint32_t* bits;
@@ -692,8 +698,8 @@ __device__ __forceinline__ uint64_t dense_tile(int n, int n_rb, const uint32_t* 
 }
 
 // The engine's dense mask tiles (every listed tile on the dense path), as
-// in tile_count_kernel's dense path.  Each wave takes kDenseTilesPerWave
-// consecutive list entries (tile order, so mostly one row block I).  With
+// in tile_count_kernel's dense path.  Each wave takes tpw consecutive list
+// entries (tile order, so mostly one row block I).  With
 // set_words != 0 (column blocks <= 32 kColSetWords) I's tile columns go into
 // an LDS ColumnSet, built when first needed for I, and the tile columns of
 // a row block J stream through it where that is cheaper than searching;
@@ -702,16 +708,16 @@ template <int K, bool PV, bool MFMA>
 __global__ __launch_bounds__(256) void dense_list_kernel(
     int n, int n_rb, const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
     const uint32_t* __restrict__ row_indices, const uint64_t* __restrict__ tiles, uint32_t w0, uint32_t w_end,
-    const uint32_t* __restrict__ tile_list, uint32_t lstride, uint32_t set_words,
+    const uint32_t* __restrict__ tile_list, uint32_t lstride, uint32_t set_words, uint32_t tpw,
     unsigned long long* __restrict__ result, int32_t* __restrict__ pv) {
     // dynamic LDS sized to the graph (dense_set_bytes): 4 waves' bits, then
     // their ranks
     extern __shared__ uint32_t dyn_lds[];
     const int lane = threadIdx.x & 63;
     const uint32_t wv = uniform(threadIdx.x >> 6);
-    const uint32_t first = w0 + (blockIdx.x * 4 + wv) * kDenseTilesPerWave;
+    const uint32_t first = w0 + (blockIdx.x * 4 + wv) * tpw;
     if (first >= w_end) return;
-    const uint32_t last = min(w_end, first + kDenseTilesPerWave);
+    const uint32_t last = min(w_end, first + tpw);
     ColumnSet cs{dyn_lds + wv * set_words, reinterpret_cast<uint16_t*>(dyn_lds + 4 * set_words) + wv * set_words, 0};
     if (set_words)
         for (uint32_t x = (uint32_t)lane; x < set_words; x += 64) cs.bits[x] = 0;
@@ -807,17 +813,18 @@ void launch_dense_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstrid
     const uint32_t set_words = ncb <= 32ull * kColSetWords ? (uint32_t)((ncb + 31) / 32) : 0u;
     // 4 waves x (set_words bit words + set_words 16-bit ranks), whole words
     const size_t lds = 4 * (size_t)set_words * 4 + (4 * (size_t)set_words * 2 + 3) / 4 * 4;
+    const uint32_t tpw = nt >= kDenseRunTiles ? 4u : 1u;
     for (uint32_t w0 = 0; w0 < nt; w0 += kMaxTilesPerLaunch) {
         const uint32_t chunk = std::min(kMaxTilesPerLaunch, nt - w0);
-        const unsigned grid = (chunk + 4 * kDenseTilesPerWave - 1) / (4 * kDenseTilesPerWave);
+        const unsigned grid = (chunk + 4 * tpw - 1) / (4 * tpw);
         if (pv)
             hipLaunchKernelGGL((dense_list_kernel<K, true, MF>), dim3(grid), dim3(256), lds, s, b->n, b->n_rb,
                                b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, w0 + chunk, list, lstride,
-                               set_words, res, pv);
+                               set_words, tpw, res, pv);
         else
             hipLaunchKernelGGL((dense_list_kernel<K, false, MF>), dim3(grid), dim3(256), lds, s, b->n, b->n_rb,
                                b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, w0 + chunk, list, lstride,
-                               set_words, res, pv);
+                               set_words, tpw, res, pv);
         BTC_CHECK_LAUNCH("dense_list_kernel");
     }
 }

@@ -780,7 +780,7 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
         };
         const uint32_t nl = (uint32_t)std::min<int64_t>(e1 - e, 0xFFFFFFFFLL);
         const bool small = wave_pivot(nl, lo, hi, t1 - t0);
-        const bool tiny = small && tiny_pivot(nl, lo, hi, t1 - t0);
+        const bool tiny = p->use_tiny && small && tiny_pivot(nl, lo, hi, t1 - t0);
         while (e < e1) {
             int64_t w = 0, f = e;
             while (f < e1 && (small || w == 0 || w + cost(f) <= chunk)) {
@@ -1005,9 +1005,15 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
         launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, p->launch.dense_kernel == 2, d_total, d_pv, s);
     if (tw) tw->start(s);
     if (p->nitems) launch_wedge(p, d_total, d_counters, d_pv, s);
-    if (tw) tw->stop(s);
-    launch_tiny(p, d_total, d_pv, s);
-    if (tm) tm->stop(s);
+    if (p->n_tiny_items) {
+        if (tw) tw->stop(s);
+        launch_tiny(p, d_total, d_pv, s);
+        if (tm) tm->stop(s);
+    } else {
+        // no event record inside tm's interval beyond the kernels
+        if (tm) tm->stop(s);
+        if (tw) tw->stop(s);
+    }
 }
 
 namespace {
@@ -1082,6 +1088,9 @@ int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_p
                     BTC_ERR_ARG, "core_ids %d (expected a multiple of 128 up to %u, 0 = default, < 0 = none)",
                     o.core_ids, btcx::kCoreIdsMax);
         p->core_ids = o.core_ids == 0 ? btcx::kCoreIdsDefault : o.core_ids < 0 ? 0u : (uint32_t)o.core_ids;
+        BTC_REQUIRE(o.tiny_items >= -1 && o.tiny_items <= 1, BTC_ERR_ARG,
+                    "tiny_items %d (expected -1, 0 or 1)", o.tiny_items);
+        p->use_tiny = o.tiny_items > 0 || (o.tiny_items == 0 && g->wedges >= btcx::kTinyMinWedges);
         btcx::check_launch_opts(o);
         p->launch = o;
         btcx::ensure_device(g, s);

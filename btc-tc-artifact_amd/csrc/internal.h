@@ -25,6 +25,7 @@ struct btc_plan_s {
     int K = 0;
     int tau = 0;
     uint32_t core_ids = 0;  // hub-core id range C of the row bitmaps (0 = none)
+    bool use_tiny = false;  // tiny pivots become tiny items (btc_count_opts.tiny_items)
     btc_count_opts launch{};  // launch-only options (btc_plan_tune)
     std::unique_ptr<btc_bitbsr_s> bsr;
     btcx::DevBuf<uint32_t> dense_list;
@@ -118,6 +119,13 @@ __host__ __device__ inline bool wave_pivot(uint32_t lists, uint32_t lo, uint32_t
 // them as wave items.
 constexpr int kTinyWords = 128;
 constexpr int kTinyLanes = 8;
+// By default only graphs with at least this many wedges get tiny items
+// (btc_count_opts.tiny_items overrides): the groups of
+// a wave drift onto different paths and largely serialise, so the kernel
+// pays only beside a long wedge kernel (R-MAT s22, 7.2e9 wedges: count
+// 3.03 vs 3.07 ms); on planted 2^17 (9.2e7 wedges) it added 0.36 ms to a
+// 0.14 ms wedge kernel.
+constexpr int64_t kTinyMinWedges = int64_t(1) << 30;
 __host__ __device__ inline bool tiny_pivot(uint32_t lists, uint32_t lo, uint32_t hi, uint32_t d) {
     return lists <= kWaveItemLists && hi - (lo & ~127u) >= 32u * kWaveWords - 1u && d <= hash_capacity(kTinyWords);
 }

@@ -239,8 +239,9 @@ __global__ void item_start_kernel(const uint32_t* __restrict__ rowptr, const int
 __global__ void items_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                              const uint32_t* __restrict__ key, const uint32_t* __restrict__ off,
                              const uint64_t* __restrict__ P, const uint32_t* __restrict__ first, uint32_t nitems,
-                             uint32_t ne, uint4* __restrict__ tmp_items, uint32_t* __restrict__ skey,
-                             uint32_t* __restrict__ sval, unsigned long long* __restrict__ stats) {
+                             uint32_t ne, bool use_tiny, uint4* __restrict__ tmp_items,
+                             uint32_t* __restrict__ skey, uint32_t* __restrict__ sval,
+                             unsigned long long* __restrict__ stats) {
     uint64_t words = 0, small_items = 0, tiny_items = 0;
     GS_LOOP(k, nitems) {
         const uint32_t eb = first[k], ee = k + 1 < nitems ? first[k + 1] : ne;
@@ -250,7 +251,7 @@ __global__ void items_kernel(const uint32_t* __restrict__ rowptr, const int32_t*
         const uint32_t w32 = (uint32_t)min(w, (uint64_t)0xFFFFFFFFu);
         const uint32_t lo = (uint32_t)col[t0], hi = (uint32_t)col[t1 - 1];
         const bool small = pivot_small(rowptr, col, off, j);
-        const bool tiny = small && tiny_pivot(off[j + 1] - off[j], lo, hi, t1 - t0);
+        const bool tiny = use_tiny && small && tiny_pivot(off[j + 1] - off[j], lo, hi, t1 - t0);
         tmp_items[2 * k] = make_uint4(t0, t1, eb, ee);
         tmp_items[2 * k + 1] = make_uint4(lo, hi, w32, 0u);
         const uint32_t w30 = (uint32_t)min(w, (uint64_t)0x3FFFFFFFu);
@@ -402,8 +403,8 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     sk1.alloc(nitems, s);
     sv1.alloc(nitems, s);
     if (nitems)
-        BTC_LAUNCH(items_kernel, nitems, s, rowptr, col, key, off.p, P.p, first.p, nitems, ne, tmp.p, sk0.p, sv0.p,
-                   st.p + 4);
+        BTC_LAUNCH(items_kernel, nitems, s, rowptr, col, key, off.p, P.p, first.p, nitems, ne,
+                   p->use_tiny, tmp.p, sk0.p, sv0.p, st.p + 4);
     const bool ialt = sort_pairs_u32(sk0.p, sv0.p, sk1.p, sv1.p, nitems, 32, s);
     p->items_dev.alloc((size_t)nitems * 2);
     if (nitems)

@@ -196,6 +196,10 @@ typedef struct {
                              more than 2^20 edges), 1 = host threads, 2 = GPU kernels */
     int32_t core_ids;     /* hub-core id range C of the rows' core bitmaps: a multiple of 128 up to
                              32768; 0 = 2048, < 0 = no core bitmaps */
+    int32_t tiny_items;   /* pivots with few lists, at most 64 ids and ids spread wider than a wave's
+                             bitmap window counted by groups of 8 lanes in their own kernel: 0 = auto
+                             (graphs with at least 2^30 wedges), 1 = always, -1 = never (they stay
+                             wave items) */
 } btc_count_opts;
 
 typedef struct btc_plan_s* btc_plan;

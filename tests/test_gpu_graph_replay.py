@@ -59,7 +59,7 @@ def test_overlapped_dense_and_wedge_kernels(gpu, orc):
     s, d = gpu.gen_rmat(18, 16, seed=18)
     g = gpu.Graph.from_coo(s, d, 1 << 18)
     want, want_pv = oracle_counts(orc, g)
-    p = gpu.Plan(g)
+    p = gpu.Plan(g, tiny_items=1)
     info = p.info
     assert info["dense_tiles"] > 0 and info["wedge_checks"] >= 1 << 24 and info["tiny_items"] > 0
     for _ in range(3):

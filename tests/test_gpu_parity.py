@@ -202,7 +202,10 @@ def test_engine_tiny_pivot_items(gpu, orc, planner):
     s, d = gpu.gen_planted(1 << 17, 128, 0.5, 4.0, seed=11)
     g = gpu.Graph.from_coo(s, d, 1 << 17)
     want, want_pv = oracle_counts(orc, g)
-    p = gpu.Plan(g, planner=planner)
+    # by default only graphs with 2^30 wedges or more get tiny items
+    assert gpu.Plan(g, planner=planner).info["tiny_items"] == 0
+    assert gpu.Plan(g, planner=planner, tiny_items=-1).info["tiny_items"] == 0
+    p = gpu.Plan(g, planner=planner, tiny_items=1)
     info = p.info
     assert 0 < info["tiny_items"] < info["work_items"]
     t, pv, _ = p.count(per_vertex=True)
@@ -220,12 +223,12 @@ def test_engine_tiny_pivot_items(gpu, orc, planner):
         p.count(per_vertex=True)
     # no dense tiles; and a smaller graph (plans below 2^24 checks run their
     # kernels one after another on one stream)
-    q = gpu.Plan(g, planner=planner, tau=100000)
+    q = gpu.Plan(g, planner=planner, tau=100000, tiny_items=1)
     assert q.count()[0] == want
     s2, d2 = gpu.gen_rmat(16, 8, seed=16)
     g2 = gpu.Graph.from_coo(s2, d2, 1 << 16)
     want2, want2_pv = oracle_counts(orc, g2)
-    t2, pv2, _ = gpu.Plan(g2, planner=planner).count(per_vertex=True)
+    t2, pv2, _ = gpu.Plan(g2, planner=planner, tiny_items=1).count(per_vertex=True)
     assert t2 == want2
     np.testing.assert_array_equal(pv2, want2_pv)
 
@@ -255,6 +258,9 @@ def test_engine_core_size_rejects(gpu):
     for bad in (100, 65536, 32768 + 128):
         with pytest.raises(gpu.BtcError):
             gpu.Plan(g, core_ids=bad)
+    for bad in (-2, 2):
+        with pytest.raises(gpu.BtcError):
+            gpu.Plan(g, tiny_items=bad)
 
 
 @pytest.mark.parametrize("flags", [0, 1, 2, 3])
