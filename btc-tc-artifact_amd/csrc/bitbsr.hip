@@ -330,6 +330,41 @@ __device__ __forceinline__ long bytes8(uint32_t b) {
     return (long)((uint64_t)(uint32_t)nib_bytes(b & 15u) | ((uint64_t)(uint32_t)nib_bytes((b >> 4) & 15u) << 32));
 }
 
+// One matched tile's operand words for lane row i (K = 32: one 32-bit
+// word; else K / 64 64-bit words), loaded apart from the MFMAs so that the
+// loads of several matches can be in flight together.
+template <int K>
+struct MfmaOp {
+    uint64_t v[K >= 64 ? K / 64 : 1];
+};
+
+template <int K>
+__device__ __forceinline__ MfmaOp<K> mfma_load(const uint64_t* __restrict__ T, uint32_t p, int i) {
+    MfmaOp<K> o;
+    if constexpr (K == 32) {
+        o.v[0] = reinterpret_cast<const uint32_t*>(T + (size_t)p * 8)[i];
+    } else {
+#pragma unroll
+        for (int h = 0; h < K / 64; ++h) o.v[h] = T[(size_t)p * (K / 4) + (K / 64) * i + h];
+    }
+    return o;
+}
+
+template <int K>
+__device__ __forceinline__ v4i mfma_mac(const MfmaOp<K>& a, const MfmaOp<K>& b, int g, v4i acc) {
+    if constexpr (K == 32) {
+        return __builtin_amdgcn_mfma_i32_16x16x32_i8(bytes8((uint32_t)(a.v[0] >> (8 * g)) & 0xFFu),
+                                                     bytes8((uint32_t)(b.v[0] >> (8 * g)) & 0xFFu), acc, 0, 0, 0);
+    } else {
+#pragma unroll
+        for (int h = 0; h < K / 64; ++h)
+            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(bytes16((uint32_t)(a.v[h] >> (16 * g)) & 0xFFFFu),
+                                                        bytes16((uint32_t)(b.v[h] >> (16 * g)) & 0xFFFFu), acc, 0, 0,
+                                                        0);
+        return acc;
+    }
+}
+
 template <int K>
 __device__ __forceinline__ v4i mfma_pair(const uint64_t* __restrict__ T, uint32_t pa, uint32_t pb, int i, int g,
                                          v4i acc) {
@@ -646,8 +681,44 @@ __device__ __forceinline__ uint64_t dense_tile(int n, int n_rb, const uint32_t* 
             if (Jo >= (uint32_t)n_rb) break;
             const uint32_t b0 = uniform(indptr[Jo]), b1 = uniform(indptr[Jo + 1]);
             v4i acc = {0, 0, 0, 0};
-            auto f = [&](uint32_t pa, uint32_t pb) { acc = mfma_pair<K>(tiles, pa, pb, c, g, acc); };
+            // matches are taken 4 at a time: their operand loads are issued
+            // together, then the MFMAs (q0 is the newest of nq queued)
+            uint32_t qa0 = 0, qb0 = 0, qa1 = 0, qb1 = 0, qa2 = 0, qb2 = 0, qa3 = 0, qb3 = 0;
+            int nq = 0;
+            auto drain = [&]() {
+                const MfmaOp<K> a0 = mfma_load<K>(tiles, qa0, c), b0p = mfma_load<K>(tiles, qb0, c);
+                MfmaOp<K> a1{}, b1p{}, a2{}, b2p{}, a3{}, b3p{};
+                if (nq > 1) {
+                    a1 = mfma_load<K>(tiles, qa1, c);
+                    b1p = mfma_load<K>(tiles, qb1, c);
+                }
+                if (nq > 2) {
+                    a2 = mfma_load<K>(tiles, qa2, c);
+                    b2p = mfma_load<K>(tiles, qb2, c);
+                }
+                if (nq > 3) {
+                    a3 = mfma_load<K>(tiles, qa3, c);
+                    b3p = mfma_load<K>(tiles, qb3, c);
+                }
+                acc = mfma_mac<K>(a0, b0p, g, acc);
+                if (nq > 1) acc = mfma_mac<K>(a1, b1p, g, acc);
+                if (nq > 2) acc = mfma_mac<K>(a2, b2p, g, acc);
+                if (nq > 3) acc = mfma_mac<K>(a3, b3p, g, acc);
+                nq = 0;
+            };
+            auto f = [&](uint32_t pa, uint32_t pb) {
+                qa3 = qa2;
+                qb3 = qb2;
+                qa2 = qa1;
+                qb2 = qb1;
+                qa1 = qa0;
+                qb1 = qb0;
+                qa0 = pa;
+                qb0 = pb;
+                if (++nq == 4) drain();
+            };
             match(b0, b1, f);
+            if (nq) drain();
             rs[0] += (M[0].bits(it * 16 + c) & 1u) ? (uint32_t)acc.x : 0u;
             rs[1] += (M[1].bits(it * 16 + c) & 1u) ? (uint32_t)acc.y : 0u;
             rs[2] += (M[2].bits(it * 16 + c) & 1u) ? (uint32_t)acc.z : 0u;
