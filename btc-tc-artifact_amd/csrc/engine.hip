@@ -838,6 +838,11 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
 
 }  // namespace
 
+// Dense-tile form of a count: i8 MFMA unless AND + popcount is asked for
+// (dense_kernel 1); the MFMA form measured equal or faster on every graph
+// timed (R-MAT s22, planted 2^17 at K = 32 / 64 / 128).
+inline bool dense_mfma(const btc_count_opts& o) { return o.dense_kernel != 1; }
+
 void check_launch_opts(const btc_count_opts& o) {
     BTC_REQUIRE(o.lookup >= 0 && o.lookup <= 2, BTC_ERR_ARG, "lookup %d (expected 0, 1 or 2)", o.lookup);
     BTC_REQUIRE(o.group_lanes == 0 || o.group_lanes == 4 || o.group_lanes == 8 || o.group_lanes == 16 ||
@@ -922,7 +927,7 @@ void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, M
     }
     for (int j = 0; j < k; ++j)
         if (P[j]->n_dense)
-            launch_dense_list(P[j]->bsr.get(), P[j]->dense_list.p, P[j]->n_dense, P[j]->launch.dense_kernel == 2,
+            launch_dense_list(P[j]->bsr.get(), P[j]->dense_list.p, P[j]->n_dense, dense_mfma(P[j]->launch),
                               ws.totals.p + j, nullptr, s);
     if (ni) {
         unsigned grid = 0;
@@ -991,7 +996,7 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
         }
         if (p->n_dense) {
             BTC_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
-            launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, p->launch.dense_kernel == 2, d_total, d_pv,
+            launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, dense_mfma(p->launch), d_total, d_pv,
                               p->side);
             BTC_HIP(hipEventRecord(p->ev_join, p->side));
             BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
@@ -1002,7 +1007,7 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
     }
     if (tm) tm->start(s);
     if (p->n_dense)
-        launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, p->launch.dense_kernel == 2, d_total, d_pv, s);
+        launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, dense_mfma(p->launch), d_total, d_pv, s);
     if (tw) tw->start(s);
     if (p->nitems) launch_wedge(p, d_total, d_counters, d_pv, s);
     if (p->n_tiny_items) {
@@ -1167,7 +1172,7 @@ int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream
             btcx::launch_wedge(p, p->scratch.p, reinterpret_cast<uint32_t*>(p->scratch.p + 1), nullptr, s, j, k);
         btcx::launch_tiny(p, p->scratch.p, nullptr, ts, j, k);
         if (nd)
-            btcx::launch_dense_list(p->bsr.get(), p->dense_list.p + j, nd, p->launch.dense_kernel == 2, p->scratch.p,
+            btcx::launch_dense_list(p->bsr.get(), p->dense_list.p + j, nd, btcx::dense_mfma(p->launch), p->scratch.p,
                                     nullptr, ds, k);
         if (ds != s) {
             BTC_HIP(hipEventRecord(p->ev_join, ds));

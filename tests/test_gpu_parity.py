@@ -109,10 +109,11 @@ def test_engine_matches_oracle(gpu, orc, name, K, tau):
     np.testing.assert_array_equal(pv, want_pv)
     t2, pv2, _ = p.count()  # a plan is reusable; totals-only launch
     assert t2 == want and pv2 is None
-    p.tune(dense_kernel=2)  # dense tiles by i8 MFMA
-    t3, pv3, _ = p.count(per_vertex=True)
-    assert t3 == want
-    np.testing.assert_array_equal(pv3, want_pv)
+    for form in (1, 2):  # dense tiles by AND + popcount, then by i8 MFMA
+        p.tune(dense_kernel=form)
+        t3, pv3, _ = p.count(per_vertex=True)
+        assert t3 == want, form
+        np.testing.assert_array_equal(pv3, want_pv)
     info = p.info
     assert info["sparse_edges"] + info["dense_edges"] == g.m
 
