@@ -194,6 +194,27 @@ def test_engine_16_and_32_bit_prefix_reads(gpu, orc, planner):
         np.testing.assert_array_equal(pv, want_pv)
 
 
+@pytest.mark.parametrize("K", [32, 64])
+def test_engine_dense_matching_without_column_set(gpu, orc, K):
+    """Every tile dense (tau = 0) on a graph with 1.5 M vertices: at K = 32 it
+    has more column blocks than the dense kernel's LDS column set holds, so
+    tile columns are matched by search; at K = 64 through the column set.
+    Both forms, totals and per-vertex counts match the oracle."""
+    n = 1_500_000
+    s, d = gpu.gen_planted(n, 16, 0.5, 1.0, seed=5)  # 93,750 communities of 16
+    g = gpu.Graph.from_coo(s, d, n)
+    ncb = (g.n + K - 1) // K  # column blocks
+    assert ncb > 43008 if K == 32 else ncb <= 43008
+    want, want_pv = oracle_counts(orc, g)
+    p = gpu.Plan(g, tile_k=K, tau=0)
+    assert p.info["dense_tiles"] >= 16384  # waves take runs of tiles
+    for form in (1, 2):
+        p.tune(dense_kernel=form)
+        t, pv, _ = p.count(per_vertex=True)
+        assert t == want, form
+        np.testing.assert_array_equal(pv, want_pv)
+
+
 @pytest.mark.parametrize("planner", [1, 2])
 def test_engine_tiny_pivot_items(gpu, orc, planner):
     """Pivots with few lists, at most 64 ids and ids spread over more than a
