@@ -80,9 +80,8 @@ struct BitmapSet {  // bit (k - base) of an LDS window; base is a multiple of 12
     uint32_t last;          // base + window bits - 1: never set
     __device__ __forceinline__ uint32_t sentinel() const { return last; }
     __device__ __forceinline__ uint32_t has(uint32_t k) const {
-        // word index by a bit-field extract (one op with the scaled add
-        // after it); v_bfe_u32 reads the bit offset from k's low 5 bits
-        return __builtin_amdgcn_ubfe(words[__builtin_amdgcn_ubfe(k, 5, 27)], k, 1);
+        // v_bfe_u32 reads the bit offset from k's low 5 bits
+        return __builtin_amdgcn_ubfe(words[k >> 5], k, 1);
     }
 };
 
@@ -238,39 +237,39 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
                     if (qq < s1) {
                         const uint32_t b = set.bits4(to_u2(*(GlobalU2)(colb + (qq << 1))));
                         // the prefix starts d0 (0..3) ids into this load and
-                        // ends e (1..4) ids into it (empty when e <= d0)
+                        // ends d1 (1..4) ids into it (empty when d1 <= d0)
                         const uint32_t d0 = s0 > qq ? s0 - qq : 0u;
-                        const uint32_t e = min(s1 - qq, 4u);
-                        hits += __popc(b & (0xFu << d0) & ((1u << e) - 1u));
+                        const uint32_t d1 = min(s1 - qq, 4u);
+                        hits += __popc(b & (0xFu << d0) & ((1u << d1) - 1u));
                     }
                 }
             }
         } else {
-        const uint32_t sent = set.sentinel();
-        const uint32_t len = s1 > s0 ? s1 - s0 : 0u;
-        for (uint32_t q = (s0 & ~3u) + 4 * gl; q < s1; q += STEP) {
-            uint4 v[U];
+            const uint32_t sent = set.sentinel();
+            const uint32_t len = s1 > s0 ? s1 - s0 : 0u;
+            for (uint32_t q = (s0 & ~3u) + 4 * gl; q < s1; q += STEP) {
+                uint4 v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t qq = q + 4 * G * u;
-                if constexpr (N16) {
-                    uint2 h = make_uint2(0u, 0u);  // ids at or past s1 are masked below
-                    if (qq < s1) h = to_u2(*(GlobalU2)(colb + (qq << 1)));
-                    v[u] = make_uint4(h.x & 0xFFFFu, h.x >> 16, h.y & 0xFFFFu, h.y >> 16);
-                } else {
-                    v[u] = make_uint4(sent, sent, sent, sent);
-                    if (qq < s1) v[u] = to_u4(*(GlobalU4)(colb + (qq << 2)));
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t qq = q + 4 * G * u;
+                    if constexpr (N16) {
+                        uint2 h = make_uint2(0u, 0u);  // ids at or past s1 are masked below
+                        if (qq < s1) h = to_u2(*(GlobalU2)(colb + (qq << 1)));
+                        v[u] = make_uint4(h.x & 0xFFFFu, h.x >> 16, h.y & 0xFFFFu, h.y >> 16);
+                    } else {
+                        v[u] = make_uint4(sent, sent, sent, sent);
+                        if (qq < s1) v[u] = to_u4(*(GlobalU4)(colb + (qq << 2)));
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t o = q + 4 * G * u - s0;  // prefix offset of v[u].x (wraps below s0)
+                    hits += set.has(o + 0 < len ? v[u].x : sent);
+                    hits += set.has(o + 1 < len ? v[u].y : sent);
+                    hits += set.has(o + 2 < len ? v[u].z : sent);
+                    hits += set.has(o + 3 < len ? v[u].w : sent);
                 }
             }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t o = q + 4 * G * u - s0;  // prefix offset of v[u].x (wraps below s0)
-                hits += set.has(o + 0 < len ? v[u].x : sent);
-                hits += set.has(o + 1 < len ? v[u].y : sent);
-                hits += set.has(o + 2 < len ? v[u].z : sent);
-                hits += set.has(o + 3 < len ? v[u].w : sent);
-            }
-        }
         }
         acc += hits;
         if (PV) {
