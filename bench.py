@@ -49,13 +49,34 @@ def make_workload(btc, name):
     raise SystemExit("unknown workload " + name)
 
 
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo) and logical CPU count."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_threads():
+    """Threads of the CPU share this job was given: OMP_NUM_THREADS when set
+    (one GPU's share of the host), else every logical CPU."""
+    return int(os.environ.get("BTC_CPU_THREADS") or os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+
 def cpu_baseline(g, gpu_total):
-    """The reference's own CPU verifier (oracle/_ref) on the same canonical L;
-    the oracle's C port when the reference build is absent."""
+    """The reference's own CPU verifier (oracle/_ref) on the same canonical L,
+    with the threads of this job's CPU share; the oracle's C port when the
+    reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    threads = int(os.environ.get("BTC_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    threads = cpu_threads()
+    model, nproc = cpu_model()
     rowptr, col = g.csr()
     t0 = time.perf_counter()
     if oracle.ref_available():
@@ -67,7 +88,10 @@ def cpu_baseline(g, gpu_total):
     dt = time.perf_counter() - t0
     return {"value": g.m / dt, "unit": "edges/s", "cores": threads, "kind": kind,
             "sample": "whole graph, same degree-ordered L as the GPU, one run (%.1f s)" % dt,
-            "seconds": dt, "total": int(total), "matches_gpu": int(total) == int(gpu_total)}
+            "seconds": dt, "total": int(total), "matches_gpu": int(total) == int(gpu_total),
+            "cpu_model": model, "host_logical_cpus": nproc,
+            "threads_from": "OMP_NUM_THREADS (this job's CPU share)" if os.environ.get("OMP_NUM_THREADS")
+            else "all logical CPUs"}
 
 
 def load_traffic(workload):
@@ -215,7 +239,7 @@ def configs_leg(btc, runs):
         for _ in range(runs):
             tp, _, ms = b.count()
             pk.append(ms)
-        threads = int(os.environ.get("BTC_CPU_THREADS", min(16, os.cpu_count() or 1)))
+        threads = cpu_threads()
         btc.count_cpu(g, threads)  # the host copy of L is made on first use
         t0 = time.perf_counter()
         tc, _ = btc.count_cpu(g, threads)
@@ -328,25 +352,13 @@ def serving_leg(btc, requests, seed=1, canon_flags=None, prep_threads=None, grou
     of requests repeat a graph submitted earlier.  A closed burst measures
     saturation; the open-loop run uses 70 % of it.  Latency = done - submit
     on the server clock."""
-    import math
     import threading
 
     rng = np.random.default_rng(seed)
+    draws = iter(range(seed * 1000, seed * 1000 + 1000))
 
     def stream(nreq, key0):
-        reqs, keys = [], []
-        for _ in range(nreq):
-            if keys and rng.random() < 0.3:
-                reqs.append((int(rng.choice(keys)), None))
-            else:
-                n = int(min(1 << 20, max(64, round(math.exp(rng.normal(math.log(1 << 13), 1.0))))))
-                sc = max(1, math.ceil(math.log2(n)))
-                s, d = btc.gen_rmat(sc, 16, 0.57, 0.19, 0.19, seed=int(rng.integers(1, 2 ** 62)))
-                keep = (s < n) & (d < n)
-                key = key0 + len(keys)
-                keys.append(key)
-                reqs.append((key, (s[keep], d[keep], n)))
-        return reqs
+        return btc.serving_workload(nreq, seed=next(draws), key0=key0)
 
     def run(reqs, qps):
         sv = btc.Server(canon_flags=canon_flags, prep_threads=prep_threads, group_graphs=group_graphs)
@@ -502,20 +514,30 @@ def main():
     log("engine: %d triangles, %.3f ms/step wall, kernel median %.3f ms, wedge kernel mean %.3f ms"
         % (total0, ms_per_step, statistics.median(kms), statistics.mean(wms)))
 
-    # roofline of the dominant kernel (wedge kernel): merge-stream bytes of
-    # its edges, B = 4(n+1) + 4 m_s + 4 W_s, over its mean event time
+    # roofline of the dominant kernel (wedge kernel): the bytes this
+    # algorithm must stream per launch (plan.stream_bytes: looked-up prefix
+    # ids at 2 or 4 B, pivot rows and hub-core words at 4 B, 16 B per list
+    # record, 32 B per work item) over its mean HIP-event time on the launch
+    # stream.  The merge-intersection stream of the same edges, B = 4(n+1) +
+    # 4 m_s + 4 W_s, is kept as context: this kernel does not read it.
     wedge_ms = statistics.mean(wms)
-    b_alg = 4 * (gi["n"] + 1) + 4 * pi["sparse_edges"] + 4 * pi["sparse_merge_work"]
-    achieved = b_alg / (wedge_ms * 1e-3) / 1e9 if wedge_ms > 0 else 0.0
+    b_merge = 4 * (gi["n"] + 1) + 4 * pi["sparse_edges"] + 4 * pi["sparse_merge_work"]
+    stream_b = pi["stream_bytes"]
+    achieved = stream_b / (wedge_ms * 1e-3) / 1e9 if wedge_ms > 0 else 0.0
     traffic = load_traffic(config["workload"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_GBps": round(traffic / (wedge_ms * 1e-3) / 1e9, 1) if traffic and wedge_ms > 0 else None,
                 "traffic_frac": round(traffic / (wedge_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
                 if traffic and wedge_ms > 0 else None,
-                "kernel": "wedge_kernel", "kernel_ms": round(wedge_ms, 4), "alg_bytes": b_alg,
-                "stream_bytes": pi["stream_bytes"],
-                "stream_GBps": round(pi["stream_bytes"] / (wedge_ms * 1e-3) / 1e9, 1) if wedge_ms > 0 else None}
+                "traffic_over_alg": round(traffic / stream_b, 3) if traffic and stream_b else None,
+                "kernel": "wedge_kernel", "kernel_ms": round(wedge_ms, 4), "alg_bytes": stream_b,
+                "alg_bytes_model": "plan.stream_bytes: 2 or 4 B per looked-up prefix id, 4 B per pivot-row id and "
+                                   "hub-core word, 16 B per list record, 32 B per work item",
+                "merge_stream_bytes": b_merge,
+                "merge_stream_GBps": round(b_merge / (wedge_ms * 1e-3) / 1e9, 1) if wedge_ms > 0 else None}
+    if roofline["frac"] > 1.0:
+        log("warning: the kernel's byte stream exceeds the HBM peak (cache reuse): %s" % roofline)
 
     # one GPU's share of a k-way split (the work each GPU would run), each
     # share on the whole chip, from a plan whose work items are cut for a

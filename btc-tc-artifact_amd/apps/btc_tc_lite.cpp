@@ -3,9 +3,10 @@
 // stdout contract (btc_tc_lite.cu:141-149), so scripts that parse the
 // reference's output read this one unchanged.
 //
-// BTC_ENGINE=plain runs the reference-semantics Bit-BSR kernels with the
-// Lite tile policy; the default runs the hybrid engine (dense tiles of the
-// same width + LDS wedge checks).
+// The counts go through the reference's own calls: the Bit-BSR of the Lite
+// tile width, then count_triangles_16x{128,32}_adaptive, which run the
+// hybrid engine (dense tiles of that Bit-BSR + LDS wedge checks);
+// BTC_ENGINE=plain runs the reference-semantics Bit-BSR kernels instead.
 #include <btc/btc.h>
 
 #include <algorithm>
@@ -49,45 +50,39 @@ int main(int argc, char** argv) try {
     std::vector<float> kernel_times((size_t)std::max(config.repeat, 1), 0.f);
     btc::CPUTimer timer_conv;
     timer.start();
-    if (plain) {
-        auto run = [&](auto& d_bcsr, int threshold) {
-            timer_conv.start();
-            if constexpr (std::is_same_v<std::decay_t<decltype(d_bcsr)>, btc::BCSR_16x128_Device>)
-                btc::convert_coo_to_bcsr_16x128_gpu(d_bcsr, A_coo);
-            else
-                btc::convert_coo_to_bcsr_16x32_gpu(d_bcsr, A_coo);
-            timer_conv.stop();
-            for (int w = 0; w < config.warmup; ++w) {
-                float tmp = 0.f;
-                btc::count_bitbsr(d_bcsr, BTC_VARIANT_AUTO, threshold, &tmp);
-            }
-            for (int r = 0; r < config.repeat; ++r)
-                count = btc::count_bitbsr(d_bcsr, BTC_VARIANT_AUTO, threshold, &kernel_times[(size_t)r]);
-            d_bcsr.free();
-        };
-        if (K == 128) {
-            btc::BCSR_16x128_Device d;
-            run(d, 2048);
-        } else {
-            btc::BCSR_16x32_Device d;
-            run(d, 40000);
-        }
-    } else {
+    // apps/btc_tc_lite.cu:74-136: convert to the Bit-BSR of width K, warm
+    // up, then time `repeat` counts.  The adaptive calls run the engine on
+    // that Bit-BSR (its plan is made with the conversion); BTC_ENGINE=plain
+    // runs the reference-semantics kernels with the reference's v4/v5
+    // (v5/v6) choice.
+    auto run = [&](auto& d_bcsr, int threshold) {
         timer_conv.start();
-        btc::Graph g = btc::graph_from_lower_coo(A_coo);
-        btc_count_opts o;
-        btc_count_opts_default(&o);
-        o.tile_k = K;
-        btc_plan p = nullptr;
-        btc::check(btc_plan_create(g.h, &o, nullptr, &p));
+        if constexpr (std::is_same_v<std::decay_t<decltype(d_bcsr)>, btc::BCSR_16x128_Device>)
+            btc::convert_coo_to_bcsr_16x128_gpu(d_bcsr, A_coo);
+        else
+            btc::convert_coo_to_bcsr_16x32_gpu(d_bcsr, A_coo);
+        if (!plain) btc::prepare_engine(d_bcsr);
         timer_conv.stop();
-        uint64_t t = 0;
-        float tmp = 0.f;
-        for (int w = 0; w < config.warmup; ++w) btc::check(btc_plan_count(p, nullptr, &t, nullptr, &tmp));
-        for (int r = 0; r < config.repeat; ++r)
-            btc::check(btc_plan_count(p, nullptr, &t, nullptr, &kernel_times[(size_t)r]));
-        count = t;
-        btc_plan_free(p);
+        auto once = [&](float* ms) {
+            if (plain) return btc::count_bitbsr(d_bcsr, BTC_VARIANT_AUTO, threshold, ms);
+            if constexpr (std::is_same_v<std::decay_t<decltype(d_bcsr)>, btc::BCSR_16x128_Device>)
+                return btc::count_triangles_16x128_adaptive(d_bcsr, threshold, ms);
+            else
+                return btc::count_triangles_16x32_adaptive(d_bcsr, threshold, ms);
+        };
+        for (int w = 0; w < config.warmup; ++w) {
+            float tmp = 0.f;
+            once(&tmp);
+        }
+        for (int r = 0; r < config.repeat; ++r) count = once(&kernel_times[(size_t)r]);
+        d_bcsr.free();
+    };
+    if (K == 128) {
+        btc::BCSR_16x128_Device d;
+        run(d, 2048);
+    } else {
+        btc::BCSR_16x32_Device d;
+        run(d, 40000);
     }
     timer.stop();
     std::vector<float> sorted(kernel_times.begin(), kernel_times.begin() + std::max(config.repeat, 0));

@@ -414,9 +414,12 @@ struct BCSR_Device {
     uint32_t num_blocks = 0;
     btc_bitbsr handle = nullptr;
     Graph graph;
+    btc_plan plan = nullptr;  // the engine's plan on this Bit-BSR, made by the first adaptive count
     unsigned long long last = 0;
 
     void free() {
+        if (plan) btc_plan_free(plan);  // before the Bit-BSR it reads
+        plan = nullptr;
         if (handle) btc_bitbsr_free(handle);
         handle = nullptr;
         graph = Graph();
@@ -463,30 +466,53 @@ unsigned long long count_bitbsr(BCSR_Device<K>& b, int variant, int threshold, f
     return t;
 }
 
-// tc_16x128_mma_v4.h:222-260, tc_16x128_mma_v5.h:213-252, tc_16x128_adaptive.h:12-29
+// The MI355X engine on the Bit-BSR's graph: dense tiles of width K from the
+// Bit-BSR itself, every other edge by LDS wedge checks; the plan is made on
+// the first call and kept in the handle, so later calls only count.
+template <int K>
+void prepare_engine(BCSR_Device<K>& b) {
+    if (!b.plan) check(btc_plan_create_on_bitbsr(b.graph.h, b.handle, nullptr, nullptr, &b.plan));
+}
+template <int K>
+unsigned long long count_engine(BCSR_Device<K>& b, float* kernel_ms) {
+    prepare_engine(b);
+    uint64_t t = 0;
+    check(btc_plan_count(b.plan, nullptr, &t, nullptr, kernel_ms));
+    b.last = t;
+    return t;
+}
+
+// tc_16x128_mma_v4.h:222-260, tc_16x128_mma_v5.h:213-252: the
+// reference-semantics kernels (one wave per mask tile, tau routing).
 inline unsigned long long count_triangles_16x128_v4(BCSR_16x128_Device& b, float* kernel_ms = nullptr) {
     return count_bitbsr(b, BTC_VARIANT_BINSEARCH, 0, kernel_ms);
 }
 inline unsigned long long count_triangles_16x128_v5(BCSR_16x128_Device& b, float* kernel_ms = nullptr) {
     return count_bitbsr(b, BTC_VARIANT_DIRECT, 0, kernel_ms);
 }
+// tc_16x128_adaptive.h:12-29.  The reference picks v4 or v5 by the number of
+// tiles (threshold_blocks); on gfx950 the call runs the engine instead, which
+// gives the same count.  threshold_blocks applies to count_bitbsr only.
 inline unsigned long long count_triangles_16x128_adaptive(BCSR_16x128_Device& b, int threshold_blocks,
                                                           float* kernel_ms) {
-    return count_bitbsr(b, BTC_VARIANT_AUTO, threshold_blocks, kernel_ms);
+    (void)threshold_blocks;
+    return count_engine(b, kernel_ms);
 }
 inline unsigned long long count_triangles_16x128_adaptive(BCSR_16x128_Device& b, int threshold_blocks = 2048) {
     return count_triangles_16x128_adaptive(b, threshold_blocks, nullptr);
 }
-// tc_16x32_mma_v5.h:246-284, tc_16x32_mma_v6.h:240-279, tc_16x32_adaptive.h:14-31
+// tc_16x32_mma_v5.h:246-284, tc_16x32_mma_v6.h:240-279
 inline unsigned long long count_triangles_16x32_v5(BCSR_16x32_Device& b, float* kernel_ms = nullptr) {
     return count_bitbsr(b, BTC_VARIANT_BINSEARCH, 0, kernel_ms);
 }
 inline unsigned long long count_triangles_16x32_v6(BCSR_16x32_Device& b, float* kernel_ms = nullptr) {
     return count_bitbsr(b, BTC_VARIANT_DIRECT, 0, kernel_ms);
 }
+// tc_16x32_adaptive.h:14-31 (v5 below 40000 tiles, else v6), on the engine
 inline unsigned long long count_triangles_16x32_adaptive(BCSR_16x32_Device& b, int threshold_blocks,
                                                          float* kernel_ms) {
-    return count_bitbsr(b, BTC_VARIANT_AUTO, threshold_blocks, kernel_ms);
+    (void)threshold_blocks;
+    return count_engine(b, kernel_ms);
 }
 inline unsigned long long count_triangles_16x32_adaptive(BCSR_16x32_Device& b, int threshold_blocks = 40000) {
     return count_triangles_16x32_adaptive(b, threshold_blocks, nullptr);

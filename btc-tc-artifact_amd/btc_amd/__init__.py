@@ -17,7 +17,7 @@ import numpy as np
 
 __all__ = [
     "BtcError", "lib", "lib_path", "Graph", "BitBSR", "Plan", "Batch", "CountOpts", "count_cpu",
-    "batch_workload", "Server", "autotune", "tune_lookup", "tune_table_save", "tune_table_load",
+    "batch_workload", "serving_workload", "Server", "autotune", "tune_lookup", "tune_table_save", "tune_table_load",
     "tune_table_clear",
     "gen_rmat", "gen_er", "gen_stencil2d", "gen_stencil3d", "gen_planted", "lite_tile_k",
     "default_tau", "version", "CANON_COMPACT", "CANON_DEGREE_ORDER", "CANON_DEFAULT", "CANON_DEVICE",
@@ -125,6 +125,8 @@ _SIGS = {
     "btc_bitbsr_free": (None, [vp]),
     "btc_bitbsr_count": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, vp, C.POINTER(C.c_uint64), vp,
                                    C.POINTER(C.c_float)]),
+    "btc_bitbsr_count64": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, vp, C.POINTER(C.c_uint64), vp,
+                                     C.POINTER(C.c_float)]),
     "btc_lite_tile_k": (C.c_int32, [C.c_int32, C.c_int64, C.c_int32]),
     "btc_default_tau": (C.c_int32, [C.c_int32]),
     "btc_count_opts_default": (None, [C.POINTER(CountOpts)]),
@@ -132,6 +134,7 @@ _SIGS = {
     "btc_plan_get_info": (C.c_int, [vp, C.POINTER(PlanInfo)]),
     "btc_plan_tune": (C.c_int, [vp, C.POINTER(CountOpts)]),
     "btc_plan_count": (C.c_int, [vp, vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
+    "btc_plan_count64": (C.c_int, [vp, vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
     "btc_plan_last_timing": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "btc_plan_free": (None, [vp]),
     "btc_plan_capture": (C.c_int, [vp, vp]),
@@ -304,10 +307,10 @@ class Graph:
         self.close()
 
 
-def _count_out(n, per_vertex):
+def _count_out(n, per_vertex, wide=False):
     total = C.c_uint64(0)
     ms = C.c_float(0)
-    pv = np.zeros(n, np.int32) if per_vertex else None
+    pv = np.zeros(n, np.int64 if wide else np.int32) if per_vertex else None
     return total, ms, pv
 
 
@@ -336,14 +339,16 @@ class BitBSR:
         _check(lib().btc_bitbsr_copy_to_host(self._h, _ptr(indptr), _ptr(indices), _ptr(rows), _ptr(tiles)))
         return indptr, indices, rows, tiles
 
-    def count(self, variant=VARIANT_AUTO, threshold_blocks=0, tau=-1, per_vertex=False, stream=None, mfma=False):
+    def count(self, variant=VARIANT_AUTO, threshold_blocks=0, tau=-1, per_vertex=False, stream=None, mfma=False,
+              wide=False):
         """Returns (total, per_vertex or None, kernel_ms); mfma=True counts
-        dense tiles with the i8 MFMA form."""
-        total, ms, pv = _count_out(self.view["n"], per_vertex)
+        dense tiles with the i8 MFMA form; wide=True returns int64
+        per-vertex counts (btc_bitbsr_count64)."""
+        total, ms, pv = _count_out(self.view["n"], per_vertex, wide)
         if mfma:
             variant |= DENSE_MFMA
-        _check(lib().btc_bitbsr_count(self._h, variant, threshold_blocks, tau, stream, C.byref(total), _ptr(pv),
-                                      C.byref(ms)))
+        fn = lib().btc_bitbsr_count64 if wide else lib().btc_bitbsr_count
+        _check(fn(self._h, variant, threshold_blocks, tau, stream, C.byref(total), _ptr(pv), C.byref(ms)))
         return total.value, pv, ms.value
 
     def close(self):
@@ -393,9 +398,12 @@ class Plan:
         o = make_opts(**opts)
         _check(lib().btc_plan_tune(self._h, C.byref(o)))
 
-    def count(self, per_vertex=False, stream=None):
-        total, ms, pv = _count_out(self.graph.n, per_vertex)
-        _check(lib().btc_plan_count(self._h, stream, C.byref(total), _ptr(pv), C.byref(ms)))
+    def count(self, per_vertex=False, stream=None, wide=False):
+        """Returns (total, per_vertex or None, kernel_ms); wide=True returns
+        int64 per-vertex counts (btc_plan_count64)."""
+        total, ms, pv = _count_out(self.graph.n, per_vertex, wide)
+        fn = lib().btc_plan_count64 if wide else lib().btc_plan_count
+        _check(fn(self._h, stream, C.byref(total), _ptr(pv), C.byref(ms)))
         return total.value, pv, ms.value
 
     def count_share(self, shares, share, stream=None):
@@ -544,6 +552,33 @@ def batch_workload(count=4096, seed=1, n_lo=256, n_hi=16384):
             kind = "stencil2d"
         out.append((src, dst, n, kind))
     return out
+
+
+def serving_workload(requests, seed=1, key0=0, repeat=0.3, median_n=1 << 13, sigma=1.0, n_cap=1 << 20,
+                     edge_factor=16):
+    """The serving request stream: `requests` exact-count queries.  With
+    probability `repeat` a query repeats a graph key drawn uniformly from
+    the keys submitted before it (no edges sent); otherwise it brings a new
+    R-MAT graph (edge factor 16, a, b, c = 0.57, 0.19, 0.19, scale ceil(log2
+    n), pairs with an id >= n dropped) with n log-normal (median 2^13, sigma
+    1 in ln n, at least 64, capped at 2^20).  Draws use numpy's PCG64 from
+    `seed`.  Returns a list of (key, (src, dst, n) or None); new keys are
+    key0, key0 + 1, ..."""
+    import math
+    rng = np.random.default_rng(seed)
+    reqs, keys = [], []
+    for _ in range(requests):
+        if keys and rng.random() < repeat:
+            reqs.append((int(rng.choice(keys)), None))
+            continue
+        n = int(min(n_cap, max(64, round(math.exp(rng.normal(math.log(median_n), sigma))))))
+        sc = max(1, math.ceil(math.log2(n)))
+        s, d = gen_rmat(sc, edge_factor, 0.57, 0.19, 0.19, seed=int(rng.integers(1, 2 ** 62)))
+        keep = (s < n) & (d < n)
+        key = key0 + len(keys)
+        keys.append(key)
+        reqs.append((key, (s[keep], d[keep], n)))
+    return reqs
 
 
 class Server:

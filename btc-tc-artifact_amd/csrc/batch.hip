@@ -24,7 +24,7 @@ struct btc_batch_s {
     btc_graph_s graph;          // the block-diagonal L
     btc_plan plan = nullptr;
     btcx::DevBuf<uint32_t> d_voff;
-    btcx::DevBuf<int32_t> d_pv;
+    btcx::PvBuf d_pv;  // per-vertex sums (64-bit slots when a row is longer than 65536)
     btcx::DevBuf<unsigned long long> d_pg;
     std::vector<uint64_t> h_pg;
     btcx::CapturedGraph count_graph;  // zero + count + per-graph sums, replayed when captured
@@ -39,13 +39,13 @@ namespace btcx {
 namespace {
 
 // per_graph[g] = sum of pv over [voff[g], voff[g + 1]) (padding vertices are 0)
-__global__ __launch_bounds__(256) void graph_sums_kernel(const int32_t* __restrict__ pv,
-                                                         const uint32_t* __restrict__ voff,
+__global__ __launch_bounds__(256) void graph_sums_kernel(PvOut pv, const uint32_t* __restrict__ voff,
                                                          unsigned long long* __restrict__ per_graph) {
     __shared__ unsigned long long part[4];
     const uint32_t g = blockIdx.x;
     uint64_t s = 0;
-    for (uint32_t v = voff[g] + threadIdx.x; v < voff[g + 1]; v += 256) s += (uint32_t)pv[v];
+    for (uint32_t v = voff[g] + threadIdx.x; v < voff[g + 1]; v += 256)
+        s += pv.p64 ? (uint64_t)pv.p64[v] : (uint64_t)(uint32_t)pv.p32[v];
     s = wave_sum_u64(s);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
@@ -62,7 +62,7 @@ void batch_finish(btc_batch_s* b, const btc_count_opts* opts, hipStream_t s) {
     check_status(btc_plan_create(&b->graph, opts, s, &b->plan));
     std::vector<uint32_t> vo32(b->voff.begin(), b->voff.end());
     b->d_voff.alloc(vo32.size());
-    b->d_pv.alloc((size_t)b->graph.n);
+    b->d_pv.alloc((size_t)b->graph.n, b->graph.max_row);
     b->d_pg.alloc((size_t)b->count);
     b->h_pg.assign((size_t)b->count, 0);
     BTC_HIP(hipMemcpyAsync(b->d_voff.p, vo32.data(), vo32.size() * 4, hipMemcpyHostToDevice, s));
@@ -83,8 +83,8 @@ btc_batch_s* batch_from_graph(btc_graph_s&& g, const std::vector<int64_t>& voff,
 }  // namespace btcx
 
 void btc_batch_s::enqueue_count(hipStream_t s) {
-    btcx::plan_enqueue(plan, s, d_pv.p, nullptr, nullptr);
-    hipLaunchKernelGGL(btcx::graph_sums_kernel, dim3((unsigned)count), dim3(256), 0, s, d_pv.p, d_voff.p, d_pg.p);
+    btcx::plan_enqueue(plan, s, d_pv.out(), nullptr, nullptr);
+    hipLaunchKernelGGL(btcx::graph_sums_kernel, dim3((unsigned)count), dim3(256), 0, s, d_pv.out(), d_voff.p, d_pg.p);
     BTC_CHECK_LAUNCH("graph_sums_kernel");
 }
 
@@ -226,7 +226,7 @@ int btc_batch_capture(btc_batch b, void* stream) {
         BTC_REQUIRE(b, BTC_ERR_ARG, "NULL handle");
         (void)stream;
         b->count_graph.capture([&](hipStream_t cs) {
-            btcx::launch_zero_u32(reinterpret_cast<uint32_t*>(b->d_pv.p), b->d_pv.n, cs);
+            btcx::launch_zero_u32(reinterpret_cast<uint32_t*>(b->d_pv.data()), b->d_pv.bytes() / 4, cs);
             b->enqueue_count(cs);
         });
     });
@@ -241,7 +241,7 @@ int btc_batch_count(btc_batch b, void* stream, uint64_t* per_graph, uint64_t* to
             tm.start(s);
             b->count_graph.launch(s);
         } else {
-            btcx::launch_zero_u32(reinterpret_cast<uint32_t*>(b->d_pv.p), b->d_pv.n, s);
+            btcx::launch_zero_u32(reinterpret_cast<uint32_t*>(b->d_pv.data()), b->d_pv.bytes() / 4, s);
             tm.start(s);
             b->enqueue_count(s);
         }

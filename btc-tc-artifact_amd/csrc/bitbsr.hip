@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void tile_count_kernel(
     int n, int n_rb, const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
     const uint32_t* __restrict__ row_indices, const uint64_t* __restrict__ tiles, uint32_t w0,
     uint32_t w_end, const uint32_t* __restrict__ tile_list, uint32_t lstride, int tau,
-    unsigned long long* __restrict__ result, int32_t* __restrict__ pv) {
+    unsigned long long* __restrict__ result, PvOut pv) {
     constexpr int WPT = K / 4;
     constexpr int SLABS = K / 16;
     const int lane = threadIdx.x & 63;
@@ -457,7 +457,7 @@ __global__ __launch_bounds__(256) void tile_count_kernel(
                 x += __shfl_xor(x, 4);
                 x += __shfl_xor(x, 8);
                 const int64_t vtx = (int64_t)I * 16 + 4 * g + v;
-                if (c == 0 && x && vtx < n) atomicAdd(pv + vtx, (int32_t)x);
+                if (c == 0 && x && vtx < n) pv.add(vtx, x);
             }
         }
         const uint64_t tot = wave_sum_u64((uint64_t)(rs[0] + rs[1] + rs[2] + rs[3]));
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(256) void tile_count_kernel(
         uint32_t rs = sum + __shfl_xor(sum, 1);
         rs += __shfl_xor(rs, 2);
         const int64_t v = (int64_t)I * 16 + r;
-        if (q == 0 && rs && v < n) atomicAdd(pv + v, (int32_t)rs);
+        if (q == 0 && rs && v < n) pv.add(v, rs);
     }
     const uint64_t tot = wave_sum_u64((uint64_t)sum);
     if (lane == 0 && tot) atomicAdd(result, (unsigned long long)tot);
@@ -652,7 +652,7 @@ __device__ __forceinline__ uint64_t dense_tile(int n, int n_rb, const uint32_t* 
                                                const uint32_t* __restrict__ indices,
                                                const uint64_t* __restrict__ tiles, uint32_t t, uint32_t I, uint32_t a0,
                                                uint32_t a1, uint32_t set_words, ColumnSet& cs, int lane,
-                                               int32_t* __restrict__ pv) {
+                                               const PvOut& pv) {
     constexpr int SLABS = K / 16;
     const uint32_t JL = uniform(indices[t]);
     // matches of I's and J's tile columns: through the column set (built
@@ -733,7 +733,7 @@ __device__ __forceinline__ uint64_t dense_tile(int n, int n_rb, const uint32_t* 
                 x += __shfl_xor(x, 4);
                 x += __shfl_xor(x, 8);
                 const int64_t vtx = (int64_t)I * 16 + 4 * g + v;
-                if (c == 0 && x && vtx < n) atomicAdd(pv + vtx, (int32_t)x);
+                if (c == 0 && x && vtx < n) pv.add(vtx, x);
             }
         }
         return (uint64_t)(rs[0] + rs[1] + rs[2] + rs[3]);
@@ -762,7 +762,7 @@ __device__ __forceinline__ uint64_t dense_tile(int n, int n_rb, const uint32_t* 
             uint32_t rsum = sum + __shfl_xor(sum, 1);
             rsum += __shfl_xor(rsum, 2);
             const int64_t v = (int64_t)I * 16 + r;
-            if (q == 0 && rsum && v < n) atomicAdd(pv + v, (int32_t)rsum);
+            if (q == 0 && rsum && v < n) pv.add(v, rsum);
         }
         return (uint64_t)sum;
     }
@@ -780,7 +780,7 @@ __global__ __launch_bounds__(256) void dense_list_kernel(
     int n, int n_rb, const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
     const uint32_t* __restrict__ row_indices, const uint64_t* __restrict__ tiles, uint32_t w0, uint32_t w_end,
     const uint32_t* __restrict__ tile_list, uint32_t lstride, uint32_t set_words, uint32_t tpw,
-    unsigned long long* __restrict__ result, int32_t* __restrict__ pv) {
+    unsigned long long* __restrict__ result, PvOut pv) {
     // dynamic LDS sized to the graph (dense_set_bytes): 4 waves' bits, then
     // their ranks
     extern __shared__ uint32_t dyn_lds[];
@@ -821,12 +821,12 @@ constexpr uint32_t kMaxTilesPerLaunch = 1u << 24;
 
 template <int K, bool BS, bool MF>
 void launch_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstride, uint32_t nt, int tau,
-              unsigned long long* res, int32_t* pv, hipStream_t s) {
+              unsigned long long* res, PvOut pv, hipStream_t s) {
     for (uint32_t w0 = 0; w0 < nt; w0 += kMaxTilesPerLaunch) {
         const uint32_t chunk = std::min(kMaxTilesPerLaunch, nt - w0);
         const unsigned grid = (chunk + 3) / 4;
         const uint32_t end = w0 + chunk;
-        if (pv)
+        if (pv.on())
             hipLaunchKernelGGL((tile_count_kernel<K, BS, true, MF>), dim3(grid), dim3(256), 0, s, b->n, b->n_rb,
                                b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, end, list, lstride, tau,
                                res, pv);
@@ -840,7 +840,7 @@ void launch_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstride, uin
 
 template <int K>
 void launch_kk(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstride, uint32_t nt, bool bs, bool mfma,
-               int tau, unsigned long long* res, int32_t* pv, hipStream_t s) {
+               int tau, unsigned long long* res, PvOut pv, hipStream_t s) {
     if (bs) {
         mfma ? launch_k<K, true, true>(b, list, lstride, nt, tau, res, pv, s)
              : launch_k<K, true, false>(b, list, lstride, nt, tau, res, pv, s);
@@ -863,11 +863,11 @@ void bitbsr_build(btc_graph_s* g, int K, hipStream_t s, btc_bitbsr_s* b) {
 }
 
 void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles, bool binsearch,
-                       bool mfma, int tau, unsigned long long* d_result, int32_t* d_pv, hipStream_t s,
+                       bool mfma, int tau, unsigned long long* d_result, PvOut d_pv, hipStream_t s,
                        uint32_t lstride) {
     if (ntiles == 0) return;
-    BTC_REQUIRE(b->max_row <= 65536, BTC_ERR_OVERFLOW,
-                "longest row %d > 65536: per-lane 32-bit sums could overflow", b->max_row);
+    BTC_REQUIRE(lane_sums_fit(b->max_row, b->K), BTC_ERR_OVERFLOW,
+                "longest row %d: a lane's 32-bit sum over %d tile columns could overflow", b->max_row, b->K / 4);
     switch (b->K) {
         case 32: launch_kk<32>(b, tile_list, lstride, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
         case 64: launch_kk<64>(b, tile_list, lstride, ntiles, binsearch, mfma, tau, d_result, d_pv, s); break;
@@ -878,7 +878,7 @@ void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_
 namespace {
 template <int K, bool MF>
 void launch_dense_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstride, uint32_t nt,
-                    unsigned long long* res, int32_t* pv, hipStream_t s) {
+                    unsigned long long* res, PvOut pv, hipStream_t s) {
     // column blocks of the graph: ids of tile columns are below this
     const uint64_t ncb = ((uint64_t)b->n + K - 1) / K;
     const uint32_t set_words = ncb <= 32ull * kColSetWords ? (uint32_t)((ncb + 31) / 32) : 0u;
@@ -888,7 +888,7 @@ void launch_dense_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstrid
     for (uint32_t w0 = 0; w0 < nt; w0 += kMaxTilesPerLaunch) {
         const uint32_t chunk = std::min(kMaxTilesPerLaunch, nt - w0);
         const unsigned grid = (chunk + 4 * tpw - 1) / (4 * tpw);
-        if (pv)
+        if (pv.on())
             hipLaunchKernelGGL((dense_list_kernel<K, true, MF>), dim3(grid), dim3(256), lds, s, b->n, b->n_rb,
                                b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, w0 + chunk, list, lstride,
                                set_words, tpw, res, pv);
@@ -902,10 +902,10 @@ void launch_dense_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstrid
 }  // namespace
 
 void launch_dense_list(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles, bool mfma,
-                       unsigned long long* d_result, int32_t* d_pv, hipStream_t s, uint32_t lstride) {
+                       unsigned long long* d_result, PvOut d_pv, hipStream_t s, uint32_t lstride) {
     if (ntiles == 0) return;
-    BTC_REQUIRE(b->max_row <= 65536, BTC_ERR_OVERFLOW,
-                "longest row %d > 65536: per-lane 32-bit sums could overflow", b->max_row);
+    BTC_REQUIRE(lane_sums_fit(b->max_row, b->K), BTC_ERR_OVERFLOW,
+                "longest row %d: a lane's 32-bit sum over %d tile columns could overflow", b->max_row, b->K / 4);
     switch (b->K) {
         case 32: mfma ? launch_dense_k<32, true>(b, tile_list, lstride, ntiles, d_result, d_pv, s)
                       : launch_dense_k<32, false>(b, tile_list, lstride, ntiles, d_result, d_pv, s); break;
@@ -989,37 +989,54 @@ int btc_bitbsr_copy_to_host(btc_bitbsr b, uint32_t* indptr, uint32_t* indices, u
 
 void btc_bitbsr_free(btc_bitbsr b) { delete b; }
 
+}  // extern "C"
+
+namespace {
+
+// btc_bitbsr_count with per-vertex output to int32 (pv32) or int64 (pv64).
+void bitbsr_count(btc_bitbsr b, int32_t variant, int32_t threshold_blocks, int32_t tau, void* stream,
+                  uint64_t* total, int32_t* pv32, int64_t* pv64, float* kernel_ms) {
+    BTC_REQUIRE(b && total, BTC_ERR_ARG, "NULL argument");
+    const bool mfma = (variant & BTC_DENSE_MFMA) != 0;
+    variant &= ~BTC_DENSE_MFMA;
+    BTC_REQUIRE(variant >= BTC_VARIANT_AUTO && variant <= BTC_VARIANT_DIRECT, BTC_ERR_ARG,
+                "unknown variant %d", variant);
+    hipStream_t s = btcx::as_stream(stream);
+    if (tau < 0) tau = btc_default_tau(b->K);
+    if (threshold_blocks <= 0) threshold_blocks = b->K == 128 ? 2048 : 40000;
+    const bool bs = variant == BTC_VARIANT_BINSEARCH ||
+                    (variant == BTC_VARIANT_AUTO && b->nblk < (uint32_t)threshold_blocks);
+    btcx::PvBuf pv;
+    if (pv32 || pv64) {
+        pv.alloc((size_t)b->n, b->max_row);
+        BTC_HIP(hipMemsetAsync(pv.data(), 0, pv.bytes(), s));
+    }
+    BTC_HIP(hipMemsetAsync(b->result.p, 0, 8, s));
+    btcx::EventTimer tm(kernel_ms != nullptr);
+    tm.start(s);
+    btcx::launch_tile_count(b, nullptr, b->nblk, bs, mfma, tau, b->result.p, pv.out(), s);
+    tm.stop(s);
+    unsigned long long h = 0;
+    BTC_HIP(hipMemcpyAsync(&h, b->result.p, 8, hipMemcpyDeviceToHost, s));
+    pv.copy_out(pv32, pv64, s);  // synchronises s
+    if (kernel_ms) *kernel_ms = tm.ms();
+    *total = h;
+}
+
+}  // namespace
+
+extern "C" {
+
 int btc_bitbsr_count(btc_bitbsr b, int32_t variant, int32_t threshold_blocks, int32_t tau, void* stream,
                      uint64_t* total, int32_t* per_vertex, float* kernel_ms) {
-    return btcx::guard([&] {
-        BTC_REQUIRE(b && total, BTC_ERR_ARG, "NULL argument");
-        const bool mfma = (variant & BTC_DENSE_MFMA) != 0;
-        variant &= ~BTC_DENSE_MFMA;
-        BTC_REQUIRE(variant >= BTC_VARIANT_AUTO && variant <= BTC_VARIANT_DIRECT, BTC_ERR_ARG,
-                    "unknown variant %d", variant);
-        hipStream_t s = btcx::as_stream(stream);
-        if (tau < 0) tau = btc_default_tau(b->K);
-        if (threshold_blocks <= 0) threshold_blocks = b->K == 128 ? 2048 : 40000;
-        const bool bs = variant == BTC_VARIANT_BINSEARCH ||
-                        (variant == BTC_VARIANT_AUTO && b->nblk < (uint32_t)threshold_blocks);
-        btcx::DevBuf<int32_t> pv;
-        if (per_vertex) {
-            pv.alloc((size_t)b->n);
-            BTC_HIP(hipMemsetAsync(pv.p, 0, (size_t)b->n * 4, s));
-        }
-        BTC_HIP(hipMemsetAsync(b->result.p, 0, 8, s));
-        btcx::EventTimer tm(kernel_ms != nullptr);
-        tm.start(s);
-        btcx::launch_tile_count(b, nullptr, b->nblk, bs, mfma, tau, b->result.p, pv.p, s);
-        tm.stop(s);
-        unsigned long long h = 0;
-        BTC_HIP(hipMemcpyAsync(&h, b->result.p, 8, hipMemcpyDeviceToHost, s));
-        if (per_vertex && b->n)
-            BTC_HIP(hipMemcpyAsync(per_vertex, pv.p, (size_t)b->n * 4, hipMemcpyDeviceToHost, s));
-        BTC_HIP(hipStreamSynchronize(s));
-        if (kernel_ms) *kernel_ms = tm.ms();
-        *total = h;
-    });
+    return btcx::guard(
+        [&] { bitbsr_count(b, variant, threshold_blocks, tau, stream, total, per_vertex, nullptr, kernel_ms); });
+}
+
+int btc_bitbsr_count64(btc_bitbsr b, int32_t variant, int32_t threshold_blocks, int32_t tau, void* stream,
+                       uint64_t* total, int64_t* per_vertex, float* kernel_ms) {
+    return btcx::guard(
+        [&] { bitbsr_count(b, variant, threshold_blocks, tau, stream, total, nullptr, per_vertex, kernel_ms); });
 }
 
 }  // extern "C"

@@ -194,6 +194,48 @@ struct EventTimer {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Per-vertex sums on the device.  A vertex's count is at most C(d, 2) for
+// its row length d, which fits int32 while d <= kPv32MaxRow; graphs with a
+// longer row sum into 64-bit slots.  At most one pointer is set (none:
+// totals only).  Every kernel adds a lane's or a group's partial sum, a u32,
+// with one atomic.
+constexpr int32_t kPv32MaxRow = 65536;
+
+struct PvOut {
+    int32_t* p32 = nullptr;
+    unsigned long long* p64 = nullptr;
+    __host__ __device__ bool on() const { return p32 != nullptr || p64 != nullptr; }
+    // the pointers are kernel arguments or plan fields: the branch is uniform
+    __device__ __forceinline__ void add(int64_t v, uint32_t x) const {
+        if (p64) atomicAdd(p64 + v, (unsigned long long)x);
+        else atomicAdd(p32 + v, (int32_t)x);
+    }
+};
+
+// A lane of the tile kernels sums its row's intersections with K / 4 tile
+// columns in 32 bits; each is at most the longest row.
+inline bool lane_sums_fit(int64_t max_row, int K) { return max_row * (K / 4) < (int64_t(1) << 32); }
+
+// Device per-vertex sums for n vertices of a graph whose longest row is
+// max_row (int32 slots when every count fits them, else 64-bit).
+struct PvBuf {
+    DevBuf<int32_t> b32;
+    DevBuf<unsigned long long> b64;
+    size_t n = 0;
+    void alloc(size_t count, int32_t max_row) {
+        n = count;
+        if (max_row <= kPv32MaxRow) b32.alloc(count);
+        else b64.alloc(count);
+    }
+    PvOut out() const { return PvOut{b32.p, b64.p}; }
+    bool wide() const { return b64.p != nullptr; }
+    void* data() const { return b64.p ? (void*)b64.p : (void*)b32.p; }
+    size_t bytes() const { return n * (b64.p ? 8 : 4); }
+    // Copies the sums to host int32 (h32; BTC_ERR_OVERFLOW when a count
+    // does not fit) or int64 (h64) after the work on s; synchronises s.
+    void copy_out(int32_t* h32, int64_t* h64, hipStream_t s) const;
+};
+
 }  // namespace btcx
 
 // Canonical lower-triangular graph L (CSR, ascending columns) on host and HBM.

@@ -151,7 +151,7 @@ struct PlanDev {
     uint32_t n_block;    // items [0, n_block) are workgroup items, the rest wave items
     uint32_t n_items;
     unsigned long long* result;
-    int32_t* pv;
+    PvOut pv;
     uint32_t core4;  // hub-core ids / 128 (uint4 words of a row bitmap)
 };
 
@@ -276,7 +276,7 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
             uint32_t c = hits;
 #pragma unroll
             for (int o = G / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, G);
-            if (gl == 0 && c) atomicAdd(a.pv + a.ur[e], (int32_t)c);
+            if (gl == 0 && c) a.pv.add(a.ur[e], c);
         }
         L = nL;
     }
@@ -618,8 +618,7 @@ void classify_dense_host(btc_plan_s* p, hipStream_t s, std::vector<uint8_t>& den
     p->info.dense_tiles = p->n_dense;
     p->info.dense_edges = dense_edges;
     if (list.empty()) return;
-    p->bsr = std::make_unique<btc_bitbsr_s>();
-    bitbsr_build(g, K, s, p->bsr.get());
+    plan_bsr(p, s);
     BTC_REQUIRE(p->bsr->nblk == ntiles[n_rb], BTC_ERR_STATE, "tile count mismatch (%u vs %u)", p->bsr->nblk,
                 ntiles[n_rb]);
     p->dense_list.alloc(list.size());
@@ -635,9 +634,7 @@ void classify_dense(btc_plan_s* p, hipStream_t s, std::vector<uint8_t>& dense) {
         classify_dense_host(p, s, dense);
         return;
     }
-    p->bsr = std::make_unique<btc_bitbsr_s>();
-    bitbsr_build(g, p->K, s, p->bsr.get());
-    btc_bitbsr_s* b = p->bsr.get();
+    const btc_bitbsr_s* b = plan_bsr(p, s);
     DevBuf<uint32_t> d_pop;
     d_pop.alloc(b->nblk);
     launch_tile_popcount(b, d_pop.p, s);
@@ -657,7 +654,7 @@ void classify_dense(btc_plan_s* p, hipStream_t s, std::vector<uint8_t>& dense) {
     p->info.dense_tiles = p->n_dense;
     p->info.dense_edges = dense_edges;
     if (list.empty()) {
-        p->bsr.reset();
+        drop_plan_bsr(p);
         return;
     }
     const int K = p->K;
@@ -870,7 +867,7 @@ WedgeFn wedge_fn(const btc_count_opts& o, bool pv, unsigned* grid) {
     return fn;
 }
 
-PlanDev plan_dev(btc_plan_s* p, unsigned long long* d_total, int32_t* d_pv) {
+PlanDev plan_dev(btc_plan_s* p, unsigned long long* d_total, PvOut d_pv) {
     return PlanDev{p->g->d_col.p, p->col16.p, p->list_p, p->rowbits_p, p->urow_p, p->items_p, p->n_block_items, p->nitems,
                    d_total, d_pv, p->core_ids / 128};
 }
@@ -881,24 +878,24 @@ uint32_t share_count(uint32_t n, uint32_t share, uint32_t shares) {
 }
 
 // Workgroup and wave items of a single plan (tiny items go to launch_tiny).
-void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counters, int32_t* d_pv, hipStream_t s,
+void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counters, PvOut d_pv, hipStream_t s,
                   uint32_t share = 0, uint32_t shares = 1) {
     const uint32_t nb = share_count(p->n_block_items, share, shares);
     const uint32_t nw = share_count(p->nitems - p->n_tiny_items - p->n_block_items, share, shares);
     if (nb + nw == 0) return;
     unsigned grid = 0;
-    WedgeFn fn = wedge_fn(p->launch, d_pv != nullptr, &grid);
+    WedgeFn fn = wedge_fn(p->launch, d_pv.on(), &grid);
     WedgeArgs a{plan_dev(p, d_total, d_pv), nullptr, nullptr, 1u, nb, nb + nw, d_counters, (int)p->launch.lookup,
                 (int)p->launch.schedule, share, shares};
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
     BTC_CHECK_LAUNCH("wedge_kernel");
 }
 
-void launch_tiny(btc_plan_s* p, unsigned long long* d_total, int32_t* d_pv, hipStream_t s, uint32_t share = 0,
+void launch_tiny(btc_plan_s* p, unsigned long long* d_total, PvOut d_pv, hipStream_t s, uint32_t share = 0,
                  uint32_t shares = 1) {
     const uint32_t n = share_count(p->n_tiny_items, share, shares);
     if (!n) return;
-    auto fn = d_pv ? tiny_kernel<true> : tiny_kernel<false>;
+    auto fn = d_pv.on() ? tiny_kernel<true> : tiny_kernel<false>;
     int per_cu = 0;
     BTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kThreads, 0));
     constexpr uint32_t kGroups = kThreads / kTinyLanes;
@@ -920,14 +917,14 @@ void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, M
     std::vector<PlanDev> pd((size_t)k);
     uint32_t nb = 0, ni = 0;
     for (int j = 0; j < k; ++j) {
-        pd[(size_t)j] = plan_dev(P[j], ws.totals.p + j, nullptr);
+        pd[(size_t)j] = plan_dev(P[j], ws.totals.p + j, PvOut{});
         nb += P[j]->n_block_items;
         ni += P[j]->nitems;
     }
     for (int j = 0; j < k; ++j)
         if (P[j]->n_dense)
-            launch_dense_list(P[j]->bsr.get(), P[j]->dense_list.p, P[j]->n_dense, dense_mfma(P[j]->launch),
-                              ws.totals.p + j, nullptr, s);
+            launch_dense_list(P[j]->bsr, P[j]->dense_list.p, P[j]->n_dense, dense_mfma(P[j]->launch),
+                              ws.totals.p + j, PvOut{}, s);
     if (ni) {
         unsigned grid = 0;
         WedgeFn fn = wedge_fn(P[0]->launch, false, &grid);
@@ -968,9 +965,9 @@ void ensure_side_stream(btc_plan_s* p) {
     BTC_HIP(hipEventCreateWithFlags(&p->ev_join2, hipEventDisableTiming));
 }
 
-void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, EventTimer* tw) {
-    BTC_REQUIRE(!d_pv || p->g->max_row <= 65536, BTC_ERR_OVERFLOW,
-                "longest row %d > 65536: per-vertex counts could overflow int32", p->g->max_row);
+void plan_enqueue(btc_plan_s* p, hipStream_t s, PvOut d_pv, EventTimer* tm, EventTimer* tw) {
+    BTC_REQUIRE(!d_pv.p32 || p->g->max_row <= kPv32MaxRow, BTC_ERR_STATE,
+                "longest row %d: per-vertex counts need 64-bit slots", p->g->max_row);
     launch_zero_u32(reinterpret_cast<uint32_t*>(p->scratch.p), 4, s);
     unsigned long long* d_total = p->scratch.p;
     uint32_t* d_counters = reinterpret_cast<uint32_t*>(p->scratch.p + 1);
@@ -995,7 +992,7 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
         }
         if (p->n_dense) {
             BTC_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
-            launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, dense_mfma(p->launch), d_total, d_pv,
+            launch_dense_list(p->bsr, p->dense_list.p, p->n_dense, dense_mfma(p->launch), d_total, d_pv,
                               p->side);
             BTC_HIP(hipEventRecord(p->ev_join, p->side));
             BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
@@ -1006,7 +1003,7 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
     }
     if (tm) tm->start(s);
     if (p->n_dense)
-        launch_dense_list(p->bsr.get(), p->dense_list.p, p->n_dense, dense_mfma(p->launch), d_total, d_pv, s);
+        launch_dense_list(p->bsr, p->dense_list.p, p->n_dense, dense_mfma(p->launch), d_total, d_pv, s);
     if (tw) tw->start(s);
     if (p->nitems) launch_wedge(p, d_total, d_counters, d_pv, s);
     if (p->n_tiny_items) {
@@ -1022,8 +1019,10 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, E
 
 namespace {
 
-void plan_count(btc_plan_s* p, hipStream_t s, uint64_t* total, int32_t* per_vertex, float* kernel_ms) {
+// One count of p; per-vertex sums go to pv32 (int32) or pv64 (int64) when set.
+void plan_count(btc_plan_s* p, hipStream_t s, uint64_t* total, int32_t* pv32, int64_t* pv64, float* kernel_ms) {
     btc_graph_s* g = p->g;
+    const bool per_vertex = pv32 || pv64;
     if (!per_vertex && p->count_graph.exec) {
         // replay of the captured totals-only count
         EventTimer tm(kernel_ms != nullptr);
@@ -1040,18 +1039,16 @@ void plan_count(btc_plan_s* p, hipStream_t s, uint64_t* total, int32_t* per_vert
         *total = h;
         return;
     }
-    DevBuf<int32_t> pv;
+    PvBuf pv;
     if (per_vertex) {
-        pv.alloc((size_t)g->n);
-        BTC_HIP(hipMemsetAsync(pv.p, 0, (size_t)g->n * 4, s));
+        pv.alloc((size_t)g->n, g->max_row);
+        BTC_HIP(hipMemsetAsync(pv.data(), 0, pv.bytes(), s));
     }
     EventTimer tm(kernel_ms != nullptr), tw(kernel_ms != nullptr);
-    plan_enqueue(p, s, pv.p, &tm, &tw);
+    plan_enqueue(p, s, pv.out(), &tm, &tw);
     unsigned long long h = 0;
     BTC_HIP(hipMemcpyAsync(&h, p->scratch.p, 8, hipMemcpyDeviceToHost, s));
-    if (per_vertex && g->n)
-        BTC_HIP(hipMemcpyAsync(per_vertex, pv.p, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
-    BTC_HIP(hipStreamSynchronize(s));
+    pv.copy_out(pv32, pv64, s);  // synchronises s
     if (kernel_ms) {
         *kernel_ms = tm.ms();
         p->last_wedge_ms = tw.ms();
@@ -1073,63 +1070,95 @@ void btc_count_opts_default(btc_count_opts* o) {
     o->schedule = 1;
 }
 
+}  // extern "C"
+
+namespace btcx {
+namespace {
+
+// btc_plan_create; with ext (a Bit-BSR of g) the dense tiles are taken from
+// it and the plan's tile width is ext's.
+btc_plan_s* plan_create(btc_graph g, const btc_count_opts* opts, hipStream_t s, const btc_bitbsr_s* ext) {
+    BTC_REQUIRE(g, BTC_ERR_ARG, "NULL argument");
+    btc_count_opts o;
+    btc_count_opts_default(&o);
+    if (opts) o = *opts;
+    btcx::tune_table_apply(g, &o);  // auto fields from the autotuner's table, when it has this graph's class
+    if (ext) {
+        BTC_REQUIRE(ext->n == g->n, BTC_ERR_ARG, "Bit-BSR of %d vertices for a graph of %d", ext->n, g->n);
+        BTC_REQUIRE(!opts || opts->tile_k == 0 || opts->tile_k == ext->K, BTC_ERR_ARG,
+                    "tile width K = %d differs from the Bit-BSR's %d", opts->tile_k, ext->K);
+        o.tile_k = ext->K;
+    }
+    BTC_REQUIRE(o.tile_k == 0 || o.tile_k == 32 || o.tile_k == 64 || o.tile_k == 128, BTC_ERR_ARG,
+                "tile width K = %d (expected 0, 32, 64 or 128)", o.tile_k);
+    auto p = std::make_unique<btc_plan_s>();
+    p->g = g;
+    p->ext_bsr = ext;
+    p->K = o.tile_k ? o.tile_k : 64;
+    p->tau = o.tau >= 0 ? o.tau : btc_default_tau(p->K);
+    BTC_REQUIRE(o.core_ids <= 0 || (o.core_ids % 128 == 0 && (uint32_t)o.core_ids <= btcx::kCoreIdsMax),
+                BTC_ERR_ARG, "core_ids %d (expected a multiple of 128 up to %u, 0 = default, < 0 = none)",
+                o.core_ids, btcx::kCoreIdsMax);
+    p->core_ids = o.core_ids == 0 ? btcx::kCoreIdsDefault : o.core_ids < 0 ? 0u : (uint32_t)o.core_ids;
+    BTC_REQUIRE(o.tiny_items >= -1 && o.tiny_items <= 1, BTC_ERR_ARG,
+                "tiny_items %d (expected -1, 0 or 1)", o.tiny_items);
+    p->use_tiny = o.tiny_items > 0 || (o.tiny_items == 0 && g->wedges >= btcx::kTinyMinWedges);
+    btcx::check_launch_opts(o);
+    p->launch = o;
+    btcx::ensure_device(g, s);
+    BTC_REQUIRE((uint64_t)g->m + btcx::kColPad < (1ull << 30), BTC_ERR_OVERFLOW,
+                "nnz(L) = %lld: the wedge kernel addresses L with 32-bit byte offsets (limit 2^30 - %zu)",
+                (long long)g->m, btcx::kColPad);
+    btcx::build_col16(p.get(), s);  // ordered before the planners' closing synchronisation
+    int64_t chunk = o.chunk_work > 0 ? o.chunk_work : 0;
+    if (!chunk) {
+        const int64_t cus = btcx::device_cu_count();
+        // about 64 items per CU, from 8192 up to 2^20 checks each (R-MAT
+        // s22: 2^20 gave 2.82 ms per count, 2^18 2.87 ms, 2^16 3.04 ms)
+        chunk = std::max<int64_t>(8192, std::min<int64_t>(1 << 20, g->wedges / std::max<int64_t>(1, cus * 64)));
+    }
+    const bool on_device = o.planner == 2 || (o.planner == 0 && (!g->on_host || g->m > btcx::kHostPlanEdges));
+    if (on_device) {
+        btcx::plan_device(p.get(), chunk, s);
+    } else {
+        btcx::ensure_host(g);
+        std::vector<uint8_t> dense;
+        btcx::classify_dense(p.get(), s, dense);
+        btcx::build_wedge_lists(p.get(), dense, chunk, s);
+    }
+    p->scratch.alloc(2);
+    p->info.tile_k = p->K;
+    p->info.tau = p->tau;
+    p->info.core_ids = p->core_ids;
+    int64_t bytes =
+        (int64_t)(p->lists.bytes() + p->items_dev.bytes() + p->dense_list.bytes() + p->scratch.bytes() +
+                  p->col16.bytes());
+    if (p->own_bsr)  // a caller's Bit-BSR is the caller's memory
+        bytes += (int64_t)(p->own_bsr->indptr.bytes() + p->own_bsr->indices.bytes() +
+                           p->own_bsr->row_indices.bytes() + p->own_bsr->tiles.bytes());
+    p->info.device_bytes = bytes;
+    return p.release();
+}
+
+}  // namespace
+}  // namespace btcx
+
+extern "C" {
+
 int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_plan* out) {
     return btcx::guard([&] {
-        BTC_REQUIRE(g && out, BTC_ERR_ARG, "NULL argument");
+        BTC_REQUIRE(out, BTC_ERR_ARG, "NULL argument");
         *out = nullptr;
-        btc_count_opts o;
-        btc_count_opts_default(&o);
-        if (opts) o = *opts;
-        btcx::tune_table_apply(g, &o);  // auto fields from the autotuner's table, when it has this graph's class
-        BTC_REQUIRE(o.tile_k == 0 || o.tile_k == 32 || o.tile_k == 64 || o.tile_k == 128, BTC_ERR_ARG,
-                    "tile width K = %d (expected 0, 32, 64 or 128)", o.tile_k);
-        hipStream_t s = btcx::as_stream(stream);
-        auto p = std::make_unique<btc_plan_s>();
-        p->g = g;
-        p->K = o.tile_k ? o.tile_k : 64;
-        p->tau = o.tau >= 0 ? o.tau : btc_default_tau(p->K);
-        BTC_REQUIRE(o.core_ids <= 0 || (o.core_ids % 128 == 0 && (uint32_t)o.core_ids <= btcx::kCoreIdsMax),
-                    BTC_ERR_ARG, "core_ids %d (expected a multiple of 128 up to %u, 0 = default, < 0 = none)",
-                    o.core_ids, btcx::kCoreIdsMax);
-        p->core_ids = o.core_ids == 0 ? btcx::kCoreIdsDefault : o.core_ids < 0 ? 0u : (uint32_t)o.core_ids;
-        BTC_REQUIRE(o.tiny_items >= -1 && o.tiny_items <= 1, BTC_ERR_ARG,
-                    "tiny_items %d (expected -1, 0 or 1)", o.tiny_items);
-        p->use_tiny = o.tiny_items > 0 || (o.tiny_items == 0 && g->wedges >= btcx::kTinyMinWedges);
-        btcx::check_launch_opts(o);
-        p->launch = o;
-        btcx::ensure_device(g, s);
-        BTC_REQUIRE((uint64_t)g->m + btcx::kColPad < (1ull << 30), BTC_ERR_OVERFLOW,
-                    "nnz(L) = %lld: the wedge kernel addresses L with 32-bit byte offsets (limit 2^30 - %zu)",
-                    (long long)g->m, btcx::kColPad);
-        btcx::build_col16(p.get(), s);  // ordered before the planners' closing synchronisation
-        int64_t chunk = o.chunk_work > 0 ? o.chunk_work : 0;
-        if (!chunk) {
-            const int64_t cus = btcx::device_cu_count();
-            // about 64 items per CU, from 8192 up to 2^20 checks each (R-MAT
-            // s22: 2^20 gave 2.82 ms per count, 2^18 2.87 ms, 2^16 3.04 ms)
-            chunk = std::max<int64_t>(8192, std::min<int64_t>(1 << 20, g->wedges / std::max<int64_t>(1, cus * 64)));
-        }
-        const bool on_device = o.planner == 2 || (o.planner == 0 && (!g->on_host || g->m > btcx::kHostPlanEdges));
-        if (on_device) {
-            btcx::plan_device(p.get(), chunk, s);
-        } else {
-            btcx::ensure_host(g);
-            std::vector<uint8_t> dense;
-            btcx::classify_dense(p.get(), s, dense);
-            btcx::build_wedge_lists(p.get(), dense, chunk, s);
-        }
-        p->scratch.alloc(2);
-        p->info.tile_k = p->K;
-        p->info.tau = p->tau;
-        p->info.core_ids = p->core_ids;
-        int64_t bytes =
-            (int64_t)(p->lists.bytes() + p->items_dev.bytes() + p->dense_list.bytes() + p->scratch.bytes() +
-                      p->col16.bytes());
-        if (p->bsr)
-            bytes += (int64_t)(p->bsr->indptr.bytes() + p->bsr->indices.bytes() + p->bsr->row_indices.bytes() +
-                               p->bsr->tiles.bytes());
-        p->info.device_bytes = bytes;
-        *out = p.release();
+        *out = btcx::plan_create(g, opts, btcx::as_stream(stream), nullptr);
+    });
+}
+
+int btc_plan_create_on_bitbsr(btc_graph g, btc_bitbsr b, const btc_count_opts* opts, void* stream,
+                              btc_plan* out) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(out && b, BTC_ERR_ARG, "NULL argument");
+        *out = nullptr;
+        *out = btcx::plan_create(g, opts, btcx::as_stream(stream), b);
     });
 }
 
@@ -1170,11 +1199,12 @@ int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream
             }
         }
         if (p->nitems)
-            btcx::launch_wedge(p, p->scratch.p, reinterpret_cast<uint32_t*>(p->scratch.p + 1), nullptr, s, j, k);
-        btcx::launch_tiny(p, p->scratch.p, nullptr, ts, j, k);
+            btcx::launch_wedge(p, p->scratch.p, reinterpret_cast<uint32_t*>(p->scratch.p + 1), btcx::PvOut{}, s, j,
+                               k);
+        btcx::launch_tiny(p, p->scratch.p, btcx::PvOut{}, ts, j, k);
         if (nd)
-            btcx::launch_dense_list(p->bsr.get(), p->dense_list.p + j, nd, btcx::dense_mfma(p->launch), p->scratch.p,
-                                    nullptr, ds, k);
+            btcx::launch_dense_list(p->bsr, p->dense_list.p + j, nd, btcx::dense_mfma(p->launch), p->scratch.p,
+                                    btcx::PvOut{}, ds, k);
         if (ds != s) {
             BTC_HIP(hipEventRecord(p->ev_join, ds));
             BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
@@ -1196,7 +1226,7 @@ int btc_plan_capture(btc_plan p, void* stream) {
     return btcx::guard([&] {
         BTC_REQUIRE(p, BTC_ERR_ARG, "NULL handle");
         (void)stream;  // captured on a private stream; replays run on the count's stream
-        p->count_graph.capture([&](hipStream_t cs) { btcx::plan_enqueue(p, cs, nullptr, nullptr, nullptr); });
+        p->count_graph.capture([&](hipStream_t cs) { btcx::plan_enqueue(p, cs, btcx::PvOut{}, nullptr, nullptr); });
     });
 }
 
@@ -1210,7 +1240,14 @@ int btc_plan_get_info(btc_plan p, btc_plan_info* info) {
 int btc_plan_count(btc_plan p, void* stream, uint64_t* total, int32_t* per_vertex, float* kernel_ms) {
     return btcx::guard([&] {
         BTC_REQUIRE(p && total, BTC_ERR_ARG, "NULL argument");
-        btcx::plan_count(p, btcx::as_stream(stream), total, per_vertex, kernel_ms);
+        btcx::plan_count(p, btcx::as_stream(stream), total, per_vertex, nullptr, kernel_ms);
+    });
+}
+
+int btc_plan_count64(btc_plan p, void* stream, uint64_t* total, int64_t* per_vertex, float* kernel_ms) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(p && total, BTC_ERR_ARG, "NULL argument");
+        btcx::plan_count(p, btcx::as_stream(stream), total, nullptr, per_vertex, kernel_ms);
     });
 }
 

@@ -27,7 +27,12 @@ struct btc_plan_s {
     uint32_t core_ids = 0;  // hub-core id range C of the row bitmaps (0 = none)
     bool use_tiny = false;  // tiny pivots become tiny items (btc_count_opts.tiny_items)
     btc_count_opts launch{};  // launch-only options (btc_plan_tune)
-    std::unique_ptr<btc_bitbsr_s> bsr;
+    // Bit-BSR of width K read by the dense-tile kernel (NULL when no tile is
+    // dense): the plan's own, or a caller's Bit-BSR of g that must outlive
+    // the plan (btc_plan_create_on_bitbsr)
+    std::unique_ptr<btc_bitbsr_s> own_bsr;
+    const btc_bitbsr_s* ext_bsr = nullptr;
+    const btc_bitbsr_s* bsr = nullptr;
     btcx::DevBuf<uint32_t> dense_list;
     uint32_t n_dense = 0;
     btcx::DevBuf<unsigned char> lists;  // items, list records, row core bitmaps, list rows
@@ -157,20 +162,37 @@ void scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, hipStream_t s)
 // Builds the 16 x K Bit-BSR of g on `s` into b (b's buffers are replaced).
 void bitbsr_build(btc_graph_s* g, int K, hipStream_t s, btc_bitbsr_s* b);
 
+// Sets p->bsr to the plan's Bit-BSR of width K: the caller's when the plan
+// was made on one, else one built on s.  drop_plan_bsr releases it.
+inline const btc_bitbsr_s* plan_bsr(btc_plan_s* p, hipStream_t s) {
+    if (p->ext_bsr) {
+        p->bsr = p->ext_bsr;
+    } else {
+        p->own_bsr = std::make_unique<btc_bitbsr_s>();
+        bitbsr_build(p->g, p->K, s, p->own_bsr.get());
+        p->bsr = p->own_bsr.get();
+    }
+    return p->bsr;
+}
+inline void drop_plan_bsr(btc_plan_s* p) {
+    p->own_bsr.reset();
+    p->bsr = nullptr;
+}
+
 // Count over mask tiles of b.  tile_list (device) selects ntiles mask tiles,
 // or all tiles when NULL (list entries lstride apart: tile_list[w * lstride]).
 // tau < 0 sends every tile down the dense path; mfma selects the i8 MFMA
 // form of the dense path over AND + popcount.  Adds the total to *d_result
-// and per-row sums to d_pv (when not NULL).
+// and per-row sums to d_pv (when it is on).
 void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles,
-                       bool binsearch, bool mfma, int tau, unsigned long long* d_result, int32_t* d_pv,
+                       bool binsearch, bool mfma, int tau, unsigned long long* d_result, PvOut d_pv,
                        hipStream_t s, uint32_t lstride = 1);
 
 // Dense-path count over a list of mask tiles (the engine's dense tiles):
 // like launch_tile_count with tau < 0, the two row blocks' tile columns
 // matched by a wave-parallel binary search instead of a two-pointer walk.
 void launch_dense_list(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles, bool mfma,
-                       unsigned long long* d_result, int32_t* d_pv, hipStream_t s, uint32_t lstride = 1);
+                       unsigned long long* d_result, PvOut d_pv, hipStream_t s, uint32_t lstride = 1);
 
 // p[0, n) = 0 by a kernel (safe inside captured graphs).
 void launch_zero_u32(uint32_t* p, size_t n, hipStream_t s);
@@ -197,8 +219,8 @@ void plans_count(btc_plan_s* const* plans, int k, hipStream_t s, uint64_t* total
 
 // Enqueues one exact count of plan p on stream s: zeroes the plan's total,
 // runs the dense-tile and wedge kernels; the u64 total lands in
-// p->scratch.p[0] and per-row sums are added into d_pv when not NULL.
+// p->scratch.p[0] and per-row sums are added into d_pv when it is on.
 // tm times both kernels, tw the wedge kernel (either may be NULL).
-void plan_enqueue(btc_plan_s* p, hipStream_t s, int32_t* d_pv, EventTimer* tm, EventTimer* tw);
+void plan_enqueue(btc_plan_s* p, hipStream_t s, PvOut d_pv, EventTimer* tm, EventTimer* tw);
 
 }  // namespace btcx

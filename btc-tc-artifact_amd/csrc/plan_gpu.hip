@@ -309,9 +309,7 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     TmpBuf<uint8_t> dense;
     p->n_dense = 0;
     if (p->tau < 16 * p->K && m > 0) {
-        p->bsr = std::make_unique<btc_bitbsr_s>();
-        bitbsr_build(g, p->K, s, p->bsr.get());
-        btc_bitbsr_s* b = p->bsr.get();
+        const btc_bitbsr_s* b = plan_bsr(p, s);
         TmpBuf<uint32_t> pop, flag, pos;
         pop.alloc(b->nblk, s);
         flag.alloc(b->nblk, s);
@@ -329,7 +327,7 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
             p->n_dense = nd;
         }
         BTC_HIP(hipStreamSynchronize(s));
-        if (!nd) p->bsr.reset();
+        if (!nd) drop_plan_bsr(p);
     }
     p->info.dense_tiles = p->n_dense;
     p->info.dense_edges = (int64_t)read_dev(st.p + 0, s);

@@ -74,6 +74,39 @@ void mem_pool_init() {
     });
 }
 
+void PvBuf::copy_out(int32_t* h32, int64_t* h64, hipStream_t s) const {
+    if (n == 0 || (!h32 && !h64)) {
+        BTC_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    if (!wide() && h32) {
+        BTC_HIP(hipMemcpyAsync(h32, b32.p, n * 4, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    if (wide() && h64) {
+        BTC_HIP(hipMemcpyAsync(h64, b64.p, n * 8, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    if (h64) {  // int32 slots widened
+        std::vector<int32_t> t(n);
+        BTC_HIP(hipMemcpyAsync(t.data(), b32.p, n * 4, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipStreamSynchronize(s));
+        for (size_t v = 0; v < n; ++v) h64[v] = t[v];
+        return;
+    }
+    std::vector<unsigned long long> t(n);  // 64-bit slots narrowed, each checked
+    BTC_HIP(hipMemcpyAsync(t.data(), b64.p, n * 8, hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    for (size_t v = 0; v < n; ++v) {
+        BTC_REQUIRE(t[v] <= (unsigned long long)INT32_MAX, BTC_ERR_OVERFLOW,
+                    "vertex %zu closes %llu triangles, beyond int32 per-vertex output (the int64 entry points "
+                    "return it)", v, t[v]);
+        h32[v] = (int32_t)t[v];
+    }
+}
+
 }  // namespace btcx
 
 extern "C" {

@@ -167,6 +167,15 @@ void btc_bitbsr_free(btc_bitbsr b);
 #define BTC_DENSE_MFMA 16
 int btc_bitbsr_count(btc_bitbsr b, int32_t variant, int32_t threshold_blocks, int32_t tau,
                      void* stream, uint64_t* total, int32_t* per_vertex, float* kernel_ms);
+/* The same with int64 per-vertex counts.  Per-vertex sums are kept in int32
+ * on the device while the longest row of L is at most 65536 (a vertex's
+ * count is at most C(d, 2) < 2^31) and in 64 bits beyond; with int32 output
+ * btc_bitbsr_count returns BTC_ERR_OVERFLOW only when a count does not fit,
+ * this variant never.  Totals are u64 for any graph whose longest row is
+ * below 2^32 / (K / 4) (a lane sums its row over K / 4 tile columns in 32
+ * bits, like the reference's per-warp u32 reduction, tc_16x128_mma_v5.h:203-210). */
+int btc_bitbsr_count64(btc_bitbsr b, int32_t variant, int32_t threshold_blocks, int32_t tau,
+                       void* stream, uint64_t* total, int64_t* per_vertex, float* kernel_ms);
 
 /* Lite tile policy (apps/btc_tc_lite.cu:45-65): returns 32 if m/n < 15 or
  * n < 50000, else 128; mode 1 forces 32, mode 2 forces 128 (BTC_MODE). */
@@ -175,12 +184,18 @@ int32_t btc_lite_tile_k(int32_t n, int64_t m, int32_t mode);
 int32_t btc_default_tau(int32_t tile_k);
 
 /* ------------------------------------------------------ hybrid count engine
- * The MI355X path for L .* (L L^T): edges of L in dense 16xK mask tiles
- * (popcount > tau) are counted over LDS-staged bit tiles with 64-bit AND +
- * popcount; every other edge (i, j) is counted as the wedges (i; j, k),
- * k < j, k in N_L(i), checked against an LDS hash of N_L(j) by one
- * workgroup per pivot j (work-stealing over pivot chunks).  Same result as
- * btc_bitbsr_count and the CPU merge, bit for bit. */
+ * The MI355X path for L .* (L L^T).  Edges of L in dense 16xK mask tiles
+ * (popcount > tau) are counted by one wave per tile over the Bit-BSR: the
+ * tile columns of the two row blocks are matched through an LDS bitmap of
+ * one row block's columns (or a wave-parallel binary search), and each
+ * matched pair adds A.B^T by v_mfma_i32_16x16x64_i8 on 0/1-unpacked tile
+ * rows (default) or by 64-bit AND + popcount.  Every other edge (i, j) is
+ * counted as the wedges (i; j, k), k < j, k in N_L(i): a persistent grid
+ * takes work items (a pivot j and a range of rows holding it) in a static
+ * stride over work-sorted items; a workgroup or a single wave loads N_L(j)
+ * into LDS as a bitmap window (else a hash table, else it searches N_L(j)
+ * in HBM) and streams the rows' prefixes through it, ANDing hub ids as
+ * bitmaps.  Same result as btc_bitbsr_count and the CPU merge, bit for bit. */
 typedef struct {
     int32_t tile_k;       /* dense tile width (32/64/128); 0 = auto */
     int32_t tau;          /* dense threshold; < 0 = default for tile_k; >= 16*K disables dense */
@@ -224,11 +239,22 @@ typedef struct {
 
 void btc_count_opts_default(btc_count_opts* opts);
 int btc_plan_create(btc_graph g, const btc_count_opts* opts, void* stream, btc_plan* out);
+/* The same on a Bit-BSR b already built from g (btc_bitbsr_build): the dense
+ * tiles are taken from b instead of a Bit-BSR of the plan's own, and the
+ * plan's tile width is b's (opts->tile_k must be 0 or b's).  b must outlive
+ * the plan.  This is how the reference's count_triangles_16x{128,32}_adaptive
+ * (tc_16x128_adaptive.h:12-29, tc_16x32_adaptive.h:14-31), which count a
+ * BCSR_16x{128,32}_Device, reach the engine. */
+int btc_plan_create_on_bitbsr(btc_graph g, btc_bitbsr b, const btc_count_opts* opts, void* stream,
+                              btc_plan* out);
 int btc_plan_get_info(btc_plan p, btc_plan_info* info);
 /* Changes the launch-only options of a plan (waves_per_eu, lookup, dense_kernel,
  * group_lanes, unroll, schedule); tile_k, tau and chunk_work need a new plan. */
 int btc_plan_tune(btc_plan p, const btc_count_opts* opts);
 int btc_plan_count(btc_plan p, void* stream, uint64_t* total, int32_t* per_vertex, float* kernel_ms);
+/* The same with int64 per-vertex counts (see btc_bitbsr_count64 for when
+ * int32 output overflows). */
+int btc_plan_count64(btc_plan p, void* stream, uint64_t* total, int64_t* per_vertex, float* kernel_ms);
 /* Event times of the last btc_plan_count with kernel_ms != NULL: wedge_ms
  * = the wedge kernel, dense_ms = the rest of the count (dense-tile and
  * tiny-pivot kernels, beside or around the wedge kernel; 0 when neither is

@@ -168,25 +168,6 @@ uint64_t orc_tc_merge(const int64_t* rowptr, const int32_t* col, int32_t n, int3
     return total;
 }
 
-/* Merge restricted to rows [r0, r1) (bounded CPU-baseline samples). */
-uint64_t orc_tc_merge_rows(const int64_t* rowptr, const int32_t* col, int32_t r0, int32_t r1,
-                           int32_t threads) {
-    uint64_t total = 0;
-#pragma omp parallel for num_threads(threads > 0 ? threads : 1) reduction(+ : total) schedule(dynamic, 256)
-    for (int32_t r = r0; r < r1; ++r) {
-        for (int64_t i = rowptr[r]; i < rowptr[r + 1]; ++i) {
-            int32_t u = col[i];
-            int64_t pr = rowptr[r], pu = rowptr[u];
-            while (pr < rowptr[r + 1] && pu < rowptr[u + 1]) {
-                if (col[pr] < col[pu]) ++pr;
-                else if (col[pr] > col[pu]) ++pu;
-                else { ++total; ++pr; ++pu; }
-            }
-        }
-    }
-    return total;
-}
-
 /* #{i > j > k : (i,j), (i,k), (j,k) in L} by a dense adjacency matrix (n <= 4096). */
 uint64_t orc_tc_brute(const int64_t* rowptr, const int32_t* col, int32_t n, int64_t* per_row) {
     if (n > 4096) return (uint64_t)-1;

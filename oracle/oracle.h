@@ -7,7 +7,6 @@ int64_t orc_canon(const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n
                   int64_t** rowptr_out, int32_t** col_out, int32_t** labels_out, int32_t* n_out);
 void orc_free(void* p);
 uint64_t orc_tc_merge(const int64_t* rowptr, const int32_t* col, int32_t n, int32_t threads, int64_t* per_row);
-uint64_t orc_tc_merge_rows(const int64_t* rowptr, const int32_t* col, int32_t r0, int32_t r1, int32_t threads);
 uint64_t orc_tc_brute(const int64_t* rowptr, const int32_t* col, int32_t n, int64_t* per_row);
 int64_t orc_bitbsr_build(const int64_t* rowptr, const int32_t* col, int32_t n, int32_t K, uint32_t* indptr,
                          uint32_t* indices, uint32_t* row_indices, uint64_t* tiles);

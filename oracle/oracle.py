@@ -41,8 +41,6 @@ def _lib():
         L.orc_free.argtypes = [vp]
         L.orc_tc_merge.restype = C.c_uint64
         L.orc_tc_merge.argtypes = [vp, vp, C.c_int32, C.c_int32, vp]
-        L.orc_tc_merge_rows.restype = C.c_uint64
-        L.orc_tc_merge_rows.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_int32]
         L.orc_tc_brute.restype = C.c_uint64
         L.orc_tc_brute.argtypes = [vp, vp, C.c_int32, vp]
         L.orc_bitbsr_build.restype = C.c_int64
@@ -88,12 +86,6 @@ def tc_merge(rowptr, col, threads=1, per_row=False):
     pr = np.zeros(n, np.int64) if per_row else None
     t = _lib().orc_tc_merge(_p(rowptr), _p(col), n, threads, _p(pr))
     return (t, pr) if per_row else t
-
-
-def tc_merge_rows(rowptr, col, r0, r1, threads=1):
-    rowptr = np.ascontiguousarray(rowptr, np.int64)
-    col = np.ascontiguousarray(col, np.int32)
-    return _lib().orc_tc_merge_rows(_p(rowptr), _p(col), r0, r1, threads)
 
 
 def tc_brute(rowptr, col, per_row=False):

@@ -126,3 +126,41 @@ def test_lite_cli_contract(gpu, tmp_path, engine, mode):
     assert int(re.search(CONTRACT[3], r.stdout).group(1)) == t
     assert "Verify: PASS" in r.stdout
     assert ("USE 128" in r.stdout) == (mode == "2")
+
+
+@pytest.fixture(scope="module")
+def adaptive_check(btc, tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("cpp") / "adaptive_check")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", PKG,
+                    os.path.join(ROOT, "tests", "cpp", "adaptive_check.cpp"), "-o", out, "-L", LIBDIR, "-lbtc_amd",
+                    "-Wl,-rpath," + LIBDIR], check=True)
+    return out
+
+
+def test_adaptive_check_builds(adaptive_check):
+    assert os.path.exists(adaptive_check)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("reorder", ["1", "0"])
+def test_reference_entry_points_reach_engine(gpu, orc, adaptive_check, tmp_path, reorder):
+    """count_triangles_16x{128,32}_adaptive (engine), v4/v5/v6 (reference
+    semantics) on the same BCSR handle: every count equals the oracle's, in
+    degree order and in the input's id order (BTC_REORDER=0)."""
+    from test_gpu_long_rows import hub_graph
+    cases = [gpu.gen_rmat(12, 16, seed=5) + (1 << 12,), hub_graph(seed=2, others=4000, degree=8)]
+    if reorder == "0":
+        cases.append(hub_graph(seed=3))  # an L-row of 66000 ids
+    for s, d, n in cases:
+        rp, c, _ = orc.canon(s, d, n, 3)
+        want = orc.tc_merge(rp, c, threads=16)
+        path = write_inputs(str(tmp_path), s, d, n)["el"]
+        env = dict(os.environ, BTC_REORDER=reorder)
+        r = subprocess.run([adaptive_check, path], capture_output=True, text=True, env=env, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        got = dict(re.findall(r"^(\w+) (\d+)$", r.stdout, re.M))
+        calls = ["v4_16x128", "v5_16x128", "adaptive_16x128", "adaptive_16x128_again", "v5_16x32", "v6_16x32",
+                 "adaptive_16x32", "adaptive_16x32_again"]
+        for k in calls:
+            assert int(got[k]) == want, (k, got, want)
+        assert got["adaptive_16x128_ms_set"] == "1"

@@ -1,5 +1,5 @@
-"""Serving: every answer is exact and equal to the graph's count alone,
-whatever it was batched with; repeats hit the resident set; a tight HBM
+"""Serving: every answer equals the oracle's count of its graph, whatever it
+was batched with; repeats hit the resident set; a tight HBM
 budget evicts idle graphs without changing answers; bad input fails only
 its own requests."""
 import time
@@ -37,14 +37,16 @@ def make_graphs(gpu, k, seed):
     return gs
 
 
-def alone(gpu, s, d, n):
-    return gpu.Plan(gpu.Graph.from_coo(s, d, n)).count()[0]
+def alone(orc, s, d, n):
+    """The graph's count by the oracle (its own canonicalisation and merge)."""
+    rp, col, _ = orc.canon(s, d, n, 3)
+    return orc.tc_merge(rp, col, threads=4)
 
 
 @pytest.mark.parametrize("canon_flags", [3, 3 | 4])  # host / GPU canonicalisation and planning
-def test_server_mix_equals_alone(gpu, canon_flags):
+def test_server_mix_equals_alone(gpu, orc, canon_flags):
     gs = make_graphs(gpu, 40, seed=1)
-    want = [alone(gpu, *g) for g in gs]
+    want = [alone(orc, *g) for g in gs]
     sv = gpu.Server(max_batch=16, prep_threads=4, canon_flags=canon_flags)
     ids = {}
     for k, (s, d, n) in enumerate(gs):
@@ -63,9 +65,9 @@ def test_server_mix_equals_alone(gpu, canon_flags):
     sv.close()
 
 
-def test_server_eviction_keeps_answers(gpu):
+def test_server_eviction_keeps_answers(gpu, orc):
     gs = make_graphs(gpu, 12, seed=2)
-    want = [alone(gpu, *g) for g in gs]
+    want = [alone(orc, *g) for g in gs]
     # host preparation: one graph per resident unit
     sv = gpu.Server(hbm_budget_bytes=3 << 20, max_batch=4, prep_threads=2, canon_flags=3)
     for rnd in range(2):
@@ -77,11 +79,11 @@ def test_server_eviction_keeps_answers(gpu):
     sv.close()
 
 
-def test_server_grouped_units_evict_and_stay_exact(gpu):
+def test_server_grouped_units_evict_and_stay_exact(gpu, orc):
     """GPU preparation groups queued graphs into one unit; a tight budget
     evicts whole idle units, and every answer stays exact."""
     gs = make_graphs(gpu, 24, seed=5)
-    want = [alone(gpu, *g) for g in gs]
+    want = [alone(orc, *g) for g in gs]
     sv = gpu.Server(hbm_budget_bytes=4 << 20, max_batch=64, prep_threads=2)
     for rnd in range(3):
         base = 1000 * rnd  # fresh keys every round: earlier units must make room
@@ -94,10 +96,10 @@ def test_server_grouped_units_evict_and_stay_exact(gpu):
     sv.close()
 
 
-def test_server_bad_graph_fails_alone(gpu):
+def test_server_bad_graph_fails_alone(gpu, orc):
     sv = gpu.Server(prep_threads=2)
     good = make_graphs(gpu, 3, seed=3)
-    want = [alone(gpu, *g) for g in good]
+    want = [alone(orc, *g) for g in good]
     bad = sv.submit(99, np.array([0, 7], np.int32), np.array([1, 2], np.int32), 5)  # id 7 >= n
     ids = {sv.submit(k, *g): k for k, g in enumerate(good)}
     res = {r["request_id"]: r for r in drain(sv, 4)}
