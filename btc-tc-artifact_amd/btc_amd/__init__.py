@@ -135,6 +135,7 @@ _SIGS = {
     "btc_plan_tune": (C.c_int, [vp, C.POINTER(CountOpts)]),
     "btc_plan_count": (C.c_int, [vp, vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
     "btc_plan_count64": (C.c_int, [vp, vp, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_float)]),
+    "btc_plan_count_noalloc": (C.c_int, [vp, vp, C.POINTER(C.c_uint64), vp]),
     "btc_plan_last_timing": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "btc_plan_free": (None, [vp]),
     "btc_plan_capture": (C.c_int, [vp, vp]),
@@ -405,6 +406,14 @@ class Plan:
         fn = lib().btc_plan_count64 if wide else lib().btc_plan_count
         _check(fn(self._h, stream, C.byref(total), _ptr(pv), C.byref(ms)))
         return total.value, pv, ms.value
+
+    def count_noalloc(self, per_vertex=False, stream=None):
+        """(total, per_vertex or None) from btc_plan_count_noalloc: the count
+        in the library's static device workspace, allocating nothing."""
+        total = C.c_uint64(0)
+        pv = np.zeros(self.graph.n, np.int32) if per_vertex else None
+        _check(lib().btc_plan_count_noalloc(self._h, stream, C.byref(total), _ptr(pv)))
+        return total.value, pv
 
     def count_share(self, shares, share, stream=None):
         """(total, kernel_ms) of share `share` of a `shares`-way split."""

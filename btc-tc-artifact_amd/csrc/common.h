@@ -194,6 +194,10 @@ struct EventTimer {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Device address of the current device's static count workspace (looked up
+// on the first call per device; btc_device_init makes that call).
+unsigned long long* noalloc_workspace();
+
 // Per-vertex sums on the device.  A vertex's count is at most C(d, 2) for
 // its row length d, which fits int32 while d <= kPv32MaxRow; graphs with a
 // longer row sum into 64-bit slots.  At most one pointer is set (none:

@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 #include "device_util.h"
@@ -965,13 +966,16 @@ void ensure_side_stream(btc_plan_s* p) {
     BTC_HIP(hipEventCreateWithFlags(&p->ev_join2, hipEventDisableTiming));
 }
 
-void plan_enqueue(btc_plan_s* p, hipStream_t s, PvOut d_pv, EventTimer* tm, EventTimer* tw) {
+void plan_enqueue(btc_plan_s* p, hipStream_t s, PvOut d_pv, EventTimer* tm, EventTimer* tw,
+                  unsigned long long* d_work, bool make_streams) {
     BTC_REQUIRE(!d_pv.p32 || p->g->max_row <= kPv32MaxRow, BTC_ERR_STATE,
                 "longest row %d: per-vertex counts need 64-bit slots", p->g->max_row);
-    launch_zero_u32(reinterpret_cast<uint32_t*>(p->scratch.p), 4, s);
-    unsigned long long* d_total = p->scratch.p;
-    uint32_t* d_counters = reinterpret_cast<uint32_t*>(p->scratch.p + 1);
-    if ((p->n_dense || p->n_tiny_items) && p->nitems && p->info.wedge_checks >= kOverlapChecks) {
+    if (!d_work) d_work = p->scratch.p;
+    launch_zero_u32(reinterpret_cast<uint32_t*>(d_work), 4, s);
+    unsigned long long* d_total = d_work;
+    uint32_t* d_counters = reinterpret_cast<uint32_t*>(d_work + 1);
+    if ((p->n_dense || p->n_tiny_items) && p->nitems && p->info.wedge_checks >= kOverlapChecks &&
+        (make_streams || p->side)) {
         // the latency-bound tiny-pivot and dense-tile kernels run on two
         // more streams while the HBM-bound wedge kernel runs on s; all add
         // into d_total (for small plans the fork and join would cost more
@@ -1269,6 +1273,92 @@ int btc_count(btc_graph g, const btc_count_opts* opts, void* stream, uint64_t* t
     rc = btc_plan_count(p, stream, total, per_vertex, kernel_ms);
     btc_plan_free(p);
     return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- count without allocation
+// btc_plan_count_noalloc keeps the count's total, work-item counters and
+// per-vertex sums in `scratch`, a static device array of the library (each
+// device holds its own copy from module load until the process ends), so a
+// count allocates nothing: for callers that cannot call hipMalloc (code
+// under stream capture, tight serving loops).  Every such call on a device
+// shares its copy; calls that overlap on one device race.
+
+namespace btcx {
+namespace {
+// [0] total, [1] two u32 work-item counters; per-vertex int32 sums from
+// byte kNoAllocPvOffset, at most kNoAllocPvBytes of them (2^22 vertices).
+constexpr size_t kNoAllocPvOffset = 256;
+constexpr size_t kNoAllocPvBytes = size_t(16) << 20;
+constexpr int kNoAllocDevices = 64;
+}  // namespace
+}  // namespace btcx
+
+static __device__ __align__(256) unsigned long long
+    scratch[(btcx::kNoAllocPvOffset + btcx::kNoAllocPvBytes) / sizeof(unsigned long long)];
+
+namespace btcx {
+namespace {
+
+// Device address of `scratch` per device (the array's name is a host-side
+// stand-in, not a device address), looked up once per device.
+struct NoAllocSlot {
+    std::once_flag once;
+    unsigned long long* ptr = nullptr;
+    hipError_t err = hipSuccess;
+};
+NoAllocSlot g_noalloc[kNoAllocDevices];
+
+}  // namespace
+
+unsigned long long* noalloc_workspace() {
+    int dev = 0;
+    BTC_HIP(hipGetDevice(&dev));
+    BTC_REQUIRE(dev >= 0 && dev < kNoAllocDevices, BTC_ERR_ARG, "device %d: the static workspace serves devices 0-%d",
+                dev, kNoAllocDevices - 1);
+    NoAllocSlot& slot = g_noalloc[dev];
+    std::call_once(slot.once, [&slot] {
+        void* p = nullptr;
+        slot.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (slot.err == hipSuccess)
+            slot.ptr = static_cast<unsigned long long*>(p);
+        else
+            (void)hipGetLastError();
+    });
+    BTC_REQUIRE(slot.ptr, BTC_ERR_HIP, "hipGetSymbolAddress of the static workspace on device %d: %s", dev,
+                hipGetErrorString(slot.err));
+    return slot.ptr;
+}
+
+}  // namespace btcx
+
+extern "C" {
+
+int btc_plan_count_noalloc(btc_plan p, void* stream, uint64_t* total, int32_t* per_vertex) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(p && total, BTC_ERR_ARG, "NULL argument");
+        hipStream_t s = btcx::as_stream(stream);
+        unsigned long long* work = btcx::noalloc_workspace();
+        const size_t n = (size_t)p->g->n;
+        btcx::PvOut pv{};
+        if (per_vertex) {
+            BTC_REQUIRE(p->g->max_row <= btcx::kPv32MaxRow, BTC_ERR_OVERFLOW,
+                        "longest row %d > %d: per-vertex sums need 64-bit slots, which this entry point does not "
+                        "keep (btc_plan_count64 returns them)", p->g->max_row, btcx::kPv32MaxRow);
+            BTC_REQUIRE(n * 4 <= btcx::kNoAllocPvBytes, BTC_ERR_ARG,
+                        "per-vertex sums of %zu vertices exceed the static workspace (at most %zu vertices)", n,
+                        btcx::kNoAllocPvBytes / 4);
+            pv.p32 = reinterpret_cast<int32_t*>(reinterpret_cast<unsigned char*>(work) + btcx::kNoAllocPvOffset);
+            btcx::launch_zero_u32(reinterpret_cast<uint32_t*>(pv.p32), n, s);
+        }
+        btcx::plan_enqueue(p, s, pv, nullptr, nullptr, work, false);
+        unsigned long long h = 0;
+        BTC_HIP(hipMemcpyAsync(&h, work, 8, hipMemcpyDeviceToHost, s));
+        if (per_vertex && n) BTC_HIP(hipMemcpyAsync(per_vertex, pv.p32, n * 4, hipMemcpyDeviceToHost, s));
+        BTC_HIP(hipStreamSynchronize(s));
+        *total = h;
+    });
 }
 
 }  // extern "C"

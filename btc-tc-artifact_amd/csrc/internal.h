@@ -217,10 +217,14 @@ struct MultiWorkspace {
 // of plans[j].  Synchronous.  Launch options are those of plans[0].
 void plans_count(btc_plan_s* const* plans, int k, hipStream_t s, uint64_t* totals, MultiWorkspace& ws);
 
-// Enqueues one exact count of plan p on stream s: zeroes the plan's total,
-// runs the dense-tile and wedge kernels; the u64 total lands in
-// p->scratch.p[0] and per-row sums are added into d_pv when it is on.
-// tm times both kernels, tw the wedge kernel (either may be NULL).
-void plan_enqueue(btc_plan_s* p, hipStream_t s, PvOut d_pv, EventTimer* tm, EventTimer* tw);
+// Enqueues one exact count of plan p on stream s: zeroes the total and the
+// work-item counters (16 bytes at d_work, default p->scratch.p), runs the
+// dense-tile, wedge and tiny-pivot kernels; the u64 total lands in
+// d_work[0] and per-row sums are added into d_pv when it is on.  tm times
+// all kernels, tw the wedge kernel (either may be NULL).  With make_streams
+// false the dense-tile and tiny-pivot kernels run beside the wedge kernel
+// only when the plan's side streams exist already (no stream is created).
+void plan_enqueue(btc_plan_s* p, hipStream_t s, PvOut d_pv, EventTimer* tm, EventTimer* tw,
+                  unsigned long long* d_work = nullptr, bool make_streams = true);
 
 }  // namespace btcx

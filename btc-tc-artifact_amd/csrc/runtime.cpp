@@ -126,6 +126,7 @@ int btc_device_init(int32_t device) {
         BTC_REQUIRE(device >= 0 && device < count, BTC_ERR_ARG, "device %d out of range [0,%d)",
                     device, count);
         BTC_HIP(hipSetDevice(device));
+        btcx::noalloc_workspace();  // once per device, with the device current
     });
 }
 

@@ -255,6 +255,19 @@ int btc_plan_count(btc_plan p, void* stream, uint64_t* total, int32_t* per_verte
 /* The same with int64 per-vertex counts (see btc_bitbsr_count64 for when
  * int32 output overflows). */
 int btc_plan_count64(btc_plan p, void* stream, uint64_t* total, int64_t* per_vertex, float* kernel_ms);
+/* btc_plan_count that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under stream
+ * capture, tight serving loops).  The total, the work-item counters and the
+ * per-vertex sums live in a static device array of the library (16 MiB +
+ * 256 B, held on each device it runs on until the process ends), so
+ * per_vertex (host int32[n], or NULL) is limited to graphs of at most 2^22
+ * vertices (BTC_ERR_ARG beyond) whose longest L-row is at most 65536
+ * (BTC_ERR_OVERFLOW beyond; btc_plan_count64 gives those); totals have no
+ * such limit.  Calls that overlap on one device share the array and race.
+ * No kernel_ms (no events are created); the dense-tile and tiny-pivot
+ * kernels run beside the wedge kernel only once an earlier btc_plan_count
+ * of the plan has created its side streams.  Synchronous on `stream`. */
+int btc_plan_count_noalloc(btc_plan p, void* stream, uint64_t* total, int32_t* per_vertex);
 /* Event times of the last btc_plan_count with kernel_ms != NULL: wedge_ms
  * = the wedge kernel, dense_ms = the rest of the count (dense-tile and
  * tiny-pivot kernels, beside or around the wedge kernel; 0 when neither is
