@@ -557,19 +557,22 @@ __device__ __forceinline__ void for_each_match(const uint32_t* __restrict__ indi
     }
 }
 
+// Dense units per wave: lists of at least kDenseRunUnits units (a dense
+// kernel beside the wedge kernel at R-MAT s22) give each wave
+// kDenseRunLong consecutive units, so column sets are reused while I stays
+// and the wedge kernel keeps most of the chip (2.83 ms per count with 16, as
+// with the earlier 4 tiles per wave); shorter lists give each wave half a
+// tile's slabs, at least 2 (planted 2^17, 16 x 128: 1 / 2 / 4 / 8 units per
+// wave took 55 / 51 / 45 / 62 us; 16 x 64: 45 / 40 / 40 / 48 us; 16 x 32 with
+// AND + popcount: 31 / 30 / 32 / 34 us).
+constexpr uint32_t kDenseRunUnits = 65536;
+constexpr uint32_t kDenseRunLong = 16;
+
 // Tile columns of one row block as an LDS bitmap over column blocks with
 // the number of set bits before each word: find(c) is the position of
 // column c in the row block's sorted list (a0 + its rank), or kNoTile.
 // Graphs with at most 32 kColSetWords column blocks use it.
 constexpr uint32_t kColSetWords = 1344;
-// Lists of at least this many dense tiles give each wave 4 consecutive
-// tiles (the column set is reused while I stays; fewer waves leave more of
-// the chip to the wedge kernel running beside): 2.86 vs 3.06 ms per count
-// at R-MAT s22 (24 K tiles).  Shorter lists take one tile per wave, which
-// keeps the chip busy (planted 2^17, 4 K tiles at K = 128: 4 per wave took
-// 0.32 ms, one 0.09 ms).
-constexpr uint32_t kDenseRunTiles = 16384;
-
 struct ColumnSet {
     uint32_t* bits;
     uint16_t* rank;  // < 2^16: a row block has fewer tiles than 32 kColSetWords
@@ -643,164 +646,252 @@ __device__ __forceinline__ void for_each_match_set(const ColumnSet& cs, const ui
     }
 }
 
-// One dense mask tile t of row block I (tile columns [a0, a1)): returns this
-// lane's part of the tile's count and adds per-vertex sums into pv.  The
-// tile columns of I and of each row block J under the tile are matched
-// through the column set cs (use_set) or by for_each_match.
-template <int K, bool PV, bool MFMA>
-__device__ __forceinline__ uint64_t dense_tile(int n, int n_rb, const uint32_t* __restrict__ indptr,
-                                               const uint32_t* __restrict__ indices,
-                                               const uint64_t* __restrict__ tiles, uint32_t t, uint32_t I, uint32_t a0,
-                                               uint32_t a1, uint32_t set_words, ColumnSet& cs, int lane,
-                                               const PvOut& pv) {
+// ------------------------------------------------------------- dense units
+// A dense unit is one slab of a dense mask tile: tile t of row block I
+// (tile columns [a0, a1) of I) and the row block Jo = indices[t] * K/16 +
+// slab under its columns (tile columns [b0, b1) of Jo).  A unit is listed
+// only when Jo holds tile columns and the mask tile has bits in the slab's
+// 16 columns (other slabs add nothing).  Two uint4 per unit: {t, I, a0, a1},
+// {Jo, b0, b1, slab}; units of a tile are consecutive, tiles in list order.
+
+// FILL = false: counts[w] = units of tile list[w]; FILL = true: writes them
+// from units[offs[w]] on.  One thread per listed tile.
+template <int K, bool FILL>
+__global__ __launch_bounds__(256) void dense_units_kernel(const uint32_t* __restrict__ list, uint32_t nt, int n_rb,
+                                                          const uint32_t* __restrict__ indptr,
+                                                          const uint32_t* __restrict__ indices,
+                                                          const uint32_t* __restrict__ row_indices,
+                                                          const uint64_t* __restrict__ tiles,
+                                                          uint32_t* __restrict__ counts,
+                                                          const uint64_t* __restrict__ offs, uint4* __restrict__ units) {
     constexpr int SLABS = K / 16;
-    const uint32_t JL = uniform(indices[t]);
-    // matches of I's and J's tile columns: through the column set (built
-    // for I on first use) when streaming J's list is cheaper, else by search
-    auto match = [&](uint32_t b0, uint32_t b1, auto&& f) {
-        if (set_words && stream_cheaper(a1 - a0, b1 - b0)) {
+    const uint32_t w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= nt) return;
+    const uint32_t t = list[w];
+    // the tile's 16-bit row chunks: row r's bits of slab s are chunk r * SLABS + s
+    const uint16_t* h = reinterpret_cast<const uint16_t*>(tiles + (size_t)t * (K / 4));
+    uint32_t occ = 0;  // bit s: slab s holds a mask bit
+    for (int r = 0; r < 16; ++r)
+        for (int sl = 0; sl < SLABS; ++sl)
+            if (h[r * SLABS + sl]) occ |= 1u << sl;
+    const uint32_t JL = indices[t];
+    uint32_t I = 0, a0 = 0, a1 = 0;
+    uint64_t o = 0;
+    if (FILL) {
+        I = row_indices[t];
+        a0 = indptr[I];
+        a1 = indptr[I + 1];
+        o = offs[w];
+    }
+    uint32_t c = 0;
+    for (int sl = 0; sl < SLABS; ++sl) {
+        const uint64_t Jo = (uint64_t)JL * SLABS + (uint64_t)sl;
+        if (!((occ >> sl) & 1u) || Jo >= (uint64_t)n_rb) continue;
+        const uint32_t b0 = indptr[Jo], b1 = indptr[Jo + 1];
+        if (b1 == b0) continue;
+        if (FILL) {
+            units[2 * o] = make_uint4(t, I, a0, a1);
+            units[2 * o + 1] = make_uint4((uint32_t)Jo, b0, b1, (uint32_t)sl);
+            ++o;
+        }
+        ++c;
+    }
+    if (!FILL) counts[w] = c;
+}
+
+// Matched tile pairs of a unit are collected in a per-wave LDS ring and
+// their tiles staged into LDS kStagePairs pairs at a time.
+constexpr int kStagePairs = 8;
+constexpr uint32_t kMatchRing = 128;  // > kStagePairs - 1 + 64 (one list chunk's matches)
+template <int K>
+constexpr int kTileBytes = 2 * K;  // 16 rows of K bits
+template <int K>
+constexpr int kStageBytes = 2 * kStagePairs * kTileBytes<K>;
+// per wave: stage, then the match ring (uint2 each)
+template <int K>
+constexpr int kWaveDenseBytes = kStageBytes<K> + (int)kMatchRing * 8;
+
+typedef const void __attribute__((address_space(1)))* GlobalVoid;
+typedef void __attribute__((address_space(3)))* LdsVoid;
+
+// The engine's dense tiles as units.  Each wave takes `run` consecutive
+// units (unit first + w * stride for w in its range; the next unit's
+// descriptor is fetched ahead).  Per unit it matches the tile columns of I
+// and Jo in the kernel: through an LDS ColumnSet of I's columns (built when
+// I changes; the first 64 entries of Jo's list are loaded before the build)
+// where streaming Jo's list is cheaper, else by search.  Matched pairs go to
+// the wave's LDS ring (a chunk's hits in parallel, ranked by mbcnt); every
+// kStagePairs of them have their A and B tiles copied into the wave's LDS
+// stage by global_load_lds_dwordx4 (each lane 16 bytes, all loads in flight
+// together, no registers), then each pair's 16 x 16 block A B^T is taken
+// from LDS: i8 MFMA (MFMA) or 64-bit AND + v_bcnt popcount.  The mask
+// tile's slab selects which block entries count.
+template <int K, bool PV, bool MFMA>
+__global__ __launch_bounds__(256) void dense_unit_kernel(int n, const uint32_t* __restrict__ indices,
+                                                         const uint64_t* __restrict__ tiles,
+                                                         const uint4* __restrict__ units, uint32_t first,
+                                                         uint32_t count, uint32_t stride, uint32_t set_words,
+                                                         uint32_t run, unsigned long long* __restrict__ slots,
+                                                         PvOut pv) {
+    constexpr int SLABS = K / 16;
+    constexpr uint32_t TB = kTileBytes<K>;
+    constexpr uint32_t WB = kWaveDenseBytes<K>;
+    // dynamic LDS: 4 waves' stage + ring, then 4 column sets' bits, then their ranks
+    extern __shared__ __align__(16) unsigned char dense_lds[];
+    __shared__ unsigned long long wg_part[4];
+    const int lane = threadIdx.x & 63;
+    const uint32_t wv = uniform(threadIdx.x >> 6);
+    const uint32_t w0 = min(count, (blockIdx.x * 4 + wv) * run);  // waves past the end reach the barrier too
+    const uint32_t w1 = min(count, w0 + run);
+    unsigned char* stage_b = dense_lds + wv * WB;
+    const uint64_t* stage = reinterpret_cast<const uint64_t*>(stage_b);
+    uint2* ring = reinterpret_cast<uint2*>(dense_lds + wv * WB + kStageBytes<K>);
+    uint32_t* sets = reinterpret_cast<uint32_t*>(dense_lds + 4 * WB);
+    ColumnSet cs{sets + wv * set_words, reinterpret_cast<uint16_t*>(sets + 4 * set_words) + wv * set_words, 0};
+    if (set_words)
+        for (uint32_t x = (uint32_t)lane; x < set_words; x += 64) cs.bits[x] = 0;
+    uint64_t tot = 0;
+    // a unit's mask chunk (lane r < 16 holds row r's 16 bits of the slab) and
+    // the first 64 entries of Jo's list; loaded for the next unit while the
+    // current one stages its tiles
+    auto head_loads = [&](const uint4& A, const uint4& B, uint32_t& mrow, uint32_t& jc) {
+        const uint32_t t = uniform(A.x), a0 = uniform(A.z), a1 = uniform(A.w);
+        const uint32_t b0 = uniform(B.y), b1 = uniform(B.z), slab = uniform(B.w);
+        mrow = 0;
+        if (lane < 16) mrow = reinterpret_cast<const uint16_t*>(tiles + (size_t)t * (K / 4))[lane * SLABS + slab];
+        jc = 0;
+        if (set_words && stream_cheaper(a1 - a0, b1 - b0) && b0 + (uint32_t)lane < b1) jc = indices[b0 + lane];
+    };
+    uint4 nA = make_uint4(0, 0, 0, 0), nB = nA;
+    uint32_t n_mrow = 0, n_jc = 0;
+    if (w0 < w1) {
+        const size_t u = (size_t)first + (size_t)w0 * stride;
+        nA = units[2 * u];
+        nB = units[2 * u + 1];
+        head_loads(nA, nB, n_mrow, n_jc);
+    }
+    for (uint32_t w = w0; w < w1; ++w) {
+        const uint32_t I = uniform(nA.y), a0 = uniform(nA.z), a1 = uniform(nA.w);
+        const uint32_t b0 = uniform(nB.y), b1 = uniform(nB.z);
+        const uint32_t mrow = n_mrow, jc = n_jc;
+        const bool use_set = set_words && stream_cheaper(a1 - a0, b1 - b0);
+        const bool more = w + 1 < w1;
+        if (more) {  // the next unit's descriptor, fetched ahead
+            const size_t u = (size_t)first + (size_t)(w + 1) * stride;
+            nA = units[2 * u];
+            nB = units[2 * u + 1];
+        }
+        uint32_t head = 0, tail = 0;  // ring positions (wave-uniform)
+        v4i acc = {0, 0, 0, 0};                 // MFMA: D[4 (l / 16) + v][l % 16]
+        uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // AND: row l / 4, columns 4 (l % 4) + v
+        // stages the tiles of ring pairs [head, head + nb) and adds their blocks
+        auto drain = [&](uint32_t nb) {
+            const uint32_t bytes = 2 * nb * TB;
+            for (uint32_t k = 0; k * 1024u < bytes; ++k) {
+                const uint32_t o = k * 1024u + (uint32_t)lane * 16u;
+                if (o < bytes) {
+                    const uint32_t sl = o / TB;
+                    const uint2 mp = ring[(head + (sl >> 1)) & (kMatchRing - 1)];
+                    const uint32_t p = (sl & 1u) ? mp.y : mp.x;
+                    const char* src = reinterpret_cast<const char*>(tiles) + (size_t)p * TB + (o % TB);
+                    __builtin_amdgcn_global_load_lds((GlobalVoid)src, (LdsVoid)(stage_b + k * 1024u), 16, 0, 0);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);  // the copies into LDS have landed
+            ColumnSet::wave_lds_sync();
+            if constexpr (MFMA) {
+                const int g = lane >> 4, c = lane & 15;
+                for (uint32_t m = 0; m < nb; ++m)
+                    acc = mfma_mac<K>(mfma_load<K>(stage, 2 * m, c), mfma_load<K>(stage, 2 * m + 1, c), g, acc);
+            } else {
+                const int r = lane >> 2, q = lane & 3;
+                for (uint32_t m = 0; m < nb; ++m) {
+                    const RowBits<K> A = RowBits<K>::load(stage, 2 * m, r);
+                    s0 += A.and_popc(RowBits<K>::load(stage, 2 * m + 1, 4 * q + 0));
+                    s1 += A.and_popc(RowBits<K>::load(stage, 2 * m + 1, 4 * q + 1));
+                    s2 += A.and_popc(RowBits<K>::load(stage, 2 * m + 1, 4 * q + 2));
+                    s3 += A.and_popc(RowBits<K>::load(stage, 2 * m + 1, 4 * q + 3));
+                }
+            }
+            ColumnSet::wave_lds_sync();  // the stage is refilled by the next batch
+            head += nb;
+        };
+        if (use_set) {
             if (cs.I != I) {
                 if (cs.I != kNoTile) cs.clear(set_words, lane);
                 cs.a0 = a0;
                 cs.build(indices, a1, set_words, lane);
                 cs.I = I;
             }
-            for_each_match_set(cs, indices, b0, b1, lane, f);
+            for (uint32_t base = b0; base < b1; base += 64) {
+                const uint32_t pb = base + (uint32_t)lane;
+                uint32_t hit = kNoTile;
+                if (pb < b1) hit = cs.find(base == b0 ? jc : indices[pb]);
+                const uint64_t m = __ballot(hit != kNoTile);
+                if (hit != kNoTile) {
+                    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    ring[(tail + rank) & (kMatchRing - 1)] = make_uint2(hit, pb);
+                }
+                tail += (uint32_t)__popcll(m);
+                ColumnSet::wave_lds_sync();
+                while (tail - head >= (uint32_t)kStagePairs) drain((uint32_t)kStagePairs);
+            }
         } else {
-            for_each_match(indices, a0, a1, b0, b1, lane, f);
+            for_each_match(indices, a0, a1, b0, b1, lane, [&](uint32_t pa, uint32_t pb) {
+                if (lane == 0) ring[tail & (kMatchRing - 1)] = make_uint2(pa, pb);
+                ++tail;
+                if (tail - head == (uint32_t)kStagePairs) {
+                    ColumnSet::wave_lds_sync();
+                    drain((uint32_t)kStagePairs);
+                }
+            });
+            ColumnSet::wave_lds_sync();
         }
-    };
-    if constexpr (MFMA) {
-        const int g = lane >> 4, c = lane & 15;
-        RowBits<K> M[4];
+        if (more) head_loads(nA, nB, n_mrow, n_jc);  // in flight with the last stage's copies
+        if (tail > head) drain(tail - head);
+        if constexpr (MFMA) {
+            const int g = lane >> 4, c = lane & 15;
+            uint32_t rv[4];
+            rv[0] = ((__shfl((int)mrow, 4 * g + 0) >> c) & 1) ? (uint32_t)acc.x : 0u;
+            rv[1] = ((__shfl((int)mrow, 4 * g + 1) >> c) & 1) ? (uint32_t)acc.y : 0u;
+            rv[2] = ((__shfl((int)mrow, 4 * g + 2) >> c) & 1) ? (uint32_t)acc.z : 0u;
+            rv[3] = ((__shfl((int)mrow, 4 * g + 3) >> c) & 1) ? (uint32_t)acc.w : 0u;
+            tot += (uint64_t)rv[0] + rv[1] + rv[2] + rv[3];
+            if (PV) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) M[v] = RowBits<K>::load(tiles, t, 4 * g + v);
-        uint32_t rs[4] = {0, 0, 0, 0};
-        for (int it = 0; it < SLABS; ++it) {
-            const uint32_t Jo = JL * SLABS + it;
-            if (Jo >= (uint32_t)n_rb) break;
-            const uint32_t b0 = uniform(indptr[Jo]), b1 = uniform(indptr[Jo + 1]);
-            v4i acc = {0, 0, 0, 0};
-            // matches are taken 4 at a time: their operand loads are issued
-            // together, then the MFMAs (q0 is the newest of nq queued)
-            uint32_t qa0 = 0, qb0 = 0, qa1 = 0, qb1 = 0, qa2 = 0, qb2 = 0, qa3 = 0, qb3 = 0;
-            int nq = 0;
-            auto drain = [&]() {
-                const MfmaOp<K> a0 = mfma_load<K>(tiles, qa0, c), b0p = mfma_load<K>(tiles, qb0, c);
-                MfmaOp<K> a1{}, b1p{}, a2{}, b2p{}, a3{}, b3p{};
-                if (nq > 1) {
-                    a1 = mfma_load<K>(tiles, qa1, c);
-                    b1p = mfma_load<K>(tiles, qb1, c);
+                for (int v = 0; v < 4; ++v) {
+                    uint64_t x = rv[v];  // 16 lanes' entries of one row: 64-bit sum
+                    x += __shfl_xor(x, 1);
+                    x += __shfl_xor(x, 2);
+                    x += __shfl_xor(x, 4);
+                    x += __shfl_xor(x, 8);
+                    const int64_t vtx = (int64_t)I * 16 + 4 * g + v;
+                    if (c == 0 && x && vtx < n) pv.add64(vtx, x);
                 }
-                if (nq > 2) {
-                    a2 = mfma_load<K>(tiles, qa2, c);
-                    b2p = mfma_load<K>(tiles, qb2, c);
-                }
-                if (nq > 3) {
-                    a3 = mfma_load<K>(tiles, qa3, c);
-                    b3p = mfma_load<K>(tiles, qb3, c);
-                }
-                acc = mfma_mac<K>(a0, b0p, g, acc);
-                if (nq > 1) acc = mfma_mac<K>(a1, b1p, g, acc);
-                if (nq > 2) acc = mfma_mac<K>(a2, b2p, g, acc);
-                if (nq > 3) acc = mfma_mac<K>(a3, b3p, g, acc);
-                nq = 0;
-            };
-            auto f = [&](uint32_t pa, uint32_t pb) {
-                qa3 = qa2;
-                qb3 = qb2;
-                qa2 = qa1;
-                qb2 = qb1;
-                qa1 = qa0;
-                qb1 = qb0;
-                qa0 = pa;
-                qb0 = pb;
-                if (++nq == 4) drain();
-            };
-            match(b0, b1, f);
-            if (nq) drain();
-            rs[0] += (M[0].bits(it * 16 + c) & 1u) ? (uint32_t)acc.x : 0u;
-            rs[1] += (M[1].bits(it * 16 + c) & 1u) ? (uint32_t)acc.y : 0u;
-            rs[2] += (M[2].bits(it * 16 + c) & 1u) ? (uint32_t)acc.z : 0u;
-            rs[3] += (M[3].bits(it * 16 + c) & 1u) ? (uint32_t)acc.w : 0u;
-        }
-        if (PV) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                uint32_t x = rs[v];
-                x += __shfl_xor(x, 1);
+            }
+        } else {
+            const int r = lane >> 2, q = lane & 3;
+            const uint32_t m4 = (__shfl((int)mrow, r) >> (4 * q)) & 0xFu;
+            const uint64_t sum = (uint64_t)((m4 & 1u) ? s0 : 0u) + ((m4 & 2u) ? s1 : 0u) + ((m4 & 4u) ? s2 : 0u) +
+                                 ((m4 & 8u) ? s3 : 0u);
+            tot += sum;
+            if (PV) {
+                uint64_t x = sum + __shfl_xor(sum, 1);
                 x += __shfl_xor(x, 2);
-                x += __shfl_xor(x, 4);
-                x += __shfl_xor(x, 8);
-                const int64_t vtx = (int64_t)I * 16 + 4 * g + v;
-                if (c == 0 && x && vtx < n) pv.add(vtx, x);
+                const int64_t v = (int64_t)I * 16 + r;
+                if (q == 0 && x && v < n) pv.add64(v, x);
             }
         }
-        return (uint64_t)(rs[0] + rs[1] + rs[2] + rs[3]);
-    } else {
-        const int r = lane >> 2, q = lane & 3;
-        const RowBits<K> S = RowBits<K>::load(tiles, t, r);
-        uint32_t sum = 0;
-        for (int it = 0; it < SLABS; ++it) {
-            const uint32_t Jo = JL * SLABS + it;
-            if (Jo >= (uint32_t)n_rb) break;
-            const uint32_t b0 = uniform(indptr[Jo]), b1 = uniform(indptr[Jo + 1]);
-            const uint32_t m4 = S.bits(it * 16 + 4 * q) & 0xFu;
-            uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
-            auto f = [&](uint32_t pa, uint32_t pb) {
-                const RowBits<K> A = RowBits<K>::load(tiles, pa, r);
-                acc0 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 0));
-                acc1 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 1));
-                acc2 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 2));
-                acc3 += A.and_popc(RowBits<K>::load(tiles, pb, 4 * q + 3));
-            };
-            match(b0, b1, f);
-            sum += ((m4 & 1u) ? acc0 : 0u) + ((m4 & 2u) ? acc1 : 0u) + ((m4 & 4u) ? acc2 : 0u) +
-                   ((m4 & 8u) ? acc3 : 0u);
-        }
-        if (PV) {
-            uint32_t rsum = sum + __shfl_xor(sum, 1);
-            rsum += __shfl_xor(rsum, 2);
-            const int64_t v = (int64_t)I * 16 + r;
-            if (q == 0 && rsum && v < n) pv.add(v, rsum);
-        }
-        return (uint64_t)sum;
     }
-}
-
-// The engine's dense mask tiles (every listed tile on the dense path), as
-// in tile_count_kernel's dense path.  Each wave takes tpw consecutive list
-// entries (tile order, so mostly one row block I).  With
-// set_words != 0 (column blocks <= 32 kColSetWords) I's tile columns go into
-// an LDS ColumnSet, built when first needed for I, and the tile columns of
-// a row block J stream through it where that is cheaper than searching;
-// otherwise the two lists are matched by for_each_match.
-template <int K, bool PV, bool MFMA>
-__global__ __launch_bounds__(256) void dense_list_kernel(
-    int n, int n_rb, const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
-    const uint32_t* __restrict__ row_indices, const uint64_t* __restrict__ tiles, uint32_t w0, uint32_t w_end,
-    const uint32_t* __restrict__ tile_list, uint32_t lstride, uint32_t set_words, uint32_t tpw,
-    unsigned long long* __restrict__ result, PvOut pv) {
-    // dynamic LDS sized to the graph (dense_set_bytes): 4 waves' bits, then
-    // their ranks
-    extern __shared__ uint32_t dyn_lds[];
-    const int lane = threadIdx.x & 63;
-    const uint32_t wv = uniform(threadIdx.x >> 6);
-    const uint32_t first = w0 + (blockIdx.x * 4 + wv) * tpw;
-    if (first >= w_end) return;
-    const uint32_t last = min(w_end, first + tpw);
-    ColumnSet cs{dyn_lds + wv * set_words, reinterpret_cast<uint16_t*>(dyn_lds + 4 * set_words) + wv * set_words, 0};
-    if (set_words)
-        for (uint32_t x = (uint32_t)lane; x < set_words; x += 64) cs.bits[x] = 0;
-    uint64_t tot = 0;
-    for (uint32_t w = first; w < last; ++w) {
-        const uint32_t t = uniform(tile_list[(size_t)w * lstride]);
-        const uint32_t I = uniform(row_indices[t]);
-        const uint32_t a0 = uniform(indptr[I]), a1 = uniform(indptr[I + 1]);
-        tot += dense_tile<K, PV, MFMA>(n, n_rb, indptr, indices, tiles, t, I, a0, a1, set_words, cs, lane, pv);
-    }
+    // one atomic per workgroup, spread over the slots
     tot = wave_sum_u64(tot);
-    if (lane == 0 && tot) atomicAdd(result, (unsigned long long)tot);
+    if (lane == 0) wg_part[wv] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long x = wg_part[0] + wg_part[1] + wg_part[2] + wg_part[3];
+        if (x) atomicAdd(slots + kSlotStride * (blockIdx.x % kSlots), x);
+    }
 }
 
 template <int K>
@@ -875,44 +966,77 @@ void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_
     }
 }
 
+uint32_t build_dense_units(const btc_bitbsr_s* b, const uint32_t* list, uint32_t nt, DevBuf<uint4>& units,
+                           hipStream_t s) {
+    units.release();
+    if (nt == 0) return 0;
+    TmpBuf<uint32_t> counts;
+    TmpBuf<uint64_t> offs;
+    counts.alloc(nt, s);
+    offs.alloc((size_t)nt + 1, s);
+    const unsigned grid = (nt + 255) / 256;
+    auto launch = [&](auto kern, uint4* out) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, list, nt, b->n_rb, b->indptr.p, b->indices.p,
+                           b->row_indices.p, b->tiles.p, counts.p, offs.p, out);
+        BTC_CHECK_LAUNCH("dense_units_kernel");
+    };
+    switch (b->K) {
+        case 32: launch(dense_units_kernel<32, false>, nullptr); break;
+        case 64: launch(dense_units_kernel<64, false>, nullptr); break;
+        default: launch(dense_units_kernel<128, false>, nullptr); break;
+    }
+    scan_u32_to_u64(counts.p, offs.p, nt, s);
+    uint64_t nu = 0;
+    BTC_HIP(hipMemcpyAsync(&nu, offs.p + nt, 8, hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    BTC_REQUIRE(nu < 0xFFFFFFFFull, BTC_ERR_OVERFLOW, "%llu dense units exceed a u32 index", (unsigned long long)nu);
+    if (nu == 0) return 0;
+    units.alloc(2 * nu);
+    switch (b->K) {
+        case 32: launch(dense_units_kernel<32, true>, units.p); break;
+        case 64: launch(dense_units_kernel<64, true>, units.p); break;
+        default: launch(dense_units_kernel<128, true>, units.p); break;
+    }
+    return (uint32_t)nu;
+}
+
 namespace {
 template <int K, bool MF>
-void launch_dense_k(const btc_bitbsr_s* b, const uint32_t* list, uint32_t lstride, uint32_t nt,
+void launch_units_k(const btc_bitbsr_s* b, const uint4* units, uint32_t first, uint32_t count, uint32_t stride,
                     unsigned long long* res, PvOut pv, hipStream_t s) {
     // column blocks of the graph: ids of tile columns are below this
     const uint64_t ncb = ((uint64_t)b->n + K - 1) / K;
     const uint32_t set_words = ncb <= 32ull * kColSetWords ? (uint32_t)((ncb + 31) / 32) : 0u;
-    // 4 waves x (set_words bit words + set_words 16-bit ranks), whole words
-    const size_t lds = 4 * (size_t)set_words * 4 + (4 * (size_t)set_words * 2 + 3) / 4 * 4;
-    const uint32_t tpw = nt >= kDenseRunTiles ? 4u : 1u;
-    for (uint32_t w0 = 0; w0 < nt; w0 += kMaxTilesPerLaunch) {
-        const uint32_t chunk = std::min(kMaxTilesPerLaunch, nt - w0);
-        const unsigned grid = (chunk + 4 * tpw - 1) / (4 * tpw);
+    // 4 waves x (stage + ring, set bit words, 16-bit ranks)
+    const size_t lds = 4 * (size_t)kWaveDenseBytes<K> + 4 * (size_t)set_words * 4 + 4 * (size_t)set_words * 2;
+    const uint32_t run = count >= kDenseRunUnits ? kDenseRunLong : (uint32_t)std::max(2, K / 32);
+    for (uint32_t w0 = 0; w0 < count; w0 += kMaxTilesPerLaunch) {
+        const uint32_t chunk = std::min(kMaxTilesPerLaunch, count - w0);
+        const unsigned grid = (chunk + 4 * run - 1) / (4 * run);
+        const uint32_t f0 = first + w0 * stride;
         if (pv.on())
-            hipLaunchKernelGGL((dense_list_kernel<K, true, MF>), dim3(grid), dim3(256), lds, s, b->n, b->n_rb,
-                               b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, w0 + chunk, list, lstride,
-                               set_words, tpw, res, pv);
+            hipLaunchKernelGGL((dense_unit_kernel<K, true, MF>), dim3(grid), dim3(256), lds, s, b->n, b->indices.p,
+                               b->tiles.p, units, f0, chunk, stride, set_words, run, res, pv);
         else
-            hipLaunchKernelGGL((dense_list_kernel<K, false, MF>), dim3(grid), dim3(256), lds, s, b->n, b->n_rb,
-                               b->indptr.p, b->indices.p, b->row_indices.p, b->tiles.p, w0, w0 + chunk, list, lstride,
-                               set_words, tpw, res, pv);
-        BTC_CHECK_LAUNCH("dense_list_kernel");
+            hipLaunchKernelGGL((dense_unit_kernel<K, false, MF>), dim3(grid), dim3(256), lds, s, b->n, b->indices.p,
+                               b->tiles.p, units, f0, chunk, stride, set_words, run, res, pv);
+        BTC_CHECK_LAUNCH("dense_unit_kernel");
     }
 }
 }  // namespace
 
-void launch_dense_list(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles, bool mfma,
-                       unsigned long long* d_result, PvOut d_pv, hipStream_t s, uint32_t lstride) {
-    if (ntiles == 0) return;
+void launch_dense_units(const btc_bitbsr_s* b, const uint4* units, uint32_t first, uint32_t count, uint32_t stride,
+                        bool mfma, unsigned long long* d_slots, PvOut d_pv, hipStream_t s) {
+    if (count == 0) return;
     BTC_REQUIRE(lane_sums_fit(b->max_row, b->K), BTC_ERR_OVERFLOW,
                 "longest row %d: a lane's 32-bit sum over %d tile columns could overflow", b->max_row, b->K / 4);
     switch (b->K) {
-        case 32: mfma ? launch_dense_k<32, true>(b, tile_list, lstride, ntiles, d_result, d_pv, s)
-                      : launch_dense_k<32, false>(b, tile_list, lstride, ntiles, d_result, d_pv, s); break;
-        case 64: mfma ? launch_dense_k<64, true>(b, tile_list, lstride, ntiles, d_result, d_pv, s)
-                      : launch_dense_k<64, false>(b, tile_list, lstride, ntiles, d_result, d_pv, s); break;
-        default: mfma ? launch_dense_k<128, true>(b, tile_list, lstride, ntiles, d_result, d_pv, s)
-                      : launch_dense_k<128, false>(b, tile_list, lstride, ntiles, d_result, d_pv, s); break;
+        case 32: mfma ? launch_units_k<32, true>(b, units, first, count, stride, d_slots, d_pv, s)
+                      : launch_units_k<32, false>(b, units, first, count, stride, d_slots, d_pv, s); break;
+        case 64: mfma ? launch_units_k<64, true>(b, units, first, count, stride, d_slots, d_pv, s)
+                      : launch_units_k<64, false>(b, units, first, count, stride, d_slots, d_pv, s); break;
+        default: mfma ? launch_units_k<128, true>(b, units, first, count, stride, d_slots, d_pv, s)
+                      : launch_units_k<128, false>(b, units, first, count, stride, d_slots, d_pv, s); break;
     }
 }
 
@@ -921,6 +1045,23 @@ __global__ void zero_u32_kernel(uint32_t* __restrict__ p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
 }
 }  // namespace
+
+namespace {
+__global__ __launch_bounds__(64) void slot_sum_kernel(unsigned long long* __restrict__ slots,
+                                                      unsigned long long* __restrict__ result) {
+    const int lane = threadIdx.x;
+    unsigned long long v = slots[kSlotStride * lane];
+    slots[kSlotStride * lane] = 0;
+    v = wave_sum_u64(v);
+    if (lane == 0 && v) atomicAdd(result, v);
+}
+}  // namespace
+
+void launch_slot_sum(unsigned long long* d_slots, unsigned long long* d_result, hipStream_t s) {
+    static_assert(kSlots == 64, "one lane per slot");
+    hipLaunchKernelGGL(slot_sum_kernel, dim3(1), dim3(64), 0, s, d_slots, d_result);
+    BTC_CHECK_LAUNCH("slot_sum_kernel");
+}
 
 void launch_zero_u32(uint32_t* p, size_t n, hipStream_t s) {
     if (n == 0) return;

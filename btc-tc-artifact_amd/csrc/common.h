@@ -198,6 +198,19 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // on the first call per device; btc_device_init makes that call).
 unsigned long long* noalloc_workspace();
 
+// Count totals of kernels with many workgroups: each workgroup adds its
+// partial sum into one of kSlots u64 slots kSlotStride words (one 128-byte
+// line) apart instead of into one address, where same-address atomics
+// serialise in L2; launch_slot_sum then folds the slots into the total and
+// clears them.  Every count clears its words (total, counters, slots) first.
+constexpr int kSlots = 64;
+constexpr int kSlotStride = 16;
+constexpr size_t kSlotWords = (size_t)kSlots * kSlotStride;
+// A count's device words: [0] total, [1] two u32 work-item counters, the
+// slots from word kWorkSlots on.
+constexpr size_t kWorkSlots = 16;
+constexpr size_t kWorkWords = kWorkSlots + kSlotWords;
+
 // Per-vertex sums on the device.  A vertex's count is at most C(d, 2) for
 // its row length d, which fits int32 while d <= kPv32MaxRow; graphs with a
 // longer row sum into 64-bit slots.  At most one pointer is set (none:
@@ -211,6 +224,11 @@ struct PvOut {
     __host__ __device__ bool on() const { return p32 != nullptr || p64 != nullptr; }
     // the pointers are kernel arguments or plan fields: the branch is uniform
     __device__ __forceinline__ void add(int64_t v, uint32_t x) const {
+        if (p64) atomicAdd(p64 + v, (unsigned long long)x);
+        else atomicAdd(p32 + v, (int32_t)x);
+    }
+    // the same for a 64-bit partial sum (int32 slots hold only counts that fit)
+    __device__ __forceinline__ void add64(int64_t v, uint64_t x) const {
         if (p64) atomicAdd(p64 + v, (unsigned long long)x);
         else atomicAdd(p32 + v, (int32_t)x);
     }

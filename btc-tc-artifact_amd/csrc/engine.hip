@@ -170,6 +170,7 @@ struct WedgeArgs {
     int lookup;          // 0 = auto, 1 = never bitmap, 2 = never hash
     int schedule;        // 0 = work stealing, 1 = static stride
     uint32_t share = 0, shares = 1;  // single plan: launch item t is plan item share + t * shares of its class
+    unsigned long long* slots = nullptr;  // single plan: workgroup sums go to these kSlots slots, not p0.result
 };
 
 // Barrier of the threads that share one LDS region: the workgroup (NT = 256)
@@ -415,6 +416,21 @@ __device__ __forceinline__ void flush(uint64_t& acc, unsigned long long* result)
     acc = 0;
 }
 
+// The workgroup's partial counts (every thread calls it) added with one
+// atomic into slot blockIdx % kSlots.
+__device__ __forceinline__ void flush_slot(uint64_t acc, unsigned long long* slots) {
+    __shared__ unsigned long long part[kThreads / 64];
+    const uint64_t s = wave_sum_u64(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long x = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w) x += part[w];
+        if (x) atomicAdd(slots + kSlotStride * (blockIdx.x % kSlots), x);
+    }
+}
+
 // Persistent grid.  Phase 1: workgroup items (heavy pivots, sorted by work,
 // 16 KiB LDS each).  Phase 2: wave items (pivots with few lists), each wave
 // with its own 4 KiB slice.  schedule 1 strides statically over the sorted
@@ -501,7 +517,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             team_sync<64>();
         }
     }
-    if (!multi) flush(acc, a.p0.result);
+    if (!multi) {
+        if (a.slots) flush_slot(acc, a.slots);
+        else flush(acc, a.p0.result);
+    }
 }
 
 // Tiny-pivot items [first, first + share + t * shares) of one plan (t < n):
@@ -512,7 +531,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 // for a team of kTinyLanes lanes stays within the group.
 template <bool PV>
 __global__ __launch_bounds__(kThreads) void tiny_kernel(PlanDev P, int lookup, uint32_t first, uint32_t n,
-                                                         uint32_t share, uint32_t shares) {
+                                                         uint32_t share, uint32_t shares,
+                                                         unsigned long long* slots) {
     constexpr uint32_t kGroups = kThreads / kTinyLanes;
     __shared__ __align__(16) uint32_t lds[kGroups * kTinyWords];
     const uint32_t tid = threadIdx.x, grp = tid / kTinyLanes, gl = tid % kTinyLanes;
@@ -525,7 +545,8 @@ __global__ __launch_bounds__(kThreads) void tiny_kernel(PlanDev P, int lookup, u
         const uint4 A = P.items[2 * li], B = P.items[2 * li + 1];
         acc += process_item<kTinyLanes, kTinyWords, kTinyLanes, 1, PV>(P, lookup, A, B, table, gl);
     }
-    flush(acc, P.result);
+    if (slots) flush_slot(acc, slots);
+    else flush(acc, P.result);
 }
 
 using WedgeFn = void (*)(WedgeArgs);
@@ -835,10 +856,13 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
 
 }  // namespace
 
-// Dense-tile form of a count: i8 MFMA unless AND + popcount is asked for
-// (dense_kernel 1); the MFMA form measured equal or faster on every graph
-// timed (R-MAT s22, planted 2^17 at K = 32 / 64 / 128).
-inline bool dense_mfma(const btc_count_opts& o) { return o.dense_kernel != 1; }
+// Dense-tile form of a count at tile width K: dense_kernel 1 = AND +
+// popcount, 2 = i8 MFMA, 0 = i8 MFMA for K = 64 and 128 and AND + popcount
+// for K = 32 (planted 2^17, dense-unit kernel: 36 vs 30 us at K = 32, equal
+// within 4 % at K = 64 and 128).
+inline bool dense_mfma(const btc_count_opts& o, int K) {
+    return o.dense_kernel == 2 || (o.dense_kernel == 0 && K != 32);
+}
 
 void check_launch_opts(const btc_count_opts& o) {
     BTC_REQUIRE(o.lookup >= 0 && o.lookup <= 2, BTC_ERR_ARG, "lookup %d (expected 0, 1 or 2)", o.lookup);
@@ -880,20 +904,20 @@ uint32_t share_count(uint32_t n, uint32_t share, uint32_t shares) {
 
 // Workgroup and wave items of a single plan (tiny items go to launch_tiny).
 void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counters, PvOut d_pv, hipStream_t s,
-                  uint32_t share = 0, uint32_t shares = 1) {
+                  uint32_t share = 0, uint32_t shares = 1, unsigned long long* d_slots = nullptr) {
     const uint32_t nb = share_count(p->n_block_items, share, shares);
     const uint32_t nw = share_count(p->nitems - p->n_tiny_items - p->n_block_items, share, shares);
     if (nb + nw == 0) return;
     unsigned grid = 0;
     WedgeFn fn = wedge_fn(p->launch, d_pv.on(), &grid);
     WedgeArgs a{plan_dev(p, d_total, d_pv), nullptr, nullptr, 1u, nb, nb + nw, d_counters, (int)p->launch.lookup,
-                (int)p->launch.schedule, share, shares};
+                (int)p->launch.schedule, share, shares, d_slots};
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
     BTC_CHECK_LAUNCH("wedge_kernel");
 }
 
 void launch_tiny(btc_plan_s* p, unsigned long long* d_total, PvOut d_pv, hipStream_t s, uint32_t share = 0,
-                 uint32_t shares = 1) {
+                 uint32_t shares = 1, unsigned long long* d_slots = nullptr) {
     const uint32_t n = share_count(p->n_tiny_items, share, shares);
     if (!n) return;
     auto fn = d_pv.on() ? tiny_kernel<true> : tiny_kernel<false>;
@@ -903,8 +927,17 @@ void launch_tiny(btc_plan_s* p, unsigned long long* d_total, PvOut d_pv, hipStre
     const unsigned grid =
         (unsigned)std::min<int64_t>((int64_t)device_cu_count() * std::max(1, per_cu), (n + kGroups - 1) / kGroups);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, plan_dev(p, d_total, d_pv), (int)p->launch.lookup,
-                       p->nitems - p->n_tiny_items, n, share, shares);
+                       p->nitems - p->n_tiny_items, n, share, shares, d_slots);
     BTC_CHECK_LAUNCH("tiny_kernel");
+}
+
+// The plan's dense tiles (share `share` of a `shares`-way split: units
+// share, share + shares, ...) on stream s.
+// Their total lands in the kSlots slots at d_slots (launch_slot_sum folds them).
+void launch_plan_dense(btc_plan_s* p, unsigned long long* d_slots, PvOut d_pv, hipStream_t s, uint32_t share = 0,
+                       uint32_t shares = 1) {
+    launch_dense_units(p->bsr, p->dense_units.p, share, share_count(p->n_units, share, shares), shares,
+                       dense_mfma(p->launch, p->K), d_slots, d_pv, s);
 }
 
 }  // namespace
@@ -923,9 +956,11 @@ void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, M
         ni += P[j]->nitems;
     }
     for (int j = 0; j < k; ++j)
-        if (P[j]->n_dense)
-            launch_dense_list(P[j]->bsr, P[j]->dense_list.p, P[j]->n_dense, dense_mfma(P[j]->launch),
-                              ws.totals.p + j, PvOut{}, s);
+        if (P[j]->n_dense) {
+            launch_zero_u32(reinterpret_cast<uint32_t*>(P[j]->scratch.p + kWorkSlots), 2 * kSlotWords, s);
+            launch_plan_dense(P[j], P[j]->scratch.p + kWorkSlots, PvOut{}, s);
+            launch_slot_sum(P[j]->scratch.p + kWorkSlots, ws.totals.p + j, s);
+        }
     if (ni) {
         unsigned grid = 0;
         WedgeFn fn = wedge_fn(P[0]->launch, false, &grid);
@@ -971,9 +1006,10 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, PvOut d_pv, EventTimer* tm, Even
     BTC_REQUIRE(!d_pv.p32 || p->g->max_row <= kPv32MaxRow, BTC_ERR_STATE,
                 "longest row %d: per-vertex counts need 64-bit slots", p->g->max_row);
     if (!d_work) d_work = p->scratch.p;
-    launch_zero_u32(reinterpret_cast<uint32_t*>(d_work), 4, s);
+    launch_zero_u32(reinterpret_cast<uint32_t*>(d_work), 2 * kWorkWords, s);  // total, counters and slots
     unsigned long long* d_total = d_work;
     uint32_t* d_counters = reinterpret_cast<uint32_t*>(d_work + 1);
+    unsigned long long* d_slots = d_work + kWorkSlots;
     if ((p->n_dense || p->n_tiny_items) && p->nitems && p->info.wedge_checks >= kOverlapChecks &&
         (make_streams || p->side)) {
         // the latency-bound tiny-pivot and dense-tile kernels run on two
@@ -987,38 +1023,32 @@ void plan_enqueue(btc_plan_s* p, hipStream_t s, PvOut d_pv, EventTimer* tm, Even
         if (tm) tm->start(s);
         BTC_HIP(hipEventRecord(p->ev_fork, s));
         if (tw) tw->start(s);
-        launch_wedge(p, d_total, d_counters, d_pv, s);
+        launch_wedge(p, d_total, d_counters, d_pv, s, 0, 1, d_slots);
         if (tw) tw->stop(s);
         if (p->n_tiny_items) {
             BTC_HIP(hipStreamWaitEvent(p->side2, p->ev_fork, 0));
-            launch_tiny(p, d_total, d_pv, p->side2);
+            launch_tiny(p, d_total, d_pv, p->side2, 0, 1, d_slots);
             BTC_HIP(hipEventRecord(p->ev_join2, p->side2));
         }
         if (p->n_dense) {
             BTC_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
-            launch_dense_list(p->bsr, p->dense_list.p, p->n_dense, dense_mfma(p->launch), d_total, d_pv,
-                              p->side);
+            launch_plan_dense(p, d_slots, d_pv, p->side);
             BTC_HIP(hipEventRecord(p->ev_join, p->side));
             BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
         }
         if (p->n_tiny_items) BTC_HIP(hipStreamWaitEvent(s, p->ev_join2, 0));
+        launch_slot_sum(d_slots, d_total, s);
         if (tm) tm->stop(s);
         return;
     }
     if (tm) tm->start(s);
-    if (p->n_dense)
-        launch_dense_list(p->bsr, p->dense_list.p, p->n_dense, dense_mfma(p->launch), d_total, d_pv, s);
+    if (p->n_dense) launch_plan_dense(p, d_slots, d_pv, s);
     if (tw) tw->start(s);
-    if (p->nitems) launch_wedge(p, d_total, d_counters, d_pv, s);
-    if (p->n_tiny_items) {
-        if (tw) tw->stop(s);
-        launch_tiny(p, d_total, d_pv, s);
-        if (tm) tm->stop(s);
-    } else {
-        // no event record inside tm's interval beyond the kernels
-        if (tm) tm->stop(s);
-        if (tw) tw->stop(s);
-    }
+    if (p->nitems) launch_wedge(p, d_total, d_counters, d_pv, s, 0, 1, d_slots);
+    if (tw) tw->stop(s);
+    if (p->n_tiny_items) launch_tiny(p, d_total, d_pv, s, 0, 1, d_slots);
+    launch_slot_sum(d_slots, d_total, s);
+    if (tm) tm->stop(s);
 }
 
 namespace {
@@ -1130,12 +1160,16 @@ btc_plan_s* plan_create(btc_graph g, const btc_count_opts* opts, hipStream_t s, 
         btcx::classify_dense(p.get(), s, dense);
         btcx::build_wedge_lists(p.get(), dense, chunk, s);
     }
-    p->scratch.alloc(2);
+    if (p->n_dense) p->n_units = btcx::build_dense_units(p->bsr, p->dense_list.p, p->n_dense, p->dense_units, s);
+    p->scratch.alloc(btcx::kWorkWords);
+    BTC_HIP(hipMemsetAsync(p->scratch.p, 0, p->scratch.bytes(), s));
+    BTC_HIP(hipStreamSynchronize(s));  // counts may run on other streams (every count clears these words again)
     p->info.tile_k = p->K;
     p->info.tau = p->tau;
     p->info.core_ids = p->core_ids;
     int64_t bytes =
-        (int64_t)(p->lists.bytes() + p->items_dev.bytes() + p->dense_list.bytes() + p->scratch.bytes() +
+        (int64_t)(p->lists.bytes() + p->items_dev.bytes() + p->dense_list.bytes() + p->dense_units.bytes() +
+                  p->scratch.bytes() +
                   p->col16.bytes());
     if (p->own_bsr)  // a caller's Bit-BSR is the caller's memory
         bytes += (int64_t)(p->own_bsr->indptr.bytes() + p->own_bsr->indices.bytes() +
@@ -1182,10 +1216,10 @@ int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream
         BTC_REQUIRE(shares >= 1 && share >= 0 && share < shares, BTC_ERR_ARG, "share %d of %d", share, shares);
         hipStream_t s = btcx::as_stream(stream);
         const uint32_t k = (uint32_t)shares, j = (uint32_t)share;
-        btcx::launch_zero_u32(reinterpret_cast<uint32_t*>(p->scratch.p), 4, s);
+        btcx::launch_zero_u32(reinterpret_cast<uint32_t*>(p->scratch.p), 2 * btcx::kWorkWords, s);
         btcx::EventTimer tm(kernel_ms != nullptr);
         tm.start(s);
-        const uint32_t nd = btcx::share_count(p->n_dense, j, k);
+        const uint32_t nd = btcx::share_count(p->n_units, j, k);
         const uint32_t nt = btcx::share_count(p->n_tiny_items, j, k);
         // the dense-tile and tiny-pivot parts beside the wedge part, as in
         // a full count
@@ -1202,13 +1236,12 @@ int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream
                 ts = p->side2;
             }
         }
+        unsigned long long* slots = p->scratch.p + btcx::kWorkSlots;
         if (p->nitems)
             btcx::launch_wedge(p, p->scratch.p, reinterpret_cast<uint32_t*>(p->scratch.p + 1), btcx::PvOut{}, s, j,
-                               k);
-        btcx::launch_tiny(p, p->scratch.p, btcx::PvOut{}, ts, j, k);
-        if (nd)
-            btcx::launch_dense_list(p->bsr, p->dense_list.p + j, nd, btcx::dense_mfma(p->launch), p->scratch.p,
-                                    btcx::PvOut{}, ds, k);
+                               k, slots);
+        btcx::launch_tiny(p, p->scratch.p, btcx::PvOut{}, ts, j, k, slots);
+        if (nd) btcx::launch_plan_dense(p, slots, btcx::PvOut{}, ds, j, k);
         if (ds != s) {
             BTC_HIP(hipEventRecord(p->ev_join, ds));
             BTC_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
@@ -1217,6 +1250,7 @@ int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream
             BTC_HIP(hipEventRecord(p->ev_join2, ts));
             BTC_HIP(hipStreamWaitEvent(s, p->ev_join2, 0));
         }
+        btcx::launch_slot_sum(slots, p->scratch.p, s);
         tm.stop(s);
         unsigned long long h = 0;
         BTC_HIP(hipMemcpyAsync(&h, p->scratch.p, 8, hipMemcpyDeviceToHost, s));
@@ -1287,9 +1321,10 @@ int btc_count(btc_graph g, const btc_count_opts* opts, void* stream, uint64_t* t
 
 namespace btcx {
 namespace {
-// [0] total, [1] two u32 work-item counters; per-vertex int32 sums from
-// byte kNoAllocPvOffset, at most kNoAllocPvBytes of them (2^22 vertices).
-constexpr size_t kNoAllocPvOffset = 256;
+// The count's words (kWorkWords: total, work-item counters, slots), then
+// per-vertex int32 sums from byte kNoAllocPvOffset, at most kNoAllocPvBytes
+// of them (2^22 vertices).
+constexpr size_t kNoAllocPvOffset = (kWorkWords * 8 + 255) / 256 * 256;
 constexpr size_t kNoAllocPvBytes = size_t(16) << 20;
 constexpr int kNoAllocDevices = 64;
 }  // namespace

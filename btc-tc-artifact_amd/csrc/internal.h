@@ -35,6 +35,8 @@ struct btc_plan_s {
     const btc_bitbsr_s* bsr = nullptr;
     btcx::DevBuf<uint32_t> dense_list;
     uint32_t n_dense = 0;
+    btcx::DevBuf<uint4> dense_units;  // 2 x uint4 per (dense tile, slab) unit (build_dense_units)
+    uint32_t n_units = 0;
     btcx::DevBuf<unsigned char> lists;  // items, list records, row core bitmaps, list rows
     btcx::DevBuf<uint4> items_dev;      // work items of a device-built plan (kept apart from lists)
     const uint4* items_p = nullptr;
@@ -45,7 +47,7 @@ struct btc_plan_s {
     uint32_t nitems = 0;
     uint32_t n_block_items = 0;
     uint32_t n_tiny_items = 0;  // the last items: tiny pivots (tiny_pivot)
-    btcx::DevBuf<unsigned long long> scratch;  // [0] total, [1] work-item counter
+    btcx::DevBuf<unsigned long long> scratch;  // kWorkWords: [0] total, [1] work-item counters, slots
     btc_plan_info info{};
     float last_dense_ms = 0.f;
     float last_wedge_ms = 0.f;
@@ -188,11 +190,20 @@ void launch_tile_count(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_
                        bool binsearch, bool mfma, int tau, unsigned long long* d_result, PvOut d_pv,
                        hipStream_t s, uint32_t lstride = 1);
 
-// Dense-path count over a list of mask tiles (the engine's dense tiles):
-// like launch_tile_count with tau < 0, the two row blocks' tile columns
-// matched by a wave-parallel binary search instead of a two-pointer walk.
-void launch_dense_list(const btc_bitbsr_s* b, const uint32_t* tile_list, uint32_t ntiles, bool mfma,
-                       unsigned long long* d_result, PvOut d_pv, hipStream_t s, uint32_t lstride = 1);
+// Units of the dense tiles list[0, nt) of Bit-BSR b (one per tile slab with
+// mask bits over a row block that has tile columns) into `units`; returns
+// their number.  Synchronous on s.
+uint32_t build_dense_units(const btc_bitbsr_s* b, const uint32_t* list, uint32_t nt, DevBuf<uint4>& units,
+                           hipStream_t s);
+
+// Counts dense units first + w * stride, w in [0, count), of Bit-BSR b: the
+// engine's dense-tile path (i8 MFMA or AND + popcount), adding into the
+// kSlots slots at d_slots and per-vertex sums into d_pv when it is on.
+void launch_dense_units(const btc_bitbsr_s* b, const uint4* units, uint32_t first, uint32_t count, uint32_t stride,
+                        bool mfma, unsigned long long* d_slots, PvOut d_pv, hipStream_t s);
+
+// *d_result += sum of the kSlots slots at d_slots, which are cleared.
+void launch_slot_sum(unsigned long long* d_slots, unsigned long long* d_result, hipStream_t s);
 
 // p[0, n) = 0 by a kernel (safe inside captured graphs).
 void launch_zero_u32(uint32_t* p, size_t n, hipStream_t s);

@@ -185,11 +185,12 @@ int32_t btc_default_tau(int32_t tile_k);
 
 /* ------------------------------------------------------ hybrid count engine
  * The MI355X path for L .* (L L^T).  Edges of L in dense 16xK mask tiles
- * (popcount > tau) are counted by one wave per tile over the Bit-BSR: the
- * tile columns of the two row blocks are matched through an LDS bitmap of
- * one row block's columns (or a wave-parallel binary search), and each
- * matched pair adds A.B^T by v_mfma_i32_16x16x64_i8 on 0/1-unpacked tile
- * rows (default) or by 64-bit AND + popcount.  Every other edge (i, j) is
+ * (popcount > tau) are counted over the Bit-BSR by units of one tile slab
+ * (the 16 columns over one row block Jo): a wave matches the tile columns
+ * of I and Jo through an LDS bitmap of I's columns (or a wave-parallel
+ * binary search), copies the matched tiles into LDS (global_load_lds) and
+ * adds each pair's A.B^T by v_mfma_i32_16x16x64_i8 on 0/1-unpacked tile rows
+ * or by 64-bit AND + popcount (dense_kernel).  Every other edge (i, j) is
  * counted as the wedges (i; j, k), k < j, k in N_L(i): a persistent grid
  * takes work items (a pivot j and a range of rows holding it) in a static
  * stride over work-sorted items; a workgroup or a single wave loads N_L(j)
@@ -205,8 +206,9 @@ typedef struct {
     int32_t group_lanes;  /* lanes streaming one prefix list: 4, 8, 16 or 32; 0 = auto (8) */
     int32_t unroll;       /* 16-byte column loads in flight per lane: 1, 2 or 4; 0 = auto */
     int32_t schedule;     /* 0 = work stealing on one counter, 1 = static stride over sorted items */
-    int32_t dense_kernel; /* dense tiles: 0 = auto (i8 MFMA), 1 = 64-bit AND + v_bcnt popcount,
-                             2 = i8 MFMA (v_mfma_i32_16x16x64_i8) on 0/1-unpacked tile rows */
+    int32_t dense_kernel; /* dense tiles: 0 = auto (i8 MFMA at K = 64 and 128, AND + popcount at K = 32),
+                             1 = 64-bit AND + v_bcnt popcount, 2 = i8 MFMA (v_mfma_i32_16x16x64_i8) on
+                             0/1-unpacked tile rows */
     int32_t planner;      /* work lists built by: 0 = auto (the GPU for graphs canonicalised there or with
                              more than 2^20 edges), 1 = host threads, 2 = GPU kernels */
     int32_t core_ids;     /* hub-core id range C of the rows' core bitmaps: a multiple of 128 up to
@@ -257,9 +259,10 @@ int btc_plan_count(btc_plan p, void* stream, uint64_t* total, int32_t* per_verte
 int btc_plan_count64(btc_plan p, void* stream, uint64_t* total, int64_t* per_vertex, float* kernel_ms);
 /* btc_plan_count that allocates nothing.  No counterpart in the reference:
  * it is for callers that cannot allocate device memory (code under stream
- * capture, tight serving loops).  The total, the work-item counters and the
- * per-vertex sums live in a static device array of the library (16 MiB +
- * 256 B, held on each device it runs on until the process ends), so
+ * capture, tight serving loops).  The total, the work-item counters, the
+ * partial-sum slots and the per-vertex sums live in a static device array of
+ * the library (16 MiB + 8.25 KiB, held on each device it runs on until the
+ * process ends), so
  * per_vertex (host int32[n], or NULL) is limited to graphs of at most 2^22
  * vertices (BTC_ERR_ARG beyond) whose longest L-row is at most 65536
  * (BTC_ERR_OVERFLOW beyond; btc_plan_count64 gives those); totals have no
@@ -274,8 +277,9 @@ int btc_plan_count_noalloc(btc_plan p, void* stream, uint64_t* total, int32_t* p
  * launched; both -1 after a replayed capture). */
 int btc_plan_last_timing(btc_plan p, float* dense_ms, float* wedge_ms);
 /* Counts share `share` of a `shares`-way split of the plan's work: the dense
- * tiles and the work items (of each class, both sorted by work) whose index
- * is share modulo shares — what one GPU of a split would run.  The totals of
+ * units (tile slabs, in tile order) and the work items (of each class, sorted
+ * by work) whose index is share modulo shares — what one GPU of a split
+ * would run.  The totals of
  * all shares add up to the plan's total.  No reference counterpart (the
  * reference counts on one GPU). */
 int btc_plan_count_share(btc_plan p, int32_t shares, int32_t share, void* stream, uint64_t* total,
