@@ -223,6 +223,8 @@ void read_mtx(Csr& m, const std::string& path) {
     long long nr = 0, nc = 0, nz = 0;
     std::istringstream hdr(line);
     if (!(hdr >> nr >> nc >> nz)) throw Error(BTC_ERR_ARG, path + ": bad size line");
+    if (nr < 0 || nc < 0 || nz < 0 || nr > INT32_MAX || nc > INT32_MAX || nz > INT32_MAX)
+        throw Error(BTC_ERR_ARG, path + ": sizes outside [0, 2^31)");
     std::vector<std::pair<int32_t, int32_t>> e;
     e.reserve((size_t)(sym ? 2 * nz : nz));
     for (long long k = 0; k < nz && std::getline(f, line); ++k) {
@@ -279,7 +281,12 @@ void read_binary(Csr& m, const std::string& path, bool with_ncol_values) {
     std::vector<int32_t> rp((size_t)nrow + 1), cl((size_t)nnz);
     f.read(reinterpret_cast<char*>(rp.data()), (std::streamsize)rp.size() * 4);
     f.read(reinterpret_cast<char*>(cl.data()), (std::streamsize)cl.size() * 4);
-    if (!f || rp[(size_t)nrow] != nnz) throw Error(BTC_ERR_ARG, path + ": truncated or rowptr[n] != nnz");
+    if (!f || rp[0] != 0 || rp[(size_t)nrow] != nnz) throw Error(BTC_ERR_ARG, path + ": truncated or rowptr[n] != nnz");
+    for (int32_t i = 0; i < nrow; ++i)
+        if (rp[(size_t)i + 1] < rp[(size_t)i]) throw Error(BTC_ERR_ARG, path + ": rowptr decreases");
+    const int32_t nc = with_ncol_values ? ncol : nrow;
+    for (int32_t c : cl)
+        if (c < 0 || c >= nc) throw Error(BTC_ERR_ARG, path + ": column id outside [0, ncol)");
     std::copy(rp.begin(), rp.end(), m.row_pointers.begin());
     std::copy(cl.begin(), cl.end(), m.column_indices.begin());
     std::fill(m.values.begin(), m.values.end(), 1);

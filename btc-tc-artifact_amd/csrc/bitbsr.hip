@@ -457,7 +457,7 @@ __global__ __launch_bounds__(256) void tile_count_kernel(
                 x += __shfl_xor(x, 4);
                 x += __shfl_xor(x, 8);
                 const int64_t vtx = (int64_t)I * 16 + 4 * g + v;
-                if (c == 0 && x && vtx < n) pv.add(vtx, x);
+                if (c == 0 && x && vtx < n) pv_add(pv, vtx, x);
             }
         }
         const uint64_t tot = wave_sum_u64((uint64_t)(rs[0] + rs[1] + rs[2] + rs[3]));
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(256) void tile_count_kernel(
         uint32_t rs = sum + __shfl_xor(sum, 1);
         rs += __shfl_xor(rs, 2);
         const int64_t v = (int64_t)I * 16 + r;
-        if (q == 0 && rs && v < n) pv.add(v, rs);
+        if (q == 0 && rs && v < n) pv_add(pv, v, rs);
     }
     const uint64_t tot = wave_sum_u64((uint64_t)sum);
     if (lane == 0 && tot) atomicAdd(result, (unsigned long long)tot);
@@ -867,7 +867,7 @@ __global__ __launch_bounds__(256) void dense_unit_kernel(int n, const uint32_t* 
                     x += __shfl_xor(x, 4);
                     x += __shfl_xor(x, 8);
                     const int64_t vtx = (int64_t)I * 16 + 4 * g + v;
-                    if (c == 0 && x && vtx < n) pv.add64(vtx, x);
+                    if (c == 0 && x && vtx < n) pv_add(pv, vtx, x);
                 }
             }
         } else {
@@ -880,7 +880,7 @@ __global__ __launch_bounds__(256) void dense_unit_kernel(int n, const uint32_t* 
                 uint64_t x = sum + __shfl_xor(sum, 1);
                 x += __shfl_xor(x, 2);
                 const int64_t v = (int64_t)I * 16 + r;
-                if (q == 0 && x && v < n) pv.add64(v, x);
+                if (q == 0 && x && v < n) pv_add(pv, v, x);
             }
         }
     }

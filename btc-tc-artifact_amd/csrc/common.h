@@ -214,24 +214,14 @@ constexpr size_t kWorkWords = kWorkSlots + kSlotWords;
 // Per-vertex sums on the device.  A vertex's count is at most C(d, 2) for
 // its row length d, which fits int32 while d <= kPv32MaxRow; graphs with a
 // longer row sum into 64-bit slots.  At most one pointer is set (none:
-// totals only).  Every kernel adds a lane's or a group's partial sum, a u32,
-// with one atomic.
+// totals only).  Every kernel adds a lane's or a group's partial sum with
+// one atomic (pv_add).
 constexpr int32_t kPv32MaxRow = 65536;
 
 struct PvOut {
     int32_t* p32 = nullptr;
     unsigned long long* p64 = nullptr;
     __host__ __device__ bool on() const { return p32 != nullptr || p64 != nullptr; }
-    // the pointers are kernel arguments or plan fields: the branch is uniform
-    __device__ __forceinline__ void add(int64_t v, uint32_t x) const {
-        if (p64) atomicAdd(p64 + v, (unsigned long long)x);
-        else atomicAdd(p32 + v, (int32_t)x);
-    }
-    // the same for a 64-bit partial sum (int32 slots hold only counts that fit)
-    __device__ __forceinline__ void add64(int64_t v, uint64_t x) const {
-        if (p64) atomicAdd(p64 + v, (unsigned long long)x);
-        else atomicAdd(p32 + v, (int32_t)x);
-    }
 };
 
 // A lane of the tile kernels sums its row's intersections with K / 4 tile

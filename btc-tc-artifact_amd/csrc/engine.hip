@@ -278,7 +278,7 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
             uint32_t c = hits;
 #pragma unroll
             for (int o = G / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, G);
-            if (gl == 0 && c) a.pv.add(a.ur[e], c);
+            if (gl == 0 && c) pv_add(a.pv, a.ur[e], c);
         }
         L = nL;
     }

@@ -71,6 +71,14 @@ struct btc_plan_s {
 
 namespace btcx {
 
+// Adds a partial per-vertex sum x to vertex v of pv (which is on).  The
+// pointers are kernel arguments or plan fields, so the branch is uniform;
+// int32 slots are used only where every count fits them.
+__device__ __forceinline__ void pv_add(const PvOut& pv, int64_t v, uint64_t x) {
+    if (pv.p64) atomicAdd(pv.p64 + v, (unsigned long long)x);
+    else atomicAdd(pv.p32 + v, (int32_t)x);
+}
+
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;
 
 // Wedge-kernel plan layout shared by the host and device planners.
