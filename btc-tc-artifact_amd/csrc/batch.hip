@@ -28,6 +28,10 @@ struct btc_batch_s {
     btcx::DevBuf<unsigned long long> d_pg;
     std::vector<uint64_t> h_pg;
     btcx::CapturedGraph count_graph;  // zero + count + per-graph sums, replayed when captured
+    // counts may put the dense-tile and tiny-pivot kernels on side streams
+    // of the plan (created on first use); a server's units count on the
+    // worker's stream alone
+    bool side_streams = true;
     ~btc_batch_s() {
         if (plan) btc_plan_free(plan);
     }
@@ -80,10 +84,12 @@ btc_batch_s* batch_from_graph(btc_graph_s&& g, const std::vector<int64_t>& voff,
     return b.release();
 }
 
+void batch_set_side_streams(btc_batch_s* b, bool on) { b->side_streams = on; }
+
 }  // namespace btcx
 
 void btc_batch_s::enqueue_count(hipStream_t s) {
-    btcx::plan_enqueue(plan, s, d_pv.out(), nullptr, nullptr);
+    btcx::plan_enqueue(plan, s, d_pv.out(), nullptr, nullptr, nullptr, side_streams);
     hipLaunchKernelGGL(btcx::graph_sums_kernel, dim3((unsigned)count), dim3(256), 0, s, d_pv.out(), d_voff.p, d_pg.p);
     BTC_CHECK_LAUNCH("graph_sums_kernel");
 }

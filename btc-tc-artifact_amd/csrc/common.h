@@ -103,14 +103,46 @@ struct DevBuf {
 // device's default pool (once per device), so temporaries are reused.
 void mem_pool_init();
 
-// Stream-ordered temporary device buffer (hipMallocAsync / hipFreeAsync on
-// one stream): freeing it does not wait for the whole device, so threads
-// that build plans side by side do not serialise on each other's kernels.
+// A device arena for the temporary buffers (TmpBuf) of one thread's work on
+// one stream: buffers are carved from one device block by a bump pointer
+// (released in LIFO order, or all at once when the scope ends) and reused
+// in stream order, so a thread that prepares many graphs (a server's
+// preparation thread) makes no allocation call per buffer.  ArenaScope
+// installs it for the calling thread and stream; buffers that do not fit
+// come from hipMallocAsync, and the block grows to the scope's demand (up
+// to kArenaMax) when the scope ends.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, top = 0, demand = 0;
+    hipStream_t s = nullptr;
+    Arena() = default;
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+    ~Arena();
+};
+constexpr size_t kArenaMax = size_t(1) << 30;
+
+struct ArenaScope {
+    Arena& a;
+    Arena* prev;
+    ArenaScope(Arena& arena, hipStream_t s);
+    ~ArenaScope();
+};
+
+// The arena installed on this thread, or NULL.
+Arena* current_arena();
+
+// Stream-ordered temporary device buffer: from the thread's arena when one
+// is installed for this stream, else hipMallocAsync / hipFreeAsync on the
+// stream (freeing does not wait for the whole device, so threads that
+// build plans side by side do not serialise on each other's kernels).
 template <class T>
 struct TmpBuf {
     T* p = nullptr;
     size_t n = 0;
     hipStream_t s = nullptr;
+    size_t off = 0, bytes = 0;
+    bool from_arena = false;
     TmpBuf() = default;
     TmpBuf(const TmpBuf&) = delete;
     TmpBuf& operator=(const TmpBuf&) = delete;
@@ -119,6 +151,19 @@ struct TmpBuf {
         release();
         if (count == 0) count = 1;
         s = stream;
+        bytes = (count * sizeof(T) + 255) & ~size_t(255);
+        Arena* a = current_arena();
+        if (a && a->s == stream) {
+            if (a->demand < a->top + bytes) a->demand = a->top + bytes;
+            if (a->top + bytes <= a->cap) {
+                off = a->top;
+                p = reinterpret_cast<T*>(a->base + off);
+                a->top += bytes;
+                from_arena = true;
+                n = count;
+                return;
+            }
+        }
         mem_pool_init();
         hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&p), count * sizeof(T), s);
         if (e != hipSuccess) {
@@ -129,10 +174,31 @@ struct TmpBuf {
         n = count;
     }
     void release() {
-        if (p) (void)hipFreeAsync(p, s);
+        if (p) {
+            if (!from_arena) {
+                (void)hipFreeAsync(p, s);
+            } else {
+                Arena* a = current_arena();
+                if (a && off + bytes == a->top) a->top = off;  // the newest buffer: pop it
+            }
+        }
         p = nullptr;
         n = 0;
+        from_arena = false;
     }
+};
+
+// Pinned host staging buffer, grown on demand (hipHostMalloc).
+struct PinnedBuf {
+    char* p = nullptr;
+    size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    char* reserve(size_t bytes);
 };
 
 // A HIP graph captured from a sequence of kernel launches, replayed with

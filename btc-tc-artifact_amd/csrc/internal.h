@@ -149,6 +149,9 @@ __host__ __device__ inline bool tiny_pivot(uint32_t lists, uint32_t lo, uint32_t
 // occupy the vertex ranges [voff[k], voff[k + 1]) (takes g over).
 btc_batch_s* batch_from_graph(btc_graph_s&& g, const std::vector<int64_t>& voff, const btc_count_opts* opts,
                               hipStream_t s);
+// Whether counts of batch b may fork their dense-tile and tiny-pivot
+// kernels onto side streams of its plan (created on first use; default on).
+void batch_set_side_streams(btc_batch_s* b, bool on);
 
 // Graphs with more edges than this are planned on the GPU unless the
 // options ask for the host planner.

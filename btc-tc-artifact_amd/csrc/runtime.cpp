@@ -74,6 +74,49 @@ void mem_pool_init() {
     });
 }
 
+static thread_local Arena* tl_arena = nullptr;
+
+Arena* current_arena() { return tl_arena; }
+
+Arena::~Arena() {
+    if (base) (void)hipFreeAsync(base, s);
+}
+
+ArenaScope::ArenaScope(Arena& arena, hipStream_t s) : a(arena), prev(tl_arena) {
+    a.s = s;
+    a.top = 0;
+    a.demand = 0;
+    tl_arena = &a;
+}
+
+ArenaScope::~ArenaScope() {
+    tl_arena = prev;
+    a.top = 0;
+    if (a.demand > a.cap && a.cap < kArenaMax) {
+        // grow to the scope's demand (a power of two, at least 64 MiB)
+        size_t c = size_t(64) << 20;
+        while (c < a.demand && c < kArenaMax) c <<= 1;
+        if (a.base) (void)hipFreeAsync(a.base, a.s);
+        a.base = nullptr;
+        a.cap = 0;
+        mem_pool_init();
+        if (hipMallocAsync(reinterpret_cast<void**>(&a.base), c, a.s) == hipSuccess) a.cap = c;
+        else (void)hipGetLastError();  // the arena stays empty: buffers come from the pool
+    }
+}
+
+char* PinnedBuf::reserve(size_t bytes) {
+    if (bytes <= cap) return p;
+    size_t c = size_t(1) << 20;
+    while (c < bytes) c <<= 1;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    BTC_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), c, hipHostMallocDefault));
+    cap = c;
+    return p;
+}
+
 void PvBuf::copy_out(int32_t* h32, int64_t* h64, hipStream_t s) const {
     if (n == 0 || (!h32 && !h64)) {
         BTC_HIP(hipStreamSynchronize(s));

@@ -378,11 +378,11 @@ typedef struct btc_server_s* btc_server;
 typedef struct {
     int64_t hbm_budget_bytes; /* resident graphs + plans; default 64 GiB */
     int32_t max_batch;        /* requests per launch; default 512 */
-    int32_t prep_threads;     /* host threads that canonicalise and plan; default 3 */
+    int32_t prep_threads;     /* host threads that canonicalise and plan; default 4 */
     int32_t canon_flags;      /* default BTC_CANON_DEFAULT | BTC_CANON_DEVICE (canonicalise and plan on the
                                  GPU, on each preparation thread's stream) */
     int32_t group_graphs;     /* with GPU preparation, a prep thread takes up to this many queued new
-                                 graphs and prepares them as one unit; default 8, 1 = one at a time */
+                                 graphs and prepares them as one unit; default 16, 1 = one at a time */
 } btc_server_opts;
 
 typedef struct {
