@@ -96,9 +96,9 @@ def cpu_baseline(g, gpu_total):
 
 def load_traffic(workload):
     """HBM bytes per launch of the wedge kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter
-    reads for a copy of known size); None when no summary exists for this
-    workload."""
+    counter summary (L2-miss read requests by size, 128 / 64 / 32 bytes, plus
+    WRITE_SIZE; checked on a device copy of known size); None when no summary
+    exists for this workload."""
     path = os.path.join(ROOT, "profiles", "wedge_kernel_%s_traffic.md" % workload)
     if not os.path.exists(path):
         return None
@@ -110,8 +110,8 @@ def load_traffic(workload):
 
 def copy_calibration(mb):
     """One device-to-device copy of `mb` MiB (HIP's blit copy kernel), so a
-    rocprofv3 counter run can scale FETCH_SIZE / WRITE_SIZE by the bytes a
-    plain copy of known size reports."""
+    rocprofv3 counter run can check its byte formulas on a kernel of known
+    traffic."""
     import ctypes as C
 
     hip = C.CDLL("libamdhip64.so")
