@@ -22,6 +22,7 @@
 // 4q..4q+3 of each 16-column J_out slab, so every lane's partial sum belongs
 // to a single vertex 16I + r.
 #include "device_util.h"
+#include "prim.h"
 #include "internal.h"
 
 namespace btcx {
@@ -560,11 +561,11 @@ __device__ __forceinline__ void for_each_match(const uint32_t* __restrict__ indi
 // Dense units per wave: lists of at least kDenseRunUnits units (a dense
 // kernel beside the wedge kernel at R-MAT s22) give each wave
 // kDenseRunLong consecutive units, so column sets are reused while I stays
-// and the wedge kernel keeps most of the chip (2.83 ms per count with 16, as
-// with the earlier 4 tiles per wave); shorter lists give each wave half a
-// tile's slabs, at least 2 (planted 2^17, 16 x 128: 1 / 2 / 4 / 8 units per
-// wave took 55 / 51 / 45 / 62 us; 16 x 64: 45 / 40 / 40 / 48 us; 16 x 32 with
-// AND + popcount: 31 / 30 / 32 / 34 us).
+// and the wedge kernel keeps most of the chip (2.81 ms per count with 16;
+// 2.86 with 4 or 8); shorter lists give each wave half a tile's slabs, at
+// least 2 (planted 2^17 with AND + popcount, tiles in descending cost:
+// 1 / 2 / 4 units per wave took 50 / 46 / 40 us at 16 x 128, 39 / 32 / 36 us
+// at 16 x 64, 26 / 23 / 29 us at 16 x 32).
 constexpr uint32_t kDenseRunUnits = 65536;
 constexpr uint32_t kDenseRunLong = 16;
 
@@ -654,19 +655,39 @@ __device__ __forceinline__ void for_each_match_set(const ColumnSet& cs, const ui
 // 16 columns (other slabs add nothing).  Two uint4 per unit: {t, I, a0, a1},
 // {Jo, b0, b1, slab}; units of a tile are consecutive, tiles in list order.
 
-// FILL = false: counts[w] = units of tile list[w]; FILL = true: writes them
-// from units[offs[w]] on.  One thread per listed tile.
+// Number of tile columns present in both sorted lists indices[a0, a1) and
+// indices[b0, b1) (one thread, two pointers; plan time only).
+__device__ uint32_t count_common(const uint32_t* __restrict__ indices, uint32_t a0, uint32_t a1, uint32_t b0,
+                                 uint32_t b1) {
+    uint32_t c = 0;
+    while (a0 < a1 && b0 < b1) {
+        const uint32_t x = indices[a0], y = indices[b0];
+        c += x == y;
+        a0 += x <= y;
+        b0 += y <= x;
+    }
+    return c;
+}
+
+// FILL = false: for tile list[w], counts[w] = its units and key[w] = ~cost,
+// cost = the sum over its units of 1 + ceil(|Jo| / 64) + ceil(matches /
+// kStagePairs) (the unit's dependent list-chunk and tile-stage steps), val[w]
+// = w.  FILL = true: thread k writes the units of tile list[order[k]] from
+// units[offs[k]] on.  One thread per listed tile.
+constexpr uint32_t kUnitStagePairs = 8;  // = kStagePairs of dense_unit_kernel
 template <int K, bool FILL>
 __global__ __launch_bounds__(256) void dense_units_kernel(const uint32_t* __restrict__ list, uint32_t nt, int n_rb,
                                                           const uint32_t* __restrict__ indptr,
                                                           const uint32_t* __restrict__ indices,
                                                           const uint32_t* __restrict__ row_indices,
                                                           const uint64_t* __restrict__ tiles,
-                                                          uint32_t* __restrict__ counts,
+                                                          uint32_t* __restrict__ counts, uint32_t* __restrict__ key,
+                                                          uint32_t* __restrict__ val, const uint32_t* __restrict__ order,
                                                           const uint64_t* __restrict__ offs, uint4* __restrict__ units) {
     constexpr int SLABS = K / 16;
-    const uint32_t w = blockIdx.x * 256 + threadIdx.x;
-    if (w >= nt) return;
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nt) return;
+    const uint32_t w = FILL ? order[k] : k;
     const uint32_t t = list[w];
     // the tile's 16-bit row chunks: row r's bits of slab s are chunk r * SLABS + s
     const uint16_t* h = reinterpret_cast<const uint16_t*>(tiles + (size_t)t * (K / 4));
@@ -675,15 +696,10 @@ __global__ __launch_bounds__(256) void dense_units_kernel(const uint32_t* __rest
         for (int sl = 0; sl < SLABS; ++sl)
             if (h[r * SLABS + sl]) occ |= 1u << sl;
     const uint32_t JL = indices[t];
-    uint32_t I = 0, a0 = 0, a1 = 0;
-    uint64_t o = 0;
-    if (FILL) {
-        I = row_indices[t];
-        a0 = indptr[I];
-        a1 = indptr[I + 1];
-        o = offs[w];
-    }
+    const uint32_t I = row_indices[t], a0 = indptr[I], a1 = indptr[I + 1];
+    uint64_t o = FILL ? offs[k] : 0;
     uint32_t c = 0;
+    uint64_t cost = 0;
     for (int sl = 0; sl < SLABS; ++sl) {
         const uint64_t Jo = (uint64_t)JL * SLABS + (uint64_t)sl;
         if (!((occ >> sl) & 1u) || Jo >= (uint64_t)n_rb) continue;
@@ -693,15 +709,29 @@ __global__ __launch_bounds__(256) void dense_units_kernel(const uint32_t* __rest
             units[2 * o] = make_uint4(t, I, a0, a1);
             units[2 * o + 1] = make_uint4((uint32_t)Jo, b0, b1, (uint32_t)sl);
             ++o;
+        } else {
+            const uint32_t m = count_common(indices, a0, a1, b0, b1);
+            cost += 1 + (b1 - b0 + 63) / 64 + (m + kUnitStagePairs - 1) / kUnitStagePairs;
         }
         ++c;
     }
-    if (!FILL) counts[w] = c;
+    if (!FILL) {
+        counts[w] = c;
+        key[w] = 0xFFFFFFFFu - (uint32_t)min<uint64_t>(cost, 0xFFFFFFFFull);
+        val[w] = w;
+    }
+}
+
+__global__ __launch_bounds__(256) void gather_u32_kernel(const uint32_t* __restrict__ in,
+                                                         const uint32_t* __restrict__ order, uint32_t n,
+                                                         uint32_t* __restrict__ out) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) out[k] = in[order[k]];
 }
 
 // Matched tile pairs of a unit are collected in a per-wave LDS ring and
 // their tiles staged into LDS kStagePairs pairs at a time.
-constexpr int kStagePairs = 8;
+constexpr int kStagePairs = (int)kUnitStagePairs;
 constexpr uint32_t kMatchRing = 128;  // > kStagePairs - 1 + 64 (one list chunk's matches)
 template <int K>
 constexpr int kTileBytes = 2 * K;  // 16 rows of K bits
@@ -970,22 +1000,33 @@ uint32_t build_dense_units(const btc_bitbsr_s* b, const uint32_t* list, uint32_t
                            hipStream_t s) {
     units.release();
     if (nt == 0) return 0;
-    TmpBuf<uint32_t> counts;
+    TmpBuf<uint32_t> counts, key, val, kt, vt, sorted_counts;
     TmpBuf<uint64_t> offs;
     counts.alloc(nt, s);
+    key.alloc(nt, s);
+    val.alloc(nt, s);
+    kt.alloc(nt, s);
+    vt.alloc(nt, s);
+    sorted_counts.alloc(nt, s);
     offs.alloc((size_t)nt + 1, s);
     const unsigned grid = (nt + 255) / 256;
-    auto launch = [&](auto kern, uint4* out) {
+    auto launch = [&](auto kern, const uint32_t* order, uint4* out) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, list, nt, b->n_rb, b->indptr.p, b->indices.p,
-                           b->row_indices.p, b->tiles.p, counts.p, offs.p, out);
+                           b->row_indices.p, b->tiles.p, counts.p, key.p, val.p, order, offs.p, out);
         BTC_CHECK_LAUNCH("dense_units_kernel");
     };
     switch (b->K) {
-        case 32: launch(dense_units_kernel<32, false>, nullptr); break;
-        case 64: launch(dense_units_kernel<64, false>, nullptr); break;
-        default: launch(dense_units_kernel<128, false>, nullptr); break;
+        case 32: launch(dense_units_kernel<32, false>, nullptr, nullptr); break;
+        case 64: launch(dense_units_kernel<64, false>, nullptr, nullptr); break;
+        default: launch(dense_units_kernel<128, false>, nullptr, nullptr); break;
     }
-    scan_u32_to_u64(counts.p, offs.p, nt, s);
+    // tiles in descending cost (stable: list order among equals), so the
+    // longest units start first and a tile's units stay consecutive
+    const bool alt = sort_pairs_u32(key.p, val.p, kt.p, vt.p, nt, 32, s);
+    const uint32_t* order = alt ? vt.p : val.p;
+    hipLaunchKernelGGL(gather_u32_kernel, dim3(grid), dim3(256), 0, s, counts.p, order, nt, sorted_counts.p);
+    BTC_CHECK_LAUNCH("gather_u32_kernel");
+    scan_u32_to_u64(sorted_counts.p, offs.p, nt, s);
     uint64_t nu = 0;
     BTC_HIP(hipMemcpyAsync(&nu, offs.p + nt, 8, hipMemcpyDeviceToHost, s));
     BTC_HIP(hipStreamSynchronize(s));
@@ -993,9 +1034,9 @@ uint32_t build_dense_units(const btc_bitbsr_s* b, const uint32_t* list, uint32_t
     if (nu == 0) return 0;
     units.alloc(2 * nu);
     switch (b->K) {
-        case 32: launch(dense_units_kernel<32, true>, units.p); break;
-        case 64: launch(dense_units_kernel<64, true>, units.p); break;
-        default: launch(dense_units_kernel<128, true>, units.p); break;
+        case 32: launch(dense_units_kernel<32, true>, order, units.p); break;
+        case 64: launch(dense_units_kernel<64, true>, order, units.p); break;
+        default: launch(dense_units_kernel<128, true>, order, units.p); break;
     }
     return (uint32_t)nu;
 }

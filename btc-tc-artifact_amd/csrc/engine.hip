@@ -856,13 +856,11 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
 
 }  // namespace
 
-// Dense-tile form of a count at tile width K: dense_kernel 1 = AND +
-// popcount, 2 = i8 MFMA, 0 = i8 MFMA for K = 64 and 128 and AND + popcount
-// for K = 32 (planted 2^17, dense-unit kernel: 36 vs 30 us at K = 32, equal
-// within 4 % at K = 64 and 128).
-inline bool dense_mfma(const btc_count_opts& o, int K) {
-    return o.dense_kernel == 2 || (o.dense_kernel == 0 && K != 32);
-}
+// Dense-tile form of a count: dense_kernel 2 = i8 MFMA, else (0 = auto,
+// 1) 64-bit AND + popcount, which the dense-unit kernel runs faster at every
+// tile width (planted 2^17, rocprofv3: 23 / 32 / 40 us against 27 / 35 / 48 us
+// with the MFMA form at K = 32 / 64 / 128).
+inline bool dense_mfma(const btc_count_opts& o) { return o.dense_kernel == 2; }
 
 void check_launch_opts(const btc_count_opts& o) {
     BTC_REQUIRE(o.lookup >= 0 && o.lookup <= 2, BTC_ERR_ARG, "lookup %d (expected 0, 1 or 2)", o.lookup);
@@ -937,7 +935,7 @@ void launch_tiny(btc_plan_s* p, unsigned long long* d_total, PvOut d_pv, hipStre
 void launch_plan_dense(btc_plan_s* p, unsigned long long* d_slots, PvOut d_pv, hipStream_t s, uint32_t share = 0,
                        uint32_t shares = 1) {
     launch_dense_units(p->bsr, p->dense_units.p, share, share_count(p->n_units, share, shares), shares,
-                       dense_mfma(p->launch, p->K), d_slots, d_pv, s);
+                       dense_mfma(p->launch), d_slots, d_pv, s);
 }
 
 }  // namespace

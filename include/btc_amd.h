@@ -206,7 +206,7 @@ typedef struct {
     int32_t group_lanes;  /* lanes streaming one prefix list: 4, 8, 16 or 32; 0 = auto (8) */
     int32_t unroll;       /* 16-byte column loads in flight per lane: 1, 2 or 4; 0 = auto */
     int32_t schedule;     /* 0 = work stealing on one counter, 1 = static stride over sorted items */
-    int32_t dense_kernel; /* dense tiles: 0 = auto (i8 MFMA at K = 64 and 128, AND + popcount at K = 32),
+    int32_t dense_kernel; /* dense tiles: 0 = auto (AND + popcount, the faster form at every K measured),
                              1 = 64-bit AND + v_bcnt popcount, 2 = i8 MFMA (v_mfma_i32_16x16x64_i8) on
                              0/1-unpacked tile rows */
     int32_t planner;      /* work lists built by: 0 = auto (the GPU for graphs canonicalised there or with
