@@ -65,7 +65,8 @@ class CountOpts(C.Structure):
     _fields_ = [("tile_k", C.c_int32), ("tau", C.c_int32), ("chunk_work", C.c_int32),
                 ("waves_per_eu", C.c_int32), ("lookup", C.c_int32), ("group_lanes", C.c_int32),
                 ("unroll", C.c_int32), ("schedule", C.c_int32), ("dense_kernel", C.c_int32),
-                ("planner", C.c_int32), ("core_ids", C.c_int32), ("tiny_items", C.c_int32)]
+                ("planner", C.c_int32), ("core_ids", C.c_int32), ("tiny_items", C.c_int32),
+                ("wide_offsets", C.c_int32)]
 
 
 class PlanInfo(C.Structure):
@@ -365,14 +366,15 @@ class BitBSR:
 
 
 def make_opts(tile_k=None, tau=None, chunk_work=None, waves_per_eu=None, lookup=None, group_lanes=None,
-              unroll=None, schedule=None, dense_kernel=None, planner=None, core_ids=None, tiny_items=None):
+              unroll=None, schedule=None, dense_kernel=None, planner=None, core_ids=None, tiny_items=None,
+              wide_offsets=None):
     """btc_count_opts with the library defaults, overridden where given."""
     o = CountOpts()
     lib().btc_count_opts_default(C.byref(o))
     for k, v in dict(tile_k=tile_k, tau=tau, chunk_work=chunk_work, waves_per_eu=waves_per_eu, lookup=lookup,
                      group_lanes=group_lanes, unroll=unroll, schedule=schedule,
                      dense_kernel=dense_kernel, planner=planner, core_ids=core_ids,
-                     tiny_items=tiny_items).items():
+                     tiny_items=tiny_items, wide_offsets=wide_offsets).items():
         if v is not None:
             setattr(o, k, v)
     return o

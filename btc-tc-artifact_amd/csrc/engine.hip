@@ -201,7 +201,16 @@ __device__ __forceinline__ void team_sync() {
 // group's first record, loaded by the caller.  For bitmap sets, wabs[w] is
 // the window's word w of ids (absolute), and the core AND runs over the
 // uint4 words [c4, we4).
-template <int NT, int G, int U, bool PV, bool N16, class Set>
+// Byte offset of position q in an array of (1 << sh)-byte ids: 32-bit
+// arithmetic (the loads take a uniform base plus a 32-bit offset) unless
+// W64, which the plans of graphs with 2^30 or more ids in L take.
+template <bool W64>
+__device__ __forceinline__ auto byte_off(uint32_t q, int sh) {
+    if constexpr (W64) return (uint64_t)q << sh;
+    else return q << sh;
+}
+
+template <int NT, int G, int U, bool PV, bool W64, bool N16, class Set>
 __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a, uint32_t tid, uint32_t eb,
                                                uint32_t ee, uint4 L, const uint32_t* wabs, uint32_t c4,
                                                uint32_t we4) {
@@ -209,7 +218,7 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
     constexpr uint32_t NG = NT / G;
     constexpr uint32_t STEP = 4 * G * U;
     const uint32_t g = tid / G, gl = tid % G;
-    // 32-bit byte offsets (nnz(L) + padding < 2^30 is enforced by the planner)
+    // 32-bit byte offsets unless W64 (positions are u32)
     const GlobalChar colb = N16 ? (GlobalChar)a.col16 : (GlobalChar)a.col;
     uint64_t acc = 0;
     for (uint32_t e = eb + g; e < ee; e += NG) {
@@ -237,7 +246,7 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
                 for (int u = 0; u < U; ++u) {
                     const uint32_t qq = q + 4 * G * u;
                     if (qq < s1) {
-                        const uint32_t b = set.bits4(to_u2(*(GlobalU2)(colb + (qq << 1))));
+                        const uint32_t b = set.bits4(to_u2(*(GlobalU2)(colb + byte_off<W64>(qq, 1))));
                         // the prefix starts d0 (0..3) ids into this load and
                         // ends d1 (1..4) ids into it (empty when d1 <= d0)
                         const uint32_t d0 = s0 > qq ? s0 - qq : 0u;
@@ -256,11 +265,11 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
                     const uint32_t qq = q + 4 * G * u;
                     if constexpr (N16) {
                         uint2 h = make_uint2(0u, 0u);  // ids at or past s1 are masked below
-                        if (qq < s1) h = to_u2(*(GlobalU2)(colb + (qq << 1)));
+                        if (qq < s1) h = to_u2(*(GlobalU2)(colb + byte_off<W64>(qq, 1)));
                         v[u] = make_uint4(h.x & 0xFFFFu, h.x >> 16, h.y & 0xFFFFu, h.y >> 16);
                     } else {
                         v[u] = make_uint4(sent, sent, sent, sent);
-                        if (qq < s1) v[u] = to_u4(*(GlobalU4)(colb + (qq << 2)));
+                        if (qq < s1) v[u] = to_u4(*(GlobalU4)(colb + byte_off<W64>(qq, 2)));
                     }
                 }
 #pragma unroll
@@ -291,7 +300,7 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
 // pivot's ids are all below kNarrowIds), else into a hash table when
 // d_L(j) <= hash_capacity(W), else stays in HBM for binary search.  The LDS
 // words are zero again on return.
-template <int NT, int W, int G, int U, bool PV>
+template <int NT, int W, int G, int U, bool PV, bool W64>
 __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, uint4 A, uint4 B, uint32_t* lds,
                                                  uint32_t tid) {
     const uint32_t t0 = A.x, t1 = A.y, eb = A.z, ee = A.w, lo = B.x, hi = B.y;
@@ -321,15 +330,15 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
         const uint32_t c4 = lo >> 7, we4 = min(a.core4, (hi >> 7) + 1u);
         if constexpr (NT == kThreads) {
             if (full16) {
-                acc = scan_lists<NT, G, U, PV, true>(FullWindow16{lds}, a, tid, eb, ee, L, lds, c4, we4);
+                acc = scan_lists<NT, G, U, PV, W64, true>(FullWindow16{lds}, a, tid, eb, ee, L, lds, c4, we4);
             } else {
-                acc = scan_lists<NT, G, U, PV, false>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, L,
+                acc = scan_lists<NT, G, U, PV, W64, false>(BitmapSet{lds - bw, base + 32u * W - 1u}, a, tid, eb, ee, L,
                                                       lds - bw, c4, we4);
             }
         } else {
             const BitmapSet set{lds - bw, base + 32u * W - 1u};
-            acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, lds - bw, c4, we4)
-                         : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, lds - bw, c4, we4);
+            acc = narrow ? scan_lists<NT, G, U, PV, W64, true>(set, a, tid, eb, ee, L, lds - bw, c4, we4)
+                         : scan_lists<NT, G, U, PV, W64, false>(set, a, tid, eb, ee, L, lds - bw, c4, we4);
         }
         team_sync<NT>();
         // zero the words that can hold bits, [0, (hi - base) / 32], with
@@ -352,14 +361,14 @@ __device__ __forceinline__ uint64_t process_item(const PlanDev& a, int lookup, u
         }
         team_sync<NT>();
         const HashSet set{lds, shift, mask};
-        acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, nullptr, 0u, 0u)
-                     : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, nullptr, 0u, 0u);
+        acc = narrow ? scan_lists<NT, G, U, PV, W64, true>(set, a, tid, eb, ee, L, nullptr, 0u, 0u)
+                     : scan_lists<NT, G, U, PV, W64, false>(set, a, tid, eb, ee, L, nullptr, 0u, 0u);
         team_sync<NT>();
         for (uint32_t x = tid; x <= mask; x += NT) lds[x] = 0;
     } else {
         const SearchSet set{a.col, t0, t1};
-        acc = narrow ? scan_lists<NT, G, U, PV, true>(set, a, tid, eb, ee, L, nullptr, 0u, 0u)
-                     : scan_lists<NT, G, U, PV, false>(set, a, tid, eb, ee, L, nullptr, 0u, 0u);
+        acc = narrow ? scan_lists<NT, G, U, PV, W64, true>(set, a, tid, eb, ee, L, nullptr, 0u, 0u)
+                     : scan_lists<NT, G, U, PV, W64, false>(set, a, tid, eb, ee, L, nullptr, 0u, 0u);
     }
     return acc;
 }
@@ -442,7 +451,7 @@ __device__ __forceinline__ void flush_slot(uint64_t acc, unsigned long long* slo
 template <int G, int U, bool PV>
 constexpr int kMinWaves = (G <= 8 && U == 1 && !PV) ? 8 : 1;
 
-template <int G, int U, bool PV>
+template <int G, int U, bool PV, bool W64>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<G, U, PV>))) void wedge_kernel(
     WedgeArgs a) {
     __shared__ __align__(16) uint32_t lds[kLdsWords];  // 16-byte reads of bitmap words
@@ -465,7 +474,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             Item nxt;
             if (nx < a.n_block) nxt = fetch_item(a, nx);
             const PlanDev P = item_plan(a, cur);
-            acc += process_item<kThreads, kLdsWords, G, U, PV>(P, a.lookup, cur.A, cur.B, lds, tid);
+            acc += process_item<kThreads, kLdsWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, lds, tid);
             if (multi) flush(acc, P.result);
             __syncthreads();
             it = nx;
@@ -479,7 +488,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             if (it >= a.n_block) break;
             const Item cur = fetch_item(a, it);
             const PlanDev P = item_plan(a, cur);
-            acc += process_item<kThreads, kLdsWords, G, U, PV>(P, a.lookup, cur.A, cur.B, lds, tid);
+            acc += process_item<kThreads, kLdsWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, lds, tid);
             if (multi) flush(acc, P.result);
             __syncthreads();
         }
@@ -498,7 +507,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             Item nxt;
             if (nx < nwave) nxt = fetch_item(a, a.n_block + nx);
             const PlanDev P = item_plan(a, cur);
-            acc += process_item<64, kWaveWords, G, U, PV>(P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
+            acc += process_item<64, kWaveWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
             if (multi) flush(acc, P.result);
             team_sync<64>();
             it = nx;
@@ -512,7 +521,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             if (it >= nwave) break;
             const Item cur = fetch_item(a, a.n_block + it);
             const PlanDev P = item_plan(a, cur);
-            acc += process_item<64, kWaveWords, G, U, PV>(P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
+            acc += process_item<64, kWaveWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
             if (multi) flush(acc, P.result);
             team_sync<64>();
         }
@@ -529,7 +538,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 // wave has 8 pivots' HBM round trips in flight where a wave item has one.
 // The groups of a wave run different items; every step of process_item
 // for a team of kTinyLanes lanes stays within the group.
-template <bool PV>
+template <bool PV, bool W64>
 __global__ __launch_bounds__(kThreads) void tiny_kernel(PlanDev P, int lookup, uint32_t first, uint32_t n,
                                                          uint32_t share, uint32_t shares,
                                                          unsigned long long* slots) {
@@ -543,7 +552,7 @@ __global__ __launch_bounds__(kThreads) void tiny_kernel(PlanDev P, int lookup, u
     for (uint32_t t = blockIdx.x * kGroups + grp; t < n; t += gridDim.x * kGroups) {
         const uint32_t li = first + share + t * shares;
         const uint4 A = P.items[2 * li], B = P.items[2 * li + 1];
-        acc += process_item<kTinyLanes, kTinyWords, kTinyLanes, 1, PV>(P, lookup, A, B, table, gl);
+        acc += process_item<kTinyLanes, kTinyWords, kTinyLanes, 1, PV, W64>(P, lookup, A, B, table, gl);
     }
     if (slots) flush_slot(acc, slots);
     else flush(acc, P.result);
@@ -553,7 +562,7 @@ using WedgeFn = void (*)(WedgeArgs);
 
 template <int G, int U>
 WedgeFn pick_pv(bool pv) {
-    return pv ? wedge_kernel<G, U, true> : wedge_kernel<G, U, false>;
+    return pv ? wedge_kernel<G, U, true, false> : wedge_kernel<G, U, false, false>;
 }
 
 template <int G>
@@ -874,20 +883,30 @@ void check_launch_opts(const btc_count_opts& o) {
     BTC_REQUIRE(o.dense_kernel >= 0 && o.dense_kernel <= 2, BTC_ERR_ARG, "dense_kernel %d (expected 0, 1 or 2)",
                 o.dense_kernel);
     BTC_REQUIRE(o.planner >= 0 && o.planner <= 2, BTC_ERR_ARG, "planner %d (expected 0, 1 or 2)", o.planner);
+    BTC_REQUIRE(o.wide_offsets == 0 || o.wide_offsets == 1, BTC_ERR_ARG, "wide_offsets %d (expected 0 or 1)",
+                o.wide_offsets);
 }
 
 namespace {
 
-WedgeFn wedge_fn(const btc_count_opts& o, bool pv, unsigned* grid) {
+// W64: 64-bit byte offsets into L, in the default shape only (8 lanes per
+// list, one load in flight).
+WedgeFn wedge_fn(const btc_count_opts& o, bool pv, bool w64, unsigned* grid) {
     const int G = o.group_lanes ? o.group_lanes : 8;
     const int U = o.unroll ? o.unroll : 1;
-    WedgeFn fn = pick_wedge(G, U, pv);
+    WedgeFn fn = w64 ? (pv ? wedge_kernel<8, 1, true, true> : wedge_kernel<8, 1, false, true>) : pick_wedge(G, U, pv);
     int per_cu = 0;
     BTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kThreads, 0));
     per_cu = std::max(1, per_cu);
     if (o.waves_per_eu > 0) per_cu = std::min(per_cu, (int)o.waves_per_eu);
     *grid = (unsigned)(device_cu_count() * per_cu);
     return fn;
+}
+
+// Whether counts of p address L with 64-bit offsets: L holds 2^30 ids or
+// more (32-bit byte offsets of 4-byte ids stop there), or the plan asks.
+bool wide_offsets(const btc_plan_s* p) {
+    return p->launch.wide_offsets != 0 || (uint64_t)p->g->m + kColPad >= (uint64_t(1) << 30);
 }
 
 PlanDev plan_dev(btc_plan_s* p, unsigned long long* d_total, PvOut d_pv) {
@@ -907,7 +926,7 @@ void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counte
     const uint32_t nw = share_count(p->nitems - p->n_tiny_items - p->n_block_items, share, shares);
     if (nb + nw == 0) return;
     unsigned grid = 0;
-    WedgeFn fn = wedge_fn(p->launch, d_pv.on(), &grid);
+    WedgeFn fn = wedge_fn(p->launch, d_pv.on(), wide_offsets(p), &grid);
     WedgeArgs a{plan_dev(p, d_total, d_pv), nullptr, nullptr, 1u, nb, nb + nw, d_counters, (int)p->launch.lookup,
                 (int)p->launch.schedule, share, shares, d_slots};
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
@@ -918,7 +937,8 @@ void launch_tiny(btc_plan_s* p, unsigned long long* d_total, PvOut d_pv, hipStre
                  uint32_t shares = 1, unsigned long long* d_slots = nullptr) {
     const uint32_t n = share_count(p->n_tiny_items, share, shares);
     if (!n) return;
-    auto fn = d_pv.on() ? tiny_kernel<true> : tiny_kernel<false>;
+    auto fn = wide_offsets(p) ? (d_pv.on() ? tiny_kernel<true, true> : tiny_kernel<false, true>)
+                              : (d_pv.on() ? tiny_kernel<true, false> : tiny_kernel<false, false>);
     int per_cu = 0;
     BTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kThreads, 0));
     constexpr uint32_t kGroups = kThreads / kTinyLanes;
@@ -961,7 +981,9 @@ void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, M
         }
     if (ni) {
         unsigned grid = 0;
-        WedgeFn fn = wedge_fn(P[0]->launch, false, &grid);
+        bool w64 = false;
+        for (int j = 0; j < k; ++j) w64 = w64 || wide_offsets(P[j]);
+        WedgeFn fn = wedge_fn(P[0]->launch, false, w64, &grid);
         WedgeArgs a{pd[0], nullptr, nullptr, 1u, nb, ni, ws.counters.p, (int)P[0]->launch.lookup,
                     (int)P[0]->launch.schedule};
         if (k > 1) {
@@ -1138,9 +1160,9 @@ btc_plan_s* plan_create(btc_graph g, const btc_count_opts* opts, hipStream_t s, 
     btcx::check_launch_opts(o);
     p->launch = o;
     btcx::ensure_device(g, s);
-    BTC_REQUIRE((uint64_t)g->m + btcx::kColPad < (1ull << 30), BTC_ERR_OVERFLOW,
-                "nnz(L) = %lld: the wedge kernel addresses L with 32-bit byte offsets (limit 2^30 - %zu)",
-                (long long)g->m, btcx::kColPad);
+    BTC_REQUIRE((uint64_t)g->m + btcx::kColPad < (1ull << 32), BTC_ERR_OVERFLOW,
+                "nnz(L) = %lld: work lists hold u32 positions of L (limit 2^32 - %zu)", (long long)g->m,
+                btcx::kColPad);
     btcx::build_col16(p.get(), s);  // ordered before the planners' closing synchronisation
     int64_t chunk = o.chunk_work > 0 ? o.chunk_work : 0;
     if (!chunk) {

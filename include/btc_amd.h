@@ -217,6 +217,8 @@ typedef struct {
                              bitmap window counted by groups of 8 lanes in their own kernel: 0 = auto
                              (graphs with at least 2^30 wedges), 1 = always, -1 = never (they stay
                              wave items) */
+    int32_t wide_offsets; /* 1 = the wedge kernel addresses L with 64-bit byte offsets (in its default
+                             shape); 0 = only when L holds 2^30 ids or more, where 32-bit offsets end */
 } btc_count_opts;
 
 typedef struct btc_plan_s* btc_plan;
