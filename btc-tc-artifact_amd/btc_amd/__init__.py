@@ -77,6 +77,13 @@ class PlanInfo(C.Structure):
                 ("tiny_items", C.c_int64)]
 
 
+class PipelineInfo(C.Structure):
+    _fields_ = [("n_cap", C.c_int32), ("graph_nodes", C.c_int32), ("nnz_cap", C.c_int64),
+                ("device_bytes", C.c_int64), ("flags", C.c_uint32), ("n", C.c_int32), ("m", C.c_int64),
+                ("wedges", C.c_int64), ("wedge_checks", C.c_int64), ("work_items", C.c_int32),
+                ("max_row", C.c_int32)]
+
+
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_batch", C.c_int32), ("prep_threads", C.c_int32),
                 ("canon_flags", C.c_int32), ("group_graphs", C.c_int32)]
@@ -158,6 +165,12 @@ _SIGS = {
     "btc_tune_table_save": (C.c_int, [C.c_char_p]),
     "btc_tune_table_load": (C.c_int, [C.c_char_p]),
     "btc_tune_table_clear": (None, []),
+    "btc_pipeline_create": (C.c_int, [C.c_int32, C.c_int64, C.c_uint32, C.POINTER(CountOpts), vp, C.POINTER(vp)]),
+    "btc_pipeline_count": (C.c_int, [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_float)]),
+    "btc_pipeline_graph": (C.c_int, [vp, vp, C.POINTER(vp)]),
+    "btc_pipeline_get_info": (C.c_int, [vp, C.POINTER(PipelineInfo)]),
+    "btc_pipeline_free": (None, [vp]),
     "btc_server_opts_default": (None, [C.POINTER(ServerOpts)]),
     "btc_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(vp)]),
     "btc_server_submit": (C.c_int, [vp, C.c_uint64, vp, vp, C.c_int64, C.c_int32, C.POINTER(C.c_uint64)]),
@@ -437,6 +450,58 @@ class Plan:
     def close(self):
         if self._h is not None and self._h.value:
             lib().btc_plan_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pipeline:
+    """COO pairs -> triangle total as one captured HIP graph, for inputs of
+    up to n_cap ids and nnz_cap pairs (btc_pipeline)."""
+
+    def __init__(self, n_cap, nnz_cap, flags=CANON_DEFAULT, stream=None, **opts):
+        self.opts = make_opts(**opts)
+        h = vp()
+        _check(lib().btc_pipeline_create(n_cap, nnz_cap, flags, C.byref(self.opts), stream, C.byref(h)))
+        self._h = h
+
+    def count(self, src, dst, n=0, stream=None):
+        """(total, kernel_ms) of the graph of host pairs (src[e], dst[e])."""
+        src = np.ascontiguousarray(src, np.int32)
+        dst = np.ascontiguousarray(dst, np.int32)
+        if src.shape != dst.shape:
+            raise ValueError("src and dst differ in length")
+        total, ms = C.c_uint64(0), C.c_float(0)
+        _check(lib().btc_pipeline_count(self._h, _ptr(src), _ptr(dst), src.size, n, 0, stream, C.byref(total),
+                                        C.byref(ms)))
+        return total.value, ms.value
+
+    def count_device(self, src_ptr, dst_ptr, nnz, n=0, stream=None):
+        """(total, kernel_ms) of int32 pairs already in HBM (device addresses)."""
+        total, ms = C.c_uint64(0), C.c_float(0)
+        _check(lib().btc_pipeline_count(self._h, vp(src_ptr), vp(dst_ptr), nnz, n, 1, stream, C.byref(total),
+                                        C.byref(ms)))
+        return total.value, ms.value
+
+    def graph(self, stream=None):
+        """The last run's canonical graph (a Graph in HBM)."""
+        h = vp()
+        _check(lib().btc_pipeline_graph(self._h, stream, C.byref(h)))
+        return Graph(h.value)
+
+    @property
+    def info(self):
+        pi = PipelineInfo()
+        _check(lib().btc_pipeline_get_info(self._h, C.byref(pi)))
+        return {k: getattr(pi, k) for k, _ in PipelineInfo._fields_}
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().btc_pipeline_free(self._h)
         self._h = None
 
     def __del__(self):

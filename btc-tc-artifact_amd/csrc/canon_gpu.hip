@@ -31,8 +31,9 @@ unsigned grid_for(int64_t n) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
 }
 
-__global__ void id_range_kernel(const int32_t* __restrict__ src, const int32_t* __restrict__ dst, int64_t nnz,
+__global__ void id_range_kernel(const int32_t* __restrict__ src, const int32_t* __restrict__ dst, Cnt nnz_,
                                 int32_t* __restrict__ lohi) {
+    const int64_t nnz = nnz_.get();
     int32_t lo = INT_MAX, hi = -1;
     for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * kT) {
         const int32_t u = src[e], v = dst[e];
@@ -60,8 +61,9 @@ __global__ void orient_kernel(const int32_t* __restrict__ src, const int32_t* __
 }
 
 // flag[e] = key e is the first of its run and not the sentinel
-__global__ void first_flag_kernel(const uint64_t* __restrict__ keys, int64_t n, uint64_t sentinel,
+__global__ void first_flag_kernel(const uint64_t* __restrict__ keys, Cnt n_, uint64_t sentinel,
                                   uint32_t* __restrict__ flag) {
+    const int64_t n = n_.get();
     for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < n; e += (int64_t)gridDim.x * kT) {
         const uint64_t k = keys[e];
         flag[e] = (k != sentinel && (e == 0 || keys[e - 1] != k)) ? 1u : 0u;
@@ -69,12 +71,14 @@ __global__ void first_flag_kernel(const uint64_t* __restrict__ keys, int64_t n, 
 }
 
 __global__ void compact_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flag,
-                               const uint32_t* __restrict__ pos, int64_t n, uint64_t* __restrict__ out) {
+                               const uint32_t* __restrict__ pos, Cnt n_, uint64_t* __restrict__ out) {
+    const int64_t n = n_.get();
     for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < n; e += (int64_t)gridDim.x * kT)
         if (flag[e]) out[pos[e]] = keys[e];
 }
 
-__global__ void mark_used_kernel(const uint64_t* __restrict__ keys, int64_t m, int b, uint32_t* __restrict__ used) {
+__global__ void mark_used_kernel(const uint64_t* __restrict__ keys, Cnt m_, int b, uint32_t* __restrict__ used) {
+    const int64_t m = m_.get();
     const uint64_t mask = (1ull << b) - 1;
     for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT) {
         const uint64_t k = keys[e];
@@ -84,20 +88,23 @@ __global__ void mark_used_kernel(const uint64_t* __restrict__ keys, int64_t m, i
 }
 
 // labels[newid[v]] = v for every used v (newid = exclusive scan of used)
-__global__ void labels_kernel(const uint32_t* __restrict__ used, const uint32_t* __restrict__ newid, int32_t n,
+__global__ void labels_kernel(const uint32_t* __restrict__ used, const uint32_t* __restrict__ newid, Cnt n_,
                               int32_t* __restrict__ labels) {
+    const int64_t n = n_.get();
     for (int64_t v = blockIdx.x * (int64_t)kT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kT)
         if (used[v]) labels[newid[v]] = (int32_t)v;
 }
 
-__global__ void iota_kernel(int32_t* __restrict__ x, int64_t n) {
+__global__ void iota_kernel(int32_t* __restrict__ x, Cnt n_) {
+    const int64_t n = n_.get();
     for (int64_t v = blockIdx.x * (int64_t)kT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kT) x[v] = (int32_t)v;
 }
 
 // Rewrites keys (row << b | col) through map (NULL = identity) into
 // max(map row, map col) << bn | min(...).
-__global__ void relabel_kernel(uint64_t* __restrict__ keys, int64_t m, int b, const uint32_t* __restrict__ map,
+__global__ void relabel_kernel(uint64_t* __restrict__ keys, Cnt m_, int b, const uint32_t* __restrict__ map,
                                int bn) {
+    const int64_t m = m_.get();
     const uint64_t mask = (1ull << b) - 1;
     for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT) {
         const uint64_t k = keys[e];
@@ -110,7 +117,8 @@ __global__ void relabel_kernel(uint64_t* __restrict__ keys, int64_t m, int b, co
     }
 }
 
-__global__ void degree_kernel(const uint64_t* __restrict__ keys, int64_t m, int b, uint32_t* __restrict__ deg) {
+__global__ void degree_kernel(const uint64_t* __restrict__ keys, Cnt m_, int b, uint32_t* __restrict__ deg) {
+    const int64_t m = m_.get();
     const uint64_t mask = (1ull << b) - 1;
     for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT) {
         const uint64_t k = keys[e];
@@ -119,11 +127,13 @@ __global__ void degree_kernel(const uint64_t* __restrict__ keys, int64_t m, int 
     }
 }
 
-// key = ~degree (ascending = descending degree), value = vertex
-__global__ void rank_keys_kernel(const uint32_t* __restrict__ deg, int32_t n, uint32_t* __restrict__ key,
+// key = top - degree (ascending = descending degree; top is at least the
+// largest degree), value = vertex
+__global__ void rank_keys_kernel(const uint32_t* __restrict__ deg, Cnt n_, uint32_t top, uint32_t* __restrict__ key,
                                  uint32_t* __restrict__ val) {
+    const int64_t n = n_.get();
     for (int64_t v = blockIdx.x * (int64_t)kT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kT) {
-        key[v] = ~deg[v];
+        key[v] = top - deg[v];
         val[v] = (uint32_t)v;
     }
 }
@@ -153,8 +163,9 @@ __global__ void gather_kernel(const uint32_t* __restrict__ map, const uint32_t* 
 }
 
 // rank[order[r]] = r; labels_out[r] = labels_in[order[r]]
-__global__ void rank_scatter_kernel(const uint32_t* __restrict__ order, int32_t n, uint32_t* __restrict__ rank,
+__global__ void rank_scatter_kernel(const uint32_t* __restrict__ order, Cnt n_, uint32_t* __restrict__ rank,
                                     const int32_t* __restrict__ labels_in, int32_t* __restrict__ labels_out) {
+    const int64_t n = n_.get();
     for (int64_t r = blockIdx.x * (int64_t)kT + threadIdx.x; r < n; r += (int64_t)gridDim.x * kT) {
         const uint32_t v = order[r];
         rank[v] = (uint32_t)r;
@@ -163,8 +174,10 @@ __global__ void rank_scatter_kernel(const uint32_t* __restrict__ order, int32_t 
 }
 
 // rowptr[r] = first e with keys[e] >= r << b (r = 0..n)
-__global__ void rowptr_kernel(const uint64_t* __restrict__ keys, int64_t m, int b, int32_t n,
+__global__ void rowptr_kernel(const uint64_t* __restrict__ keys, Cnt m_, int b, Cnt n_,
                               uint32_t* __restrict__ rowptr) {
+    const int64_t m = m_.get();
+    const int64_t n = n_.get();
     for (int64_t r = blockIdx.x * (int64_t)kT + threadIdx.x; r <= n; r += (int64_t)gridDim.x * kT) {
         const uint64_t t = (uint64_t)r << b;
         int64_t lo = 0, hi = m;
@@ -177,15 +190,17 @@ __global__ void rowptr_kernel(const uint64_t* __restrict__ keys, int64_t m, int 
     }
 }
 
-__global__ void col_kernel(const uint64_t* __restrict__ keys, int64_t m, int b, int32_t* __restrict__ col) {
+__global__ void col_kernel(const uint64_t* __restrict__ keys, Cnt m_, int b, int32_t* __restrict__ col) {
+    const int64_t m = m_.get();
     const uint64_t mask = (1ull << b) - 1;
     for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT)
         col[e] = (int32_t)(keys[e] & mask);
 }
 
 // out: [0] longest row, [1] sum C(d, 2), [2] sum over edges of d(i) + d(j)
-__global__ void row_stats_kernel(const uint32_t* __restrict__ rowptr, int32_t n,
+__global__ void row_stats_kernel(const uint32_t* __restrict__ rowptr, Cnt n_,
                                  unsigned long long* __restrict__ out) {
+    const int64_t n = n_.get();
     uint32_t mx = 0;
     uint64_t w = 0, work = 0;
     for (int64_t i = blockIdx.x * (int64_t)kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
@@ -204,8 +219,9 @@ __global__ void row_stats_kernel(const uint32_t* __restrict__ rowptr, int32_t n,
     }
 }
 
-__global__ void edge_stats_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int64_t m,
+__global__ void edge_stats_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col, Cnt m_,
                                   unsigned long long* __restrict__ out) {
+    const int64_t m = m_.get();
     uint64_t work = 0;
     for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT) {
         const int32_t j = col[e];
@@ -213,6 +229,77 @@ __global__ void edge_stats_kernel(const uint32_t* __restrict__ rowptr, const int
     }
     work = wave_sum_u64(work);
     if ((threadIdx.x & 63) == 0) atomicAdd(&out[2], (unsigned long long)work);
+}
+
+// ------------------------------------------------- capturable form kernels
+// Clears a run's words (keeping the two the host wrote) and opens the id
+// range.
+__global__ void pipe_reset_kernel(PipeCtl* __restrict__ c) {
+    constexpr uint32_t kWords = sizeof(PipeCtl) / 4;
+    uint32_t* w = reinterpret_cast<uint32_t*>(c);
+    for (uint32_t i = threadIdx.x + 2; i < kWords; i += blockDim.x) w[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c->lo = INT_MAX;
+        c->hi = -1;
+    }
+}
+
+// n = the given count, or the largest id + 1; flags ids out of range.
+__global__ void pipe_resolve_kernel(PipeCtl* __restrict__ c, int32_t n_cap) {
+    if (threadIdx.x != 0) return;
+    const int32_t lo = c->nnz ? c->lo : 0, hi = c->hi;
+    int64_t n = c->n_in > 0 ? (int64_t)c->n_in : (int64_t)hi + 1;
+    uint32_t st = 0;
+    if (lo < 0) st |= kPipeNegativeId;
+    if (hi >= n) st |= kPipeIdRange;
+    if (n > n_cap) {
+        st |= kPipeTooManyIds;
+        n = n_cap;
+    }
+    c->n = (uint32_t)n;
+    c->status = st;
+}
+
+// orient_kernel for unchecked input: pairs with an id outside [0, n) become
+// the sentinel too (the run is refused then, but its kernels stay in
+// bounds).
+__global__ void orient_checked_kernel(const int32_t* __restrict__ src, const int32_t* __restrict__ dst, Cnt nnz_,
+                                      const uint32_t* __restrict__ d_n, int b, uint64_t sentinel,
+                                      uint64_t* __restrict__ keys) {
+    const int64_t nnz = nnz_.get();
+    const uint32_t n = *d_n;
+    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * kT) {
+        const uint32_t u = (uint32_t)src[e], v = (uint32_t)dst[e];
+        keys[e] = (u == v || u >= n || v >= n) ? sentinel : ((uint64_t)max(u, v) << b) | (uint64_t)min(u, v);
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, Cnt n_, uint32_t v) {
+    const int64_t n = n_.get();
+    for (int64_t i = blockIdx.x * (int64_t)kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) p[i] = v;
+}
+
+// labels_kernel of the compacted ids; with no edge nothing is compacted
+// (labels = identity, nc = n), as in canonicalise_device.
+__global__ void pipe_labels_kernel(const uint32_t* __restrict__ used, const uint32_t* __restrict__ newid, Cnt n_,
+                                   PipeCtl* __restrict__ c, int32_t* __restrict__ labels) {
+    const int64_t n = n_.get();
+    const bool none = c->m == 0;
+    for (int64_t v = blockIdx.x * (int64_t)kT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kT) {
+        if (none) labels[v] = (int32_t)v;
+        else if (used[v]) labels[newid[v]] = (int32_t)v;
+    }
+    if (none && blockIdx.x == 0 && threadIdx.x == 0) c->nc = (uint32_t)n;
+}
+
+__global__ void pipe_copy_n_kernel(PipeCtl* __restrict__ c) {
+    if (threadIdx.x == 0) c->nc = c->n;
+}
+
+// col[m, m + kColPad) = 0
+__global__ void col_pad_kernel(int32_t* __restrict__ col, const uint32_t* __restrict__ d_m) {
+    if (threadIdx.x < kColPad) col[(size_t)*d_m + threadIdx.x] = 0;
 }
 
 #define BTC_LAUNCH(kernel, n, s, ...)                                                   \
@@ -330,7 +417,7 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
         lab2.alloc((size_t)nc);
         BTC_HIP(hipMemsetAsync(deg.p, 0, (size_t)nc * 4, s));
         if (m) BTC_LAUNCH(degree_kernel, m, s, keys, m, bn, deg.p);
-        BTC_LAUNCH(rank_keys_kernel, nc, s, deg.p, nc, k0.p, v0.p);
+        BTC_LAUNCH(rank_keys_kernel, nc, s, deg.p, nc, 0xFFFFFFFFu, k0.p, v0.p);
         bool alt = sort_pairs_u32(k0.p, v0.p, k1.p, v1.p, (size_t)nc, 32, s);
         if (segs) {
             // then a stable sort by segment: (segment, -degree, id)
@@ -374,6 +461,79 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
     g->wedges = (int64_t)h[1];
     g->merge_work = (int64_t)h[2];
     g->on_host = false;
+}
+
+namespace {
+
+// Grid of a capacity-sized launch: its workgroups stride over the live
+// count, so a few per CU suffice.
+unsigned cap_grid(int64_t cap) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + kT - 1) / kT, 2048));
+}
+
+}  // namespace
+
+#define CAP_LAUNCH(kernel, cap, s, ...)                                                \
+    do {                                                                               \
+        hipLaunchKernelGGL(kernel, dim3(cap_grid(cap)), dim3(kT), 0, s, __VA_ARGS__);  \
+        BTC_CHECK_LAUNCH(#kernel);                                                     \
+    } while (0)
+
+int32_t* canonicalise_capacity(const CanonCap& c, hipStream_t s) {
+    PipeCtl* ctl = c.ctl;
+    const int64_t E = c.nnz_cap, N = c.n_cap;
+    const int b = id_bits(N);
+    const uint64_t sentinel = (1ull << (2 * b)) - 1;
+    const Cnt nnz = Cnt::dev(&ctl->nnz), n = Cnt::dev(&ctl->n), m = Cnt::dev(&ctl->m), nc = Cnt::dev(&ctl->nc);
+    hipLaunchKernelGGL(pipe_reset_kernel, dim3(1), dim3(64), 0, s, ctl);
+    BTC_CHECK_LAUNCH("pipe_reset_kernel");
+    CAP_LAUNCH(id_range_kernel, E, s, c.src, c.dst, nnz, &ctl->lo);
+    hipLaunchKernelGGL(pipe_resolve_kernel, dim3(1), dim3(64), 0, s, ctl, c.n_cap);
+    BTC_CHECK_LAUNCH("pipe_resolve_kernel");
+
+    // oriented keys, sorted, deduplicated
+    CAP_LAUNCH(orient_checked_kernel, E, s, c.src, c.dst, nnz, &ctl->n, b, sentinel, c.ka);
+    uint64_t* sorted = dsort_u64(c.ka, c.kb, (size_t)E, &ctl->nnz, 2 * b, c.ws, s);
+    uint64_t* keys = sorted == c.ka ? c.kb : c.ka;
+    uint64_t* spare = sorted;
+    CAP_LAUNCH(first_flag_kernel, E, s, sorted, nnz, sentinel, c.flag);
+    dscan_u32(c.flag, c.pos, (size_t)E, &ctl->nnz, &ctl->m, c.ws, s);
+    CAP_LAUNCH(compact_kernel, E, s, sorted, c.flag, c.pos, nnz, keys);
+
+    // order-preserving compaction of the used ids (ids keep b bits)
+    int32_t* labels = c.lab0;
+    if (c.flags & BTC_CANON_COMPACT) {
+        CAP_LAUNCH(fill_u32_kernel, N, s, c.used, n, 0u);
+        CAP_LAUNCH(mark_used_kernel, E, s, keys, m, b, c.used);
+        dscan_u32(c.used, c.newid, (size_t)N, &ctl->n, &ctl->nc, c.ws, s);
+        CAP_LAUNCH(pipe_labels_kernel, N, s, c.used, c.newid, n, ctl, labels);
+        CAP_LAUNCH(relabel_kernel, E, s, keys, m, b, c.newid, b);
+    } else {
+        hipLaunchKernelGGL(pipe_copy_n_kernel, dim3(1), dim3(64), 0, s, ctl);
+        BTC_CHECK_LAUNCH("pipe_copy_n_kernel");
+        CAP_LAUNCH(iota_kernel, N, s, labels, n);
+    }
+
+    if (c.flags & BTC_CANON_DEGREE_ORDER) {
+        // keys N - 1 - degree < 2^b: descending degree, ties by id
+        CAP_LAUNCH(fill_u32_kernel, N, s, c.deg, nc, 0u);
+        CAP_LAUNCH(degree_kernel, E, s, keys, m, b, c.deg);
+        CAP_LAUNCH(rank_keys_kernel, N, s, c.deg, nc, (uint32_t)(N - 1), c.k0, c.v0);
+        const bool alt = dsort_pairs_u32(c.k0, c.v0, c.k1, c.v1, (size_t)N, &ctl->nc, b, c.ws, s);
+        CAP_LAUNCH(rank_scatter_kernel, N, s, alt ? c.v1 : c.v0, nc, c.rank, labels, c.lab1);
+        CAP_LAUNCH(relabel_kernel, E, s, keys, m, b, c.rank, b);
+        keys = dsort_u64(keys, spare, (size_t)E, &ctl->m, 2 * b, c.ws, s);
+        labels = c.lab1;
+    }
+
+    // CSR of L and its statistics
+    CAP_LAUNCH(rowptr_kernel, N + 1, s, keys, m, b, nc, c.rowptr);
+    CAP_LAUNCH(col_kernel, E, s, keys, m, b, c.col);
+    hipLaunchKernelGGL(col_pad_kernel, dim3(1), dim3(64), 0, s, c.col, &ctl->m);
+    BTC_CHECK_LAUNCH("col_pad_kernel");
+    CAP_LAUNCH(row_stats_kernel, N, s, c.rowptr, nc, ctl->rs);
+    CAP_LAUNCH(edge_stats_kernel, E, s, c.rowptr, c.col, m, ctl->rs);
+    return labels;
 }
 
 void ensure_host(btc_graph_s* g) {

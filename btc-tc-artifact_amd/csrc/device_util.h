@@ -9,6 +9,21 @@ namespace btcx {
 
 constexpr int kWave = 64;
 
+// Length of a kernel's index range: a value fixed at launch, or one read
+// from device memory when the kernel runs (Cnt::dev, plus `add`), which a
+// captured launch needs when a preceding kernel computes it.
+struct Cnt {
+    int64_t v = 0;
+    const uint32_t* p = nullptr;
+    Cnt(int64_t x) : v(x) {}
+    static Cnt dev(const uint32_t* q, int64_t add = 0) {
+        Cnt c(add);
+        c.p = q;
+        return c;
+    }
+    __device__ __forceinline__ int64_t get() const { return p ? (int64_t)*p + v : v; }
+};
+
 // Make a value the compiler treats as per-lane into a wave-uniform one, so
 // index loads derived from it go through the scalar unit.
 __device__ __forceinline__ uint32_t uniform(uint32_t v) {

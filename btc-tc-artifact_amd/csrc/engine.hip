@@ -171,6 +171,7 @@ struct WedgeArgs {
     int schedule;        // 0 = work stealing, 1 = static stride
     uint32_t share = 0, shares = 1;  // single plan: launch item t is plan item share + t * shares of its class
     unsigned long long* slots = nullptr;  // single plan: workgroup sums go to these kSlots slots, not p0.result
+    const uint32_t* dcounts = nullptr;    // single plan, set: {n_block, n_items} read from device memory
 };
 
 // Barrier of the threads that share one LDS region: the workgroup (NT = 256)
@@ -393,7 +394,8 @@ struct Item {
     uint4 A, B;
 };
 
-__device__ __forceinline__ Item fetch_item(const WedgeArgs& a, uint32_t t) {
+// nb = the launch's workgroup items, pnb = the plan's.
+__device__ __forceinline__ Item fetch_item(const WedgeArgs& a, uint32_t t, uint32_t nb, uint32_t pnb) {
     Item r;
     const uint4* items = a.p0.items;
     uint32_t li = t;
@@ -401,7 +403,7 @@ __device__ __forceinline__ Item fetch_item(const WedgeArgs& a, uint32_t t) {
     if (a.nplans == 1) {
         // single plan: every shares-th item of each class from share on
         // (the plan's own numbering when shares = 1)
-        li = t < a.n_block ? a.share + t * a.shares : a.p0.n_block + a.share + (t - a.n_block) * a.shares;
+        li = t < nb ? a.share + t * a.shares : pnb + a.share + (t - nb) * a.shares;
     } else {
         const uint2 m = a.map[t];
         r.plan = uniform(m.x);
@@ -460,6 +462,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
     const int lane = tid & 63;
     const uint32_t wid = uniform(tid >> 6);
     const bool multi = a.nplans != 1;
+    const uint32_t n_block = a.dcounts ? uniform(a.dcounts[0]) : a.n_block;
+    const uint32_t n_items = a.dcounts ? uniform(a.dcounts[1]) : a.n_items;
+    const uint32_t pnb = a.dcounts ? n_block : a.p0.n_block;
     for (int i = tid; i < kLdsWords; i += kThreads) lds[i] = 0;
     __syncthreads();
     uint64_t acc = 0;
@@ -468,11 +473,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
     if (a.schedule == 1) {
         uint32_t it = blockIdx.x;
         Item cur;
-        if (it < a.n_block) cur = fetch_item(a, it);
-        while (it < a.n_block) {
+        if (it < n_block) cur = fetch_item(a, it, n_block, pnb);
+        while (it < n_block) {
             const uint32_t nx = it + gridDim.x;
             Item nxt;
-            if (nx < a.n_block) nxt = fetch_item(a, nx);
+            if (nx < n_block) nxt = fetch_item(a, nx, n_block, pnb);
             const PlanDev P = item_plan(a, cur);
             acc += process_item<kThreads, kLdsWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, lds, tid);
             if (multi) flush(acc, P.result);
@@ -485,8 +490,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             if (tid == 0) s_item = atomicAdd(a.counters, 1u);
             __syncthreads();
             const uint32_t it = uniform(s_item);
-            if (it >= a.n_block) break;
-            const Item cur = fetch_item(a, it);
+            if (it >= n_block) break;
+            const Item cur = fetch_item(a, it, n_block, pnb);
             const PlanDev P = item_plan(a, cur);
             acc += process_item<kThreads, kLdsWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, lds, tid);
             if (multi) flush(acc, P.result);
@@ -496,16 +501,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 
     // phase 2: wave items, LDS split into four zeroed slices
     uint32_t* slice = lds + wid * kWaveWords;
-    const uint32_t nwave = a.n_items - a.n_block;
+    const uint32_t nwave = n_items - n_block;
     if (a.schedule == 1) {
         const uint32_t stride = gridDim.x * (kThreads / 64);
         uint32_t it = blockIdx.x * (kThreads / 64) + wid;
         Item cur;
-        if (it < nwave) cur = fetch_item(a, a.n_block + it);
+        if (it < nwave) cur = fetch_item(a, n_block + it, n_block, pnb);
         while (it < nwave) {
             const uint32_t nx = it + stride;
             Item nxt;
-            if (nx < nwave) nxt = fetch_item(a, a.n_block + nx);
+            if (nx < nwave) nxt = fetch_item(a, n_block + nx, n_block, pnb);
             const PlanDev P = item_plan(a, cur);
             acc += process_item<64, kWaveWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
             if (multi) flush(acc, P.result);
@@ -519,7 +524,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             if (lane == 0) it = atomicAdd(a.counters + 1, 1u);
             it = uniform(__shfl(it, 0));
             if (it >= nwave) break;
-            const Item cur = fetch_item(a, a.n_block + it);
+            const Item cur = fetch_item(a, n_block + it, n_block, pnb);
             const PlanDev P = item_plan(a, cur);
             acc += process_item<64, kWaveWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
             if (multi) flush(acc, P.result);
@@ -590,7 +595,8 @@ namespace btcx {
 namespace {
 
 // col16[q] = low 16 bits of col[q], over L and its padding.
-__global__ void narrow_col_kernel(const int32_t* __restrict__ col, uint32_t n, uint16_t* __restrict__ col16) {
+__global__ void narrow_col_kernel(const int32_t* __restrict__ col, Cnt n_, uint16_t* __restrict__ col16) {
+    const uint32_t n = (uint32_t)n_.get();
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
         col16[q] = (uint16_t)col[q];
 }
@@ -959,6 +965,26 @@ void launch_plan_dense(btc_plan_s* p, unsigned long long* d_slots, PvOut d_pv, h
 }
 
 }  // namespace
+
+void pipeline_count_enqueue(const PlanCap& c, uint16_t* col16, unsigned long long* d_work,
+                            const btc_count_opts& o, hipStream_t s) {
+    launch_zero_u32(reinterpret_cast<uint32_t*>(d_work), 2 * kWorkWords, s);
+    const int64_t span = c.m_cap + (int64_t)kColPad;
+    const unsigned g = (unsigned)std::min<int64_t>((span + kThreads - 1) / kThreads, 2048);
+    hipLaunchKernelGGL(narrow_col_kernel, dim3(g), dim3(kThreads), 0, s, c.col,
+                       Cnt::dev(&c.ctl->m, (int64_t)kColPad), col16);
+    BTC_CHECK_LAUNCH("narrow_col_kernel");
+    const bool w64 = o.wide_offsets != 0 || (uint64_t)span >= (uint64_t(1) << 30);
+    unsigned grid = 0;
+    WedgeFn fn = wedge_fn(o, false, w64, &grid);
+    const PlanDev pd{c.col, col16, c.lists, reinterpret_cast<const uint4*>(c.rowbits), c.ur, c.items, 0u, 0u,
+                     d_work, PvOut{}, c.core_ids / 128};
+    WedgeArgs a{pd, nullptr, nullptr, 1u, 0u, 0u, reinterpret_cast<uint32_t*>(d_work + 1), (int)o.lookup,
+                (int)o.schedule, 0u, 1u, d_work + kWorkSlots, c.ctl->item_counts};
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
+    BTC_CHECK_LAUNCH("wedge_kernel");
+    launch_slot_sum(d_work + kWorkSlots, d_work, s);
+}
 
 void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, MultiWorkspace& ws) {
     if (k <= 0) return;

@@ -166,6 +166,121 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s);
 // a row may run up to 3 ids past the last entry.
 constexpr size_t kColPad = 16;
 
+// ---------------------------------------------------------------- pipeline
+// Device words of one capturable build + count (btc_pipeline): the host
+// writes nnz and n_in before a replay and reads the block back after it.
+struct PipeCtl {
+    uint32_t nnz;    // input pairs
+    int32_t n_in;    // vertices as given (<= 0: largest id + 1)
+    int32_t lo, hi;  // smallest and largest id of the input
+    uint32_t n;      // vertices of the input
+    uint32_t status; // kPipe* bits: why the input was refused
+    uint32_t m;      // nnz(L)
+    uint32_t nc;     // vertices of L (after compaction)
+    uint32_t ne;     // prefix lists
+    uint32_t nbm;    // rows with a hub-core bitmap
+    uint32_t nbm_words;
+    uint32_t nitems;           // work items
+    uint32_t item_counts[2];   // {workgroup items, workgroup + wave items}
+    uint32_t reserved[2];
+    unsigned long long rs[3];  // longest row, sum C(d, 2), merge work
+    unsigned long long st[8];  // planner statistics (as plan_device's)
+    unsigned long long chunk;  // wedge checks per work item
+    unsigned long long checks; // wedge checks in all
+};
+constexpr uint32_t kPipeNegativeId = 1, kPipeIdRange = 2, kPipeTooManyIds = 4;
+
+// Buffers of a capturable canonicalisation of up to nnz_cap pairs over up
+// to n_cap ids (element counts in brackets).
+struct CanonCap {
+    const int32_t* src;
+    const int32_t* dst;  // [nnz_cap] each
+    int64_t nnz_cap;
+    int32_t n_cap;
+    uint32_t flags;      // BTC_CANON_*
+    PipeCtl* ctl;
+    uint64_t* ka;
+    uint64_t* kb;        // [nnz_cap] each
+    uint32_t* flag;      // [nnz_cap]
+    uint32_t* pos;       // [nnz_cap + 1]
+    uint32_t* used;      // [n_cap]
+    uint32_t* newid;     // [n_cap + 1]
+    uint32_t* deg;
+    uint32_t* k0;
+    uint32_t* v0;
+    uint32_t* k1;
+    uint32_t* v1;
+    uint32_t* rank;      // [n_cap] each
+    int32_t* lab0;
+    int32_t* lab1;       // [n_cap] each
+    uint32_t* rowptr;    // [n_cap + 1]
+    int32_t* col;        // [nnz_cap + kColPad]
+    void* ws;            // dsort / dscan workspace for max(nnz_cap, n_cap)
+};
+
+// Enqueues the canonicalisation of ctl->nnz pairs (the same CSR as
+// canonicalise_device makes) with kernels only; counts land in *ctl.
+// Returns the buffer that receives the labels (lab0 or lab1).
+int32_t* canonicalise_capacity(const CanonCap& c, hipStream_t s);
+
+// Buffers of a capturable wedge-kernel plan of a canonical L of up to
+// m_cap entries over up to n_cap rows (dense tiles and tiny items off).
+struct PlanCap {
+    const uint32_t* rowptr;  // [n_cap + 1]
+    const int32_t* col;      // [m_cap + kColPad]
+    int64_t m_cap;
+    int32_t n_cap;
+    uint32_t core_ids;       // C (0 = no hub-core bitmaps)
+    int cus;                 // compute units (items of about 64 per CU)
+    PipeCtl* ctl;
+    uint32_t* erow;
+    uint32_t* lflag;         // [m_cap] each
+    uint32_t* lpos;          // [m_cap + 1]
+    uint32_t* k0;
+    uint32_t* v0;
+    uint32_t* k1;
+    uint32_t* v1;            // [m_cap] each
+    uint32_t* off;           // [n_cap + 1]
+    uint32_t* cend;
+    uint32_t* has;           // [n_cap] each
+    uint32_t* rbi;           // [n_cap + 1]
+    uint4* lists;            // [m_cap]
+    uint32_t* rowbits;       // [nbm_cap * C / 32]
+    int32_t* ur;             // [m_cap]
+    uint64_t* cost;          // [m_cap]
+    uint64_t* P;             // [m_cap + 1]
+    uint32_t* start;         // [m_cap]
+    uint32_t* spos;          // [m_cap + 1]
+    size_t items_cap;        // m_cap (every work item opens at a distinct list)
+    uint32_t* first;         // [items_cap]
+    uint4* tmp_items;        // [2 * items_cap]
+    uint32_t* sk0;
+    uint32_t* sv0;
+    uint32_t* sk1;
+    uint32_t* sv1;           // [items_cap] each
+    uint4* items;            // [2 * items_cap]
+    void* ws;                // dsort / dscan workspace for max(m_cap, items_cap)
+};
+
+// Rows that can hold a hub-core bitmap: at least kCoreRowMin entries each.
+inline size_t pipe_bitmap_rows(int64_t m_cap, int32_t n_cap) {
+    return (size_t)std::min<int64_t>(n_cap, m_cap / kCoreRowMin + 1);
+}
+
+// Enqueues the plan (as plan_device with tau above every tile and no tiny
+// items; the item cut is about 64 items per CU with no upper bound on an
+// item's checks) with kernels only; counts land in *ctl.
+void plan_capacity(const PlanCap& c, hipStream_t s);
+
+// Checks the launch-shape fields of o (BTC_ERR_ARG on a bad value).
+void check_launch_opts(const btc_count_opts& o);
+
+// Enqueues the count of a plan_capacity plan: clears the work words at
+// d_work (kWorkWords), builds col16 over L, runs the wedge kernel on the
+// item counts in ctl and folds the total into d_work[0].
+void pipeline_count_enqueue(const PlanCap& c, uint16_t* col16, unsigned long long* d_work,
+                            const btc_count_opts& o, hipStream_t s);
+
 // Uploads L (rowptr narrowed to u32) to HBM on first use.
 void ensure_device(btc_graph_s* g, hipStream_t s);
 

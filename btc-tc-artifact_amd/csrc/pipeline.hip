@@ -1,0 +1,301 @@
+// btc_pipeline: COO pairs -> triangle total as one captured HIP graph.
+//
+// The graph holds the device canonicalisation (canonicalise_capacity), the
+// wedge-kernel planner (plan_capacity) and the count
+// (pipeline_count_enqueue), each built from kernels that read their live
+// sizes -- pairs, nnz(L), vertices, lists, items -- from a control block in
+// HBM (PipeCtl) that earlier kernels of the same graph fill.  Every buffer
+// is carved at creation from one allocation sized for n_cap ids and nnz_cap
+// pairs, so a run is: copy the pairs and two header words in, one
+// hipGraphLaunch, copy the control block and the total out.  The steps are
+// those of the reference's preprocess_for_triangle_counting
+// (btc/operations/convert.h:601-692) followed by planning and counting; the
+// reference runs them from the host for every graph.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
+#include "internal.h"
+#include "prim.h"
+
+struct btc_pipeline_s {
+    int32_t n_cap = 0;
+    int64_t nnz_cap = 0;
+    uint32_t flags = 0;
+    btc_count_opts opts{};
+    btcx::DevBuf<unsigned char> mem;
+    btcx::PipeCtl* ctl = nullptr;
+    unsigned long long* work = nullptr;
+    int32_t* src = nullptr;
+    int32_t* dst = nullptr;
+    uint16_t* col16 = nullptr;
+    int32_t* labels = nullptr;  // the canonicalisation's label buffer
+    btcx::CanonCap canon{};
+    btcx::PlanCap plan{};
+    btcx::CapturedGraph graph;
+    int32_t graph_nodes = 0;
+    btcx::PinnedBuf pin;  // [0, 4 KiB): control block and total; then staged pairs
+    btcx::PipeCtl last{};
+    bool ran = false;     // `last` holds an accepted run
+    std::mutex mu;        // one run at a time
+};
+
+namespace btcx {
+namespace {
+
+constexpr size_t kPinHeader = 4096;
+static_assert(sizeof(PipeCtl) + 8 <= kPinHeader, "control block fits the pinned header");
+
+// Bump layout over one allocation (base NULL: sizing pass).
+struct Carve {
+    unsigned char* base = nullptr;
+    size_t top = 0;
+    template <class T>
+    T* take(size_t count) {
+        const size_t at = top;
+        top += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~size_t(255);
+        return base ? reinterpret_cast<T*>(base + at) : nullptr;
+    }
+};
+
+int id_bits(int64_t n) {
+    int b = 1;
+    while (b < 31 && (1ll << b) < n) ++b;
+    return b;
+}
+
+void layout(btc_pipeline_s* p, Carve& c) {
+    const size_t E = (size_t)p->nnz_cap, N = (size_t)p->n_cap, M = E;
+    const uint32_t C = p->opts.core_ids == 0 ? kCoreIdsDefault : p->opts.core_ids < 0 ? 0u : (uint32_t)p->opts.core_ids;
+    p->ctl = c.take<PipeCtl>(1);
+    p->work = c.take<unsigned long long>(kWorkWords);
+    p->src = c.take<int32_t>(E);
+    p->dst = c.take<int32_t>(E);
+    CanonCap& k = p->canon;
+    k.src = p->src;
+    k.dst = p->dst;
+    k.nnz_cap = (int64_t)E;
+    k.n_cap = p->n_cap;
+    k.flags = p->flags;
+    k.ctl = p->ctl;
+    k.ka = c.take<uint64_t>(E);
+    k.kb = c.take<uint64_t>(E);
+    k.flag = c.take<uint32_t>(E);
+    k.pos = c.take<uint32_t>(E + 1);
+    k.used = c.take<uint32_t>(N);
+    k.newid = c.take<uint32_t>(N + 1);
+    k.deg = c.take<uint32_t>(N);
+    k.k0 = c.take<uint32_t>(N);
+    k.v0 = c.take<uint32_t>(N);
+    k.k1 = c.take<uint32_t>(N);
+    k.v1 = c.take<uint32_t>(N);
+    k.rank = c.take<uint32_t>(N);
+    k.lab0 = c.take<int32_t>(N);
+    k.lab1 = c.take<int32_t>(N);
+    k.rowptr = c.take<uint32_t>(N + 1);
+    k.col = c.take<int32_t>(E + kColPad);
+    PlanCap& q = p->plan;
+    q.rowptr = k.rowptr;
+    q.col = k.col;
+    q.m_cap = (int64_t)M;
+    q.n_cap = p->n_cap;
+    q.core_ids = C;
+    q.cus = device_cu_count();
+    q.ctl = p->ctl;
+    // the canonicalisation's keys and flags are dead once L is written:
+    // the planner's list keys and flags reuse them
+    q.k0 = reinterpret_cast<uint32_t*>(k.ka);
+    q.v0 = q.k0 ? q.k0 + M : nullptr;
+    q.k1 = reinterpret_cast<uint32_t*>(k.kb);
+    q.v1 = q.k1 ? q.k1 + M : nullptr;
+    q.lflag = k.flag;
+    q.lpos = k.pos;
+    q.erow = c.take<uint32_t>(M);
+    q.off = c.take<uint32_t>(N + 1);
+    q.cend = c.take<uint32_t>(N);
+    q.has = c.take<uint32_t>(N);
+    q.rbi = c.take<uint32_t>(N + 1);
+    q.lists = c.take<uint4>(M);
+    q.rowbits = c.take<uint32_t>(C ? pipe_bitmap_rows((int64_t)M, p->n_cap) * (C / 32) : 1);
+    q.ur = c.take<int32_t>(M);
+    q.cost = c.take<uint64_t>(M);
+    q.P = c.take<uint64_t>(M + 1);
+    q.start = c.take<uint32_t>(M);
+    q.spos = c.take<uint32_t>(M + 1);
+    q.items_cap = M;
+    q.first = c.take<uint32_t>(M);
+    q.tmp_items = c.take<uint4>(2 * M);
+    q.sk0 = c.take<uint32_t>(M);
+    q.sv0 = c.take<uint32_t>(M);
+    q.sk1 = c.take<uint32_t>(M);
+    q.sv1 = c.take<uint32_t>(M);
+    q.items = c.take<uint4>(2 * M);
+    p->col16 = c.take<uint16_t>(E + kColPad);
+    const size_t big = std::max(E, N);
+    void* ws = c.take<unsigned char>(std::max(dsort_ws_bytes(big), dscan_ws_bytes(big)));
+    k.ws = ws;
+    q.ws = ws;
+}
+
+// Carves the buffers and captures the graph.
+void pipeline_build(btc_pipeline_s* p, hipStream_t s) {
+    Carve sizing;
+    layout(p, sizing);
+    p->mem.alloc(sizing.top);
+    Carve real;
+    real.base = p->mem.p;
+    layout(p, real);
+    // the count's words start cleared; the captured kernels clear them again
+    BTC_HIP(hipMemsetAsync(p->work, 0, kWorkWords * 8, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    p->graph.capture([&](hipStream_t cs) {
+        p->labels = canonicalise_capacity(p->canon, cs);
+        plan_capacity(p->plan, cs);
+        pipeline_count_enqueue(p->plan, p->col16, p->work, p->opts, cs);
+    });
+    size_t nodes = 0;
+    BTC_HIP(hipGraphGetNodes(p->graph.graph, nullptr, &nodes));
+    p->graph_nodes = (int32_t)nodes;
+    p->pin.reserve(kPinHeader);
+}
+
+void pipeline_run(btc_pipeline_s* p, const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n,
+                  bool device_pairs, hipStream_t s, uint64_t* total, float* kernel_ms) {
+    BTC_REQUIRE(nnz >= 0 && nnz <= p->nnz_cap, BTC_ERR_ARG, "%lld pairs (pipeline capacity %lld)", (long long)nnz,
+                (long long)p->nnz_cap);
+    BTC_REQUIRE(n <= p->n_cap, BTC_ERR_ARG, "n = %d exceeds the pipeline's %d ids", n, p->n_cap);
+    BTC_REQUIRE(nnz == 0 || (src && dst), BTC_ERR_ARG, "src/dst is NULL");
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->ran = false;
+    const size_t pair_bytes = (size_t)nnz * 4;
+    char* pin = p->pin.reserve(kPinHeader + (device_pairs ? 0 : 2 * pair_bytes));
+    PipeCtl* h = reinterpret_cast<PipeCtl*>(pin);
+    if (nnz) {
+        if (device_pairs) {
+            BTC_HIP(hipMemcpyAsync(p->src, src, pair_bytes, hipMemcpyDeviceToDevice, s));
+            BTC_HIP(hipMemcpyAsync(p->dst, dst, pair_bytes, hipMemcpyDeviceToDevice, s));
+        } else {
+            std::memcpy(pin + kPinHeader, src, pair_bytes);
+            std::memcpy(pin + kPinHeader + pair_bytes, dst, pair_bytes);
+            BTC_HIP(hipMemcpyAsync(p->src, pin + kPinHeader, pair_bytes, hipMemcpyHostToDevice, s));
+            BTC_HIP(hipMemcpyAsync(p->dst, pin + kPinHeader + pair_bytes, pair_bytes, hipMemcpyHostToDevice, s));
+        }
+    }
+    std::memset(h, 0, sizeof(PipeCtl));
+    h->nnz = (uint32_t)nnz;
+    h->n_in = n;
+    BTC_HIP(hipMemcpyAsync(p->ctl, h, 8, hipMemcpyHostToDevice, s));
+    EventTimer tm(kernel_ms != nullptr);
+    tm.start(s);
+    p->graph.launch(s);
+    tm.stop(s);
+    unsigned long long* htotal = reinterpret_cast<unsigned long long*>(pin + sizeof(PipeCtl));
+    BTC_HIP(hipMemcpyAsync(h, p->ctl, sizeof(PipeCtl), hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipMemcpyAsync(htotal, p->work, 8, hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    const PipeCtl c = *h;
+    BTC_REQUIRE(!(c.status & kPipeNegativeId), BTC_ERR_ARG, "negative vertex id %d", c.lo);
+    BTC_REQUIRE(!(c.status & kPipeIdRange), BTC_ERR_ARG, "vertex id %d >= n = %d", c.hi, n);
+    BTC_REQUIRE(!(c.status & kPipeTooManyIds), BTC_ERR_ARG, "vertex id %d: more than the pipeline's %d ids", c.hi,
+                p->n_cap);
+    if (kernel_ms) *kernel_ms = tm.ms();
+    *total = *htotal;
+    p->last = c;
+    p->ran = true;
+}
+
+btc_graph_s* pipeline_graph(btc_pipeline_s* p, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    BTC_REQUIRE(p->ran, BTC_ERR_STATE, "no accepted run yet");
+    const PipeCtl& c = p->last;
+    auto g = std::make_unique<btc_graph_s>();
+    g->n = (int32_t)c.nc;
+    g->n_input = (int32_t)c.n;
+    g->m = c.m;
+    g->flags = p->flags;
+    g->max_row = (int32_t)c.rs[0];
+    g->wedges = (int64_t)c.rs[1];
+    g->merge_work = (int64_t)c.rs[2];
+    g->on_host = false;
+    g->d_rowptr.alloc((size_t)g->n + 1);
+    g->d_col.alloc((size_t)g->m + kColPad);
+    g->d_labels.alloc((size_t)std::max(g->n, 1));
+    BTC_HIP(hipMemcpyAsync(g->d_rowptr.p, p->canon.rowptr, ((size_t)g->n + 1) * 4, hipMemcpyDeviceToDevice, s));
+    BTC_HIP(hipMemcpyAsync(g->d_col.p, p->canon.col, ((size_t)g->m + kColPad) * 4, hipMemcpyDeviceToDevice, s));
+    if (g->n) BTC_HIP(hipMemcpyAsync(g->d_labels.p, p->labels, (size_t)g->n * 4, hipMemcpyDeviceToDevice, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    return g.release();
+}
+
+}  // namespace
+}  // namespace btcx
+
+extern "C" {
+
+int btc_pipeline_create(int32_t n_cap, int64_t nnz_cap, uint32_t flags, const btc_count_opts* opts, void* stream,
+                        btc_pipeline* out) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(out, BTC_ERR_ARG, "out is NULL");
+        *out = nullptr;
+        BTC_REQUIRE(n_cap >= 1, BTC_ERR_ARG, "n_cap %d < 1", n_cap);
+        BTC_REQUIRE(nnz_cap >= 1 && nnz_cap + (int64_t)btcx::kColPad < (int64_t)0xFFFFFFFFLL, BTC_ERR_ARG,
+                    "nnz_cap %lld (expected 1 .. 2^32 - %zu)", (long long)nnz_cap, btcx::kColPad + 1);
+        BTC_REQUIRE((flags & ~(BTC_CANON_DEFAULT | BTC_CANON_DEVICE)) == 0, BTC_ERR_ARG, "unknown flags 0x%x",
+                    flags);
+        auto p = std::make_unique<btc_pipeline_s>();
+        p->n_cap = n_cap;
+        p->nnz_cap = nnz_cap;
+        p->flags = flags & BTC_CANON_DEFAULT;
+        btc_count_opts_default(&p->opts);
+        if (opts) p->opts = *opts;
+        BTC_REQUIRE(p->opts.core_ids <= 0 ||
+                        (p->opts.core_ids % 128 == 0 && (uint32_t)p->opts.core_ids <= btcx::kCoreIdsMax),
+                    BTC_ERR_ARG, "core_ids %d (expected a multiple of 128 up to %u, 0 = default, < 0 = none)",
+                    p->opts.core_ids, btcx::kCoreIdsMax);
+        btcx::check_launch_opts(p->opts);
+        btcx::pipeline_build(p.get(), btcx::as_stream(stream));
+        *out = p.release();
+    });
+}
+
+int btc_pipeline_count(btc_pipeline p, const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n,
+                       int32_t device_pairs, void* stream, uint64_t* total, float* kernel_ms) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(p && total, BTC_ERR_ARG, "NULL argument");
+        btcx::pipeline_run(p, src, dst, nnz, n, device_pairs != 0, btcx::as_stream(stream), total, kernel_ms);
+    });
+}
+
+int btc_pipeline_graph(btc_pipeline p, void* stream, btc_graph* out) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(p && out, BTC_ERR_ARG, "NULL argument");
+        *out = nullptr;
+        *out = btcx::pipeline_graph(p, btcx::as_stream(stream));
+    });
+}
+
+int btc_pipeline_get_info(btc_pipeline p, btc_pipeline_info* info) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(p && info, BTC_ERR_ARG, "NULL argument");
+        std::lock_guard<std::mutex> lk(p->mu);
+        std::memset(info, 0, sizeof(*info));
+        info->n_cap = p->n_cap;
+        info->nnz_cap = p->nnz_cap;
+        info->flags = p->flags;
+        info->graph_nodes = p->graph_nodes;
+        info->device_bytes = (int64_t)p->mem.bytes();
+        if (p->ran) {
+            const btcx::PipeCtl& c = p->last;
+            info->n = (int32_t)c.nc;
+            info->m = c.m;
+            info->max_row = (int32_t)c.rs[0];
+            info->wedges = (int64_t)c.rs[1];
+            info->wedge_checks = (int64_t)c.checks;
+            info->work_items = (int32_t)c.nitems;
+        }
+    });
+}
+
+void btc_pipeline_free(btc_pipeline p) { delete p; }
+
+}  // extern "C"

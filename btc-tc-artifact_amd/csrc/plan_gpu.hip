@@ -61,7 +61,8 @@ __device__ __forceinline__ uint32_t upper_bound_dev(const T* a, uint32_t lo, uin
 }
 
 // erow[e] = row of entry e: one wave per row, lanes stride over its entries.
-__global__ void edge_rows_kernel(const uint32_t* __restrict__ rowptr, int32_t n, uint32_t* __restrict__ erow) {
+__global__ void edge_rows_kernel(const uint32_t* __restrict__ rowptr, Cnt n_, uint32_t* __restrict__ erow) {
+    const int64_t n = (int64_t)n_.get();
     const int64_t waves = ((int64_t)gridDim.x * kT) >> 6;
     const int lane = threadIdx.x & 63;
     for (int64_t i = (blockIdx.x * (int64_t)kT + threadIdx.x) >> 6; i < n; i += waves)
@@ -81,8 +82,9 @@ __global__ void dense_tiles_kernel(const uint32_t* __restrict__ pop, uint32_t nb
     if ((threadIdx.x & 63) == 0 && sum) atomicAdd(dense_edges, (unsigned long long)sum);
 }
 
-__global__ void select_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int64_t n,
+__global__ void select_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, Cnt n_,
                               uint32_t* __restrict__ out) {
+    const int64_t n = (int64_t)n_.get();
     GS_LOOP(t, n) if (flag[t]) out[pos[t]] = (uint32_t)t;
 }
 
@@ -112,8 +114,9 @@ __device__ __forceinline__ bool clip(const uint32_t* __restrict__ rowptr, const 
 
 // Pass over the edges: list flag per edge, sparse edge count and merge work.
 __global__ void clip_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                            const uint32_t* __restrict__ erow, const uint8_t* __restrict__ dense, int64_t m,
+                            const uint32_t* __restrict__ erow, const uint8_t* __restrict__ dense, Cnt m_,
                             uint32_t* __restrict__ flag, unsigned long long* __restrict__ stats) {
+    const int64_t m = (int64_t)m_.get();
     uint64_t se = 0, sw = 0;
     GS_LOOP(q, m) {
         uint32_t f = 0;
@@ -135,8 +138,9 @@ __global__ void clip_kernel(const uint32_t* __restrict__ rowptr, const int32_t* 
 }
 
 __global__ void list_keys_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
-                                 const int32_t* __restrict__ col, int64_t m, uint32_t* __restrict__ key,
+                                 const int32_t* __restrict__ col, Cnt m_, uint32_t* __restrict__ key,
                                  uint32_t* __restrict__ val) {
+    const int64_t m = (int64_t)m_.get();
     GS_LOOP(q, m) if (flag[q]) {
         key[pos[q]] = (uint32_t)col[q];
         val[pos[q]] = (uint32_t)q;
@@ -144,14 +148,17 @@ __global__ void list_keys_kernel(const uint32_t* __restrict__ flag, const uint32
 }
 
 // off[j] = first list of pivot j (j = 0..n) in the pivot-sorted keys.
-__global__ void pivot_offsets_kernel(const uint32_t* __restrict__ key, uint32_t ne, int32_t n,
+__global__ void pivot_offsets_kernel(const uint32_t* __restrict__ key, Cnt ne_, Cnt n_,
                                      uint32_t* __restrict__ off) {
+    const uint32_t ne = (uint32_t)ne_.get();
+    const int64_t n = (int64_t)n_.get();
     GS_LOOP(j, (int64_t)n + 1) off[j] = lower_bound_dev(key, 0u, ne, (uint32_t)j);
 }
 
 // Core ids (< C) per row; rows with at least kCoreRowMin get a bitmap.
-__global__ void core_rows_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int32_t n,
+__global__ void core_rows_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col, Cnt n_,
                                  uint32_t C, uint32_t* __restrict__ cend, uint32_t* __restrict__ has) {
+    const int64_t n = (int64_t)n_.get();
     GS_LOOP(i, n) {
         const uint32_t r0 = rowptr[i];
         const uint32_t c = lower_bound_dev(col, r0, rowptr[i + 1], (int32_t)C);
@@ -162,8 +169,9 @@ __global__ void core_rows_kernel(const uint32_t* __restrict__ rowptr, const int3
 
 __global__ void core_fill_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                  const uint32_t* __restrict__ cend, const uint32_t* __restrict__ has,
-                                 const uint32_t* __restrict__ rbi, int32_t n, uint32_t CW,
+                                 const uint32_t* __restrict__ rbi, Cnt n_, uint32_t CW,
                                  uint32_t* __restrict__ rowbits) {
+    const int64_t n = (int64_t)n_.get();
     const int64_t waves = ((int64_t)gridDim.x * kT) >> 6;
     const int lane = threadIdx.x & 63;
     for (int64_t i = (blockIdx.x * (int64_t)kT + threadIdx.x) >> 6; i < n; i += waves) {
@@ -179,11 +187,12 @@ __global__ void core_fill_kernel(const uint32_t* __restrict__ rowptr, const int3
 // List record of every pivot-sorted list e (see the wedge kernel's
 // scan_lists), its row, and its cost (ids looked up + core words ANDed).
 __global__ void records_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                               const uint32_t* __restrict__ erow, const uint32_t* __restrict__ val, uint32_t ne,
+                               const uint32_t* __restrict__ erow, const uint32_t* __restrict__ val, Cnt ne_,
                                const uint32_t* __restrict__ cend, const uint32_t* __restrict__ has,
                                const uint32_t* __restrict__ rbi, uint32_t C, uint4* __restrict__ lst,
                                int32_t* __restrict__ ur, uint64_t* __restrict__ cost,
                                unsigned long long* __restrict__ ncore, unsigned long long* __restrict__ nnarrow) {
+    const uint32_t ne = (uint32_t)ne_.get();
     uint64_t nc = 0, nn = 0;
     GS_LOOP(e, ne) {
         const uint32_t q = val[e], i = erow[q], j = (uint32_t)col[q];
@@ -224,8 +233,10 @@ __device__ __forceinline__ bool pivot_small(const uint32_t* __restrict__ rowptr,
 // pivots) the running cost crosses a multiple of chunk.
 __global__ void item_start_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                   const uint32_t* __restrict__ key, const uint32_t* __restrict__ off,
-                                  const uint64_t* __restrict__ P, uint32_t ne, uint64_t chunk,
-                                  uint32_t* __restrict__ start) {
+                                  const uint64_t* __restrict__ P, Cnt ne_, uint64_t chunk_,
+                                  const unsigned long long* __restrict__ d_chunk, uint32_t* __restrict__ start) {
+    const uint32_t ne = (uint32_t)ne_.get();
+    const uint64_t chunk = d_chunk ? (uint64_t)*d_chunk : chunk_;
     GS_LOOP(e, ne) {
         const uint32_t j = key[e], eb = off[j];
         uint32_t st = e == eb;
@@ -238,10 +249,12 @@ __global__ void item_start_kernel(const uint32_t* __restrict__ rowptr, const int
 // before wave items before tiny items, each by descending work.
 __global__ void items_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                              const uint32_t* __restrict__ key, const uint32_t* __restrict__ off,
-                             const uint64_t* __restrict__ P, const uint32_t* __restrict__ first, uint32_t nitems,
-                             uint32_t ne, bool use_tiny, uint4* __restrict__ tmp_items,
+                             const uint64_t* __restrict__ P, const uint32_t* __restrict__ first, Cnt nitems_,
+                             Cnt ne_, bool use_tiny, uint4* __restrict__ tmp_items,
                              uint32_t* __restrict__ skey, uint32_t* __restrict__ sval,
                              unsigned long long* __restrict__ stats) {
+    const uint32_t nitems = (uint32_t)nitems_.get();
+    const uint32_t ne = (uint32_t)ne_.get();
     uint64_t words = 0, small_items = 0, tiny_items = 0;
     GS_LOOP(k, nitems) {
         const uint32_t eb = first[k], ee = k + 1 < nitems ? first[k + 1] : ne;
@@ -272,11 +285,35 @@ __global__ void items_kernel(const uint32_t* __restrict__ rowptr, const int32_t*
 }
 
 __global__ void place_items_kernel(const uint4* __restrict__ tmp_items, const uint32_t* __restrict__ order,
-                                   uint32_t nitems, uint4* __restrict__ items) {
+                                   Cnt nitems_, uint4* __restrict__ items) {
+    const uint32_t nitems = (uint32_t)nitems_.get();
     GS_LOOP(r, nitems) {
         const uint32_t k = order[r];
         items[2 * r] = tmp_items[2 * k];
         items[2 * r + 1] = tmp_items[2 * k + 1];
+    }
+}
+
+// ------------------------------------------------- capturable form kernels
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, Cnt n_, uint32_t v) {
+    const int64_t n = n_.get();
+    GS_LOOP(i, n) p[i] = v;
+}
+
+__global__ void pipe_bitmap_words_kernel(PipeCtl* __restrict__ c, uint32_t CW) {
+    if (threadIdx.x == 0) c->nbm_words = c->nbm * CW;
+}
+
+// About 64 items per CU: chunk = max(8192, checks / (64 CUs)).
+__global__ void pipe_chunk_kernel(PipeCtl* __restrict__ c, uint64_t items) {
+    if (threadIdx.x == 0) c->chunk = max(8192ull, (c->checks + items - 1) / items);
+}
+
+// Wave items (st[5]) follow the workgroup items; there are no tiny items.
+__global__ void pipe_items_kernel(PipeCtl* __restrict__ c) {
+    if (threadIdx.x == 0) {
+        c->item_counts[0] = c->nitems - (uint32_t)c->st[5];
+        c->item_counts[1] = c->nitems;
     }
 }
 
@@ -388,7 +425,7 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     TmpBuf<uint32_t> start, spos;
     start.alloc(ne, s);
     spos.alloc((size_t)ne + 1, s);
-    if (ne) BTC_LAUNCH(item_start_kernel, ne, s, rowptr, col, key, off.p, P.p, ne, (uint64_t)chunk, start.p);
+    if (ne) BTC_LAUNCH(item_start_kernel, ne, s, rowptr, col, key, off.p, P.p, ne, (uint64_t)chunk, nullptr, start.p);
     exclusive_scan_u32(start.p, spos.p, ne, s);
     const uint32_t nitems = read_dev(spos.p + ne, s);
     TmpBuf<uint32_t> first, sk0, sv0, sk1, sv1;
@@ -432,6 +469,71 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     // rows (4 B), list records (16 B), items (32 B)
     p->info.stream_bytes = 4 * ((int64_t)checks + (int64_t)h[4]) - 2 * (int64_t)h[6] + 16 * (int64_t)ne +
                            32 * (int64_t)nitems;
+}
+
+namespace {
+
+unsigned cap_grid(int64_t cap) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + kT - 1) / kT, 2048));
+}
+
+}  // namespace
+
+#define CAP_LAUNCH(kernel, cap, s, ...)                                                \
+    do {                                                                               \
+        hipLaunchKernelGGL(kernel, dim3(cap_grid(cap)), dim3(kT), 0, s, __VA_ARGS__);  \
+        BTC_CHECK_LAUNCH(#kernel);                                                     \
+    } while (0)
+
+#define ONE_LAUNCH(kernel, s, ...)                                          \
+    do {                                                                    \
+        hipLaunchKernelGGL(kernel, dim3(1), dim3(64), 0, s, __VA_ARGS__);   \
+        BTC_CHECK_LAUNCH(#kernel);                                          \
+    } while (0)
+
+void plan_capacity(const PlanCap& c, hipStream_t s) {
+    PipeCtl* ctl = c.ctl;
+    const int64_t M = c.m_cap, N = c.n_cap, I = (int64_t)c.items_cap;
+    const Cnt m = Cnt::dev(&ctl->m), nc = Cnt::dev(&ctl->nc), ne = Cnt::dev(&ctl->ne),
+              nitems = Cnt::dev(&ctl->nitems);
+    const uint32_t* rowptr = c.rowptr;
+    const int32_t* col = c.col;
+    unsigned long long* st = ctl->st;  // cleared with the rest of *ctl
+
+    CAP_LAUNCH(edge_rows_kernel, N * 64, s, rowptr, nc, c.erow);
+    // prefix lists: flag, compact (pivot, edge), stable sort by pivot
+    CAP_LAUNCH(clip_kernel, M, s, rowptr, col, c.erow, nullptr, m, c.lflag, st + 1);
+    dscan_u32(c.lflag, c.lpos, (size_t)M, &ctl->m, &ctl->ne, c.ws, s);
+    CAP_LAUNCH(list_keys_kernel, M, s, c.lflag, c.lpos, col, m, c.k0, c.v0);
+    const bool alt = dsort_pairs_u32(c.k0, c.v0, c.k1, c.v1, (size_t)M, &ctl->ne, id_bits(N), c.ws, s);
+    const uint32_t* key = alt ? c.k1 : c.k0;
+    const uint32_t* val = alt ? c.v1 : c.v0;
+    CAP_LAUNCH(pivot_offsets_kernel, N + 1, s, key, ne, nc, c.off);
+
+    // hub-core bitmaps
+    const uint32_t C = c.core_ids;
+    CAP_LAUNCH(core_rows_kernel, N, s, rowptr, col, nc, C, c.cend, c.has);
+    dscan_u32(c.has, c.rbi, (size_t)N, &ctl->nc, &ctl->nbm, c.ws, s);
+    if (C) {
+        ONE_LAUNCH(pipe_bitmap_words_kernel, s, ctl, C / 32);
+        CAP_LAUNCH(fill_u32_kernel, (int64_t)pipe_bitmap_rows(M, c.n_cap) * (C / 32), s, c.rowbits,
+                   Cnt::dev(&ctl->nbm_words), 0u);
+        CAP_LAUNCH(core_fill_kernel, N * 64, s, rowptr, col, c.cend, c.has, c.rbi, nc, C / 32, c.rowbits);
+    }
+    CAP_LAUNCH(records_kernel, M, s, rowptr, col, c.erow, val, ne, c.cend, c.has, c.rbi, C, c.lists, c.ur, c.cost,
+               st + 3, st + 6);
+    dscan_u64(c.cost, c.P, (size_t)M, &ctl->ne, reinterpret_cast<uint64_t*>(&ctl->checks), c.ws, s);
+
+    // work items (each opens at a list, so there are at most ne of them)
+    ONE_LAUNCH(pipe_chunk_kernel, s, ctl, (uint64_t)64 * (uint64_t)c.cus);
+    CAP_LAUNCH(item_start_kernel, M, s, rowptr, col, key, c.off, c.P, ne, (uint64_t)0, &ctl->chunk, c.start);
+    dscan_u32(c.start, c.spos, (size_t)M, &ctl->ne, &ctl->nitems, c.ws, s);
+    CAP_LAUNCH(select_kernel, M, s, c.start, c.spos, ne, c.first);
+    CAP_LAUNCH(items_kernel, I, s, rowptr, col, key, c.off, c.P, c.first, nitems, ne, false, c.tmp_items, c.sk0,
+               c.sv0, st + 4);
+    const bool ialt = dsort_pairs_u32(c.sk0, c.sv0, c.sk1, c.sv1, (size_t)I, &ctl->nitems, 32, c.ws, s);
+    CAP_LAUNCH(place_items_kernel, I, s, c.tmp_items, ialt ? c.sv1 : c.sv0, nitems, c.items);
+    ONE_LAUNCH(pipe_items_kernel, s, ctl);
 }
 
 }  // namespace btcx

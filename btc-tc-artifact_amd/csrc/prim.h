@@ -25,4 +25,26 @@ bool sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* kt, uint32_t* vt, 
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, hipStream_t s);
 void exclusive_scan_u64(const uint64_t* in, uint64_t* out, size_t n, hipStream_t s);
 
+// Capturable forms (devprim.hip).  The element count n is read from device
+// memory (*d_n) when the kernels run, buffers hold `cap` elements (n <= cap),
+// and the caller's workspace `ws` (dsort_ws_bytes / dscan_ws_bytes of cap,
+// 256-byte aligned) holds every temporary: the kernels are the only work
+// enqueued, so one stream capture serves every n up to cap.
+
+size_t dsort_ws_bytes(size_t cap);
+size_t dscan_ws_bytes(size_t cap);
+
+// out[i] = sum of in[0, i) for i <= n; out[n] and *d_total (when set) get
+// the total.
+void dscan_u32(const uint32_t* in, uint32_t* out, size_t cap, const uint32_t* d_n, uint32_t* d_total, void* ws,
+               hipStream_t s);
+void dscan_u64(const uint64_t* in, uint64_t* out, size_t cap, const uint32_t* d_n, uint64_t* d_total, void* ws,
+               hipStream_t s);
+
+// Stable radix sorts on bits [0, end_bit), as sort_u64 / sort_pairs_u32.
+uint64_t* dsort_u64(uint64_t* keys, uint64_t* tmp, size_t cap, const uint32_t* d_n, int end_bit, void* ws,
+                    hipStream_t s);
+bool dsort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* kt, uint32_t* vt, size_t cap, const uint32_t* d_n,
+                     int end_bit, void* ws, hipStream_t s);
+
 }  // namespace btcx

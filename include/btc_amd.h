@@ -364,6 +364,55 @@ int btc_batch_autotune(btc_batch b, const btc_tune_opts* topts, void* stream, bt
 int btc_batch_capture(btc_batch b, void* stream);
 void btc_batch_free(btc_batch b);
 
+/* ------------------------------------------------------------- pipeline
+ * The whole path from COO pairs to the triangle total -- the device
+ * canonicalisation (as BTC_CANON_DEVICE), the wedge-kernel plan and the
+ * count -- captured once into a HIP graph for inputs of up to n_cap ids and
+ * nnz_cap pairs, then replayed for each input with one hipGraphLaunch:
+ * every kernel reads the live sizes (pairs, nnz(L), vertices, lists, work
+ * items) from device memory, so a run makes no allocation and no host
+ * round trip between its steps.  Buffers (about 180 bytes per pair and
+ * 60 per id of capacity) are allocated once, at creation.  The plan has no
+ * dense tiles and no tiny items (the wedge kernel counts every edge), cut
+ * into about 64 work items per CU; of opts, group_lanes, unroll, lookup,
+ * schedule, waves_per_eu, core_ids and wide_offsets apply.  Counts equal
+ * btc_count on btc_graph_from_coo_device's graph of the same pairs.  No
+ * reference counterpart: the reference runs preprocess_for_triangle_counting
+ * (convert.h:601-692), the Bit-BSR build (bcsr_16x128.h:137-241) and the
+ * count from the host for every graph. */
+typedef struct btc_pipeline_s* btc_pipeline;
+
+typedef struct {
+    int32_t n_cap;
+    int32_t graph_nodes;  /* nodes of the captured HIP graph */
+    int64_t nnz_cap;
+    int64_t device_bytes; /* buffers held for the capacity */
+    uint32_t flags;       /* BTC_CANON_* of the canonicalisation */
+    /* the last accepted run */
+    int32_t n;            /* vertices of L */
+    int64_t m;            /* nnz(L) */
+    int64_t wedges;       /* sum_i d_L(i)(d_L(i)-1)/2 */
+    int64_t wedge_checks; /* ids looked up + core words ANDed by the plan */
+    int32_t work_items;
+    int32_t max_row;
+} btc_pipeline_info;
+
+int btc_pipeline_create(int32_t n_cap, int64_t nnz_cap, uint32_t flags, const btc_count_opts* opts, void* stream,
+                        btc_pipeline* out);
+/* Triangles of the graph of nnz pairs (src[e], dst[e]) over ids [0, n)
+ * (n <= 0: the largest id + 1): host arrays, or HBM when device_pairs is 1.
+ * An id out of range, nnz > nnz_cap or more than n_cap ids returns
+ * BTC_ERR_ARG, as btc_graph_from_coo_device would.  Runs on one pipeline
+ * are serialised; kernel_ms (may be NULL) times the replay. */
+int btc_pipeline_count(btc_pipeline p, const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n,
+                       int32_t device_pairs, void* stream, uint64_t* total, float* kernel_ms);
+/* A copy of the last accepted run's canonical graph, in HBM: the same L and
+ * labels as btc_graph_from_coo_device of the same pairs. */
+int btc_pipeline_graph(btc_pipeline p, void* stream, btc_graph* out);
+int btc_pipeline_get_info(btc_pipeline p, btc_pipeline_info* info);
+void btc_pipeline_free(btc_pipeline p);
+
+
 /* -------------------------------------------------------------- serving
  * A request loop with continuous batching.  btc_server_submit queues an
  * exact-count query for a graph identified by the caller's key; a graph not
