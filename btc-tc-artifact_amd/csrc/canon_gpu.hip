@@ -33,6 +33,7 @@ unsigned grid_for(int64_t n) {
 
 __global__ void id_range_kernel(const int32_t* __restrict__ src, const int32_t* __restrict__ dst, Cnt nnz_,
                                 int32_t* __restrict__ lohi) {
+    __shared__ int32_t s_lo[kT / 64], s_hi[kT / 64];
     const int64_t nnz = nnz_.get();
     int32_t lo = INT_MAX, hi = -1;
     for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * kT) {
@@ -45,8 +46,17 @@ __global__ void id_range_kernel(const int32_t* __restrict__ src, const int32_t* 
         hi = max(hi, __shfl_xor(hi, o));
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicMin(&lohi[0], lo);
-        atomicMax(&lohi[1], hi);
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kT / 64; ++w) {
+            lo = min(lo, s_lo[w]);
+            hi = max(hi, s_hi[w]);
+        }
+        if (lo != INT_MAX) atomicMin(&lohi[0], lo);
+        if (hi != -1) atomicMax(&lohi[1], hi);
     }
 }
 
@@ -117,14 +127,45 @@ __global__ void relabel_kernel(uint64_t* __restrict__ keys, Cnt m_, int b, const
     }
 }
 
-__global__ void degree_kernel(const uint64_t* __restrict__ keys, Cnt m_, int b, uint32_t* __restrict__ deg) {
+// deg[v] += entries of v in the ascending keys [0, m) (row r = k >> b,
+// column c = k & mask).  Rows come in runs, so a wave adds each run of its
+// 64 keys with one atomic; columns of ids below kDegWindow (where hub
+// vertices of skewed graphs concentrate) are counted in LDS first, so a hub
+// gets one global atomic per workgroup rather than one per edge.
+constexpr uint32_t kDegWindow = 8192;
+
+__global__ __launch_bounds__(kT) void degree_kernel(const uint64_t* __restrict__ keys, Cnt m_, int b,
+                                                    uint32_t* __restrict__ deg) {
+    __shared__ uint32_t h[kDegWindow];
+    for (uint32_t i = threadIdx.x; i < kDegWindow; i += kT) h[i] = 0;
+    __syncthreads();
     const int64_t m = m_.get();
     const uint64_t mask = (1ull << b) - 1;
-    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT) {
-        const uint64_t k = keys[e];
-        atomicAdd(&deg[k >> b], 1u);
-        atomicAdd(&deg[k & mask], 1u);
+    const uint32_t lane = threadIdx.x & 63;
+    for (int64_t base = blockIdx.x * (int64_t)kT + (threadIdx.x & ~63u); base < m; base += (int64_t)gridDim.x * kT) {
+        const int64_t e = base + lane;
+        const bool ok = e < m;
+        const uint64_t k = ok ? keys[e] : ~0ull;
+        const uint32_t r = (uint32_t)(k >> b);
+        const uint32_t prev = __shfl_up(r, 1), next = __shfl_down(r, 1);
+        const bool first = lane == 0 || prev != r;
+        const bool last = lane == 63 || next != r;
+        const uint64_t starts = __ballot(first);
+        const uint32_t run0 = 63 - __clzll(starts & ((2ull << lane) - 1));  // first lane of my run
+        if (ok && last) {
+            const uint32_t cnt = lane - run0 + 1;
+            if (r < kDegWindow) atomicAdd(&h[r], cnt);
+            else atomicAdd(&deg[r], cnt);
+        }
+        if (ok) {
+            const uint32_t c = (uint32_t)(k & mask);
+            if (c < kDegWindow) atomicAdd(&h[c], 1u);
+            else atomicAdd(&deg[c], 1u);
+        }
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kDegWindow; i += kT)
+        if (h[i]) atomicAdd(&deg[i], h[i]);
 }
 
 // key = top - degree (ascending = descending degree; top is at least the
@@ -173,12 +214,17 @@ __global__ void rank_scatter_kernel(const uint32_t* __restrict__ order, Cnt n_, 
     }
 }
 
-// rowptr[r] = first e with keys[e] >= r << b (r = 0..n)
-__global__ void rowptr_kernel(const uint64_t* __restrict__ keys, Cnt m_, int b, Cnt n_,
-                              uint32_t* __restrict__ rowptr) {
+// L from the ascending keys: rowptr[r] = first e with keys[e] >= r << b
+// (r = 0..n), col[e] = the key's low b bits, col[m, m + kColPad) = 0; col16
+// (when set) gets the low 16 bits of col over the same m + kColPad entries.
+__global__ void csr_kernel(const uint64_t* __restrict__ keys, Cnt m_, int b, Cnt n_, uint32_t* __restrict__ rowptr,
+                           int32_t* __restrict__ col, uint16_t* __restrict__ col16) {
     const int64_t m = m_.get();
     const int64_t n = n_.get();
-    for (int64_t r = blockIdx.x * (int64_t)kT + threadIdx.x; r <= n; r += (int64_t)gridDim.x * kT) {
+    const uint64_t mask = (1ull << b) - 1;
+    const int64_t stride = (int64_t)gridDim.x * kT;
+    const int64_t t0 = blockIdx.x * (int64_t)kT + threadIdx.x;
+    for (int64_t r = t0; r <= n; r += stride) {
         const uint64_t t = (uint64_t)r << b;
         int64_t lo = 0, hi = m;
         while (lo < hi) {
@@ -188,47 +234,45 @@ __global__ void rowptr_kernel(const uint64_t* __restrict__ keys, Cnt m_, int b, 
         }
         rowptr[r] = (uint32_t)lo;
     }
-}
-
-__global__ void col_kernel(const uint64_t* __restrict__ keys, Cnt m_, int b, int32_t* __restrict__ col) {
-    const int64_t m = m_.get();
-    const uint64_t mask = (1ull << b) - 1;
-    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT)
-        col[e] = (int32_t)(keys[e] & mask);
+    for (int64_t e = t0; e < m + (int64_t)kColPad; e += stride) {
+        const int32_t c = e < m ? (int32_t)(keys[e] & mask) : 0;
+        col[e] = c;
+        if (col16) col16[e] = (uint16_t)c;
+    }
 }
 
 // out: [0] longest row, [1] sum C(d, 2), [2] sum over edges of d(i) + d(j)
-__global__ void row_stats_kernel(const uint32_t* __restrict__ rowptr, Cnt n_,
-                                 unsigned long long* __restrict__ out) {
+// (rows [0, n), edges [0, m) of L; one atomic per workgroup and word)
+__global__ __launch_bounds__(kT) void stats_kernel(const uint32_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col, Cnt n_, Cnt m_,
+                                                   unsigned long long* __restrict__ out) {
+    __shared__ uint64_t lds[kT / 64];
     const int64_t n = n_.get();
+    const int64_t m = m_.get();
+    const int64_t stride = (int64_t)gridDim.x * kT;
     uint32_t mx = 0;
     uint64_t w = 0, work = 0;
-    for (int64_t i = blockIdx.x * (int64_t)kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    for (int64_t i = blockIdx.x * (int64_t)kT + threadIdx.x; i < n; i += stride) {
         const uint64_t d = rowptr[i + 1] - rowptr[i];
         mx = max(mx, (uint32_t)d);
         w += d * (d - (d > 0)) / 2;
         work += d * d;
     }
-    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
-    w = wave_sum_u64(w);
-    work = wave_sum_u64(work);
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(&out[0], (unsigned long long)mx);
-        atomicAdd(&out[1], (unsigned long long)w);
-        atomicAdd(&out[2], (unsigned long long)work);
-    }
-}
-
-__global__ void edge_stats_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col, Cnt m_,
-                                  unsigned long long* __restrict__ out) {
-    const int64_t m = m_.get();
-    uint64_t work = 0;
-    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += (int64_t)gridDim.x * kT) {
+    for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < m; e += stride) {
         const int32_t j = col[e];
         work += rowptr[j + 1] - rowptr[j];
     }
-    work = wave_sum_u64(work);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&out[2], (unsigned long long)work);
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+    __shared__ uint32_t s_mx[kT / 64];
+    if ((threadIdx.x & 63) == 0) s_mx[threadIdx.x >> 6] = mx;
+    w = block_sum_u64<kT>(w, lds);
+    work = block_sum_u64<kT>(work, lds);
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kT / 64; ++k) mx = max(mx, s_mx[k]);
+        if (mx) atomicMax(&out[0], (unsigned long long)mx);
+        if (w) atomicAdd(&out[1], (unsigned long long)w);
+        if (work) atomicAdd(&out[2], (unsigned long long)work);
+    }
 }
 
 // ------------------------------------------------- capturable form kernels
@@ -245,30 +289,28 @@ __global__ void pipe_reset_kernel(PipeCtl* __restrict__ c) {
     }
 }
 
-// n = the given count, or the largest id + 1; flags ids out of range.
-__global__ void pipe_resolve_kernel(PipeCtl* __restrict__ c, int32_t n_cap) {
-    if (threadIdx.x != 0) return;
-    const int32_t lo = c->nnz ? c->lo : 0, hi = c->hi;
-    int64_t n = c->n_in > 0 ? (int64_t)c->n_in : (int64_t)hi + 1;
+// orient_kernel for unchecked input.  Every workgroup resolves n itself (the
+// given count, or the largest id + 1) and workgroup 0 records it with the
+// refusal bits; pairs with an id outside [0, n) become the sentinel too (the
+// run is refused then, but its kernels stay in bounds).
+__global__ void orient_checked_kernel(const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
+                                      PipeCtl* __restrict__ c, int32_t n_cap, int b, uint64_t sentinel,
+                                      uint64_t* __restrict__ keys) {
+    const int64_t nnz = c->nnz;
+    const int32_t lo = nnz ? c->lo : 0, hi = c->hi;
+    int64_t nn = c->n_in > 0 ? (int64_t)c->n_in : (int64_t)hi + 1;
     uint32_t st = 0;
     if (lo < 0) st |= kPipeNegativeId;
-    if (hi >= n) st |= kPipeIdRange;
-    if (n > n_cap) {
+    if (hi >= nn) st |= kPipeIdRange;
+    if (nn > n_cap) {
         st |= kPipeTooManyIds;
-        n = n_cap;
+        nn = n_cap;
     }
-    c->n = (uint32_t)n;
-    c->status = st;
-}
-
-// orient_kernel for unchecked input: pairs with an id outside [0, n) become
-// the sentinel too (the run is refused then, but its kernels stay in
-// bounds).
-__global__ void orient_checked_kernel(const int32_t* __restrict__ src, const int32_t* __restrict__ dst, Cnt nnz_,
-                                      const uint32_t* __restrict__ d_n, int b, uint64_t sentinel,
-                                      uint64_t* __restrict__ keys) {
-    const int64_t nnz = nnz_.get();
-    const uint32_t n = *d_n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->n = (uint32_t)nn;
+        c->status = st;
+    }
+    const uint32_t n = (uint32_t)nn;
     for (int64_t e = blockIdx.x * (int64_t)kT + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * kT) {
         const uint32_t u = (uint32_t)src[e], v = (uint32_t)dst[e];
         keys[e] = (u == v || u >= n || v >= n) ? sentinel : ((uint64_t)max(u, v) << b) | (uint64_t)min(u, v);
@@ -280,26 +322,34 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, Cnt n_, uint32_t v) {
     for (int64_t i = blockIdx.x * (int64_t)kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) p[i] = v;
 }
 
-// labels_kernel of the compacted ids; with no edge nothing is compacted
-// (labels = identity, nc = n), as in canonicalise_device.
-__global__ void pipe_labels_kernel(const uint32_t* __restrict__ used, const uint32_t* __restrict__ newid, Cnt n_,
-                                   PipeCtl* __restrict__ c, int32_t* __restrict__ labels) {
-    const int64_t n = n_.get();
-    const bool none = c->m == 0;
-    for (int64_t v = blockIdx.x * (int64_t)kT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kT) {
+// The compacted ids: labels (labels[newid[v]] = v for used v) and the keys
+// rewritten through newid (monotone, so they stay sorted).  With no edge
+// nothing is compacted (labels = identity, nc = n), as in
+// canonicalise_device.
+__global__ void pipe_compact_ids_kernel(const uint32_t* __restrict__ used, const uint32_t* __restrict__ newid,
+                                        PipeCtl* __restrict__ c, int32_t* __restrict__ labels,
+                                        uint64_t* __restrict__ keys, int b) {
+    const int64_t n = c->n, m = c->m;
+    const int64_t stride = (int64_t)gridDim.x * kT, t0 = blockIdx.x * (int64_t)kT + threadIdx.x;
+    const bool none = m == 0;
+    for (int64_t v = t0; v < n; v += stride) {
         if (none) labels[v] = (int32_t)v;
         else if (used[v]) labels[newid[v]] = (int32_t)v;
     }
-    if (none && blockIdx.x == 0 && threadIdx.x == 0) c->nc = (uint32_t)n;
+    if (none && t0 == 0) c->nc = (uint32_t)n;
+    const uint64_t mask = (1ull << b) - 1;
+    for (int64_t e = t0; e < m; e += stride) {
+        const uint64_t k = keys[e];
+        keys[e] = ((uint64_t)newid[k >> b] << b) | (uint64_t)newid[k & mask];
+    }
 }
 
-__global__ void pipe_copy_n_kernel(PipeCtl* __restrict__ c) {
-    if (threadIdx.x == 0) c->nc = c->n;
-}
-
-// col[m, m + kColPad) = 0
-__global__ void col_pad_kernel(int32_t* __restrict__ col, const uint32_t* __restrict__ d_m) {
-    if (threadIdx.x < kColPad) col[(size_t)*d_m + threadIdx.x] = 0;
+// No compaction: labels = identity and nc = n.
+__global__ void pipe_identity_kernel(PipeCtl* __restrict__ c, int32_t* __restrict__ labels) {
+    const int64_t n = c->n;
+    const int64_t t0 = blockIdx.x * (int64_t)kT + threadIdx.x;
+    for (int64_t v = t0; v < n; v += (int64_t)gridDim.x * kT) labels[v] = (int32_t)v;
+    if (t0 == 0) c->nc = (uint32_t)n;
 }
 
 #define BTC_LAUNCH(kernel, n, s, ...)                                                   \
@@ -416,7 +466,10 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
         rank.alloc((size_t)nc, s);
         lab2.alloc((size_t)nc);
         BTC_HIP(hipMemsetAsync(deg.p, 0, (size_t)nc * 4, s));
-        if (m) BTC_LAUNCH(degree_kernel, m, s, keys, m, bn, deg.p);
+        if (m) {
+            hipLaunchKernelGGL(degree_kernel, dim3(std::min(grid_for(m), 1024u)), dim3(kT), 0, s, keys, m, bn, deg.p);
+            BTC_CHECK_LAUNCH("degree_kernel");
+        }
         BTC_LAUNCH(rank_keys_kernel, nc, s, deg.p, nc, 0xFFFFFFFFu, k0.p, v0.p);
         bool alt = sort_pairs_u32(k0.p, v0.p, k1.p, v1.p, (size_t)nc, 32, s);
         if (segs) {
@@ -443,17 +496,15 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
     g->m = m;
     g->d_rowptr.alloc((size_t)nc + 1);
     g->d_col.alloc((size_t)m + kColPad);
-    BTC_HIP(hipMemsetAsync(g->d_col.p + m, 0, kColPad * 4, s));
-    BTC_LAUNCH(rowptr_kernel, (int64_t)nc + 1, s, keys, m, bn, nc, g->d_rowptr.p);
-    if (m) BTC_LAUNCH(col_kernel, m, s, keys, m, bn, g->d_col.p);
+    BTC_LAUNCH(csr_kernel, std::max<int64_t>((int64_t)nc + 1, m + (int64_t)kColPad), s, keys, m, bn, nc,
+               g->d_rowptr.p, g->d_col.p, (uint16_t*)nullptr);
     g->d_labels = std::move(labels);
 
     // derived statistics (graph_finalize on the device)
     TmpBuf<unsigned long long> st;
     st.alloc(3, s);
     BTC_HIP(hipMemsetAsync(st.p, 0, 24, s));
-    if (nc) BTC_LAUNCH(row_stats_kernel, nc, s, g->d_rowptr.p, nc, st.p);
-    if (m) BTC_LAUNCH(edge_stats_kernel, m, s, g->d_rowptr.p, g->d_col.p, m, st.p);
+    if (nc) BTC_LAUNCH(stats_kernel, std::max<int64_t>(nc, m), s, g->d_rowptr.p, g->d_col.p, nc, m, st.p);
     unsigned long long h[3] = {0, 0, 0};
     BTC_HIP(hipMemcpyAsync(h, st.p, 24, hipMemcpyDeviceToHost, s));
     BTC_HIP(hipStreamSynchronize(s));
@@ -488,11 +539,9 @@ int32_t* canonicalise_capacity(const CanonCap& c, hipStream_t s) {
     hipLaunchKernelGGL(pipe_reset_kernel, dim3(1), dim3(64), 0, s, ctl);
     BTC_CHECK_LAUNCH("pipe_reset_kernel");
     CAP_LAUNCH(id_range_kernel, E, s, c.src, c.dst, nnz, &ctl->lo);
-    hipLaunchKernelGGL(pipe_resolve_kernel, dim3(1), dim3(64), 0, s, ctl, c.n_cap);
-    BTC_CHECK_LAUNCH("pipe_resolve_kernel");
 
     // oriented keys, sorted, deduplicated
-    CAP_LAUNCH(orient_checked_kernel, E, s, c.src, c.dst, nnz, &ctl->n, b, sentinel, c.ka);
+    CAP_LAUNCH(orient_checked_kernel, E, s, c.src, c.dst, ctl, c.n_cap, b, sentinel, c.ka);
     uint64_t* sorted = dsort_u64(c.ka, c.kb, (size_t)E, &ctl->nnz, 2 * b, c.ws, s);
     uint64_t* keys = sorted == c.ka ? c.kb : c.ka;
     uint64_t* spare = sorted;
@@ -506,18 +555,16 @@ int32_t* canonicalise_capacity(const CanonCap& c, hipStream_t s) {
         CAP_LAUNCH(fill_u32_kernel, N, s, c.used, n, 0u);
         CAP_LAUNCH(mark_used_kernel, E, s, keys, m, b, c.used);
         dscan_u32(c.used, c.newid, (size_t)N, &ctl->n, &ctl->nc, c.ws, s);
-        CAP_LAUNCH(pipe_labels_kernel, N, s, c.used, c.newid, n, ctl, labels);
-        CAP_LAUNCH(relabel_kernel, E, s, keys, m, b, c.newid, b);
+        CAP_LAUNCH(pipe_compact_ids_kernel, std::max(N, E), s, c.used, c.newid, ctl, labels, keys, b);
     } else {
-        hipLaunchKernelGGL(pipe_copy_n_kernel, dim3(1), dim3(64), 0, s, ctl);
-        BTC_CHECK_LAUNCH("pipe_copy_n_kernel");
-        CAP_LAUNCH(iota_kernel, N, s, labels, n);
+        CAP_LAUNCH(pipe_identity_kernel, N, s, ctl, labels);
     }
 
     if (c.flags & BTC_CANON_DEGREE_ORDER) {
         // keys N - 1 - degree < 2^b: descending degree, ties by id
         CAP_LAUNCH(fill_u32_kernel, N, s, c.deg, nc, 0u);
-        CAP_LAUNCH(degree_kernel, E, s, keys, m, b, c.deg);
+        hipLaunchKernelGGL(degree_kernel, dim3(std::min(cap_grid(E), 1024u)), dim3(kT), 0, s, keys, m, b, c.deg);
+        BTC_CHECK_LAUNCH("degree_kernel");
         CAP_LAUNCH(rank_keys_kernel, N, s, c.deg, nc, (uint32_t)(N - 1), c.k0, c.v0);
         const bool alt = dsort_pairs_u32(c.k0, c.v0, c.k1, c.v1, (size_t)N, &ctl->nc, b, c.ws, s);
         CAP_LAUNCH(rank_scatter_kernel, N, s, alt ? c.v1 : c.v0, nc, c.rank, labels, c.lab1);
@@ -526,13 +573,9 @@ int32_t* canonicalise_capacity(const CanonCap& c, hipStream_t s) {
         labels = c.lab1;
     }
 
-    // CSR of L and its statistics
-    CAP_LAUNCH(rowptr_kernel, N + 1, s, keys, m, b, nc, c.rowptr);
-    CAP_LAUNCH(col_kernel, E, s, keys, m, b, c.col);
-    hipLaunchKernelGGL(col_pad_kernel, dim3(1), dim3(64), 0, s, c.col, &ctl->m);
-    BTC_CHECK_LAUNCH("col_pad_kernel");
-    CAP_LAUNCH(row_stats_kernel, N, s, c.rowptr, nc, ctl->rs);
-    CAP_LAUNCH(edge_stats_kernel, E, s, c.rowptr, c.col, m, ctl->rs);
+    // CSR of L (with its 16-bit copy) and its statistics
+    CAP_LAUNCH(csr_kernel, std::max(N + 1, E + (int64_t)kColPad), s, keys, m, b, nc, c.rowptr, c.col, c.col16);
+    CAP_LAUNCH(stats_kernel, std::max(N, E), s, c.rowptr, c.col, nc, m, ctl->rs);
     return labels;
 }
 

@@ -42,6 +42,21 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
+// Workgroup sum of v (blockDim.x = NT, a multiple of 64), valid in thread
+// 0; every thread calls it (lds: NT / 64 words, free again on return).
+template <int NT>
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* lds) {
+    v = wave_sum_u64(v);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t s = 0;
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) s += lds[w];
+    __syncthreads();
+    return s;
+}
+
 __device__ __forceinline__ uint32_t group_min_u32(uint32_t v, int width) {
     for (int o = width >> 1; o > 0; o >>= 1) {
         uint32_t w = __shfl_xor(v, o, width);

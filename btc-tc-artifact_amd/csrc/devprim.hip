@@ -5,10 +5,12 @@
 //
 // Scan: tile sums, one workgroup scanning the tile sums, then each tile
 // applies its offset.  Sort: least-significant-digit radix sort, 8 bits a
-// pass; each pass takes a digit histogram per tile (digit-major, so one
-// flat scan gives every (digit, tile) its output offset), then ranks each
-// tile stably in LDS (rocPRIM's block_radix_rank, match form: per-wave
-// ballots of equal digits) and writes it out run by run.
+// pass, in one sweep per pass: one kernel takes the digit totals of every
+// pass in a single read of the keys; each pass kernel then ranks a tile
+// stably in LDS (rocPRIM's block_radix_rank, match form: per-wave ballots
+// of equal digits) and finds the tile's offset for each digit by
+// decoupled look-back over the tiles before it -- 2 + passes kernels a
+// sort.
 #include <rocprim/block/block_radix_rank.hpp>
 
 #include "common.h"
@@ -120,9 +122,13 @@ __global__ __launch_bounds__(kT) void dscan_apply(const T* __restrict__ in, cons
     }
 }
 
+// A scan's tile sums start past the first 256 bytes of its workspace, which
+// hold a sort's control words when the two share one workspace.
+constexpr size_t kScanSumsAt = 256;
+
 template <class T>
 void dscan_t(const T* in, T* out, size_t cap, const uint32_t* d_n, T* d_total, void* ws, hipStream_t s) {
-    T* sums = static_cast<T*>(ws);
+    T* sums = reinterpret_cast<T*>(static_cast<unsigned char*>(ws) + kScanSumsAt);
     const unsigned g = grid_of(cap, kScanTile);
     hipLaunchKernelGGL(dscan_tiles<T>, dim3(g), dim3(kT), 0, s, in, d_n, sums);
     BTC_CHECK_LAUNCH("dscan_tiles");
@@ -133,64 +139,105 @@ void dscan_t(const T* in, T* out, size_t cap, const uint32_t* d_n, T* d_total, v
 }
 
 // ------------------------------------------------------------------ sort
-// ctl[0] = sort tiles of the count, ctl[1] = digits x tiles (the length of
-// the digit-count array a pass scans).
-__global__ void dsort_setup(const uint32_t* __restrict__ d_n, uint32_t* __restrict__ ctl) {
-    if (threadIdx.x == 0) {
-        const uint32_t nt = (*d_n + kSortTile - 1) / kSortTile;
-        ctl[0] = nt;
-        ctl[1] = nt * kDigits;
-    }
-}
+// Workspace control words: [0] tiles of the count, [1 + p] pass p's tile
+// ticket, [kMaxPasses + 1] set when a look-back ran past its spin limit.
+constexpr int kMaxPasses = 8;
+constexpr uint32_t kCtlWords = 64;  // 256 bytes
+constexpr uint32_t kErrWord = kMaxPasses + 1;
+static_assert(kCtlWords * 4 <= kScanSumsAt, "a scan sharing the workspace leaves the control words alone");
+// Look-back status of (tile, digit): flag << 32 | count; flag 1 = the
+// tile's own count, 2 = inclusive of every earlier tile.
+constexpr uint64_t kAgg = 1ull << 32, kInc = 2ull << 32;
+constexpr uint32_t kSpinMax = 1u << 22;
 
 template <class K>
 __device__ __forceinline__ uint32_t digit_of(K k, int shift) {
     return (uint32_t)(k >> shift) & (kDigits - 1);
 }
 
-// counts[d * nt + t] = keys of tile t with digit d.
+// One workgroup: clears the digit totals and the tickets, counts tiles
+// (the error word stays set once set).
+__global__ __launch_bounds__(kT) void dsort_setup(const uint32_t* __restrict__ d_n, uint32_t* __restrict__ ctl,
+                                                  uint32_t* __restrict__ totals, int passes) {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)passes * kDigits; i += kT) totals[i] = 0;
+    if (threadIdx.x < kErrWord) ctl[threadIdx.x] = threadIdx.x == 0 ? (*d_n + kSortTile - 1) / kSortTile : 0u;
+}
+
+// Digit totals of every pass in one read of the keys (per tile in LDS,
+// then one atomic per non-zero (pass, digit)), and the tile's look-back
+// statuses of every pass cleared.
 template <class K>
 __global__ __launch_bounds__(kT) void dsort_hist(const K* __restrict__ keys, const uint32_t* __restrict__ d_n,
-                                                 const uint32_t* __restrict__ ctl, int shift,
-                                                 uint32_t* __restrict__ counts) {
-    __shared__ uint32_t h[kDigits];
+                                                 const uint32_t* __restrict__ ctl, int passes,
+                                                 uint32_t* __restrict__ totals, uint64_t* __restrict__ status,
+                                                 size_t status_pass) {
+    __shared__ uint32_t h[kMaxPasses][kDigits];
     const uint32_t n = *d_n, nt = ctl[0];
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
-        h[threadIdx.x] = 0;
+        for (int p = 0; p < passes; ++p) h[p][threadIdx.x] = 0;
         __syncthreads();
         const size_t base = (size_t)t * kSortTile;
 #pragma unroll
         for (int k = 0; k < kSortPer; ++k) {
             const size_t e = base + (size_t)k * kT + threadIdx.x;
-            if (e < n) atomicAdd(&h[digit_of(keys[e], shift)], 1u);
+            if (e < n) {
+                const K x = keys[e];
+                for (int p = 0; p < passes; ++p) atomicAdd(&h[p][digit_of(x, p * kDigitBits)], 1u);
+            }
         }
         __syncthreads();
-        counts[(size_t)threadIdx.x * nt + t] = h[threadIdx.x];
+        for (int p = 0; p < passes; ++p) {
+            const uint32_t c = h[p][threadIdx.x];
+            if (c) atomicAdd(&totals[p * kDigits + threadIdx.x], c);
+            status[(size_t)p * status_pass + (size_t)t * kDigits + threadIdx.x] = 0;
+        }
         __syncthreads();
     }
 }
 
-// Stable scatter of one pass.  Key i of a tile sits at index
-// wave * 64 * kSortPer + item * 64 + lane, the order in which the match
-// ranking breaks ties, so equal digits keep their input order; padding keys
-// (all bits set) rank after every real key of the tile.
+__device__ __forceinline__ uint64_t load_status(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_status(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One stable pass.  Workgroups take tiles in ticket order; a tile ranks its
+// keys in LDS (rocPRIM's block_radix_rank, match form), publishes its count
+// of each digit and looks back over earlier tiles for the digit's offset
+// (thread d owns digit d), then writes its keys out run by run.  Key i of a
+// tile sits at index wave * 64 * kSortPer + item * 64 + lane, the order in
+// which the match ranking breaks ties, so equal digits keep their input
+// order; padding keys (all bits set) rank after every real key of the tile.
+// Every look-back ends: a ticket is only taken by a running workgroup,
+// which publishes before it waits on anything.
 template <class K, bool V>
-__global__ __launch_bounds__(kT) void dsort_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                    K* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                    const uint32_t* __restrict__ d_n,
-                                                    const uint32_t* __restrict__ ctl, int shift,
-                                                    const uint32_t* __restrict__ offs) {
+__global__ __launch_bounds__(kT) void dsort_onesweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                     K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                     const uint32_t* __restrict__ d_n, uint32_t* __restrict__ ctl,
+                                                     int pass, const uint32_t* __restrict__ totals,
+                                                     uint64_t* __restrict__ status) {
     using Rank = rocprim::block_radix_rank<kT, kDigitBits, rocprim::block_radix_rank_algorithm::match>;
     static_assert(Rank::digits_per_thread == 1, "one digit per thread");
     __shared__ typename Rank::storage_type rank_lds;
     __shared__ K sk[kSortTile];
     __shared__ uint32_t sv[V ? kSortTile : 1];
     __shared__ uint32_t s_pre[kDigits], s_out[kDigits];
+    __shared__ uint32_t s_scan[kT / 64];
+    __shared__ uint32_t s_tile;
+    const int shift = pass * kDigitBits;
     const uint32_t n = *d_n, nt = ctl[0];
-    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6, d = threadIdx.x;
+    uint32_t all;
+    const uint32_t dstart = block_scan_ex(totals[pass * kDigits + d], s_scan, &all);
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&ctl[1 + pass], 1u);
+        __syncthreads();
+        const uint32_t t = s_tile;
+        if (t >= nt) break;
         const size_t base = (size_t)t * kSortTile;
-        const uint32_t valid = (uint32_t)std::min<size_t>(kSortTile, n - base);
+        const uint32_t valid = (uint32_t)min((size_t)kSortTile, (size_t)n - base);
         K k[kSortPer];
         uint32_t v[kSortPer], r[kSortPer];
 #pragma unroll
@@ -201,18 +248,40 @@ __global__ __launch_bounds__(kT) void dsort_scatter(const K* __restrict__ kin, c
         }
         uint32_t pre[1], cnt[1];
         Rank().rank_keys(k, r, rank_lds, [shift](const K& x) { return digit_of(x, shift); }, pre, cnt);
+        uint64_t* mine = status + (size_t)t * kDigits + d;
+        uint32_t excl = 0;
+        if (t == 0) {
+            store_status(mine, kInc | cnt[0]);
+        } else {
+            store_status(mine, kAgg | cnt[0]);
+            for (int64_t j = (int64_t)t - 1; j >= 0; --j) {
+                const uint64_t* at = status + (size_t)j * kDigits + d;
+                uint64_t x = load_status(at);
+                for (uint32_t spin = 0; (x >> 32) == 0 && spin < kSpinMax; ++spin) {
+                    __builtin_amdgcn_s_sleep(1);
+                    x = load_status(at);
+                }
+                if ((x >> 32) == 0) {  // not reached: flags the sort as failed rather than wait on
+                    atomicOr(&ctl[kErrWord], 1u);
+                    break;
+                }
+                excl += (uint32_t)x;
+                if ((x >> 32) == 2) break;
+            }
+            store_status(mine, kInc | (uint32_t)(excl + cnt[0]));
+        }
 #pragma unroll
         for (int i = 0; i < kSortPer; ++i) {
             sk[r[i]] = k[i];
             if constexpr (V) sv[r[i]] = v[i];
         }
-        s_pre[threadIdx.x] = pre[0];
-        s_out[threadIdx.x] = offs[(size_t)threadIdx.x * nt + t];
+        s_pre[d] = pre[0];
+        s_out[d] = dstart + excl;
         __syncthreads();
         for (uint32_t idx = threadIdx.x; idx < valid; idx += kT) {
             const K x = sk[idx];
-            const uint32_t d = digit_of(x, shift);
-            const uint32_t pos = s_out[d] + idx - s_pre[d];
+            const uint32_t dx = digit_of(x, shift);
+            const uint32_t pos = s_out[dx] + idx - s_pre[dx];
             if (pos >= n) continue;  // unreachable when the counts agree; keeps a bad count in bounds
             kout[pos] = x;
             if constexpr (V) vout[pos] = sv[idx];
@@ -221,24 +290,22 @@ __global__ __launch_bounds__(kT) void dsort_scatter(const K* __restrict__ kin, c
     }
 }
 
-// Workspace: [0, 256) control words, then the digit counts and their
-// offsets (digits x tiles + 1 each), then the scan's tile sums.
+// Workspace: control words, digit totals (kMaxPasses x 256), then the
+// look-back statuses (per pass: tiles x 256 words).
 struct SortWs {
     uint32_t* ctl;
-    uint32_t* counts;
-    uint32_t* offs;
-    void* scan;
+    uint32_t* totals;
+    uint64_t* status;
+    size_t status_pass;  // words per pass
 };
 
 SortWs sort_ws(void* ws, size_t cap) {
-    const size_t nc = tiles(cap, kSortTile) * kDigits + 1;
     auto* b = static_cast<unsigned char*>(ws);
     SortWs w;
     w.ctl = reinterpret_cast<uint32_t*>(b);
-    w.counts = reinterpret_cast<uint32_t*>(b + 256);
-    w.offs = w.counts + nc;
-    const size_t at = (256 + 2 * nc * 4 + 255) & ~size_t(255);
-    w.scan = b + at;
+    w.totals = w.ctl + kCtlWords;
+    w.status = reinterpret_cast<uint64_t*>(b + (kCtlWords + kMaxPasses * kDigits) * 4);
+    w.status_pass = tiles(cap, kSortTile) * kDigits;
     return w;
 }
 
@@ -248,23 +315,23 @@ bool dsort_t(K* keys, uint32_t* vals, K* kt, uint32_t* vt, size_t cap, const uin
              hipStream_t s) {
     const int passes = (std::max(end_bit, 0) + kDigitBits - 1) / kDigitBits;
     if (passes == 0) return false;
+    BTC_REQUIRE(passes <= kMaxPasses && end_bit <= (int)(8 * sizeof(K)), BTC_ERR_ARG, "sort on %d bits", end_bit);
     const SortWs w = sort_ws(ws, cap);
-    const size_t ncap = tiles(cap, kSortTile) * kDigits;
     const unsigned g = grid_of(cap, kSortTile);
-    hipLaunchKernelGGL(dsort_setup, dim3(1), dim3(64), 0, s, d_n, w.ctl);
+    const unsigned gp = std::min<unsigned>(g, 2048);  // workgroups take tiles by ticket
+    hipLaunchKernelGGL(dsort_setup, dim3(1), dim3(kT), 0, s, d_n, w.ctl, w.totals, passes);
     BTC_CHECK_LAUNCH("dsort_setup");
+    hipLaunchKernelGGL(dsort_hist<K>, dim3(g), dim3(kT), 0, s, keys, d_n, w.ctl, passes, w.totals, w.status,
+                       w.status_pass);
+    BTC_CHECK_LAUNCH("dsort_hist");
     K* ki = keys;
     K* ko = kt;
     uint32_t* vi = vals;
     uint32_t* vo = vt;
     for (int p = 0; p < passes; ++p) {
-        const int shift = p * kDigitBits;
-        hipLaunchKernelGGL(dsort_hist<K>, dim3(g), dim3(kT), 0, s, ki, d_n, w.ctl, shift, w.counts);
-        BTC_CHECK_LAUNCH("dsort_hist");
-        dscan_t<uint32_t>(w.counts, w.offs, ncap, w.ctl + 1, nullptr, w.scan, s);
-        hipLaunchKernelGGL((dsort_scatter<K, V>), dim3(g), dim3(kT), 0, s, ki, vi, ko, vo, d_n, w.ctl, shift,
-                           w.offs);
-        BTC_CHECK_LAUNCH("dsort_scatter");
+        hipLaunchKernelGGL((dsort_onesweep<K, V>), dim3(gp), dim3(kT), 0, s, ki, vi, ko, vo, d_n, w.ctl, p,
+                           w.totals, w.status + (size_t)p * w.status_pass);
+        BTC_CHECK_LAUNCH("dsort_onesweep");
         std::swap(ki, ko);
         std::swap(vi, vo);
     }
@@ -273,11 +340,12 @@ bool dsort_t(K* keys, uint32_t* vals, K* kt, uint32_t* vt, size_t cap, const uin
 
 }  // namespace
 
-size_t dscan_ws_bytes(size_t cap) { return tiles(cap, kScanTile) * 8 + 256; }
+size_t dscan_ws_bytes(size_t cap) { return kScanSumsAt + tiles(cap, kScanTile) * 8; }
+
+const uint32_t* dsort_error_word(const void* ws) { return static_cast<const uint32_t*>(ws) + kErrWord; }
 
 size_t dsort_ws_bytes(size_t cap) {
-    const size_t nc = tiles(cap, kSortTile) * kDigits + 1;
-    return ((256 + 2 * nc * 4 + 255) & ~size_t(255)) + dscan_ws_bytes(nc);
+    return (kCtlWords + kMaxPasses * kDigits) * 4 + (size_t)kMaxPasses * tiles(cap, kSortTile) * kDigits * 8;
 }
 
 void dscan_u32(const uint32_t* in, uint32_t* out, size_t cap, const uint32_t* d_n, uint32_t* d_total, void* ws,

@@ -966,24 +966,19 @@ void launch_plan_dense(btc_plan_s* p, unsigned long long* d_slots, PvOut d_pv, h
 
 }  // namespace
 
-void pipeline_count_enqueue(const PlanCap& c, uint16_t* col16, unsigned long long* d_work,
-                            const btc_count_opts& o, hipStream_t s) {
-    launch_zero_u32(reinterpret_cast<uint32_t*>(d_work), 2 * kWorkWords, s);
+void pipeline_count_enqueue(const PlanCap& c, const uint16_t* col16, const btc_count_opts& o, hipStream_t s) {
     const int64_t span = c.m_cap + (int64_t)kColPad;
-    const unsigned g = (unsigned)std::min<int64_t>((span + kThreads - 1) / kThreads, 2048);
-    hipLaunchKernelGGL(narrow_col_kernel, dim3(g), dim3(kThreads), 0, s, c.col,
-                       Cnt::dev(&c.ctl->m, (int64_t)kColPad), col16);
-    BTC_CHECK_LAUNCH("narrow_col_kernel");
     const bool w64 = o.wide_offsets != 0 || (uint64_t)span >= (uint64_t(1) << 30);
     unsigned grid = 0;
     WedgeFn fn = wedge_fn(o, false, w64, &grid);
+    unsigned long long* work = c.work;
     const PlanDev pd{c.col, col16, c.lists, reinterpret_cast<const uint4*>(c.rowbits), c.ur, c.items, 0u, 0u,
-                     d_work, PvOut{}, c.core_ids / 128};
-    WedgeArgs a{pd, nullptr, nullptr, 1u, 0u, 0u, reinterpret_cast<uint32_t*>(d_work + 1), (int)o.lookup,
-                (int)o.schedule, 0u, 1u, d_work + kWorkSlots, c.ctl->item_counts};
+                     work, PvOut{}, c.core_ids / 128};
+    WedgeArgs a{pd, nullptr, nullptr, 1u, 0u, 0u, reinterpret_cast<uint32_t*>(work + 1), (int)o.lookup,
+                (int)o.schedule, 0u, 1u, work + kWorkSlots, c.ctl->item_counts};
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
     BTC_CHECK_LAUNCH("wedge_kernel");
-    launch_slot_sum(d_work + kWorkSlots, d_work, s);
+    launch_slot_sum(work + kWorkSlots, work, s);
 }
 
 void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, MultiWorkspace& ws) {

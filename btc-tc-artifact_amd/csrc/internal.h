@@ -215,6 +215,7 @@ struct CanonCap {
     int32_t* lab1;       // [n_cap] each
     uint32_t* rowptr;    // [n_cap + 1]
     int32_t* col;        // [nnz_cap + kColPad]
+    uint16_t* col16;     // [nnz_cap + kColPad]: low 16 bits of col
     void* ws;            // dsort / dscan workspace for max(nnz_cap, n_cap)
 };
 
@@ -259,6 +260,7 @@ struct PlanCap {
     uint32_t* sk1;
     uint32_t* sv1;           // [items_cap] each
     uint4* items;            // [2 * items_cap]
+    unsigned long long* work;  // the count's words (kWorkWords): total and cursors cleared with the items
     void* ws;                // dsort / dscan workspace for max(m_cap, items_cap)
 };
 
@@ -275,11 +277,11 @@ void plan_capacity(const PlanCap& c, hipStream_t s);
 // Checks the launch-shape fields of o (BTC_ERR_ARG on a bad value).
 void check_launch_opts(const btc_count_opts& o);
 
-// Enqueues the count of a plan_capacity plan: clears the work words at
-// d_work (kWorkWords), builds col16 over L, runs the wedge kernel on the
-// item counts in ctl and folds the total into d_work[0].
-void pipeline_count_enqueue(const PlanCap& c, uint16_t* col16, unsigned long long* d_work,
-                            const btc_count_opts& o, hipStream_t s);
+// Enqueues the count of a plan_capacity plan (whose last kernel cleared the
+// total and cursors at c.work; the slots are left cleared by every count):
+// the wedge kernel on the item counts in ctl, then the slot sum into
+// c.work[0].  col16 is L's 16-bit copy.
+void pipeline_count_enqueue(const PlanCap& c, const uint16_t* col16, const btc_count_opts& o, hipStream_t s);
 
 // Uploads L (rowptr narrowed to u32) to HBM on first use.
 void ensure_device(btc_graph_s* g, hipStream_t s);

@@ -44,7 +44,7 @@ namespace btcx {
 namespace {
 
 constexpr size_t kPinHeader = 4096;
-static_assert(sizeof(PipeCtl) + 8 <= kPinHeader, "control block fits the pinned header");
+static_assert(sizeof(PipeCtl) + 16 <= kPinHeader, "control block, total and error word fit the pinned header");
 
 // Bump layout over one allocation (base NULL: sizing pass).
 struct Carve {
@@ -94,6 +94,8 @@ void layout(btc_pipeline_s* p, Carve& c) {
     k.lab1 = c.take<int32_t>(N);
     k.rowptr = c.take<uint32_t>(N + 1);
     k.col = c.take<int32_t>(E + kColPad);
+    p->col16 = c.take<uint16_t>(E + kColPad);
+    k.col16 = p->col16;
     PlanCap& q = p->plan;
     q.rowptr = k.rowptr;
     q.col = k.col;
@@ -130,7 +132,7 @@ void layout(btc_pipeline_s* p, Carve& c) {
     q.sk1 = c.take<uint32_t>(M);
     q.sv1 = c.take<uint32_t>(M);
     q.items = c.take<uint4>(2 * M);
-    p->col16 = c.take<uint16_t>(E + kColPad);
+    q.work = p->work;
     const size_t big = std::max(E, N);
     void* ws = c.take<unsigned char>(std::max(dsort_ws_bytes(big), dscan_ws_bytes(big)));
     k.ws = ws;
@@ -145,13 +147,15 @@ void pipeline_build(btc_pipeline_s* p, hipStream_t s) {
     Carve real;
     real.base = p->mem.p;
     layout(p, real);
-    // the count's words start cleared; the captured kernels clear them again
+    // the count's words start cleared (the captured kernels clear them again);
+    // so does the sorts' error word, which only a failed sort sets
     BTC_HIP(hipMemsetAsync(p->work, 0, kWorkWords * 8, s));
+    BTC_HIP(hipMemsetAsync(const_cast<uint32_t*>(dsort_error_word(p->canon.ws)), 0, 4, s));
     BTC_HIP(hipStreamSynchronize(s));
     p->graph.capture([&](hipStream_t cs) {
         p->labels = canonicalise_capacity(p->canon, cs);
         plan_capacity(p->plan, cs);
-        pipeline_count_enqueue(p->plan, p->col16, p->work, p->opts, cs);
+        pipeline_count_enqueue(p->plan, p->col16, p->opts, cs);
     });
     size_t nodes = 0;
     BTC_HIP(hipGraphGetNodes(p->graph.graph, nullptr, &nodes));
@@ -192,7 +196,10 @@ void pipeline_run(btc_pipeline_s* p, const int32_t* src, const int32_t* dst, int
     unsigned long long* htotal = reinterpret_cast<unsigned long long*>(pin + sizeof(PipeCtl));
     BTC_HIP(hipMemcpyAsync(h, p->ctl, sizeof(PipeCtl), hipMemcpyDeviceToHost, s));
     BTC_HIP(hipMemcpyAsync(htotal, p->work, 8, hipMemcpyDeviceToHost, s));
+    uint32_t* herr = reinterpret_cast<uint32_t*>(htotal + 1);
+    BTC_HIP(hipMemcpyAsync(herr, dsort_error_word(p->canon.ws), 4, hipMemcpyDeviceToHost, s));
     BTC_HIP(hipStreamSynchronize(s));
+    BTC_REQUIRE(*herr == 0, BTC_ERR_HIP, "a device sort of the pipeline failed (look-back spin limit)");
     const PipeCtl c = *h;
     BTC_REQUIRE(!(c.status & kPipeNegativeId), BTC_ERR_ARG, "negative vertex id %d", c.lo);
     BTC_REQUIRE(!(c.status & kPipeIdRange), BTC_ERR_ARG, "vertex id %d >= n = %d", c.hi, n);

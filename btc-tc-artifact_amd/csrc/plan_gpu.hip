@@ -78,8 +78,9 @@ __global__ void dense_tiles_kernel(const uint32_t* __restrict__ pop, uint32_t nb
         flag[t] = d;
         sum += d ? pop[t] : 0u;
     }
-    sum = wave_sum_u64(sum);
-    if ((threadIdx.x & 63) == 0 && sum) atomicAdd(dense_edges, (unsigned long long)sum);
+    __shared__ uint64_t lds[kT / 64];
+    sum = block_sum_u64<kT>(sum, lds);
+    if (threadIdx.x == 0 && sum) atomicAdd(dense_edges, (unsigned long long)sum);
 }
 
 __global__ void select_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, Cnt n_,
@@ -129,11 +130,12 @@ __global__ void clip_kernel(const uint32_t* __restrict__ rowptr, const int32_t* 
         }
         flag[q] = f;
     }
-    se = wave_sum_u64(se);
-    sw = wave_sum_u64(sw);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&stats[0], (unsigned long long)se);
-        atomicAdd(&stats[1], (unsigned long long)sw);
+    __shared__ uint64_t lds[kT / 64];
+    se = block_sum_u64<kT>(se, lds);
+    sw = block_sum_u64<kT>(sw, lds);
+    if (threadIdx.x == 0) {
+        if (se) atomicAdd(&stats[0], (unsigned long long)se);
+        if (sw) atomicAdd(&stats[1], (unsigned long long)sw);
     }
 }
 
@@ -217,10 +219,11 @@ __global__ void records_kernel(const uint32_t* __restrict__ rowptr, const int32_
         cost[e] = (uint64_t)(s1 > sc ? s1 - sc : 0u) + words;
         if (hi < kNarrowIds) nn += s1 > sc ? s1 - sc : 0u;
     }
-    nc = wave_sum_u64(nc);
-    nn = wave_sum_u64(nn);
-    if ((threadIdx.x & 63) == 0 && nc) atomicAdd(ncore, (unsigned long long)nc);
-    if ((threadIdx.x & 63) == 0 && nn) atomicAdd(nnarrow, (unsigned long long)nn);
+    __shared__ uint64_t lds[kT / 64];
+    nc = block_sum_u64<kT>(nc, lds);
+    nn = block_sum_u64<kT>(nn, lds);
+    if (threadIdx.x == 0 && nc) atomicAdd(ncore, (unsigned long long)nc);
+    if (threadIdx.x == 0 && nn) atomicAdd(nnarrow, (unsigned long long)nn);
 }
 
 __device__ __forceinline__ bool pivot_small(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
@@ -234,9 +237,12 @@ __device__ __forceinline__ bool pivot_small(const uint32_t* __restrict__ rowptr,
 __global__ void item_start_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                   const uint32_t* __restrict__ key, const uint32_t* __restrict__ off,
                                   const uint64_t* __restrict__ P, Cnt ne_, uint64_t chunk_,
-                                  const unsigned long long* __restrict__ d_chunk, uint32_t* __restrict__ start) {
+                                  const unsigned long long* __restrict__ d_checks, uint64_t target_items,
+                                  uint32_t* __restrict__ start) {
     const uint32_t ne = (uint32_t)ne_.get();
-    const uint64_t chunk = d_chunk ? (uint64_t)*d_chunk : chunk_;
+    // with d_checks: about target_items items, at least 8192 checks each
+    const uint64_t chunk =
+        d_checks ? max((uint64_t)8192, ((uint64_t)*d_checks + target_items - 1) / target_items) : chunk_;
     GS_LOOP(e, ne) {
         const uint32_t j = key[e], eb = off[j];
         uint32_t st = e == eb;
@@ -250,7 +256,7 @@ __global__ void item_start_kernel(const uint32_t* __restrict__ rowptr, const int
 __global__ void items_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                              const uint32_t* __restrict__ key, const uint32_t* __restrict__ off,
                              const uint64_t* __restrict__ P, const uint32_t* __restrict__ first, Cnt nitems_,
-                             Cnt ne_, bool use_tiny, uint4* __restrict__ tmp_items,
+                             Cnt ne_, bool use_tiny, bool coarse, uint4* __restrict__ tmp_items,
                              uint32_t* __restrict__ skey, uint32_t* __restrict__ sval,
                              unsigned long long* __restrict__ stats) {
     const uint32_t nitems = (uint32_t)nitems_.get();
@@ -267,26 +273,45 @@ __global__ void items_kernel(const uint32_t* __restrict__ rowptr, const int32_t*
         const bool tiny = use_tiny && small && tiny_pivot(off[j + 1] - off[j], lo, hi, t1 - t0);
         tmp_items[2 * k] = make_uint4(t0, t1, eb, ee);
         tmp_items[2 * k + 1] = make_uint4(lo, hi, w32, 0u);
-        const uint32_t w30 = (uint32_t)min(w, (uint64_t)0x3FFFFFFFu);
-        skey[k] = ((tiny ? 2u : small ? 1u : 0u) << 30) | (0x3FFFFFFFu - w30);
+        const uint32_t cls = tiny ? 2u : small ? 1u : 0u;
+        if (coarse) {
+            // 8-bit key: class, then work by half octaves, heaviest first
+            const uint32_t len = w ? 64 - __clzll(w) : 0u;
+            const uint32_t half = len >= 2 ? (uint32_t)(w >> (len - 2)) & 1u : 0u;
+            skey[k] = (cls << 6) | (63u - min(63u, 2 * len + half));
+        } else {
+            const uint32_t w30 = (uint32_t)min(w, (uint64_t)0x3FFFFFFFu);
+            skey[k] = (cls << 30) | (0x3FFFFFFFu - w30);
+        }
         sval[k] = (uint32_t)k;
         words += t1 - t0;
         small_items += small;
         tiny_items += tiny;
     }
-    words = wave_sum_u64(words);
-    small_items = wave_sum_u64(small_items);
-    tiny_items = wave_sum_u64(tiny_items);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&stats[0], (unsigned long long)words);
-        atomicAdd(&stats[1], (unsigned long long)small_items);
-        atomicAdd(&stats[3], (unsigned long long)tiny_items);
+    __shared__ uint64_t lds[kT / 64];
+    words = block_sum_u64<kT>(words, lds);
+    small_items = block_sum_u64<kT>(small_items, lds);
+    tiny_items = block_sum_u64<kT>(tiny_items, lds);
+    if (threadIdx.x == 0) {
+        if (words) atomicAdd(&stats[0], (unsigned long long)words);
+        if (small_items) atomicAdd(&stats[1], (unsigned long long)small_items);
+        if (tiny_items) atomicAdd(&stats[3], (unsigned long long)tiny_items);
     }
 }
 
+// With c set (a capacity plan), workgroup 0 also records the item counts of
+// the wedge kernel's launch (wave items, st[5], follow the workgroup items;
+// there are no tiny items) and clears the count's total and cursors.
 __global__ void place_items_kernel(const uint4* __restrict__ tmp_items, const uint32_t* __restrict__ order,
-                                   Cnt nitems_, uint4* __restrict__ items) {
+                                   Cnt nitems_, uint4* __restrict__ items, PipeCtl* __restrict__ c,
+                                   unsigned long long* __restrict__ work) {
     const uint32_t nitems = (uint32_t)nitems_.get();
+    if (c && blockIdx.x == 0 && threadIdx.x == 0) {
+        c->item_counts[0] = c->nitems - (uint32_t)c->st[5];
+        c->item_counts[1] = c->nitems;
+        work[0] = 0;
+        work[1] = 0;
+    }
     GS_LOOP(r, nitems) {
         const uint32_t k = order[r];
         items[2 * r] = tmp_items[2 * k];
@@ -295,26 +320,27 @@ __global__ void place_items_kernel(const uint4* __restrict__ tmp_items, const ui
 }
 
 // ------------------------------------------------- capturable form kernels
-__global__ void fill_u32_kernel(uint32_t* __restrict__ p, Cnt n_, uint32_t v) {
+// pivot_offsets_kernel and core_rows_kernel in one launch.
+__global__ void pivot_core_kernel(const uint32_t* __restrict__ key, Cnt ne_, Cnt n_,
+                                  const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col, uint32_t C,
+                                  uint32_t* __restrict__ off, uint32_t* __restrict__ cend,
+                                  uint32_t* __restrict__ has) {
+    const uint32_t ne = (uint32_t)ne_.get();
     const int64_t n = n_.get();
-    GS_LOOP(i, n) p[i] = v;
-}
-
-__global__ void pipe_bitmap_words_kernel(PipeCtl* __restrict__ c, uint32_t CW) {
-    if (threadIdx.x == 0) c->nbm_words = c->nbm * CW;
-}
-
-// About 64 items per CU: chunk = max(8192, checks / (64 CUs)).
-__global__ void pipe_chunk_kernel(PipeCtl* __restrict__ c, uint64_t items) {
-    if (threadIdx.x == 0) c->chunk = max(8192ull, (c->checks + items - 1) / items);
-}
-
-// Wave items (st[5]) follow the workgroup items; there are no tiny items.
-__global__ void pipe_items_kernel(PipeCtl* __restrict__ c) {
-    if (threadIdx.x == 0) {
-        c->item_counts[0] = c->nitems - (uint32_t)c->st[5];
-        c->item_counts[1] = c->nitems;
+    GS_LOOP(j, n + 1) off[j] = lower_bound_dev(key, 0u, ne, (uint32_t)j);
+    GS_LOOP(i, n) {
+        const uint32_t r0 = rowptr[i];
+        const uint32_t c = lower_bound_dev(col, r0, rowptr[i + 1], (int32_t)C);
+        cend[i] = c;
+        has[i] = c - r0 >= kCoreRowMin;
     }
+}
+
+// rowbits[0, nbm * CW) = 0
+__global__ void rowbits_clear_kernel(uint32_t* __restrict__ rowbits, const uint32_t* __restrict__ d_nbm,
+                                     uint32_t CW) {
+    const int64_t n = (int64_t)*d_nbm * CW;
+    GS_LOOP(i, n) rowbits[i] = 0;
 }
 
 template <class T>
@@ -425,7 +451,8 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     TmpBuf<uint32_t> start, spos;
     start.alloc(ne, s);
     spos.alloc((size_t)ne + 1, s);
-    if (ne) BTC_LAUNCH(item_start_kernel, ne, s, rowptr, col, key, off.p, P.p, ne, (uint64_t)chunk, nullptr, start.p);
+    if (ne) BTC_LAUNCH(item_start_kernel, ne, s, rowptr, col, key, off.p, P.p, ne, (uint64_t)chunk, nullptr, (uint64_t)1,
+                       start.p);
     exclusive_scan_u32(start.p, spos.p, ne, s);
     const uint32_t nitems = read_dev(spos.p + ne, s);
     TmpBuf<uint32_t> first, sk0, sv0, sk1, sv1;
@@ -439,11 +466,12 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     sv1.alloc(nitems, s);
     if (nitems)
         BTC_LAUNCH(items_kernel, nitems, s, rowptr, col, key, off.p, P.p, first.p, nitems, ne,
-                   p->use_tiny, tmp.p, sk0.p, sv0.p, st.p + 4);
+                   p->use_tiny, false, tmp.p, sk0.p, sv0.p, st.p + 4);
     const bool ialt = sort_pairs_u32(sk0.p, sv0.p, sk1.p, sv1.p, nitems, 32, s);
     p->items_dev.alloc((size_t)nitems * 2);
     if (nitems)
-        BTC_LAUNCH(place_items_kernel, nitems, s, tmp.p, ialt ? sv1.p : sv0.p, nitems, p->items_dev.p);
+        BTC_LAUNCH(place_items_kernel, nitems, s, tmp.p, ialt ? sv1.p : sv0.p, nitems, p->items_dev.p, nullptr,
+                   nullptr);
 
     // st: [0] dense edges, [1] sparse edges, [2] their merge work, [3] core
     // lists, [4] pivot row words over items, [5] wave items (tiny ones
@@ -485,11 +513,6 @@ unsigned cap_grid(int64_t cap) {
         BTC_CHECK_LAUNCH(#kernel);                                                     \
     } while (0)
 
-#define ONE_LAUNCH(kernel, s, ...)                                          \
-    do {                                                                    \
-        hipLaunchKernelGGL(kernel, dim3(1), dim3(64), 0, s, __VA_ARGS__);   \
-        BTC_CHECK_LAUNCH(#kernel);                                          \
-    } while (0)
 
 void plan_capacity(const PlanCap& c, hipStream_t s) {
     PipeCtl* ctl = c.ctl;
@@ -508,32 +531,30 @@ void plan_capacity(const PlanCap& c, hipStream_t s) {
     const bool alt = dsort_pairs_u32(c.k0, c.v0, c.k1, c.v1, (size_t)M, &ctl->ne, id_bits(N), c.ws, s);
     const uint32_t* key = alt ? c.k1 : c.k0;
     const uint32_t* val = alt ? c.v1 : c.v0;
-    CAP_LAUNCH(pivot_offsets_kernel, N + 1, s, key, ne, nc, c.off);
 
-    // hub-core bitmaps
+    // pivot offsets and hub-core rows, bitmaps
     const uint32_t C = c.core_ids;
-    CAP_LAUNCH(core_rows_kernel, N, s, rowptr, col, nc, C, c.cend, c.has);
+    CAP_LAUNCH(pivot_core_kernel, N + 1, s, key, ne, nc, rowptr, col, C, c.off, c.cend, c.has);
     dscan_u32(c.has, c.rbi, (size_t)N, &ctl->nc, &ctl->nbm, c.ws, s);
     if (C) {
-        ONE_LAUNCH(pipe_bitmap_words_kernel, s, ctl, C / 32);
-        CAP_LAUNCH(fill_u32_kernel, (int64_t)pipe_bitmap_rows(M, c.n_cap) * (C / 32), s, c.rowbits,
-                   Cnt::dev(&ctl->nbm_words), 0u);
+        CAP_LAUNCH(rowbits_clear_kernel, (int64_t)pipe_bitmap_rows(M, c.n_cap) * (C / 32), s, c.rowbits, &ctl->nbm,
+                   C / 32);
         CAP_LAUNCH(core_fill_kernel, N * 64, s, rowptr, col, c.cend, c.has, c.rbi, nc, C / 32, c.rowbits);
     }
     CAP_LAUNCH(records_kernel, M, s, rowptr, col, c.erow, val, ne, c.cend, c.has, c.rbi, C, c.lists, c.ur, c.cost,
                st + 3, st + 6);
     dscan_u64(c.cost, c.P, (size_t)M, &ctl->ne, reinterpret_cast<uint64_t*>(&ctl->checks), c.ws, s);
 
-    // work items (each opens at a list, so there are at most ne of them)
-    ONE_LAUNCH(pipe_chunk_kernel, s, ctl, (uint64_t)64 * (uint64_t)c.cus);
-    CAP_LAUNCH(item_start_kernel, M, s, rowptr, col, key, c.off, c.P, ne, (uint64_t)0, &ctl->chunk, c.start);
+    // work items (each opens at a list, so there are at most ne of them):
+    // about 64 per CU, sorted by class and then work in half octaves
+    CAP_LAUNCH(item_start_kernel, M, s, rowptr, col, key, c.off, c.P, ne, (uint64_t)0, &ctl->checks,
+               (uint64_t)64 * (uint64_t)c.cus, c.start);
     dscan_u32(c.start, c.spos, (size_t)M, &ctl->ne, &ctl->nitems, c.ws, s);
     CAP_LAUNCH(select_kernel, M, s, c.start, c.spos, ne, c.first);
-    CAP_LAUNCH(items_kernel, I, s, rowptr, col, key, c.off, c.P, c.first, nitems, ne, false, c.tmp_items, c.sk0,
-               c.sv0, st + 4);
-    const bool ialt = dsort_pairs_u32(c.sk0, c.sv0, c.sk1, c.sv1, (size_t)I, &ctl->nitems, 32, c.ws, s);
-    CAP_LAUNCH(place_items_kernel, I, s, c.tmp_items, ialt ? c.sv1 : c.sv0, nitems, c.items);
-    ONE_LAUNCH(pipe_items_kernel, s, ctl);
+    CAP_LAUNCH(items_kernel, I, s, rowptr, col, key, c.off, c.P, c.first, nitems, ne, false, true, c.tmp_items,
+               c.sk0, c.sv0, st + 4);
+    const bool ialt = dsort_pairs_u32(c.sk0, c.sv0, c.sk1, c.sv1, (size_t)I, &ctl->nitems, 8, c.ws, s);
+    CAP_LAUNCH(place_items_kernel, I, s, c.tmp_items, ialt ? c.sv1 : c.sv0, nitems, c.items, ctl, c.work);
 }
 
 }  // namespace btcx

@@ -41,6 +41,11 @@ void dscan_u32(const uint32_t* in, uint32_t* out, size_t cap, const uint32_t* d_
 void dscan_u64(const uint64_t* in, uint64_t* out, size_t cap, const uint32_t* d_n, uint64_t* d_total, void* ws,
                hipStream_t s);
 
+// Word of ws that a sort sets (and nothing clears) if a look-back ever ran
+// past its spin limit -- a sort whose result cannot be trusted.  The caller
+// clears it before first use.
+const uint32_t* dsort_error_word(const void* ws);
+
 // Stable radix sorts on bits [0, end_bit), as sort_u64 / sort_pairs_u32.
 uint64_t* dsort_u64(uint64_t* keys, uint64_t* tmp, size_t cap, const uint32_t* d_n, int end_bit, void* ws,
                     hipStream_t s);
