@@ -346,7 +346,7 @@ def batched_leg(btc, count, steps, sample, tune_ms):
             "gpu_build_from_pairs_s": round(t_gpu_build, 3), "gpu_build_exact": gpu_build_exact}
 
 
-def serving_leg(btc, requests, seed=1, canon_flags=None, prep_threads=None, group_graphs=None):
+def serving_leg(btc, requests, seed=1, canon_flags=None, prep_threads=None, group_graphs=None, captured_prep=None):
     """Poisson stream of exact-count queries: R-MAT graphs (edge factor 16)
     with n log-normal (median 2^13, sigma 1 in ln n, capped at 2^20), 30 %
     of requests repeat a graph submitted earlier.  A closed burst measures
@@ -363,7 +363,8 @@ def serving_leg(btc, requests, seed=1, canon_flags=None, prep_threads=None, grou
     warm = stream(48, 3 << 40)
 
     def run(reqs, qps):
-        sv = btc.Server(canon_flags=canon_flags, prep_threads=prep_threads, group_graphs=group_graphs)
+        sv = btc.Server(canon_flags=canon_flags, prep_threads=prep_threads, group_graphs=group_graphs,
+                        captured_prep=captured_prep)
         for key, g in warm:
             if g is not None:
                 sv.submit(key, g[0], g[1], g[2])
@@ -423,7 +424,7 @@ def serving_leg(btc, requests, seed=1, canon_flags=None, prep_threads=None, grou
            "distinct_graphs": len(graphs),
            "p50_prepare_ms": round(1e3 * float(np.percentile(wait, 50)), 3) if wait.size else None,
            "p50_launch_ms": round(1e3 * float(np.percentile(run_, 50)), 3)}
-    log("serving (canon flags %s): %s" % (canon_flags, out))
+    log("serving (canon flags %s, captured preparation %s): %s" % (canon_flags, captured_prep, out))
     return out
 
 
@@ -448,6 +449,9 @@ def main():
     ap.add_argument("--shares", type=int, default=1, help="time one GPU's share of 2-, 4- and 8-way splits")
     ap.add_argument("--serve-host-prep", type=int, default=1,
                     help="also run the serving leg with canonicalisation and planning on host threads")
+    ap.add_argument("--serve-eager-prep", type=int, default=1,
+                    help="also run the serving leg with eager (uncaptured, grouped) GPU preparation")
+    ap.add_argument("--serve-threads", type=int, default=0, help="prep threads of the serving legs (0 = default)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one D2D copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -602,8 +606,11 @@ def main():
 
     batched = (batched_leg(btc, args.batch, max(3, args.steps // 2), args.batch_sample, args.batch_tune_ms)
                if args.batch > 0 else None)
-    serving = serving_leg(btc, args.serve) if args.serve > 0 else None
-    serving_host = (serving_leg(btc, args.serve, canon_flags=btc.CANON_DEFAULT)
+    pt = args.serve_threads or None
+    serving = serving_leg(btc, args.serve, prep_threads=pt) if args.serve > 0 else None
+    serving_eager = (serving_leg(btc, args.serve, prep_threads=pt, captured_prep=0)
+                     if args.serve > 0 and args.serve_eager_prep else None)
+    serving_host = (serving_leg(btc, args.serve, canon_flags=btc.CANON_DEFAULT, prep_threads=pt)
                     if args.serve > 0 and args.serve_host_prep else None)
     dense = dense_leg(btc, args.dense) if args.dense > 0 else None
     configs = configs_leg(btc, args.configs) if args.configs > 0 else None
@@ -617,7 +624,8 @@ def main():
         "triangles": int(total0), "triangles_per_s": total0 / (wall / args.steps),
         "kernel_ms_median": statistics.median(kms), "canon_ms": round(canon_ms, 1), "plan_ms": round(build_ms, 1),
         "roofline": roofline, "cpu_baseline": cpu, "plain_hybrid": plain, "end_to_end": e2e, "sharding": sharding,
-        "batched": batched, "serving": serving, "serving_host_prep": serving_host, "dense_tiles": dense,
+        "batched": batched, "serving": serving, "serving_eager_prep": serving_eager,
+        "serving_host_prep": serving_host, "dense_tiles": dense,
         "other_configs": configs,
     }
     print(json.dumps(out), flush=True)

@@ -86,7 +86,7 @@ class PipelineInfo(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_batch", C.c_int32), ("prep_threads", C.c_int32),
-                ("canon_flags", C.c_int32), ("group_graphs", C.c_int32)]
+                ("canon_flags", C.c_int32), ("group_graphs", C.c_int32), ("captured_prep", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -169,6 +169,7 @@ _SIGS = {
     "btc_pipeline_count": (C.c_int, [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_float)]),
     "btc_pipeline_graph": (C.c_int, [vp, vp, C.POINTER(vp)]),
+    "btc_pipeline_export": (C.c_int, [vp, vp, C.POINTER(vp), C.POINTER(vp)]),
     "btc_pipeline_get_info": (C.c_int, [vp, C.POINTER(PipelineInfo)]),
     "btc_pipeline_free": (None, [vp]),
     "btc_server_opts_default": (None, [C.POINTER(ServerOpts)]),
@@ -493,6 +494,17 @@ class Pipeline:
         _check(lib().btc_pipeline_graph(self._h, stream, C.byref(h)))
         return Graph(h.value)
 
+    def export(self, stream=None):
+        """(Graph, Plan) of the last run, in buffers of their own."""
+        g, q = vp(), vp()
+        _check(lib().btc_pipeline_export(self._h, stream, C.byref(g), C.byref(q)))
+        graph = Graph(g.value)
+        plan = Plan.__new__(Plan)
+        plan.graph = graph
+        plan.opts = self.opts
+        plan._h = q
+        return graph, plan
+
     @property
     def info(self):
         pi = PipelineInfo()
@@ -661,11 +673,11 @@ class Server:
     """Request loop with continuous batching (btc_server_*)."""
 
     def __init__(self, hbm_budget_bytes=None, max_batch=None, prep_threads=None, canon_flags=None,
-                 group_graphs=None):
+                 group_graphs=None, captured_prep=None):
         o = ServerOpts()
         lib().btc_server_opts_default(C.byref(o))
         for k, v in dict(hbm_budget_bytes=hbm_budget_bytes, max_batch=max_batch, prep_threads=prep_threads,
-                         canon_flags=canon_flags, group_graphs=group_graphs).items():
+                         canon_flags=canon_flags, group_graphs=group_graphs, captured_prep=captured_prep).items():
             if v is not None:
                 setattr(o, k, v)
         h = vp()

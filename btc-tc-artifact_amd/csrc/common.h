@@ -70,13 +70,29 @@ template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool owned = true;  // false: a view into memory that another buffer owns (view())
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), owned(o.owned) { o.p = nullptr; o.n = 0; o.owned = true; }
     DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            owned = o.owned;
+            o.p = nullptr;
+            o.n = 0;
+            o.owned = true;
+        }
         return *this;
+    }
+    // Points at count elements at q, which the buffer does not free.
+    void view(T* q, size_t count) {
+        release();
+        p = q;
+        n = count;
+        owned = false;
     }
     ~DevBuf() { release(); }
     void alloc(size_t count) {
@@ -92,9 +108,10 @@ struct DevBuf {
         n = count;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        owned = true;
     }
     size_t bytes() const { return n * sizeof(T); }
 };
@@ -332,6 +349,9 @@ struct btc_graph_s {
     btcx::DevBuf<uint32_t> d_rowptr;  // n + 1 (m < 2^32 is enforced)
     btcx::DevBuf<int32_t> d_col;      // m + padding
     btcx::DevBuf<int32_t> d_labels;   // n, for graphs canonicalised on the device
+    // one allocation behind the device arrays above (views into it) and,
+    // for an exported pipeline run, behind its plan's arrays too
+    btcx::DevBuf<unsigned char> block;
 };
 
 namespace btcx {

@@ -211,27 +211,110 @@ void pipeline_run(btc_pipeline_s* p, const int32_t* src, const int32_t* dst, int
     p->ran = true;
 }
 
-btc_graph_s* pipeline_graph(btc_pipeline_s* p, hipStream_t s) {
-    std::lock_guard<std::mutex> lk(p->mu);
+// Copies of the last accepted run: its graph (with_plan false) or its graph
+// and wedge-kernel plan, carved from one allocation that the graph owns (the
+// plan's arrays are views into it, so the plan must go first).  The plan
+// counts like any other and later runs of the pipeline do not touch it.
+void pipeline_copy_locked(btc_pipeline_s* p, hipStream_t s, bool with_plan, btc_graph_s** gout,
+                          btc_plan_s** pout) {
     BTC_REQUIRE(p->ran, BTC_ERR_STATE, "no accepted run yet");
     const PipeCtl& c = p->last;
+    const PlanCap& pc = p->plan;
+    const size_t n = c.nc, m = c.m, ne = c.ne, ni = c.nitems, CW = pc.core_ids / 32;
+    const size_t lb = ne * sizeof(uint4), bb = (size_t)c.nbm * CW * 4, rb = ne * 4;
     auto g = std::make_unique<btc_graph_s>();
-    g->n = (int32_t)c.nc;
+    std::unique_ptr<btc_plan_s> q;
+    if (with_plan) q = std::make_unique<btc_plan_s>();
+    auto carve = [&](Carve& cv) {
+        uint32_t* rowptr = cv.take<uint32_t>(n + 1);
+        int32_t* col = cv.take<int32_t>(m + kColPad);
+        int32_t* labels = cv.take<int32_t>(std::max<size_t>(n, 1));
+        if (cv.base) {
+            g->d_rowptr.view(rowptr, n + 1);
+            g->d_col.view(col, m + kColPad);
+            g->d_labels.view(labels, std::max<size_t>(n, 1));
+        }
+        if (!with_plan) return;
+        unsigned char* L = cv.take<unsigned char>(lb + bb + rb);
+        uint4* items = cv.take<uint4>(std::max<size_t>(2 * ni, 1));
+        uint16_t* col16 = cv.take<uint16_t>(m + kColPad);
+        unsigned long long* scratch = cv.take<unsigned long long>(kWorkWords);
+        if (cv.base) {
+            q->lists.view(L, lb + bb + rb);
+            q->items_dev.view(items, std::max<size_t>(2 * ni, 1));
+            q->col16.view(col16, m + kColPad);
+            q->scratch.view(scratch, kWorkWords);
+        }
+    };
+    Carve sizing;
+    carve(sizing);
+    g->block.alloc(sizing.top);
+    Carve real;
+    real.base = g->block.p;
+    carve(real);
+    g->n = (int32_t)n;
     g->n_input = (int32_t)c.n;
-    g->m = c.m;
+    g->m = (int64_t)m;
     g->flags = p->flags;
     g->max_row = (int32_t)c.rs[0];
     g->wedges = (int64_t)c.rs[1];
     g->merge_work = (int64_t)c.rs[2];
     g->on_host = false;
-    g->d_rowptr.alloc((size_t)g->n + 1);
-    g->d_col.alloc((size_t)g->m + kColPad);
-    g->d_labels.alloc((size_t)std::max(g->n, 1));
-    BTC_HIP(hipMemcpyAsync(g->d_rowptr.p, p->canon.rowptr, ((size_t)g->n + 1) * 4, hipMemcpyDeviceToDevice, s));
-    BTC_HIP(hipMemcpyAsync(g->d_col.p, p->canon.col, ((size_t)g->m + kColPad) * 4, hipMemcpyDeviceToDevice, s));
-    if (g->n) BTC_HIP(hipMemcpyAsync(g->d_labels.p, p->labels, (size_t)g->n * 4, hipMemcpyDeviceToDevice, s));
+    BTC_HIP(hipMemcpyAsync(g->d_rowptr.p, p->canon.rowptr, (n + 1) * 4, hipMemcpyDeviceToDevice, s));
+    BTC_HIP(hipMemcpyAsync(g->d_col.p, p->canon.col, (m + kColPad) * 4, hipMemcpyDeviceToDevice, s));
+    if (n) BTC_HIP(hipMemcpyAsync(g->d_labels.p, p->labels, n * 4, hipMemcpyDeviceToDevice, s));
+    if (with_plan) {
+        unsigned char* L = q->lists.p;
+        if (lb) BTC_HIP(hipMemcpyAsync(L, pc.lists, lb, hipMemcpyDeviceToDevice, s));
+        if (bb) BTC_HIP(hipMemcpyAsync(L + lb, pc.rowbits, bb, hipMemcpyDeviceToDevice, s));
+        if (rb) BTC_HIP(hipMemcpyAsync(L + lb + bb, pc.ur, rb, hipMemcpyDeviceToDevice, s));
+        if (ni) BTC_HIP(hipMemcpyAsync(q->items_dev.p, pc.items, 2 * ni * sizeof(uint4), hipMemcpyDeviceToDevice, s));
+        BTC_HIP(hipMemcpyAsync(q->col16.p, p->col16, (m + kColPad) * 2, hipMemcpyDeviceToDevice, s));
+        BTC_HIP(hipMemsetAsync(q->scratch.p, 0, q->scratch.bytes(), s));
+    }
     BTC_HIP(hipStreamSynchronize(s));
-    return g.release();
+    if (with_plan) {
+        q->g = g.get();
+        q->K = 64;
+        q->tau = 16 * q->K;  // no dense tiles
+        q->core_ids = pc.core_ids;
+        q->use_tiny = false;
+        q->launch = p->opts;
+        q->list_p = reinterpret_cast<const uint4*>(q->lists.p);
+        q->rowbits_p = reinterpret_cast<const uint4*>(q->lists.p + lb);
+        q->urow_p = reinterpret_cast<const int32_t*>(q->lists.p + lb + bb);
+        q->items_p = q->items_dev.p;
+        q->nitems = (uint32_t)ni;
+        q->n_block_items = c.item_counts[0];
+        q->n_tiny_items = 0;
+        btc_plan_info& in = q->info;
+        in.sparse_edges = (int64_t)c.st[1];
+        in.sparse_merge_work = (int64_t)c.st[2];
+        in.core_lists = (int64_t)c.st[3];
+        in.core_rows = c.nbm;
+        in.wedge_checks = (int64_t)c.checks;
+        in.work_items = (int64_t)ni;
+        in.tile_k = q->K;
+        in.tau = q->tau;
+        in.core_ids = pc.core_ids;
+        in.stream_bytes = 4 * ((int64_t)c.checks + (int64_t)c.st[4]) - 2 * (int64_t)c.st[6] + 16 * (int64_t)ne +
+                          32 * (int64_t)ni;
+        in.device_bytes = (int64_t)(q->lists.bytes() + q->items_dev.bytes() + q->col16.bytes() + q->scratch.bytes());
+        *pout = q.release();
+    }
+    *gout = g.release();
+}
+
+btc_graph_s* pipeline_graph(btc_pipeline_s* p, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    btc_graph_s* g = nullptr;
+    pipeline_copy_locked(p, s, false, &g, nullptr);
+    return g;
+}
+
+void pipeline_export(btc_pipeline_s* p, hipStream_t s, btc_graph_s** gout, btc_plan_s** pout) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    pipeline_copy_locked(p, s, true, gout, pout);
 }
 
 }  // namespace
@@ -278,6 +361,15 @@ int btc_pipeline_graph(btc_pipeline p, void* stream, btc_graph* out) {
         BTC_REQUIRE(p && out, BTC_ERR_ARG, "NULL argument");
         *out = nullptr;
         *out = btcx::pipeline_graph(p, btcx::as_stream(stream));
+    });
+}
+
+int btc_pipeline_export(btc_pipeline p, void* stream, btc_graph* graph, btc_plan* plan) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(p && graph && plan, BTC_ERR_ARG, "NULL argument");
+        *graph = nullptr;
+        *plan = nullptr;
+        btcx::pipeline_export(p, btcx::as_stream(stream), graph, plan);
     });
 }
 

@@ -13,6 +13,12 @@
 //             graph)
 //          -> completion queue (btc_server_poll)
 //
+// With captured preparation (the default) a prep thread instead builds and
+// counts each new graph with one replay of a captured HIP graph
+// (btc_pipeline, one per size class per thread), answers the requests
+// waiting for it right away, and then publishes a copy of its L and plan
+// as a resident unit for later requests.
+//
 // Requests for a graph key that is resident (or being prepared) reuse it.
 // While a launch is on the GPU, new arrivals are prepared concurrently on
 // the prep streams.  Under light load units hold one graph; under heavy
@@ -45,6 +51,69 @@ using Clock = std::chrono::steady_clock;
 constexpr int kGroupGraphsMax = 64;
 constexpr size_t kGroupPairs = size_t(1) << 22;
 constexpr int64_t kGroupVertices = int64_t(1) << 25;  // keeps a group's vertex ids well inside int32
+// Captured preparation: graphs up to these sizes go through a pipeline of
+// their size class; a thread keeps at most kPipeCacheBytes of pipelines.
+constexpr int64_t kPipeMaxPairs = int64_t(1) << 23;
+constexpr int64_t kPipeMaxIds = int64_t(1) << 24;
+constexpr int64_t kPipeCacheBytes = int64_t(4) << 30;
+// Captured preparation turns to groups beyond this many queued graphs per
+// prep thread.
+constexpr size_t kBacklogPerThread = 8;
+
+int64_t pow2_at_least(int64_t x, int64_t lo) {
+    int64_t c = lo;
+    while (c < x) c <<= 1;
+    return c;
+}
+
+// A prep thread's pipelines by size class (ids, pairs), least recently used
+// dropped first beyond kPipeCacheBytes.
+struct PipeCache {
+    struct Slot {
+        btc_pipeline p = nullptr;
+        int64_t bytes = 0;
+        uint64_t used = 0;
+    };
+    std::vector<std::pair<std::pair<int32_t, int64_t>, Slot>> slots;
+    int64_t bytes = 0;
+    uint64_t tick = 0;
+    PipeCache() = default;
+    PipeCache(const PipeCache&) = delete;
+    PipeCache& operator=(const PipeCache&) = delete;
+    ~PipeCache() {
+        for (auto& x : slots) btc_pipeline_free(x.second.p);
+    }
+    // The pipeline of class (ncap, ecap), created on s when missing.
+    int get(int32_t ncap, int64_t ecap, uint32_t flags, const btc_count_opts& co, hipStream_t s, btc_pipeline* out) {
+        for (auto& x : slots)
+            if (x.first.first == ncap && x.first.second == ecap) {
+                x.second.used = ++tick;
+                *out = x.second.p;
+                return BTC_OK;
+            }
+        btc_pipeline p = nullptr;
+        int rc = btc_pipeline_create(ncap, ecap, flags, &co, s, &p);
+        if (rc != BTC_OK) return rc;
+        btc_pipeline_info pi;
+        btc_pipeline_get_info(p, &pi);
+        while (!slots.empty() && bytes + pi.device_bytes > kPipeCacheBytes) {
+            size_t old = 0;
+            for (size_t k = 1; k < slots.size(); ++k)
+                if (slots[k].second.used < slots[old].second.used) old = k;
+            bytes -= slots[old].second.bytes;
+            btc_pipeline_free(slots[old].second.p);
+            slots.erase(slots.begin() + (long)old);
+        }
+        Slot sl;
+        sl.p = p;
+        sl.bytes = pi.device_bytes;
+        sl.used = ++tick;
+        slots.emplace_back(std::make_pair(ncap, ecap), sl);
+        bytes += sl.bytes;
+        *out = p;
+        return BTC_OK;
+    }
+};
 
 struct Unit {  // what is prepared, resident and evicted together
     btc_graph graph = nullptr;  // a single graph: its L and plan
@@ -247,6 +316,74 @@ struct btc_server_s {
         return BTC_OK;
     }
 
+    // Captured preparation of one graph that fits a pipeline: the count
+    // answers the waiting requests at once; the exported L and plan become
+    // the resident unit.  Returns false when the graph is too large for a
+    // pipeline (nothing done then).
+    bool prepare_captured(PrepJob& job, uint32_t cf, const btc_count_opts& co, hipStream_t s, PipeCache& cache_p,
+                          std::shared_ptr<Unit>& out, int& rc) {
+        const int64_t nnz = (int64_t)job.src.size();
+        const int64_t nv = job_vertices(job);
+        if (nnz > kPipeMaxPairs || nv > kPipeMaxIds) return false;
+        if (nv < 0) {
+            rc = BTC_ERR_ARG;
+            return true;
+        }
+        btc_pipeline pipe = nullptr;
+        rc = cache_p.get((int32_t)pow2_at_least(nv, 1024), pow2_at_least(nnz, 4096), cf & BTC_CANON_DEFAULT, co, s,
+                         &pipe);
+        if (rc != BTC_OK) return true;
+        const double t_start = now();
+        uint64_t tri = 0;
+        rc = btc_pipeline_count(pipe, job.src.data(), job.dst.data(), nnz, job.n, 0, s, &tri, nullptr);
+        std::vector<int32_t>().swap(job.src);
+        std::vector<int32_t>().swap(job.dst);
+        if (rc != BTC_OK) return true;
+        {
+            // the requests waiting so far get their answer now
+            std::lock_guard<std::mutex> lk(mu);
+            Entry& e = cache[job.key];
+            auto w = waiting.find(job.key);
+            if (w != waiting.end()) {
+                for (Request& r : w->second) {
+                    --e.pins;
+                    r.t_start = t_start;
+                    finish(r, BTC_OK, tri, 1);
+                    ++stats.launched_requests;
+                }
+                waiting.erase(w);
+            }
+            ++stats.launches;
+            cv_done.notify_all();
+        }
+        btc_pipeline_info pi;
+        btc_pipeline_get_info(pipe, &pi);
+        const int64_t est = 24 * pi.m + 36 * (int64_t)pi.work_items + 12 * (int64_t)pi.n + (1 << 16);
+        if (!admit(est)) {
+            rc = BTC_ERR_STATE;
+            return true;
+        }
+        btc_graph g = nullptr;
+        btc_plan p = nullptr;
+        rc = btc_pipeline_export(pipe, s, &g, &p);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            resident_bytes -= est;
+        }
+        cv_prep.notify_all();
+        if (rc != BTC_OK) return true;
+        btc_plan_info info;
+        btc_plan_get_info(p, &info);
+        btc_graph_info gi;
+        btc_graph_get_info(g, &gi);
+        out = std::make_shared<Unit>();
+        out->graph = g;
+        out->plan = p;
+        out->bytes = info.device_bytes + 4 * (gi.m + 2 * (int64_t)gi.n + 1);
+        out->keys = {job.key};
+        return true;
+    }
+
     // One GPU canonicalisation and one block-diagonal plan for several graphs.
     int prepare_group(std::vector<PrepJob>& jobs, const std::vector<int64_t>& nv, uint32_t cf,
                       const btc_count_opts& co, hipStream_t s, btcx::PinnedBuf& pin, std::shared_ptr<Unit>& out) {
@@ -329,6 +466,8 @@ struct btc_server_s {
             // of the pairs, released before their stream
             btcx::Arena arena;
             btcx::PinnedBuf pin;
+            PipeCache pipes;
+            const bool captured = opts.captured_prep != 0 && (cf & BTC_CANON_DEVICE);
             for (;;) {
                 std::vector<PrepJob> jobs;
                 {
@@ -336,8 +475,14 @@ struct btc_server_s {
                     cv_prep.wait(lk, [&] { return stop || !prep_queue.empty(); });
                     if (stop) break;
                     size_t pairs = 0;
-                    // graphs are grouped only when canonicalised on the GPU
-                    const size_t cap = (cf & BTC_CANON_DEVICE) ? (size_t)opts.group_graphs : 1;
+                    // graphs are grouped only when canonicalised on the GPU; with
+                    // captured preparation only under a backlog (more than
+                    // kBacklogPerThread queued graphs per prep thread): a pipeline
+                    // answers one graph soonest, a group prepares many graphs
+                    // fastest
+                    const bool backlog = prep_queue.size() > kBacklogPerThread * (size_t)opts.prep_threads;
+                    const size_t cap =
+                        (cf & BTC_CANON_DEVICE) && (!captured || backlog) ? (size_t)opts.group_graphs : 1;
                     while (!prep_queue.empty() && jobs.size() < cap &&
                            (jobs.empty() || pairs + prep_queue.front().src.size() <= kGroupPairs)) {
                         pairs += prep_queue.front().src.size();
@@ -350,7 +495,9 @@ struct btc_server_s {
                 std::unique_ptr<btcx::ArenaScope> scope(new btcx::ArenaScope(arena, s));
                 if (jobs.size() == 1) {
                     std::shared_ptr<Unit> u;
-                    const int rc = prepare_single(jobs[0], cf, co, s, pin, u);
+                    int rc = BTC_OK;
+                    if (!captured || !prepare_captured(jobs[0], cf, co, s, pipes, u, rc))
+                        rc = prepare_single(jobs[0], cf, co, s, pin, u);
                     if (rc == BTC_OK) units.push_back(u);
                     else failed.emplace_back(jobs[0].key, rc);
                 } else {
@@ -499,6 +646,7 @@ void btc_server_opts_default(btc_server_opts* o) {
     o->prep_threads = 4;
     o->canon_flags = BTC_CANON_DEFAULT | BTC_CANON_DEVICE;
     o->group_graphs = 16;
+    o->captured_prep = 1;
 }
 
 int btc_server_create(const btc_server_opts* opts, btc_server* out) {
@@ -525,6 +673,13 @@ int btc_server_submit(btc_server sv, uint64_t graph_key, const int32_t* src, con
     return btcx::guard([&] {
         BTC_REQUIRE(sv && request_id, BTC_ERR_ARG, "NULL argument");
         BTC_REQUIRE(nnz >= 0 && (nnz == 0 || (src && dst)) , BTC_ERR_ARG, "bad edge arrays");
+        // the pairs are copied before the lock is taken: preparation threads
+        // and the GPU worker keep running meanwhile
+        PrepJob job;
+        job.key = graph_key;
+        job.src.assign(src, src + nnz);
+        job.dst.assign(dst, dst + nnz);
+        job.n = n;
         std::lock_guard<std::mutex> lk(sv->mu);
         BTC_REQUIRE(!sv->stop, BTC_ERR_STATE, "server is stopping");
         Request r;
@@ -539,11 +694,6 @@ int btc_server_submit(btc_server sv, uint64_t graph_key, const int32_t* src, con
                         (unsigned long long)graph_key);
             Entry& e = sv->cache[graph_key];
             e.pins = 1;
-            PrepJob job;
-            job.key = graph_key;
-            job.src.assign(src, src + nnz);
-            job.dst.assign(dst, dst + nnz);
-            job.n = n;
             sv->prep_queue.push_back(std::move(job));
             sv->waiting[graph_key].push_back(r);
             // prep threads and admission waiters share cv_prep: wake them all so

@@ -409,6 +409,11 @@ int btc_pipeline_count(btc_pipeline p, const int32_t* src, const int32_t* dst, i
 /* A copy of the last accepted run's canonical graph, in HBM: the same L and
  * labels as btc_graph_from_coo_device of the same pairs. */
 int btc_pipeline_graph(btc_pipeline p, void* stream, btc_graph* out);
+/* The last accepted run's L and plan, copied into buffers of their own:
+ * *graph is btc_pipeline_graph's graph, *plan a plan of it (no dense tiles,
+ * no tiny items) that btc_plan_count and the others count like any plan,
+ * unaffected by later runs.  Free the plan before the graph. */
+int btc_pipeline_export(btc_pipeline p, void* stream, btc_graph* graph, btc_plan* plan);
 int btc_pipeline_get_info(btc_pipeline p, btc_pipeline_info* info);
 void btc_pipeline_free(btc_pipeline p);
 
@@ -432,8 +437,16 @@ typedef struct {
     int32_t prep_threads;     /* host threads that canonicalise and plan; default 4 */
     int32_t canon_flags;      /* default BTC_CANON_DEFAULT | BTC_CANON_DEVICE (canonicalise and plan on the
                                  GPU, on each preparation thread's stream) */
-    int32_t group_graphs;     /* with GPU preparation, a prep thread takes up to this many queued new
-                                 graphs and prepares them as one unit; default 16, 1 = one at a time */
+    int32_t group_graphs;     /* with GPU preparation (and, when captured_prep is 1, a backlog), a prep
+                                 thread takes up to this many queued new graphs and prepares them as one
+                                 unit; default 16, 1 = one at a time */
+    int32_t captured_prep;    /* 1 (default, with GPU preparation): while at most eight new graphs per
+                                 prep thread are queued, a prep thread builds and counts each new graph of up to
+                                 2^23 pairs with one replay of a btc_pipeline of its size class (ids and
+                                 pairs rounded up to powers of two), answers the requests waiting for it at
+                                 once, then keeps a copy of its L and plan resident for later requests
+                                 (btc_pipeline_export); a longer queue is prepared in groups as with 0.
+                                 0: eager preparation (grouped), counts by the GPU worker */
 } btc_server_opts;
 
 typedef struct {

@@ -137,12 +137,13 @@ def serving_stream(gpu, orc):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("canon_flags", [3 | 4, 3])  # GPU / host preparation
-def test_serving_stream_matches_oracle(gpu, serving_stream, canon_flags):
+# GPU preparation captured (pipelines) or eager; host preparation
+@pytest.mark.parametrize("canon_flags,captured", [(3 | 4, 1), (3 | 4, 0), (3, 0)])
+def test_serving_stream_matches_oracle(gpu, serving_stream, canon_flags, captured):
     reqs, want = serving_stream
     rng = np.random.default_rng(5)
     gaps = rng.exponential(1.0 / 300.0, len(reqs))  # Poisson arrivals, 300 requests/s
-    sv = gpu.Server(canon_flags=canon_flags)
+    sv = gpu.Server(canon_flags=canon_flags, captured_prep=captured)
     t0 = time.perf_counter()
 
     def submit():

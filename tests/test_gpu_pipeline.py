@@ -131,3 +131,27 @@ def test_pipeline_rmat18(gpu, orc):
         assert p.count(s, d, 1 << 18)[0] == want
     same_graph(host, p.graph())
     p.close()
+
+
+def test_pipeline_export_is_independent(gpu, orc):
+    """An exported graph and plan count like any plan (totals, per-vertex
+    sums, replay, multi-plan launches) and keep doing so after the pipeline
+    runs other inputs."""
+    p = pipe(gpu, 3)
+    kept = []
+    for name in ["rmat15", "planted_dense", "karate", "star"]:
+        s, d, n = case(name, gpu)
+        want, want_pv = oracle_counts(orc, gpu.Graph.from_coo(s, d, n))
+        assert p.count(s, d, n)[0] == want
+        g, plan = p.export()
+        kept.append((name, g, plan, want, want_pv))
+    for name, g, plan, want, want_pv in kept:  # the pipeline has run other graphs since
+        t, pv, _ = plan.count(per_vertex=True)
+        assert t == want, name
+        np.testing.assert_array_equal(pv, want_pv)
+        info = plan.info
+        assert info["dense_tiles"] == 0 and info["work_items"] <= g.m
+        plan.capture()
+        assert plan.count()[0] == want
+    b = gpu.Batch([k[1] for k in kept])  # the exported graphs as ordinary graphs
+    np.testing.assert_array_equal(b.count()[1], [k[3] for k in kept])

@@ -19,6 +19,18 @@ def drain(sv, n, timeout=60.0):
     return out
 
 
+def settled(sv, prepared, timeout=30.0):
+    """Server stats once `prepared` graphs are resident: with captured
+    preparation a graph's requests are answered before its copy is
+    published."""
+    t0 = time.time()
+    st = sv.stats()
+    while st["prepared"] < prepared and time.time() - t0 < timeout:
+        time.sleep(0.01)
+        st = sv.stats()
+    return st
+
+
 def make_graphs(gpu, k, seed):
     rng = np.random.default_rng(seed)
     gs = []
@@ -43,11 +55,12 @@ def alone(orc, s, d, n):
     return orc.tc_merge(rp, col, threads=4)
 
 
-@pytest.mark.parametrize("canon_flags", [3, 3 | 4])  # host / GPU canonicalisation and planning
-def test_server_mix_equals_alone(gpu, orc, canon_flags):
+# host preparation; GPU preparation, eager (grouped units) or captured (pipelines)
+@pytest.mark.parametrize("canon_flags,captured", [(3, 0), (3 | 4, 0), (3 | 4, 1)])
+def test_server_mix_equals_alone(gpu, orc, canon_flags, captured):
     gs = make_graphs(gpu, 40, seed=1)
     want = [alone(orc, *g) for g in gs]
-    sv = gpu.Server(max_batch=16, prep_threads=4, canon_flags=canon_flags)
+    sv = gpu.Server(max_batch=16, prep_threads=4, canon_flags=canon_flags, captured_prep=captured)
     ids = {}
     for k, (s, d, n) in enumerate(gs):
         ids[sv.submit(k, s, d, n)] = k
@@ -59,7 +72,7 @@ def test_server_mix_equals_alone(gpu, orc, canon_flags):
         assert r["status"] == 0
         assert r["triangles"] == want[ids[r["request_id"]]]
         assert r["t_submit"] <= r["t_start"] <= r["t_done"]
-    st = sv.stats()
+    st = settled(sv, 40)
     assert st["completed"] == len(ids) and st["prepared"] == 40 and st["resident_hits"] == 14
     assert st["launches"] >= 1 and st["failed"] == 0
     sv.close()
@@ -79,25 +92,28 @@ def test_server_eviction_keeps_answers(gpu, orc):
     sv.close()
 
 
-def test_server_grouped_units_evict_and_stay_exact(gpu, orc):
-    """GPU preparation groups queued graphs into one unit; a tight budget
-    evicts whole idle units, and every answer stays exact."""
+@pytest.mark.parametrize("captured", [0, 1])
+def test_server_grouped_units_evict_and_stay_exact(gpu, orc, captured):
+    """Eager GPU preparation groups queued graphs into one unit, captured
+    preparation makes one unit per graph; a tight budget evicts idle units,
+    and every answer stays exact."""
     gs = make_graphs(gpu, 24, seed=5)
     want = [alone(orc, *g) for g in gs]
-    sv = gpu.Server(hbm_budget_bytes=4 << 20, max_batch=64, prep_threads=2)
+    sv = gpu.Server(hbm_budget_bytes=4 << 20, max_batch=64, prep_threads=2, captured_prep=captured)
     for rnd in range(3):
         base = 1000 * rnd  # fresh keys every round: earlier units must make room
         ids = {sv.submit(base + k, *g): k for k, g in enumerate(gs)}
         ids.update({sv.submit(base + k): k for k in range(0, 24, 5)})  # repeats
         for r in drain(sv, len(ids)):
             assert r["status"] == 0 and r["triangles"] == want[ids[r["request_id"]]]
-    st = sv.stats()
+    st = settled(sv, 72)
     assert st["evictions"] > 0 and st["prepared"] == 72 and st["failed"] == 0
     sv.close()
 
 
-def test_server_bad_graph_fails_alone(gpu, orc):
-    sv = gpu.Server(prep_threads=2)
+@pytest.mark.parametrize("captured", [0, 1])
+def test_server_bad_graph_fails_alone(gpu, orc, captured):
+    sv = gpu.Server(prep_threads=2, captured_prep=captured)
     good = make_graphs(gpu, 3, seed=3)
     want = [alone(orc, *g) for g in good]
     bad = sv.submit(99, np.array([0, 7], np.int32), np.array([1, 2], np.int32), 5)  # id 7 >= n
