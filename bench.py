@@ -428,6 +428,51 @@ def serving_leg(btc, requests, seed=1, canon_flags=None, prep_threads=None, grou
     return out
 
 
+def pipeline_leg(btc, graphs, seed=3):
+    """Captured build + count (btc_pipeline) against the eager device path on
+    `graphs` new graphs of the serving workload: one pipeline per size class
+    (ids and pairs rounded up to powers of two), every graph run through it
+    twice (the second run timed: wall from host pairs to total, and the
+    replay by HIP events) and built eagerly (btc_graph_from_coo_device path +
+    plan + count, wall); every total equal to the eager one."""
+    work = [g for _, g in btc.serving_workload(3 * graphs, seed=seed) if g is not None][:graphs]
+
+    def cls(n, nnz):
+        return 1 << max(10, (n - 1).bit_length()), 1 << max(12, (max(nnz, 1) - 1).bit_length())
+
+    pipes = {}
+    t0 = time.perf_counter()
+    for s, d, n in work:
+        c = cls(n, s.size)
+        if c not in pipes:
+            pipes[c] = btc.Pipeline(c[0], c[1], btc.CANON_DEFAULT)
+    create_s = time.perf_counter() - t0
+    eager, wall, replay, exact = [], [], [], True
+    for s, d, n in work:
+        p = pipes[cls(n, s.size)]
+        p.count(s, d, n)
+        t1 = time.perf_counter()
+        got, ms = p.count(s, d, n)
+        t2 = time.perf_counter()
+        want = btc.Plan(btc.Graph.from_coo(s, d, n, btc.CANON_DEFAULT | btc.CANON_DEVICE)).count()[0]
+        t3 = time.perf_counter()
+        exact = exact and got == want
+        wall.append(t2 - t1)
+        eager.append(t3 - t2)
+        replay.append(ms)
+    nodes = sorted({p.info["graph_nodes"] for p in pipes.values()})
+    out = {"graphs": len(work), "size_classes": len(pipes), "graph_nodes": nodes,
+           "create_s": round(create_s, 3), "median_pairs": int(np.median([g[0].size for g in work])),
+           "pipeline_wall_ms_p50": round(1e3 * float(np.median(wall)), 3),
+           "replay_ms_p50": round(float(np.median(replay)), 3),
+           "eager_wall_ms_p50": round(1e3 * float(np.median(eager)), 3),
+           "speedup_p50": round(float(np.median(eager)) / float(np.median(wall)), 2), "exact": exact}
+    log("pipeline: %s" % out)
+    for p in pipes.values():
+        p.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -452,6 +497,8 @@ def main():
     ap.add_argument("--serve-eager-prep", type=int, default=1,
                     help="also run the serving leg with eager (uncaptured, grouped) GPU preparation")
     ap.add_argument("--serve-threads", type=int, default=0, help="prep threads of the serving legs (0 = default)")
+    ap.add_argument("--pipeline", type=int, default=96,
+                    help="serving-workload graphs run through captured build + count pipelines (0 = skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one D2D copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -612,6 +659,7 @@ def main():
                      if args.serve > 0 and args.serve_eager_prep else None)
     serving_host = (serving_leg(btc, args.serve, canon_flags=btc.CANON_DEFAULT, prep_threads=pt)
                     if args.serve > 0 and args.serve_host_prep else None)
+    pipeline = pipeline_leg(btc, args.pipeline) if args.pipeline > 0 else None
     dense = dense_leg(btc, args.dense) if args.dense > 0 else None
     configs = configs_leg(btc, args.configs) if args.configs > 0 else None
 
@@ -624,7 +672,7 @@ def main():
         "triangles": int(total0), "triangles_per_s": total0 / (wall / args.steps),
         "kernel_ms_median": statistics.median(kms), "canon_ms": round(canon_ms, 1), "plan_ms": round(build_ms, 1),
         "roofline": roofline, "cpu_baseline": cpu, "plain_hybrid": plain, "end_to_end": e2e, "sharding": sharding,
-        "batched": batched, "serving": serving, "serving_eager_prep": serving_eager,
+        "batched": batched, "serving": serving, "serving_eager_prep": serving_eager, "pipeline": pipeline,
         "serving_host_prep": serving_host, "dense_tiles": dense,
         "other_configs": configs,
     }

@@ -185,8 +185,9 @@ int btc_batch_from_coo(const int32_t* src, const int32_t* dst, const int64_t* ed
             ds.alloc((size_t)pairs, s);
             dd.alloc((size_t)pairs, s);
             if (pairs) {
-                BTC_HIP(hipMemcpyAsync(ds.p, hs.data(), (size_t)pairs * 4, hipMemcpyHostToDevice, s));
-                BTC_HIP(hipMemcpyAsync(dd.p, hd.data(), (size_t)pairs * 4, hipMemcpyHostToDevice, s));
+                const void* h[2] = {hs.data(), hd.data()};
+                void* d[2] = {ds.p, dd.p};
+                btcx::upload_arrays(h, d, 2, (size_t)pairs * 4, s);
             }
             btcx::canonicalise_device(ds.p, dd.p, pairs, (int32_t)segs.back(), flags & ~BTC_CANON_DEVICE, s, &G,
                                       &segs, &csegs);

@@ -108,8 +108,9 @@ int btc_graph_from_coo(const int32_t* src, const int32_t* dst, int64_t nnz, int3
             ds.alloc((size_t)nnz);
             dd.alloc((size_t)nnz);
             if (nnz) {
-                BTC_HIP(hipMemcpy(ds.p, src, (size_t)nnz * 4, hipMemcpyHostToDevice));
-                BTC_HIP(hipMemcpy(dd.p, dst, (size_t)nnz * 4, hipMemcpyHostToDevice));
+                const void* h[2] = {src, dst};
+                void* d[2] = {ds.p, dd.p};
+                btcx::upload_arrays(h, d, 2, (size_t)nnz * 4, nullptr);
             }
             auto g = std::make_unique<btc_graph_s>();
             btcx::canonicalise_device(ds.p, dd.p, nnz, n, flags, nullptr, g.get(), nullptr, nullptr);

@@ -19,6 +19,8 @@ namespace btcx {
 namespace {
 
 constexpr int kT = 256;
+constexpr unsigned kRedBlocks = 1024;    // eager reductions
+constexpr unsigned kCapRedBlocks = 256;  // capacity-sized reductions: about one per CU
 
 int id_bits(int64_t n) {
     int b = 1;
@@ -358,6 +360,14 @@ __global__ void pipe_identity_kernel(PipeCtl* __restrict__ c, int32_t* __restric
         BTC_CHECK_LAUNCH(#kernel);                                                      \
     } while (0)
 
+// Kernels that end in same-address atomics (one per workgroup) run on at
+// most kRedBlocks workgroups.
+#define BTC_LAUNCH_RED(kernel, n, s, ...)                                                   \
+    do {                                                                                \
+        hipLaunchKernelGGL(kernel, dim3(std::min(grid_for(n), kRedBlocks)), dim3(kT), 0, s, __VA_ARGS__);     \
+        BTC_CHECK_LAUNCH(#kernel);                                                      \
+    } while (0)
+
 }  // namespace
 
 void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n, uint32_t flags,
@@ -376,7 +386,7 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
         TmpBuf<int32_t> lohi;
         lohi.alloc(2, s);
         BTC_HIP(hipMemcpyAsync(lohi.p, h_lohi, 8, hipMemcpyHostToDevice, s));
-        if (nnz) BTC_LAUNCH(id_range_kernel, nnz, s, src, dst, nnz, lohi.p);
+        if (nnz) BTC_LAUNCH_RED(id_range_kernel, nnz, s, src, dst, nnz, lohi.p);
         BTC_HIP(hipMemcpyAsync(h_lohi, lohi.p, 8, hipMemcpyDeviceToHost, s));
         BTC_HIP(hipStreamSynchronize(s));
     }
@@ -467,7 +477,7 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
         lab2.alloc((size_t)nc);
         BTC_HIP(hipMemsetAsync(deg.p, 0, (size_t)nc * 4, s));
         if (m) {
-            hipLaunchKernelGGL(degree_kernel, dim3(std::min(grid_for(m), 1024u)), dim3(kT), 0, s, keys, m, bn, deg.p);
+            hipLaunchKernelGGL(degree_kernel, dim3(std::min(grid_for(m), kRedBlocks)), dim3(kT), 0, s, keys, m, bn, deg.p);
             BTC_CHECK_LAUNCH("degree_kernel");
         }
         BTC_LAUNCH(rank_keys_kernel, nc, s, deg.p, nc, 0xFFFFFFFFu, k0.p, v0.p);
@@ -504,7 +514,7 @@ void canonicalise_device(const int32_t* src, const int32_t* dst, int64_t nnz, in
     TmpBuf<unsigned long long> st;
     st.alloc(3, s);
     BTC_HIP(hipMemsetAsync(st.p, 0, 24, s));
-    if (nc) BTC_LAUNCH(stats_kernel, std::max<int64_t>(nc, m), s, g->d_rowptr.p, g->d_col.p, nc, m, st.p);
+    if (nc) BTC_LAUNCH_RED(stats_kernel, std::max<int64_t>(nc, m), s, g->d_rowptr.p, g->d_col.p, nc, m, st.p);
     unsigned long long h[3] = {0, 0, 0};
     BTC_HIP(hipMemcpyAsync(h, st.p, 24, hipMemcpyDeviceToHost, s));
     BTC_HIP(hipStreamSynchronize(s));
@@ -530,6 +540,12 @@ unsigned cap_grid(int64_t cap) {
         BTC_CHECK_LAUNCH(#kernel);                                                     \
     } while (0)
 
+#define CAP_LAUNCH_RED(kernel, cap, s, ...)                                                \
+    do {                                                                               \
+        hipLaunchKernelGGL(kernel, dim3(std::min(cap_grid(cap), kCapRedBlocks)), dim3(kT), 0, s, __VA_ARGS__);  \
+        BTC_CHECK_LAUNCH(#kernel);                                                     \
+    } while (0)
+
 int32_t* canonicalise_capacity(const CanonCap& c, hipStream_t s) {
     PipeCtl* ctl = c.ctl;
     const int64_t E = c.nnz_cap, N = c.n_cap;
@@ -538,7 +554,7 @@ int32_t* canonicalise_capacity(const CanonCap& c, hipStream_t s) {
     const Cnt nnz = Cnt::dev(&ctl->nnz), n = Cnt::dev(&ctl->n), m = Cnt::dev(&ctl->m), nc = Cnt::dev(&ctl->nc);
     hipLaunchKernelGGL(pipe_reset_kernel, dim3(1), dim3(64), 0, s, ctl);
     BTC_CHECK_LAUNCH("pipe_reset_kernel");
-    CAP_LAUNCH(id_range_kernel, E, s, c.src, c.dst, nnz, &ctl->lo);
+    CAP_LAUNCH_RED(id_range_kernel, E, s, c.src, c.dst, nnz, &ctl->lo);
 
     // oriented keys, sorted, deduplicated
     CAP_LAUNCH(orient_checked_kernel, E, s, c.src, c.dst, ctl, c.n_cap, b, sentinel, c.ka);
@@ -563,7 +579,7 @@ int32_t* canonicalise_capacity(const CanonCap& c, hipStream_t s) {
     if (c.flags & BTC_CANON_DEGREE_ORDER) {
         // keys N - 1 - degree < 2^b: descending degree, ties by id
         CAP_LAUNCH(fill_u32_kernel, N, s, c.deg, nc, 0u);
-        hipLaunchKernelGGL(degree_kernel, dim3(std::min(cap_grid(E), 1024u)), dim3(kT), 0, s, keys, m, b, c.deg);
+        hipLaunchKernelGGL(degree_kernel, dim3(std::min(cap_grid(E), kCapRedBlocks)), dim3(kT), 0, s, keys, m, b, c.deg);
         BTC_CHECK_LAUNCH("degree_kernel");
         CAP_LAUNCH(rank_keys_kernel, N, s, c.deg, nc, (uint32_t)(N - 1), c.k0, c.v0);
         const bool alt = dsort_pairs_u32(c.k0, c.v0, c.k1, c.v1, (size_t)N, &ctl->nc, b, c.ws, s);
@@ -575,7 +591,7 @@ int32_t* canonicalise_capacity(const CanonCap& c, hipStream_t s) {
 
     // CSR of L (with its 16-bit copy) and its statistics
     CAP_LAUNCH(csr_kernel, std::max(N + 1, E + (int64_t)kColPad), s, keys, m, b, nc, c.rowptr, c.col, c.col16);
-    CAP_LAUNCH(stats_kernel, std::max(N, E), s, c.rowptr, c.col, nc, m, ctl->rs);
+    CAP_LAUNCH_RED(stats_kernel, std::max(N, E), s, c.rowptr, c.col, nc, m, ctl->rs);
     return labels;
 }
 

@@ -218,6 +218,12 @@ struct PinnedBuf {
     char* reserve(size_t bytes);
 };
 
+// Copies count host arrays of `bytes` each into device memory (dev[k]) on
+// stream s and waits: through a thread's cached pinned staging, two 16 MiB
+// chunks at a time for large arrays (host threads fill one chunk while the
+// DMA engine drains the other).
+void upload_arrays(const void* const* host, void* const* dev, int count, size_t bytes, hipStream_t s);
+
 // A HIP graph captured from a sequence of kernel launches, replayed with
 // one hipGraphLaunch.  Captured work contains kernels only (buffers are
 // cleared by fill kernels, not memset nodes).

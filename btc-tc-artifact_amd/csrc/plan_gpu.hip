@@ -19,6 +19,8 @@ namespace btcx {
 namespace {
 
 constexpr int kT = 256;
+constexpr unsigned kRedBlocks = 1024;    // eager reductions
+constexpr unsigned kCapRedBlocks = 256;  // capacity-sized reductions: about one per CU
 
 unsigned grid_for(int64_t n) {
     const int64_t g = (n + kT - 1) / kT;
@@ -36,6 +38,14 @@ int id_bits(int64_t n) {
 #define BTC_LAUNCH(kernel, n, s, ...)                                               \
     do {                                                                            \
         hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(kT), 0, s, __VA_ARGS__); \
+        BTC_CHECK_LAUNCH(#kernel);                                                  \
+    } while (0)
+
+// Kernels that end in same-address atomics (one per workgroup) run on at
+// most kRedBlocks workgroups.
+#define BTC_LAUNCH_RED(kernel, n, s, ...)                                               \
+    do {                                                                            \
+        hipLaunchKernelGGL(kernel, dim3(std::min(grid_for(n), kRedBlocks)), dim3(kT), 0, s, __VA_ARGS__); \
         BTC_CHECK_LAUNCH(#kernel);                                                  \
     } while (0)
 
@@ -378,7 +388,7 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
         flag.alloc(b->nblk, s);
         pos.alloc((size_t)b->nblk + 1, s);
         launch_tile_popcount(b, pop.p, s);
-        BTC_LAUNCH(dense_tiles_kernel, b->nblk, s, pop.p, b->nblk, p->tau, flag.p, st.p + 0);
+        BTC_LAUNCH_RED(dense_tiles_kernel, b->nblk, s, pop.p, b->nblk, p->tau, flag.p, st.p + 0);
         exclusive_scan_u32(flag.p, pos.p, b->nblk, s);
         const uint32_t nd = read_dev(pos.p + b->nblk, s);
         if (nd) {
@@ -399,7 +409,7 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     TmpBuf<uint32_t> lflag, lpos;
     lflag.alloc((size_t)m, s);
     lpos.alloc((size_t)m + 1, s);
-    if (m) BTC_LAUNCH(clip_kernel, m, s, rowptr, col, erow.p, p->n_dense ? dense.p : nullptr, m, lflag.p, st.p + 1);
+    if (m) BTC_LAUNCH_RED(clip_kernel, m, s, rowptr, col, erow.p, p->n_dense ? dense.p : nullptr, m, lflag.p, st.p + 1);
     exclusive_scan_u32(lflag.p, lpos.p, (size_t)m, s);
     const uint32_t ne = read_dev(lpos.p + m, s);
     TmpBuf<uint32_t> k0, v0, k1, v1;
@@ -442,7 +452,7 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     cost.alloc(ne, s);
     P.alloc((size_t)ne + 1, s);
     if (ne)
-        BTC_LAUNCH(records_kernel, ne, s, rowptr, col, erow.p, val, ne, cend.p, has.p, rbi.p, C, lst, ur, cost.p,
+        BTC_LAUNCH_RED(records_kernel, ne, s, rowptr, col, erow.p, val, ne, cend.p, has.p, rbi.p, C, lst, ur, cost.p,
                    st.p + 3, st.p + 6);
     exclusive_scan_u64(cost.p, P.p, ne, s);
     erow.release();
@@ -465,7 +475,7 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     sk1.alloc(nitems, s);
     sv1.alloc(nitems, s);
     if (nitems)
-        BTC_LAUNCH(items_kernel, nitems, s, rowptr, col, key, off.p, P.p, first.p, nitems, ne,
+        BTC_LAUNCH_RED(items_kernel, nitems, s, rowptr, col, key, off.p, P.p, first.p, nitems, ne,
                    p->use_tiny, false, tmp.p, sk0.p, sv0.p, st.p + 4);
     const bool ialt = sort_pairs_u32(sk0.p, sv0.p, sk1.p, sv1.p, nitems, 32, s);
     p->items_dev.alloc((size_t)nitems * 2);
@@ -513,6 +523,12 @@ unsigned cap_grid(int64_t cap) {
         BTC_CHECK_LAUNCH(#kernel);                                                     \
     } while (0)
 
+#define CAP_LAUNCH_RED(kernel, cap, s, ...)                                                \
+    do {                                                                               \
+        hipLaunchKernelGGL(kernel, dim3(std::min(cap_grid(cap), kCapRedBlocks)), dim3(kT), 0, s, __VA_ARGS__);  \
+        BTC_CHECK_LAUNCH(#kernel);                                                     \
+    } while (0)
+
 
 void plan_capacity(const PlanCap& c, hipStream_t s) {
     PipeCtl* ctl = c.ctl;
@@ -525,7 +541,7 @@ void plan_capacity(const PlanCap& c, hipStream_t s) {
 
     CAP_LAUNCH(edge_rows_kernel, N * 64, s, rowptr, nc, c.erow);
     // prefix lists: flag, compact (pivot, edge), stable sort by pivot
-    CAP_LAUNCH(clip_kernel, M, s, rowptr, col, c.erow, nullptr, m, c.lflag, st + 1);
+    CAP_LAUNCH_RED(clip_kernel, M, s, rowptr, col, c.erow, nullptr, m, c.lflag, st + 1);
     dscan_u32(c.lflag, c.lpos, (size_t)M, &ctl->m, &ctl->ne, c.ws, s);
     CAP_LAUNCH(list_keys_kernel, M, s, c.lflag, c.lpos, col, m, c.k0, c.v0);
     const bool alt = dsort_pairs_u32(c.k0, c.v0, c.k1, c.v1, (size_t)M, &ctl->ne, id_bits(N), c.ws, s);
@@ -541,7 +557,7 @@ void plan_capacity(const PlanCap& c, hipStream_t s) {
                    C / 32);
         CAP_LAUNCH(core_fill_kernel, N * 64, s, rowptr, col, c.cend, c.has, c.rbi, nc, C / 32, c.rowbits);
     }
-    CAP_LAUNCH(records_kernel, M, s, rowptr, col, c.erow, val, ne, c.cend, c.has, c.rbi, C, c.lists, c.ur, c.cost,
+    CAP_LAUNCH_RED(records_kernel, M, s, rowptr, col, c.erow, val, ne, c.cend, c.has, c.rbi, C, c.lists, c.ur, c.cost,
                st + 3, st + 6);
     dscan_u64(c.cost, c.P, (size_t)M, &ctl->ne, reinterpret_cast<uint64_t*>(&ctl->checks), c.ws, s);
 
@@ -551,7 +567,7 @@ void plan_capacity(const PlanCap& c, hipStream_t s) {
                (uint64_t)64 * (uint64_t)c.cus, c.start);
     dscan_u32(c.start, c.spos, (size_t)M, &ctl->ne, &ctl->nitems, c.ws, s);
     CAP_LAUNCH(select_kernel, M, s, c.start, c.spos, ne, c.first);
-    CAP_LAUNCH(items_kernel, I, s, rowptr, col, key, c.off, c.P, c.first, nitems, ne, false, true, c.tmp_items,
+    CAP_LAUNCH_RED(items_kernel, I, s, rowptr, col, key, c.off, c.P, c.first, nitems, ne, false, true, c.tmp_items,
                c.sk0, c.sv0, st + 4);
     const bool ialt = dsort_pairs_u32(c.sk0, c.sv0, c.sk1, c.sv1, (size_t)I, &ctl->nitems, 8, c.ws, s);
     CAP_LAUNCH(place_items_kernel, I, s, c.tmp_items, ialt ? c.sv1 : c.sv0, nitems, c.items, ctl, c.work);

@@ -1,5 +1,6 @@
 // Error state, version, device selection, event timing and the scalar
 // dispatch rules shared by every path.
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
@@ -115,6 +116,59 @@ char* PinnedBuf::reserve(size_t bytes) {
     BTC_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), c, hipHostMallocDefault));
     cap = c;
     return p;
+}
+
+void upload_arrays(const void* const* host, void* const* dev, int count, size_t bytes, hipStream_t s) {
+    constexpr size_t kChunk = size_t(16) << 20;
+    if (bytes == 0) return;
+    if (bytes <= kChunk / 4) {  // small: one staging copy each
+        static thread_local PinnedBuf small;
+        char* b = small.reserve(bytes * (size_t)count);
+        for (int k = 0; k < count; ++k) {
+            std::memcpy(b + (size_t)k * bytes, host[k], bytes);
+            BTC_HIP(hipMemcpyAsync(dev[k], b + (size_t)k * bytes, bytes, hipMemcpyHostToDevice, s));
+        }
+        BTC_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    // two pinned chunks: host threads fill one while the DMA engine drains
+    // the other (each chunk's event says when it may be refilled)
+    static thread_local PinnedBuf stage;
+    char* buf = stage.reserve(2 * kChunk);
+    hipEvent_t done[2] = {nullptr, nullptr};
+    BTC_HIP(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+    hipError_t e = hipEventCreateWithFlags(&done[1], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(done[0]);
+        BTC_HIP(e);
+    }
+    bool used[2] = {false, false};
+    int slot = 0;
+    e = hipSuccess;
+    for (int k = 0; k < count && e == hipSuccess; ++k) {
+        const char* src = static_cast<const char*>(host[k]);
+        char* dst = static_cast<char*>(dev[k]);
+        for (size_t off = 0; off < bytes && e == hipSuccess; off += kChunk) {
+            const size_t len = std::min(kChunk, bytes - off);
+            char* c = buf + (size_t)slot * kChunk;
+            if (used[slot]) e = hipEventSynchronize(done[slot]);
+            if (e != hipSuccess) break;
+            const int parts = (int)std::max<size_t>(1, std::min<size_t>(16, len >> 20));
+#pragma omp parallel for num_threads(parts) schedule(static)
+            for (int t = 0; t < parts; ++t) {
+                const size_t a = len * (size_t)t / (size_t)parts, b = len * (size_t)(t + 1) / (size_t)parts;
+                std::memcpy(c + a, src + off + a, b - a);
+            }
+            e = hipMemcpyAsync(dst + off, c, len, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipEventRecord(done[slot], s);
+            used[slot] = true;
+            slot ^= 1;
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipEventDestroy(done[0]);
+    (void)hipEventDestroy(done[1]);
+    BTC_HIP(e);
 }
 
 void PvBuf::copy_out(int32_t* h32, int64_t* h64, hipStream_t s) const {
