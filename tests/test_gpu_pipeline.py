@@ -155,3 +155,22 @@ def test_pipeline_export_is_independent(gpu, orc):
         assert plan.count()[0] == want
     b = gpu.Batch([k[1] for k in kept])  # the exported graphs as ordinary graphs
     np.testing.assert_array_equal(b.count()[1], [k[3] for k in kept])
+
+
+@pytest.mark.parametrize("nnz", [1, 2047, 2048, 2049, 4095, 4096, 4097, 8191, 12289])
+def test_pipeline_tile_boundaries(gpu, orc, nnz):
+    """Pair counts around the sort's 2048-key tiles and the scan's 4096-id
+    tiles, ids around 1024 / 4096, duplicates and loops mixed in: one
+    pipeline replays them all and matches the host canonicaliser and the
+    oracle every time."""
+    p = pipe(gpu, 3, n_cap=1 << 13, nnz_cap=16384)
+    rng = np.random.default_rng(nnz)
+    for n in (1023, 1024, 1025, 4097, 8192):
+        s = rng.integers(0, n, nnz).astype(np.int32)
+        d = rng.integers(0, n, nnz).astype(np.int32)
+        d[: nnz // 7] = s[: nnz // 7]  # self-loops
+        s[nnz // 2:] = np.minimum(s[nnz // 2:], 40)  # a hub and many duplicates
+        host = gpu.Graph.from_coo(s, d, n, 3)
+        want, _ = oracle_counts(orc, host)
+        assert p.count(s, d, n)[0] == want, (nnz, n)
+        same_graph(host, p.graph())
