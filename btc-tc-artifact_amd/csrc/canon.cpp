@@ -103,10 +103,11 @@ int btc_graph_from_coo(const int32_t* src, const int32_t* dst, int64_t nnz, int3
         BTC_REQUIRE(nnz == 0 || (src && dst), BTC_ERR_ARG, "src/dst is NULL");
         BTC_REQUIRE((flags & ~(BTC_CANON_DEFAULT | BTC_CANON_DEVICE)) == 0, BTC_ERR_ARG, "unknown flags 0x%x", flags);
         if (flags & BTC_CANON_DEVICE) {
-            // stage the pairs in HBM and canonicalise there
-            btcx::DevBuf<int32_t> ds, dd;
-            ds.alloc((size_t)nnz);
-            dd.alloc((size_t)nnz);
+            // stage the pairs in HBM (stream-ordered buffers: freeing them
+            // does not wait for the whole device) and canonicalise there
+            btcx::TmpBuf<int32_t> ds, dd;
+            ds.alloc((size_t)nnz, nullptr);
+            dd.alloc((size_t)nnz, nullptr);
             if (nnz) {
                 const void* h[2] = {src, dst};
                 void* d[2] = {ds.p, dd.p};

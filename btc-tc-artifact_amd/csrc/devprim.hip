@@ -3,14 +3,21 @@
 // no host synchronisation -- so a stream capture of them replays for any
 // count up to the capacity their buffers were sized for (see prim.h).
 //
-// Scan: tile sums, one workgroup scanning the tile sums, then each tile
-// applies its offset.  Sort: least-significant-digit radix sort, 8 bits a
+// Scan: one kernel; workgroups take 4096-element tiles in ticket order,
+// publish each tile's sum and find its offset by decoupled look-back over
+// the tiles before it.  Sort: least-significant-digit radix sort, 8 bits a
 // pass, in one sweep per pass: one kernel takes the digit totals of every
 // pass in a single read of the keys; each pass kernel then ranks a tile
 // stably in LDS (rocPRIM's block_radix_rank, match form: per-wave ballots
-// of equal digits) and finds the tile's offset for each digit by
-// decoupled look-back over the tiles before it -- 2 + passes kernels a
-// sort.
+// of equal digits) and finds the tile's offset for each digit by decoupled
+// look-back -- 1 + passes kernels a sort.
+//
+// Every look-back waits only on tiles whose tickets were taken by running
+// workgroups, which publish before they wait, so every wait ends (a spin
+// limit sets an error word rather than waiting on).  The workspace starts
+// zeroed and every scan and sort leaves its tickets, totals and statuses
+// zeroed again: the last workgroup to finish a scan, or a sort's last pass,
+// clears them, so no clearing kernel or memset is needed between uses.
 #include <rocprim/block/block_radix_rank.hpp>
 
 #include "common.h"
@@ -57,53 +64,84 @@ __device__ T block_scan_ex(T v, T* lds, T* total) {
     return base + x - v;
 }
 
-template <class T>
-__global__ __launch_bounds__(kT) void dscan_tiles(const T* __restrict__ in, const uint32_t* __restrict__ d_n,
-                                                  T* __restrict__ sums) {
-    __shared__ T lds[kT / 64];
-    const uint32_t n = *d_n;
-    const uint32_t nt = (n + kScanTile - 1) / kScanTile;
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
-        const size_t base = (size_t)t * kScanTile + (size_t)threadIdx.x * kScanPer;
-        T v = 0;
-#pragma unroll
-        for (int k = 0; k < kScanPer; ++k)
-            if (base + k < n) v += in[base + k];
-        T tot;
-        block_scan_ex(v, lds, &tot);
-        if (threadIdx.x == 0) sums[t] = tot;
-    }
+// Workspace layout (256-byte aligned parts): control words, then the sort's
+// digit totals (kMaxPasses x 256 u32), then the scan's tile statuses, then
+// the sort's tile statuses (per pass: tiles x 256).
+constexpr int kMaxPasses = 8;
+constexpr uint32_t kCtlWords = 64;  // 256 bytes
+// control words: [0] sort tiles, [1 + p] sort pass p's ticket, then:
+constexpr uint32_t kErrWord = kMaxPasses + 1;     // a look-back ran past its spin limit (stays set)
+constexpr uint32_t kSortDone = kMaxPasses + 2;    // workgroups of a sort's last pass that have finished
+constexpr uint32_t kScanTicket = kMaxPasses + 3;
+constexpr uint32_t kScanDone = kMaxPasses + 4;
+static_assert(kScanDone < kCtlWords, "control words");
+constexpr size_t kTotalsAt = kCtlWords * 4;
+constexpr size_t kScanStatusAt = kTotalsAt + (size_t)kMaxPasses * kDigits * 4;
+static_assert(kScanStatusAt % 256 == 0, "aligned parts");
+
+// Look-back status of a tile (and digit): flag << 62 | value; flag 1 = the
+// tile's own value, 2 = inclusive of every earlier tile.  Values stay below
+// 2^62 (element counts are u32; scanned u64 values are check counts).
+constexpr uint64_t kAgg = 1ull << 62, kInc = 2ull << 62, kValMask = (1ull << 62) - 1;
+constexpr uint32_t kSpinMax = 1u << 22;
+
+__device__ __forceinline__ uint64_t load_status(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One workgroup: tile sums -> tile offsets; out[n] and *d_total = the total.
-template <class T>
-__global__ __launch_bounds__(kT) void dscan_sums(T* __restrict__ sums, const uint32_t* __restrict__ d_n,
-                                                 T* __restrict__ out, T* __restrict__ d_total) {
-    __shared__ T lds[kT / 64];
-    const uint32_t n = *d_n;
-    const uint32_t nt = (n + kScanTile - 1) / kScanTile;
-    T carry = 0;
-    for (uint32_t b0 = 0; b0 < nt; b0 += kT) {
-        const uint32_t b = b0 + threadIdx.x;
-        const T v = b < nt ? sums[b] : T(0);
-        T tot;
-        const T ex = block_scan_ex(v, lds, &tot);
-        if (b < nt) sums[b] = carry + ex;
-        carry += tot;
+__device__ __forceinline__ void store_status(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of the values of tiles [0, t) for look-back cell j0 = t - 1 downwards
+// (stride apart): waits on each flag, stops at an inclusive one.
+__device__ __forceinline__ uint64_t look_back(const uint64_t* cells, int64_t j0, size_t stride, uint32_t* err) {
+    uint64_t excl = 0;
+    for (int64_t j = j0; j >= 0; --j) {
+        const uint64_t* at = cells + (size_t)j * stride;
+        uint64_t x = load_status(at);
+        for (uint32_t spin = 0; (x >> 62) == 0 && spin < kSpinMax; ++spin) {
+            __builtin_amdgcn_s_sleep(1);
+            x = load_status(at);
+        }
+        if ((x >> 62) == 0) {  // not reached: flags the result as failed rather than wait on
+            atomicOr(err, 1u);
+            break;
+        }
+        excl += x & kValMask;
+        if ((x >> 62) == 2) break;
     }
+    return excl;
+}
+
+// Thread 0 of the workgroup that finishes last (every thread calls it, at
+// the kernel's end); the others see false.
+__device__ __forceinline__ bool last_to_finish(uint32_t* done) {
+    __shared__ bool s_last;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        out[n] = carry;
-        if (d_total) *d_total = carry;
+        __threadfence();
+        s_last = atomicAdd(done, 1u) == gridDim.x - 1;
     }
+    __syncthreads();
+    return s_last;
 }
 
+// Exclusive scan of in[0, n) into out, out[n] and *d_total = the total.
 template <class T>
-__global__ __launch_bounds__(kT) void dscan_apply(const T* __restrict__ in, const uint32_t* __restrict__ d_n,
-                                                  const T* __restrict__ sums, T* __restrict__ out) {
+__global__ __launch_bounds__(kT) void dscan_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                                  const uint32_t* __restrict__ d_n, T* __restrict__ d_total,
+                                                  uint32_t* __restrict__ ctl, uint64_t* __restrict__ status) {
     __shared__ T lds[kT / 64];
+    __shared__ uint32_t s_tile;
+    __shared__ T s_excl;
     const uint32_t n = *d_n;
     const uint32_t nt = (n + kScanTile - 1) / kScanTile;
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&ctl[kScanTicket], 1u);
+        __syncthreads();
+        const uint32_t t = s_tile;
+        if (t >= nt) break;
         const size_t base = (size_t)t * kScanTile + (size_t)threadIdx.x * kScanPer;
         T v[kScanPer];
         T sum = 0;
@@ -113,66 +151,72 @@ __global__ __launch_bounds__(kT) void dscan_apply(const T* __restrict__ in, cons
             sum += v[k];
         }
         T tot;
-        T run = sums[t] + block_scan_ex(sum, lds, &tot);
+        const T ex = block_scan_ex(sum, lds, &tot);
+        if (threadIdx.x == 0) {
+            T excl = 0;
+            if (t == 0) {
+                store_status(status, kInc | (uint64_t)tot);
+            } else {
+                store_status(status + t, kAgg | (uint64_t)tot);
+                excl = (T)look_back(status, (int64_t)t - 1, 1, &ctl[kErrWord]);
+                store_status(status + t, kInc | (uint64_t)(excl + tot));
+            }
+            if (t == nt - 1) {
+                out[n] = excl + tot;
+                if (d_total) *d_total = excl + tot;
+            }
+            s_excl = excl;
+        }
+        __syncthreads();
+        T run = s_excl + ex;
 #pragma unroll
         for (int k = 0; k < kScanPer; ++k) {
             if (base + k < n) out[base + k] = run;
             run += v[k];
         }
+        __syncthreads();
+    }
+    if (last_to_finish(&ctl[kScanDone])) {
+        for (uint32_t i = threadIdx.x; i < nt; i += kT) status[i] = 0;
+        if (threadIdx.x == 0) {
+            ctl[kScanTicket] = 0;
+            ctl[kScanDone] = 0;
+            if (nt == 0) {
+                out[0] = 0;
+                if (d_total) *d_total = 0;
+            }
+        }
     }
 }
 
-// A scan's tile sums start past the first 256 bytes of its workspace, which
-// hold a sort's control words when the two share one workspace.
-constexpr size_t kScanSumsAt = 256;
-
 template <class T>
-void dscan_t(const T* in, T* out, size_t cap, const uint32_t* d_n, T* d_total, void* ws, hipStream_t s) {
-    T* sums = reinterpret_cast<T*>(static_cast<unsigned char*>(ws) + kScanSumsAt);
-    const unsigned g = grid_of(cap, kScanTile);
-    hipLaunchKernelGGL(dscan_tiles<T>, dim3(g), dim3(kT), 0, s, in, d_n, sums);
-    BTC_CHECK_LAUNCH("dscan_tiles");
-    hipLaunchKernelGGL(dscan_sums<T>, dim3(1), dim3(kT), 0, s, sums, d_n, out, d_total);
-    BTC_CHECK_LAUNCH("dscan_sums");
-    hipLaunchKernelGGL(dscan_apply<T>, dim3(g), dim3(kT), 0, s, in, d_n, sums, out);
-    BTC_CHECK_LAUNCH("dscan_apply");
+void dscan_t(const T* in, T* out, size_t cap, const uint32_t* d_n, T* d_total, const DevWs& ws, hipStream_t s) {
+    BTC_REQUIRE(cap <= ws.cap, BTC_ERR_ARG, "scan of %zu elements in a workspace for %zu", cap, ws.cap);
+    auto* b = static_cast<unsigned char*>(ws.p);
+    const unsigned g = std::min<unsigned>(grid_of(cap, kScanTile), 2048);  // tiles go by ticket
+    hipLaunchKernelGGL(dscan_kernel<T>, dim3(g), dim3(kT), 0, s, in, out, d_n, d_total,
+                       reinterpret_cast<uint32_t*>(b), reinterpret_cast<uint64_t*>(b + kScanStatusAt));
+    BTC_CHECK_LAUNCH("dscan_kernel");
 }
 
 // ------------------------------------------------------------------ sort
-// Workspace control words: [0] tiles of the count, [1 + p] pass p's tile
-// ticket, [kMaxPasses + 1] set when a look-back ran past its spin limit.
-constexpr int kMaxPasses = 8;
-constexpr uint32_t kCtlWords = 64;  // 256 bytes
-constexpr uint32_t kErrWord = kMaxPasses + 1;
-static_assert(kCtlWords * 4 <= kScanSumsAt, "a scan sharing the workspace leaves the control words alone");
-// Look-back status of (tile, digit): flag << 32 | count; flag 1 = the
-// tile's own count, 2 = inclusive of every earlier tile.
-constexpr uint64_t kAgg = 1ull << 32, kInc = 2ull << 32;
-constexpr uint32_t kSpinMax = 1u << 22;
-
 template <class K>
 __device__ __forceinline__ uint32_t digit_of(K k, int shift) {
     return (uint32_t)(k >> shift) & (kDigits - 1);
 }
 
-// One workgroup: clears the digit totals and the tickets, counts tiles
-// (the error word stays set once set).
-__global__ __launch_bounds__(kT) void dsort_setup(const uint32_t* __restrict__ d_n, uint32_t* __restrict__ ctl,
-                                                  uint32_t* __restrict__ totals, int passes) {
-    for (uint32_t i = threadIdx.x; i < (uint32_t)passes * kDigits; i += kT) totals[i] = 0;
-    if (threadIdx.x < kErrWord) ctl[threadIdx.x] = threadIdx.x == 0 ? (*d_n + kSortTile - 1) / kSortTile : 0u;
-}
-
 // Digit totals of every pass in one read of the keys (per tile in LDS,
-// then one atomic per non-zero (pass, digit)), and the tile's look-back
-// statuses of every pass cleared.
+// then one atomic per non-zero (pass, digit), into totals the previous
+// sort left cleared), the tile's look-back statuses of every pass cleared,
+// and the tile count recorded for the pass kernels.
 template <class K>
 __global__ __launch_bounds__(kT) void dsort_hist(const K* __restrict__ keys, const uint32_t* __restrict__ d_n,
-                                                 const uint32_t* __restrict__ ctl, int passes,
+                                                 uint32_t* __restrict__ ctl, int passes,
                                                  uint32_t* __restrict__ totals, uint64_t* __restrict__ status,
                                                  size_t status_pass) {
     __shared__ uint32_t h[kMaxPasses][kDigits];
-    const uint32_t n = *d_n, nt = ctl[0];
+    const uint32_t n = *d_n, nt = (n + kSortTile - 1) / kSortTile;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl[0] = nt;
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
         for (int p = 0; p < passes; ++p) h[p][threadIdx.x] = 0;
         __syncthreads();
@@ -195,14 +239,6 @@ __global__ __launch_bounds__(kT) void dsort_hist(const K* __restrict__ keys, con
     }
 }
 
-__device__ __forceinline__ uint64_t load_status(const uint64_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void store_status(uint64_t* p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // One stable pass.  Workgroups take tiles in ticket order; a tile ranks its
 // keys in LDS (rocPRIM's block_radix_rank, match form), publishes its count
 // of each digit and looks back over earlier tiles for the digit's offset
@@ -216,7 +252,7 @@ template <class K, bool V>
 __global__ __launch_bounds__(kT) void dsort_onesweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                      K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                      const uint32_t* __restrict__ d_n, uint32_t* __restrict__ ctl,
-                                                     int pass, const uint32_t* __restrict__ totals,
+                                                     int pass, int passes, uint32_t* __restrict__ totals,
                                                      uint64_t* __restrict__ status) {
     using Rank = rocprim::block_radix_rank<kT, kDigitBits, rocprim::block_radix_rank_algorithm::match>;
     static_assert(Rank::digits_per_thread == 1, "one digit per thread");
@@ -254,21 +290,8 @@ __global__ __launch_bounds__(kT) void dsort_onesweep(const K* __restrict__ kin, 
             store_status(mine, kInc | cnt[0]);
         } else {
             store_status(mine, kAgg | cnt[0]);
-            for (int64_t j = (int64_t)t - 1; j >= 0; --j) {
-                const uint64_t* at = status + (size_t)j * kDigits + d;
-                uint64_t x = load_status(at);
-                for (uint32_t spin = 0; (x >> 32) == 0 && spin < kSpinMax; ++spin) {
-                    __builtin_amdgcn_s_sleep(1);
-                    x = load_status(at);
-                }
-                if ((x >> 32) == 0) {  // not reached: flags the sort as failed rather than wait on
-                    atomicOr(&ctl[kErrWord], 1u);
-                    break;
-                }
-                excl += (uint32_t)x;
-                if ((x >> 32) == 2) break;
-            }
-            store_status(mine, kInc | (uint32_t)(excl + cnt[0]));
+            excl = (uint32_t)look_back(status + d, (int64_t)t - 1, kDigits, &ctl[kErrWord]);
+            store_status(mine, kInc | (uint64_t)(excl + cnt[0]));
         }
 #pragma unroll
         for (int i = 0; i < kSortPer; ++i) {
@@ -288,10 +311,14 @@ __global__ __launch_bounds__(kT) void dsort_onesweep(const K* __restrict__ kin, 
         }
         __syncthreads();
     }
+    // after the last pass the sort's totals and tickets go back to zero
+    if (pass == passes - 1 && last_to_finish(&ctl[kSortDone])) {
+        for (uint32_t i = threadIdx.x; i < (uint32_t)passes * kDigits; i += kT) totals[i] = 0;
+        if (threadIdx.x <= (uint32_t)passes) ctl[threadIdx.x] = 0;  // tile count and tickets
+        if (threadIdx.x == 0) ctl[kSortDone] = 0;
+    }
 }
 
-// Workspace: control words, digit totals (kMaxPasses x 256), then the
-// look-back statuses (per pass: tiles x 256 words).
 struct SortWs {
     uint32_t* ctl;
     uint32_t* totals;
@@ -299,28 +326,29 @@ struct SortWs {
     size_t status_pass;  // words per pass
 };
 
-SortWs sort_ws(void* ws, size_t cap) {
-    auto* b = static_cast<unsigned char*>(ws);
+size_t scan_status_bytes(size_t cap) { return (tiles(cap, kScanTile) * 8 + 255) & ~size_t(255); }
+
+SortWs sort_ws(const DevWs& ws) {
+    auto* b = static_cast<unsigned char*>(ws.p);
     SortWs w;
     w.ctl = reinterpret_cast<uint32_t*>(b);
-    w.totals = w.ctl + kCtlWords;
-    w.status = reinterpret_cast<uint64_t*>(b + (kCtlWords + kMaxPasses * kDigits) * 4);
-    w.status_pass = tiles(cap, kSortTile) * kDigits;
+    w.totals = reinterpret_cast<uint32_t*>(b + kTotalsAt);
+    w.status = reinterpret_cast<uint64_t*>(b + kScanStatusAt + scan_status_bytes(ws.cap));
+    w.status_pass = tiles(ws.cap, kSortTile) * kDigits;
     return w;
 }
 
 // Returns whether the result is in (kt, vt).
 template <class K, bool V>
-bool dsort_t(K* keys, uint32_t* vals, K* kt, uint32_t* vt, size_t cap, const uint32_t* d_n, int end_bit, void* ws,
-             hipStream_t s) {
+bool dsort_t(K* keys, uint32_t* vals, K* kt, uint32_t* vt, size_t cap, const uint32_t* d_n, int end_bit,
+             const DevWs& ws, hipStream_t s) {
     const int passes = (std::max(end_bit, 0) + kDigitBits - 1) / kDigitBits;
     if (passes == 0) return false;
     BTC_REQUIRE(passes <= kMaxPasses && end_bit <= (int)(8 * sizeof(K)), BTC_ERR_ARG, "sort on %d bits", end_bit);
-    const SortWs w = sort_ws(ws, cap);
+    BTC_REQUIRE(cap <= ws.cap, BTC_ERR_ARG, "sort of %zu keys in a workspace for %zu", cap, ws.cap);
+    const SortWs w = sort_ws(ws);
     const unsigned g = grid_of(cap, kSortTile);
     const unsigned gp = std::min<unsigned>(g, 2048);  // workgroups take tiles by ticket
-    hipLaunchKernelGGL(dsort_setup, dim3(1), dim3(kT), 0, s, d_n, w.ctl, w.totals, passes);
-    BTC_CHECK_LAUNCH("dsort_setup");
     hipLaunchKernelGGL(dsort_hist<K>, dim3(g), dim3(kT), 0, s, keys, d_n, w.ctl, passes, w.totals, w.status,
                        w.status_pass);
     BTC_CHECK_LAUNCH("dsort_hist");
@@ -329,7 +357,7 @@ bool dsort_t(K* keys, uint32_t* vals, K* kt, uint32_t* vt, size_t cap, const uin
     uint32_t* vi = vals;
     uint32_t* vo = vt;
     for (int p = 0; p < passes; ++p) {
-        hipLaunchKernelGGL((dsort_onesweep<K, V>), dim3(gp), dim3(kT), 0, s, ki, vi, ko, vo, d_n, w.ctl, p,
+        hipLaunchKernelGGL((dsort_onesweep<K, V>), dim3(gp), dim3(kT), 0, s, ki, vi, ko, vo, d_n, w.ctl, p, passes,
                            w.totals, w.status + (size_t)p * w.status_pass);
         BTC_CHECK_LAUNCH("dsort_onesweep");
         std::swap(ki, ko);
@@ -340,31 +368,29 @@ bool dsort_t(K* keys, uint32_t* vals, K* kt, uint32_t* vt, size_t cap, const uin
 
 }  // namespace
 
-size_t dscan_ws_bytes(size_t cap) { return kScanSumsAt + tiles(cap, kScanTile) * 8; }
-
-const uint32_t* dsort_error_word(const void* ws) { return static_cast<const uint32_t*>(ws) + kErrWord; }
-
-size_t dsort_ws_bytes(size_t cap) {
-    return (kCtlWords + kMaxPasses * kDigits) * 4 + (size_t)kMaxPasses * tiles(cap, kSortTile) * kDigits * 8;
+size_t dprim_ws_bytes(size_t cap) {
+    return kScanStatusAt + scan_status_bytes(cap) + (size_t)kMaxPasses * tiles(cap, kSortTile) * kDigits * 8;
 }
 
-void dscan_u32(const uint32_t* in, uint32_t* out, size_t cap, const uint32_t* d_n, uint32_t* d_total, void* ws,
+const uint32_t* dsort_error_word(const DevWs& ws) { return static_cast<const uint32_t*>(ws.p) + kErrWord; }
+
+void dscan_u32(const uint32_t* in, uint32_t* out, size_t cap, const uint32_t* d_n, uint32_t* d_total, const DevWs& ws,
                hipStream_t s) {
     dscan_t<uint32_t>(in, out, cap, d_n, d_total, ws, s);
 }
 
-void dscan_u64(const uint64_t* in, uint64_t* out, size_t cap, const uint32_t* d_n, uint64_t* d_total, void* ws,
+void dscan_u64(const uint64_t* in, uint64_t* out, size_t cap, const uint32_t* d_n, uint64_t* d_total, const DevWs& ws,
                hipStream_t s) {
     dscan_t<uint64_t>(in, out, cap, d_n, d_total, ws, s);
 }
 
-uint64_t* dsort_u64(uint64_t* keys, uint64_t* tmp, size_t cap, const uint32_t* d_n, int end_bit, void* ws,
+uint64_t* dsort_u64(uint64_t* keys, uint64_t* tmp, size_t cap, const uint32_t* d_n, int end_bit, const DevWs& ws,
                     hipStream_t s) {
     return dsort_t<uint64_t, false>(keys, nullptr, tmp, nullptr, cap, d_n, end_bit, ws, s) ? tmp : keys;
 }
 
 bool dsort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* kt, uint32_t* vt, size_t cap, const uint32_t* d_n,
-                     int end_bit, void* ws, hipStream_t s) {
+                     int end_bit, const DevWs& ws, hipStream_t s) {
     return dsort_t<uint32_t, true>(keys, vals, kt, vt, cap, d_n, end_bit, ws, s);
 }
 

@@ -4,6 +4,7 @@
 #include <memory>
 
 #include "common.h"
+#include "prim.h"
 
 // Bit-BSR of L with 16 x K tiles (K in {32, 64, 128}); see btc_amd.h.
 struct btc_bitbsr_s {
@@ -216,7 +217,7 @@ struct CanonCap {
     uint32_t* rowptr;    // [n_cap + 1]
     int32_t* col;        // [nnz_cap + kColPad]
     uint16_t* col16;     // [nnz_cap + kColPad]: low 16 bits of col
-    void* ws;            // dsort / dscan workspace for max(nnz_cap, n_cap)
+    DevWs ws;            // dsort / dscan workspace for max(nnz_cap, n_cap)
 };
 
 // Enqueues the canonicalisation of ctl->nnz pairs (the same CSR as
@@ -261,7 +262,7 @@ struct PlanCap {
     uint32_t* sv1;           // [items_cap] each
     uint4* items;            // [2 * items_cap]
     unsigned long long* work;  // the count's words (kWorkWords): total and cursors cleared with the items
-    void* ws;                // dsort / dscan workspace for max(m_cap, items_cap)
+    DevWs ws;                // dsort / dscan workspace for max(m_cap, items_cap)
 };
 
 // Rows that can hold a hub-core bitmap: at least kCoreRowMin entries each.

@@ -36,6 +36,7 @@ struct btc_pipeline_s {
     int32_t graph_nodes = 0;
     btcx::PinnedBuf pin;  // [0, 4 KiB): control block and total; then staged pairs
     btcx::PipeCtl last{};
+    size_t ws_bytes = 0;  // the sorts' and scans' workspace (zeroed at creation)
     bool ran = false;     // `last` holds an accepted run
     std::mutex mu;        // one run at a time
 };
@@ -134,7 +135,8 @@ void layout(btc_pipeline_s* p, Carve& c) {
     q.items = c.take<uint4>(2 * M);
     q.work = p->work;
     const size_t big = std::max(E, N);
-    void* ws = c.take<unsigned char>(std::max(dsort_ws_bytes(big), dscan_ws_bytes(big)));
+    p->ws_bytes = dprim_ws_bytes(big);
+    const DevWs ws{c.take<unsigned char>(p->ws_bytes), big};
     k.ws = ws;
     q.ws = ws;
 }
@@ -147,10 +149,10 @@ void pipeline_build(btc_pipeline_s* p, hipStream_t s) {
     Carve real;
     real.base = p->mem.p;
     layout(p, real);
-    // the count's words start cleared (the captured kernels clear them again);
-    // so does the sorts' error word, which only a failed sort sets
+    // the count's words and the sorts' and scans' workspace start cleared;
+    // every replay leaves them cleared again
     BTC_HIP(hipMemsetAsync(p->work, 0, kWorkWords * 8, s));
-    BTC_HIP(hipMemsetAsync(const_cast<uint32_t*>(dsort_error_word(p->canon.ws)), 0, 4, s));
+    BTC_HIP(hipMemsetAsync(p->canon.ws.p, 0, p->ws_bytes, s));
     BTC_HIP(hipStreamSynchronize(s));
     p->graph.capture([&](hipStream_t cs) {
         p->labels = canonicalise_capacity(p->canon, cs);

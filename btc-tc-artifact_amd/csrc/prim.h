@@ -27,29 +27,37 @@ void exclusive_scan_u64(const uint64_t* in, uint64_t* out, size_t n, hipStream_t
 
 // Capturable forms (devprim.hip).  The element count n is read from device
 // memory (*d_n) when the kernels run, buffers hold `cap` elements (n <= cap),
-// and the caller's workspace `ws` (dsort_ws_bytes / dscan_ws_bytes of cap,
-// 256-byte aligned) holds every temporary: the kernels are the only work
-// enqueued, so one stream capture serves every n up to cap.
+// and the caller's workspace `ws` (dprim_ws_bytes(cap) bytes, 256-byte
+// aligned, zeroed once before its first use) holds every temporary: the
+// kernels are the only work enqueued, so one stream capture serves every n
+// up to cap.  Each call leaves the workspace's tickets, totals and statuses
+// zeroed again for the next call on the same stream; calls sharing a
+// workspace must not overlap.
 
-size_t dsort_ws_bytes(size_t cap);
-size_t dscan_ws_bytes(size_t cap);
+size_t dprim_ws_bytes(size_t cap);
+
+// The workspace: its device address and the capacity it was sized for
+// (its layout follows from that, whatever the capacity of a call).
+struct DevWs {
+    void* p = nullptr;
+    size_t cap = 0;
+};
 
 // out[i] = sum of in[0, i) for i <= n; out[n] and *d_total (when set) get
 // the total.
-void dscan_u32(const uint32_t* in, uint32_t* out, size_t cap, const uint32_t* d_n, uint32_t* d_total, void* ws,
-               hipStream_t s);
-void dscan_u64(const uint64_t* in, uint64_t* out, size_t cap, const uint32_t* d_n, uint64_t* d_total, void* ws,
-               hipStream_t s);
+void dscan_u32(const uint32_t* in, uint32_t* out, size_t cap, const uint32_t* d_n, uint32_t* d_total,
+               const DevWs& ws, hipStream_t s);
+void dscan_u64(const uint64_t* in, uint64_t* out, size_t cap, const uint32_t* d_n, uint64_t* d_total,
+               const DevWs& ws, hipStream_t s);
 
-// Word of ws that a sort sets (and nothing clears) if a look-back ever ran
-// past its spin limit -- a sort whose result cannot be trusted.  The caller
-// clears it before first use.
-const uint32_t* dsort_error_word(const void* ws);
+// Word of ws that a sort or scan sets (and nothing clears) if a look-back
+// ever ran past its spin limit -- a result that cannot be trusted.
+const uint32_t* dsort_error_word(const DevWs& ws);
 
 // Stable radix sorts on bits [0, end_bit), as sort_u64 / sort_pairs_u32.
-uint64_t* dsort_u64(uint64_t* keys, uint64_t* tmp, size_t cap, const uint32_t* d_n, int end_bit, void* ws,
+uint64_t* dsort_u64(uint64_t* keys, uint64_t* tmp, size_t cap, const uint32_t* d_n, int end_bit, const DevWs& ws,
                     hipStream_t s);
 bool dsort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* kt, uint32_t* vt, size_t cap, const uint32_t* d_n,
-                     int end_bit, void* ws, hipStream_t s);
+                     int end_bit, const DevWs& ws, hipStream_t s);
 
 }  // namespace btcx
