@@ -81,7 +81,7 @@ class PipelineInfo(C.Structure):
     _fields_ = [("n_cap", C.c_int32), ("graph_nodes", C.c_int32), ("nnz_cap", C.c_int64),
                 ("device_bytes", C.c_int64), ("flags", C.c_uint32), ("n", C.c_int32), ("m", C.c_int64),
                 ("wedges", C.c_int64), ("wedge_checks", C.c_int64), ("work_items", C.c_int32),
-                ("max_row", C.c_int32)]
+                ("max_row", C.c_int32), ("max_graphs", C.c_int32), ("reserved", C.c_int32)]
 
 
 class ServerOpts(C.Structure):
@@ -168,6 +168,11 @@ _SIGS = {
     "btc_pipeline_create": (C.c_int, [C.c_int32, C.c_int64, C.c_uint32, C.POINTER(CountOpts), vp, C.POINTER(vp)]),
     "btc_pipeline_count": (C.c_int, [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_float)]),
+    "btc_pipeline_create_batch": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.c_uint32, C.POINTER(CountOpts), vp,
+                                            C.POINTER(vp)]),
+    "btc_pipeline_count_batch": (C.c_int, [vp, vp, vp, vp, vp, C.c_int32, vp, vp, C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_float)]),
+    "btc_pipeline_export_batch": (C.c_int, [vp, vp, C.POINTER(vp)]),
     "btc_pipeline_graph": (C.c_int, [vp, vp, C.POINTER(vp)]),
     "btc_pipeline_export": (C.c_int, [vp, vp, C.POINTER(vp), C.POINTER(vp)]),
     "btc_pipeline_get_info": (C.c_int, [vp, C.POINTER(PipelineInfo)]),
@@ -464,11 +469,38 @@ class Pipeline:
     """COO pairs -> triangle total as one captured HIP graph, for inputs of
     up to n_cap ids and nnz_cap pairs (btc_pipeline)."""
 
-    def __init__(self, n_cap, nnz_cap, flags=CANON_DEFAULT, stream=None, **opts):
+    def __init__(self, n_cap, nnz_cap, flags=CANON_DEFAULT, stream=None, max_graphs=1, **opts):
         self.opts = make_opts(**opts)
         h = vp()
-        _check(lib().btc_pipeline_create(n_cap, nnz_cap, flags, C.byref(self.opts), stream, C.byref(h)))
+        _check(lib().btc_pipeline_create_batch(n_cap, nnz_cap, max_graphs, flags, C.byref(self.opts), stream,
+                                               C.byref(h)))
         self._h = h
+
+    def count_batch(self, graphs, stream=None):
+        """(total, per-graph uint64 array, kernel_ms) of [(src, dst, n), ...]
+        in one run (btc_pipeline_count_batch)."""
+        k = len(graphs)
+        off = np.zeros(k + 1, np.int64)
+        off[1:] = np.cumsum([len(g[0]) for g in graphs])
+        src = np.ascontiguousarray(np.concatenate([np.asarray(g[0], np.int32) for g in graphs]), np.int32)
+        dst = np.ascontiguousarray(np.concatenate([np.asarray(g[1], np.int32) for g in graphs]), np.int32)
+        nv = np.ascontiguousarray([g[2] for g in graphs], np.int32)
+        per = np.zeros(k, np.uint64)
+        total, ms = C.c_uint64(0), C.c_float(0)
+        _check(lib().btc_pipeline_count_batch(self._h, _ptr(src), _ptr(dst), _ptr(off), _ptr(nv), k, stream,
+                                              _ptr(per), C.byref(total), C.byref(ms)))
+        return total.value, per, ms.value
+
+    def export_batch(self, stream=None):
+        """The last run of several graphs as a Batch."""
+        h = vp()
+        _check(lib().btc_pipeline_export_batch(self._h, stream, C.byref(h)))
+        b = Batch.__new__(Batch)
+        b.n_graphs = self.info["max_graphs"]
+        b.opts = self.opts
+        b._h = h
+        b.n_graphs = b.info["graphs"]
+        return b
 
     def count(self, src, dst, n=0, stream=None):
         """(total, kernel_ms) of the graph of host pairs (src[e], dst[e])."""

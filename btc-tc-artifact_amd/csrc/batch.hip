@@ -84,6 +84,25 @@ btc_batch_s* batch_from_graph(btc_graph_s&& g, const std::vector<int64_t>& voff,
     return b.release();
 }
 
+btc_batch_s* batch_from_plan(btc_graph_s&& g, btc_plan_s* plan, const std::vector<int64_t>& voff, hipStream_t s) {
+    std::unique_ptr<btc_plan_s> own(plan);
+    BTC_REQUIRE(voff.size() >= 2 && voff.front() == 0 && voff.back() == g.n, BTC_ERR_ARG, "bad graph ranges");
+    auto b = std::make_unique<btc_batch_s>();
+    b->count = (int32_t)voff.size() - 1;
+    b->voff = voff;
+    b->graph = std::move(g);
+    b->plan = own.release();
+    b->plan->g = &b->graph;
+    std::vector<uint32_t> vo32(voff.begin(), voff.end());
+    b->d_voff.alloc(vo32.size());
+    b->d_pv.alloc((size_t)b->graph.n, b->graph.max_row);
+    b->d_pg.alloc((size_t)b->count);
+    b->h_pg.assign((size_t)b->count, 0);
+    BTC_HIP(hipMemcpyAsync(b->d_voff.p, vo32.data(), vo32.size() * 4, hipMemcpyHostToDevice, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    return b.release();
+}
+
 void batch_set_side_streams(btc_batch_s* b, bool on) { b->side_streams = on; }
 
 }  // namespace btcx

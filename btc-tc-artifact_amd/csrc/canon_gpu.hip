@@ -278,12 +278,12 @@ __global__ __launch_bounds__(kT) void stats_kernel(const uint32_t* __restrict__ 
 }
 
 // ------------------------------------------------- capturable form kernels
-// Clears a run's words (keeping the two the host wrote) and opens the id
+// Clears a run's words (keeping the ones the host wrote) and opens the id
 // range.
 __global__ void pipe_reset_kernel(PipeCtl* __restrict__ c) {
     constexpr uint32_t kWords = sizeof(PipeCtl) / 4;
     uint32_t* w = reinterpret_cast<uint32_t*>(c);
-    for (uint32_t i = threadIdx.x + 2; i < kWords; i += blockDim.x) w[i] = 0;
+    for (uint32_t i = threadIdx.x + kPipeHostWords; i < kWords; i += blockDim.x) w[i] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
         c->lo = INT_MAX;
@@ -343,6 +343,34 @@ __global__ void pipe_compact_ids_kernel(const uint32_t* __restrict__ used, const
     for (int64_t e = t0; e < m; e += stride) {
         const uint64_t k = keys[e];
         keys[e] = ((uint64_t)newid[k >> b] << b) | (uint64_t)newid[k & mask];
+    }
+}
+
+// First vertex of L of every graph: its first input id through newid
+// (none: no compaction, as when there is no edge).
+__global__ void pipe_csegs_kernel(const uint32_t* __restrict__ segs, const uint32_t* __restrict__ newid,
+                                  const PipeCtl* __restrict__ c, uint32_t* __restrict__ csegs) {
+    const uint32_t k = c->graphs;
+    const bool map = newid && c->m != 0;
+    for (uint32_t g = threadIdx.x; g <= k; g += blockDim.x) csegs[g] = map ? newid[segs[g]] : segs[g];
+}
+
+// rank_keys_kernel within graphs: key = graph << b | (top - degree), so one
+// sort orders by (graph, descending degree, id).
+__global__ void pipe_rank_keys_kernel(const uint32_t* __restrict__ deg, const PipeCtl* __restrict__ c,
+                                      const uint32_t* __restrict__ csegs, uint32_t top, int b,
+                                      uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+    const int64_t n = c->nc;
+    const uint32_t k = c->graphs;
+    for (int64_t v = blockIdx.x * (int64_t)kT + threadIdx.x; v < n; v += (int64_t)gridDim.x * kT) {
+        uint32_t lo = 0, hi = k;  // the last graph whose first vertex is <= v
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (csegs[mid] <= (uint32_t)v) lo = mid;
+            else hi = mid;
+        }
+        key[v] = (lo << b) | (top - deg[v]);
+        val[v] = (uint32_t)v;
     }
 }
 
@@ -565,7 +593,9 @@ int32_t* canonicalise_capacity(const CanonCap& c, hipStream_t s) {
     dscan_u32(c.flag, c.pos, (size_t)E, &ctl->nnz, &ctl->m, c.ws, s);
     CAP_LAUNCH(compact_kernel, E, s, sorted, c.flag, c.pos, nnz, keys);
 
-    // order-preserving compaction of the used ids (ids keep b bits)
+    // order-preserving compaction of the used ids (ids keep b bits); the
+    // graphs' ranges follow it
+    const bool groups = c.max_graphs > 1;
     int32_t* labels = c.lab0;
     if (c.flags & BTC_CANON_COMPACT) {
         CAP_LAUNCH(fill_u32_kernel, N, s, c.used, n, 0u);
@@ -575,14 +605,29 @@ int32_t* canonicalise_capacity(const CanonCap& c, hipStream_t s) {
     } else {
         CAP_LAUNCH(pipe_identity_kernel, N, s, ctl, labels);
     }
+    if (groups) {
+        hipLaunchKernelGGL(pipe_csegs_kernel, dim3(1), dim3(64), 0, s, c.segs,
+                           (c.flags & BTC_CANON_COMPACT) ? c.newid : nullptr, ctl, c.csegs);
+        BTC_CHECK_LAUNCH("pipe_csegs_kernel");
+    }
 
     if (c.flags & BTC_CANON_DEGREE_ORDER) {
         // keys N - 1 - degree < 2^b: descending degree, ties by id
         CAP_LAUNCH(fill_u32_kernel, N, s, c.deg, nc, 0u);
         hipLaunchKernelGGL(degree_kernel, dim3(std::min(cap_grid(E), kCapRedBlocks)), dim3(kT), 0, s, keys, m, b, c.deg);
         BTC_CHECK_LAUNCH("degree_kernel");
-        CAP_LAUNCH(rank_keys_kernel, N, s, c.deg, nc, (uint32_t)(N - 1), c.k0, c.v0);
-        const bool alt = dsort_pairs_u32(c.k0, c.v0, c.k1, c.v1, (size_t)N, &ctl->nc, b, c.ws, s);
+        int kb = b;  // key bits: degree rank, then (several graphs) the graph
+        if (groups) {
+            int gb = 1;
+            while ((1 << gb) < c.max_graphs) ++gb;
+            BTC_REQUIRE(b + gb <= 32, BTC_ERR_OVERFLOW, "%d ids and %d graphs exceed 32-bit relabel keys", c.n_cap,
+                        c.max_graphs);
+            CAP_LAUNCH(pipe_rank_keys_kernel, N, s, c.deg, ctl, c.csegs, (uint32_t)(N - 1), b, c.k0, c.v0);
+            kb = b + gb;
+        } else {
+            CAP_LAUNCH(rank_keys_kernel, N, s, c.deg, nc, (uint32_t)(N - 1), c.k0, c.v0);
+        }
+        const bool alt = dsort_pairs_u32(c.k0, c.v0, c.k1, c.v1, (size_t)N, &ctl->nc, kb, c.ws, s);
         CAP_LAUNCH(rank_scatter_kernel, N, s, alt ? c.v1 : c.v0, nc, c.rank, labels, c.lab1);
         CAP_LAUNCH(relabel_kernel, E, s, keys, m, b, c.rank, b);
         keys = dsort_u64(keys, spare, (size_t)E, &ctl->m, 2 * b, c.ws, s);

@@ -966,19 +966,57 @@ void launch_plan_dense(btc_plan_s* p, unsigned long long* d_slots, PvOut d_pv, h
 
 }  // namespace
 
-void pipeline_count_enqueue(const PlanCap& c, const uint16_t* col16, const btc_count_opts& o, hipStream_t s) {
+namespace {
+
+// pv[0, ctl->nc) = 0
+__global__ void zero_pv_kernel(unsigned long long* __restrict__ pv, const PipeCtl* __restrict__ c) {
+    const uint32_t n = c->nc;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) pv[v] = 0;
+}
+
+// per_graph[g] = sum of pv over [csegs[g], csegs[g + 1]), g = blockIdx < ctl->graphs
+__global__ __launch_bounds__(kThreads) void segment_sums_kernel(const unsigned long long* __restrict__ pv,
+                                                                const uint32_t* __restrict__ csegs,
+                                                                const PipeCtl* __restrict__ c,
+                                                                unsigned long long* __restrict__ per_graph) {
+    __shared__ uint64_t lds[kThreads / 64];
+    const uint32_t g = blockIdx.x;
+    if (g >= c->graphs) return;  // workgroup-uniform
+    uint64_t x = 0;
+    for (uint32_t v = csegs[g] + threadIdx.x; v < csegs[g + 1]; v += kThreads) x += pv[v];
+    x = block_sum_u64<kThreads>(x, lds);
+    if (threadIdx.x == 0) per_graph[g] = x;
+}
+
+}  // namespace
+
+void pipeline_count_enqueue(const PlanCap& c, const uint16_t* col16, const btc_count_opts& o, hipStream_t s,
+                            unsigned long long* pv, const uint32_t* csegs, unsigned long long* per_graph,
+                            int max_graphs) {
     const int64_t span = c.m_cap + (int64_t)kColPad;
     const bool w64 = o.wide_offsets != 0 || (uint64_t)span >= (uint64_t(1) << 30);
+    PvOut pvo{};
+    if (pv) {
+        pvo.p64 = pv;
+        const unsigned g = (unsigned)std::min<int64_t>((c.n_cap + kThreads - 1) / kThreads, 2048);
+        hipLaunchKernelGGL(zero_pv_kernel, dim3(g), dim3(kThreads), 0, s, pv, c.ctl);
+        BTC_CHECK_LAUNCH("zero_pv_kernel");
+    }
     unsigned grid = 0;
-    WedgeFn fn = wedge_fn(o, false, w64, &grid);
+    WedgeFn fn = wedge_fn(o, pvo.on(), w64, &grid);
     unsigned long long* work = c.work;
     const PlanDev pd{c.col, col16, c.lists, reinterpret_cast<const uint4*>(c.rowbits), c.ur, c.items, 0u, 0u,
-                     work, PvOut{}, c.core_ids / 128};
+                     work, pvo, c.core_ids / 128};
     WedgeArgs a{pd, nullptr, nullptr, 1u, 0u, 0u, reinterpret_cast<uint32_t*>(work + 1), (int)o.lookup,
                 (int)o.schedule, 0u, 1u, work + kWorkSlots, c.ctl->item_counts};
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
     BTC_CHECK_LAUNCH("wedge_kernel");
     launch_slot_sum(work + kWorkSlots, work, s);
+    if (pv) {
+        hipLaunchKernelGGL(segment_sums_kernel, dim3((unsigned)max_graphs), dim3(kThreads), 0, s, pv, csegs, c.ctl,
+                           per_graph);
+        BTC_CHECK_LAUNCH("segment_sums_kernel");
+    }
 }
 
 void plans_count(btc_plan_s* const* P, int k, hipStream_t s, uint64_t* totals, MultiWorkspace& ws) {

@@ -150,6 +150,9 @@ __host__ __device__ inline bool tiny_pivot(uint32_t lists, uint32_t lo, uint32_t
 // occupy the vertex ranges [voff[k], voff[k + 1]) (takes g over).
 btc_batch_s* batch_from_graph(btc_graph_s&& g, const std::vector<int64_t>& voff, const btc_count_opts* opts,
                               hipStream_t s);
+// A batch over g (graphs on the vertex ranges voff) with an existing plan of
+// g, which the batch takes over (plan->g is pointed at the batch's copy).
+btc_batch_s* batch_from_plan(btc_graph_s&& g, btc_plan_s* plan, const std::vector<int64_t>& voff, hipStream_t s);
 // Whether counts of batch b may fork their dense-tile and tiny-pivot
 // kernels onto side streams of its plan (created on first use; default on).
 void batch_set_side_streams(btc_batch_s* b, bool on);
@@ -171,8 +174,10 @@ constexpr size_t kColPad = 16;
 // Device words of one capturable build + count (btc_pipeline): the host
 // writes nnz and n_in before a replay and reads the block back after it.
 struct PipeCtl {
-    uint32_t nnz;    // input pairs
-    int32_t n_in;    // vertices as given (<= 0: largest id + 1)
+    uint32_t nnz;     // input pairs
+    int32_t n_in;     // vertices as given (<= 0: largest id + 1)
+    uint32_t graphs;  // graphs of the run (vertex ranges segs[0, graphs]; 1 for a single graph)
+    uint32_t host_reserved;
     int32_t lo, hi;  // smallest and largest id of the input
     uint32_t n;      // vertices of the input
     uint32_t status; // kPipe* bits: why the input was refused
@@ -183,13 +188,14 @@ struct PipeCtl {
     uint32_t nbm_words;
     uint32_t nitems;           // work items
     uint32_t item_counts[2];   // {workgroup items, workgroup + wave items}
-    uint32_t reserved[2];
     unsigned long long rs[3];  // longest row, sum C(d, 2), merge work
     unsigned long long st[8];  // planner statistics (as plan_device's)
     unsigned long long chunk;  // wedge checks per work item
     unsigned long long checks; // wedge checks in all
 };
 constexpr uint32_t kPipeNegativeId = 1, kPipeIdRange = 2, kPipeTooManyIds = 4;
+constexpr uint32_t kPipeHostWords = 4;  // words of PipeCtl the host writes before a run
+static_assert(sizeof(PipeCtl) % 8 == 0, "u64 fields aligned");
 
 // Buffers of a capturable canonicalisation of up to nnz_cap pairs over up
 // to n_cap ids (element counts in brackets).
@@ -218,6 +224,12 @@ struct CanonCap {
     int32_t* col;        // [nnz_cap + kColPad]
     uint16_t* col16;     // [nnz_cap + kColPad]: low 16 bits of col
     DevWs ws;            // dsort / dscan workspace for max(nnz_cap, n_cap)
+    // several graphs (max_graphs > 1): graph g's ids are segs[g] ..
+    // segs[g + 1] - 1 of the input (host-written), csegs[g] its first vertex
+    // of L; the degree relabel keeps every graph a contiguous range
+    int32_t max_graphs = 1;
+    const uint32_t* segs = nullptr;  // [max_graphs + 1]
+    uint32_t* csegs = nullptr;       // [max_graphs + 1]
 };
 
 // Enqueues the canonicalisation of ctl->nnz pairs (the same CSR as
@@ -281,8 +293,13 @@ void check_launch_opts(const btc_count_opts& o);
 // Enqueues the count of a plan_capacity plan (whose last kernel cleared the
 // total and cursors at c.work; the slots are left cleared by every count):
 // the wedge kernel on the item counts in ctl, then the slot sum into
-// c.work[0].  col16 is L's 16-bit copy.
-void pipeline_count_enqueue(const PlanCap& c, const uint16_t* col16, const btc_count_opts& o, hipStream_t s);
+// c.work[0].  col16 is L's 16-bit copy.  With pv set (several graphs), the
+// per-vertex sums go to pv (cleared first over ctl->nc vertices) and
+// per_graph[g] gets the sum over vertices [csegs[g], csegs[g + 1]) for g <
+// ctl->graphs (launched over max_graphs workgroups).
+void pipeline_count_enqueue(const PlanCap& c, const uint16_t* col16, const btc_count_opts& o, hipStream_t s,
+                            unsigned long long* pv = nullptr, const uint32_t* csegs = nullptr,
+                            unsigned long long* per_graph = nullptr, int max_graphs = 1);
 
 // Uploads L (rowptr narrowed to u32) to HBM on first use.
 void ensure_device(btc_graph_s* g, hipStream_t s);

@@ -21,6 +21,7 @@
 struct btc_pipeline_s {
     int32_t n_cap = 0;
     int64_t nnz_cap = 0;
+    int32_t max_graphs = 1;  // graphs one run may hold (vertex ranges of one L)
     uint32_t flags = 0;
     btc_count_opts opts{};
     btcx::DevBuf<unsigned char> mem;
@@ -30,6 +31,12 @@ struct btc_pipeline_s {
     int32_t* dst = nullptr;
     uint16_t* col16 = nullptr;
     int32_t* labels = nullptr;  // the canonicalisation's label buffer
+    // several graphs: input ranges (host-written), ranges in L, per-vertex
+    // and per-graph sums
+    uint32_t* segs = nullptr;
+    uint32_t* csegs = nullptr;
+    unsigned long long* pv = nullptr;
+    unsigned long long* per_graph = nullptr;
     btcx::CanonCap canon{};
     btcx::PlanCap plan{};
     btcx::CapturedGraph graph;
@@ -44,8 +51,14 @@ struct btc_pipeline_s {
 namespace btcx {
 namespace {
 
+// Pinned staging: control block, total and error word from 0, the graphs'
+// ranges at kPinSegs, per-graph sums at kPinPerGraph, pairs from kPinHeader.
 constexpr size_t kPinHeader = 4096;
-static_assert(sizeof(PipeCtl) + 16 <= kPinHeader, "control block, total and error word fit the pinned header");
+constexpr size_t kPinSegs = 1024, kPinPerGraph = 2048;
+constexpr int kMaxPipelineGraphs = 64;
+static_assert(sizeof(PipeCtl) + 16 <= kPinSegs && kPinSegs + 4 * (kMaxPipelineGraphs + 1) <= kPinPerGraph &&
+                  kPinPerGraph + 8 * kMaxPipelineGraphs <= kPinHeader,
+              "pinned header layout");
 
 // Bump layout over one allocation (base NULL: sizing pass).
 struct Carve {
@@ -139,6 +152,16 @@ void layout(btc_pipeline_s* p, Carve& c) {
     const DevWs ws{c.take<unsigned char>(p->ws_bytes), big};
     k.ws = ws;
     q.ws = ws;
+    if (p->max_graphs > 1) {
+        const size_t G = (size_t)p->max_graphs;
+        p->segs = c.take<uint32_t>(G + 1);
+        p->csegs = c.take<uint32_t>(G + 1);
+        p->per_graph = c.take<unsigned long long>(G);
+        p->pv = c.take<unsigned long long>(N);
+        k.max_graphs = p->max_graphs;
+        k.segs = p->segs;
+        k.csegs = p->csegs;
+    }
 }
 
 // Carves the buffers and captures the graph.
@@ -157,7 +180,7 @@ void pipeline_build(btc_pipeline_s* p, hipStream_t s) {
     p->graph.capture([&](hipStream_t cs) {
         p->labels = canonicalise_capacity(p->canon, cs);
         plan_capacity(p->plan, cs);
-        pipeline_count_enqueue(p->plan, p->col16, p->opts, cs);
+        pipeline_count_enqueue(p->plan, p->col16, p->opts, cs, p->pv, p->csegs, p->per_graph, p->max_graphs);
     });
     size_t nodes = 0;
     BTC_HIP(hipGraphGetNodes(p->graph.graph, nullptr, &nodes));
@@ -165,17 +188,68 @@ void pipeline_build(btc_pipeline_s* p, hipStream_t s) {
     p->pin.reserve(kPinHeader);
 }
 
+// One replay, the pairs already in p->src / p->dst (and, for several graphs,
+// their ranges at pin + kPinSegs): header words in, control block, total,
+// error word and per-graph sums out.  Caller holds p->mu.
+void replay(btc_pipeline_s* p, char* pin, uint32_t nnz, int32_t n_in, uint32_t graphs, hipStream_t s,
+            uint64_t* total, uint64_t* per_graph, float* kernel_ms) {
+    PipeCtl* h = reinterpret_cast<PipeCtl*>(pin);
+    std::memset(h, 0, sizeof(PipeCtl));
+    h->nnz = nnz;
+    h->n_in = n_in;
+    h->graphs = graphs;
+    BTC_HIP(hipMemcpyAsync(p->ctl, h, kPipeHostWords * 4, hipMemcpyHostToDevice, s));
+    if (p->max_graphs > 1)
+        BTC_HIP(hipMemcpyAsync(p->segs, pin + kPinSegs, ((size_t)graphs + 1) * 4, hipMemcpyHostToDevice, s));
+    EventTimer tm(kernel_ms != nullptr);
+    tm.start(s);
+    p->graph.launch(s);
+    tm.stop(s);
+    unsigned long long* htotal = reinterpret_cast<unsigned long long*>(pin + sizeof(PipeCtl));
+    uint32_t* herr = reinterpret_cast<uint32_t*>(htotal + 1);
+    unsigned long long* hpg = reinterpret_cast<unsigned long long*>(pin + kPinPerGraph);
+    BTC_HIP(hipMemcpyAsync(h, p->ctl, sizeof(PipeCtl), hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipMemcpyAsync(htotal, p->work, 8, hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipMemcpyAsync(herr, dsort_error_word(p->canon.ws), 4, hipMemcpyDeviceToHost, s));
+    if (p->max_graphs > 1)
+        BTC_HIP(hipMemcpyAsync(hpg, p->per_graph, (size_t)graphs * 8, hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    BTC_REQUIRE(*herr == 0, BTC_ERR_HIP, "a device sort of the pipeline failed (look-back spin limit)");
+    const PipeCtl c = *h;
+    BTC_REQUIRE(!(c.status & kPipeNegativeId), BTC_ERR_ARG, "negative vertex id %d", c.lo);
+    BTC_REQUIRE(!(c.status & kPipeIdRange), BTC_ERR_ARG, "vertex id %d >= n = %d", c.hi, n_in);
+    BTC_REQUIRE(!(c.status & kPipeTooManyIds), BTC_ERR_ARG, "vertex id %d: more than the pipeline's %d ids", c.hi,
+                p->n_cap);
+    if (kernel_ms) *kernel_ms = tm.ms();
+    *total = *htotal;
+    if (per_graph) {
+        if (p->max_graphs > 1) std::memcpy(per_graph, hpg, (size_t)graphs * 8);
+        else per_graph[0] = *htotal;
+    }
+    p->last = c;
+    p->ran = true;
+}
+
+void pipeline_run_batch(btc_pipeline_s* p, const int32_t* src, const int32_t* dst, const int64_t* eoff,
+                        const int32_t* npg, int32_t count, hipStream_t s, uint64_t* per_graph, uint64_t* total,
+                        float* kernel_ms);
+
 void pipeline_run(btc_pipeline_s* p, const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n,
                   bool device_pairs, hipStream_t s, uint64_t* total, float* kernel_ms) {
     BTC_REQUIRE(nnz >= 0 && nnz <= p->nnz_cap, BTC_ERR_ARG, "%lld pairs (pipeline capacity %lld)", (long long)nnz,
                 (long long)p->nnz_cap);
     BTC_REQUIRE(n <= p->n_cap, BTC_ERR_ARG, "n = %d exceeds the pipeline's %d ids", n, p->n_cap);
     BTC_REQUIRE(nnz == 0 || (src && dst), BTC_ERR_ARG, "src/dst is NULL");
+    if (p->max_graphs > 1) {  // one graph as a run of one range (ids checked on the host)
+        BTC_REQUIRE(!device_pairs, BTC_ERR_ARG, "a pipeline for several graphs takes host pairs");
+        const int64_t eoff[2] = {0, nnz};
+        pipeline_run_batch(p, src, dst, eoff, &n, 1, s, nullptr, total, kernel_ms);
+        return;
+    }
     std::lock_guard<std::mutex> lk(p->mu);
     p->ran = false;
     const size_t pair_bytes = (size_t)nnz * 4;
     char* pin = p->pin.reserve(kPinHeader + (device_pairs ? 0 : 2 * pair_bytes));
-    PipeCtl* h = reinterpret_cast<PipeCtl*>(pin);
     if (nnz) {
         if (device_pairs) {
             BTC_HIP(hipMemcpyAsync(p->src, src, pair_bytes, hipMemcpyDeviceToDevice, s));
@@ -187,30 +261,65 @@ void pipeline_run(btc_pipeline_s* p, const int32_t* src, const int32_t* dst, int
             BTC_HIP(hipMemcpyAsync(p->dst, pin + kPinHeader + pair_bytes, pair_bytes, hipMemcpyHostToDevice, s));
         }
     }
-    std::memset(h, 0, sizeof(PipeCtl));
-    h->nnz = (uint32_t)nnz;
-    h->n_in = n;
-    BTC_HIP(hipMemcpyAsync(p->ctl, h, 8, hipMemcpyHostToDevice, s));
-    EventTimer tm(kernel_ms != nullptr);
-    tm.start(s);
-    p->graph.launch(s);
-    tm.stop(s);
-    unsigned long long* htotal = reinterpret_cast<unsigned long long*>(pin + sizeof(PipeCtl));
-    BTC_HIP(hipMemcpyAsync(h, p->ctl, sizeof(PipeCtl), hipMemcpyDeviceToHost, s));
-    BTC_HIP(hipMemcpyAsync(htotal, p->work, 8, hipMemcpyDeviceToHost, s));
-    uint32_t* herr = reinterpret_cast<uint32_t*>(htotal + 1);
-    BTC_HIP(hipMemcpyAsync(herr, dsort_error_word(p->canon.ws), 4, hipMemcpyDeviceToHost, s));
-    BTC_HIP(hipStreamSynchronize(s));
-    BTC_REQUIRE(*herr == 0, BTC_ERR_HIP, "a device sort of the pipeline failed (look-back spin limit)");
-    const PipeCtl c = *h;
-    BTC_REQUIRE(!(c.status & kPipeNegativeId), BTC_ERR_ARG, "negative vertex id %d", c.lo);
-    BTC_REQUIRE(!(c.status & kPipeIdRange), BTC_ERR_ARG, "vertex id %d >= n = %d", c.hi, n);
-    BTC_REQUIRE(!(c.status & kPipeTooManyIds), BTC_ERR_ARG, "vertex id %d: more than the pipeline's %d ids", c.hi,
-                p->n_cap);
-    if (kernel_ms) *kernel_ms = tm.ms();
-    *total = *htotal;
-    p->last = c;
-    p->ran = true;
+    replay(p, pin, (uint32_t)nnz, n, 1, s, total, nullptr, kernel_ms);
+}
+
+// Several graphs in one run: graph g's pairs are src/dst[eoff[g], eoff[g +
+// 1]) over ids [0, npg[g]) (<= 0: its largest id + 1), checked here and
+// shifted into one id space while they are staged.
+void pipeline_run_batch(btc_pipeline_s* p, const int32_t* src, const int32_t* dst, const int64_t* eoff,
+                        const int32_t* npg, int32_t count, hipStream_t s, uint64_t* per_graph, uint64_t* total,
+                        float* kernel_ms) {
+    BTC_REQUIRE(eoff && npg && count >= 1, BTC_ERR_ARG, "bad argument");
+    BTC_REQUIRE(count <= p->max_graphs, BTC_ERR_ARG, "%d graphs (the pipeline takes %d)", count, p->max_graphs);
+    BTC_REQUIRE(eoff[0] == 0, BTC_ERR_ARG, "edge_offsets[0] != 0");
+    const int64_t pairs = eoff[count];
+    BTC_REQUIRE(pairs >= 0 && pairs <= p->nnz_cap, BTC_ERR_ARG, "%lld pairs (pipeline capacity %lld)",
+                (long long)pairs, (long long)p->nnz_cap);
+    BTC_REQUIRE(pairs == 0 || (src && dst), BTC_ERR_ARG, "src/dst is NULL");
+    if (p->max_graphs == 1) {
+        uint64_t t = 0;
+        pipeline_run(p, src, dst, pairs, npg[0], false, s, &t, kernel_ms);
+        *total = t;
+        if (per_graph) per_graph[0] = t;
+        return;
+    }
+    std::vector<int64_t> segs((size_t)count + 1, 0);
+    for (int32_t g = 0; g < count; ++g) {
+        const int64_t e0 = eoff[g], e1 = eoff[g + 1];
+        BTC_REQUIRE(e0 <= e1, BTC_ERR_ARG, "edge_offsets not ascending at graph %d", g);
+        int32_t lo = 0, hi = -1;
+        for (int64_t e = e0; e < e1; ++e) {
+            lo = std::min(lo, std::min(src[e], dst[e]));
+            hi = std::max(hi, std::max(src[e], dst[e]));
+        }
+        const int64_t n = npg[g] > 0 ? npg[g] : (int64_t)hi + 1;
+        BTC_REQUIRE(lo >= 0 && hi < n, BTC_ERR_ARG, "graph %d: vertex id out of [0, %lld)", g, (long long)n);
+        segs[(size_t)g + 1] = segs[(size_t)g] + n;
+    }
+    BTC_REQUIRE(segs[(size_t)count] <= p->n_cap, BTC_ERR_ARG, "%lld ids in all (the pipeline takes %d)",
+                (long long)segs[(size_t)count], p->n_cap);
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->ran = false;
+    const size_t pair_bytes = (size_t)pairs * 4;
+    char* pin = p->pin.reserve(kPinHeader + 2 * pair_bytes);
+    int32_t* hs = reinterpret_cast<int32_t*>(pin + kPinHeader);
+    int32_t* hd = hs + pairs;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t g = 0; g < count; ++g) {
+        const int32_t v0 = (int32_t)segs[(size_t)g];
+        for (int64_t e = eoff[g]; e < eoff[g + 1]; ++e) {
+            hs[e] = src[e] + v0;
+            hd[e] = dst[e] + v0;
+        }
+    }
+    uint32_t* hsegs = reinterpret_cast<uint32_t*>(pin + kPinSegs);
+    for (int32_t g = 0; g <= count; ++g) hsegs[g] = (uint32_t)segs[(size_t)g];
+    if (pairs) {
+        BTC_HIP(hipMemcpyAsync(p->src, hs, pair_bytes, hipMemcpyHostToDevice, s));
+        BTC_HIP(hipMemcpyAsync(p->dst, hd, pair_bytes, hipMemcpyHostToDevice, s));
+    }
+    replay(p, pin, (uint32_t)pairs, (int32_t)segs[(size_t)count], (uint32_t)count, s, total, per_graph, kernel_ms);
 }
 
 // Copies of the last accepted run: its graph (with_plan false) or its graph
@@ -319,24 +428,52 @@ void pipeline_export(btc_pipeline_s* p, hipStream_t s, btc_graph_s** gout, btc_p
     pipeline_copy_locked(p, s, true, gout, pout);
 }
 
+// The last run of several graphs as a batch (its L, plan and the graphs'
+// vertex ranges in L).
+btc_batch_s* pipeline_export_batch(btc_pipeline_s* p, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    BTC_REQUIRE(p->max_graphs > 1, BTC_ERR_STATE, "a pipeline of single graphs (btc_pipeline_export)");
+    BTC_REQUIRE(p->ran, BTC_ERR_STATE, "no accepted run yet");
+    const uint32_t k = p->last.graphs;
+    std::vector<uint32_t> cs((size_t)k + 1);
+    BTC_HIP(hipMemcpyAsync(cs.data(), p->csegs, cs.size() * 4, hipMemcpyDeviceToHost, s));
+    BTC_HIP(hipStreamSynchronize(s));
+    btc_graph_s* g = nullptr;
+    btc_plan_s* q = nullptr;
+    pipeline_copy_locked(p, s, true, &g, &q);
+    std::unique_ptr<btc_graph_s> own_g(g);
+    const std::vector<int64_t> voff(cs.begin(), cs.end());
+    return batch_from_plan(std::move(*own_g), q, voff, s);
+}
+
 }  // namespace
 }  // namespace btcx
 
 extern "C" {
 
-int btc_pipeline_create(int32_t n_cap, int64_t nnz_cap, uint32_t flags, const btc_count_opts* opts, void* stream,
-                        btc_pipeline* out) {
+int btc_pipeline_create_batch(int32_t n_cap, int64_t nnz_cap, int32_t max_graphs, uint32_t flags,
+                              const btc_count_opts* opts, void* stream, btc_pipeline* out) {
     return btcx::guard([&] {
         BTC_REQUIRE(out, BTC_ERR_ARG, "out is NULL");
         *out = nullptr;
         BTC_REQUIRE(n_cap >= 1, BTC_ERR_ARG, "n_cap %d < 1", n_cap);
         BTC_REQUIRE(nnz_cap >= 1 && nnz_cap + (int64_t)btcx::kColPad < (int64_t)0xFFFFFFFFLL, BTC_ERR_ARG,
                     "nnz_cap %lld (expected 1 .. 2^32 - %zu)", (long long)nnz_cap, btcx::kColPad + 1);
+        BTC_REQUIRE(max_graphs >= 1 && max_graphs <= btcx::kMaxPipelineGraphs, BTC_ERR_ARG,
+                    "max_graphs %d (expected 1 .. %d)", max_graphs, btcx::kMaxPipelineGraphs);
         BTC_REQUIRE((flags & ~(BTC_CANON_DEFAULT | BTC_CANON_DEVICE)) == 0, BTC_ERR_ARG, "unknown flags 0x%x",
                     flags);
+        if (max_graphs > 1 && (flags & BTC_CANON_DEGREE_ORDER)) {
+            int b = 1, gb = 1;
+            while (b < 31 && (1ll << b) < n_cap) ++b;
+            while ((1 << gb) < max_graphs) ++gb;
+            BTC_REQUIRE(b + gb <= 32, BTC_ERR_OVERFLOW, "%d ids and %d graphs exceed 32-bit relabel keys", n_cap,
+                        max_graphs);
+        }
         auto p = std::make_unique<btc_pipeline_s>();
         p->n_cap = n_cap;
         p->nnz_cap = nnz_cap;
+        p->max_graphs = max_graphs;
         p->flags = flags & BTC_CANON_DEFAULT;
         btc_count_opts_default(&p->opts);
         if (opts) p->opts = *opts;
@@ -350,11 +487,34 @@ int btc_pipeline_create(int32_t n_cap, int64_t nnz_cap, uint32_t flags, const bt
     });
 }
 
+int btc_pipeline_create(int32_t n_cap, int64_t nnz_cap, uint32_t flags, const btc_count_opts* opts, void* stream,
+                        btc_pipeline* out) {
+    return btc_pipeline_create_batch(n_cap, nnz_cap, 1, flags, opts, stream, out);
+}
+
 int btc_pipeline_count(btc_pipeline p, const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n,
                        int32_t device_pairs, void* stream, uint64_t* total, float* kernel_ms) {
     return btcx::guard([&] {
         BTC_REQUIRE(p && total, BTC_ERR_ARG, "NULL argument");
         btcx::pipeline_run(p, src, dst, nnz, n, device_pairs != 0, btcx::as_stream(stream), total, kernel_ms);
+    });
+}
+
+int btc_pipeline_count_batch(btc_pipeline p, const int32_t* src, const int32_t* dst, const int64_t* edge_offsets,
+                             const int32_t* n_per_graph, int32_t count, void* stream, uint64_t* per_graph,
+                             uint64_t* total, float* kernel_ms) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(p && total, BTC_ERR_ARG, "NULL argument");
+        btcx::pipeline_run_batch(p, src, dst, edge_offsets, n_per_graph, count, btcx::as_stream(stream), per_graph,
+                                 total, kernel_ms);
+    });
+}
+
+int btc_pipeline_export_batch(btc_pipeline p, void* stream, btc_batch* out) {
+    return btcx::guard([&] {
+        BTC_REQUIRE(p && out, BTC_ERR_ARG, "NULL argument");
+        *out = nullptr;
+        *out = btcx::pipeline_export_batch(p, btcx::as_stream(stream));
     });
 }
 
@@ -382,6 +542,7 @@ int btc_pipeline_get_info(btc_pipeline p, btc_pipeline_info* info) {
         std::memset(info, 0, sizeof(*info));
         info->n_cap = p->n_cap;
         info->nnz_cap = p->nnz_cap;
+        info->max_graphs = p->max_graphs;
         info->flags = p->flags;
         info->graph_nodes = p->graph_nodes;
         info->device_bytes = (int64_t)p->mem.bytes();

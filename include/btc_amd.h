@@ -395,10 +395,20 @@ typedef struct {
     int64_t wedge_checks; /* ids looked up + core words ANDed by the plan */
     int32_t work_items;
     int32_t max_row;
+    int32_t max_graphs;   /* graphs one run may hold */
+    int32_t reserved;
 } btc_pipeline_info;
 
 int btc_pipeline_create(int32_t n_cap, int64_t nnz_cap, uint32_t flags, const btc_count_opts* opts, void* stream,
                         btc_pipeline* out);
+/* A pipeline whose runs hold up to max_graphs (<= 64) graphs with n_cap ids
+ * and nnz_cap pairs in all: canonicalised as separate vertex ranges (each
+ * compacted and relabelled by degree within itself, as btc_batch_from_coo),
+ * planned as one block-diagonal L and counted with per-vertex sums summed
+ * per graph.  With BTC_CANON_DEGREE_ORDER, bits(n_cap) + bits(max_graphs)
+ * must not exceed 32. */
+int btc_pipeline_create_batch(int32_t n_cap, int64_t nnz_cap, int32_t max_graphs, uint32_t flags,
+                              const btc_count_opts* opts, void* stream, btc_pipeline* out);
 /* Triangles of the graph of nnz pairs (src[e], dst[e]) over ids [0, n)
  * (n <= 0: the largest id + 1): host arrays, or HBM when device_pairs is 1.
  * An id out of range, nnz > nnz_cap or more than n_cap ids returns
@@ -406,6 +416,13 @@ int btc_pipeline_create(int32_t n_cap, int64_t nnz_cap, uint32_t flags, const bt
  * are serialised; kernel_ms (may be NULL) times the replay. */
 int btc_pipeline_count(btc_pipeline p, const int32_t* src, const int32_t* dst, int64_t nnz, int32_t n,
                        int32_t device_pairs, void* stream, uint64_t* total, float* kernel_ms);
+/* count graphs in one run (host pairs, the layout of btc_batch_from_coo:
+ * graph k's pairs are src/dst[edge_offsets[k], edge_offsets[k+1]) over ids
+ * [0, n_per_graph[k]), n <= 0: its largest id + 1; ids are checked on the
+ * host); per_graph[k] (may be NULL) = graph k's triangles, *total = all. */
+int btc_pipeline_count_batch(btc_pipeline p, const int32_t* src, const int32_t* dst, const int64_t* edge_offsets,
+                             const int32_t* n_per_graph, int32_t count, void* stream, uint64_t* per_graph,
+                             uint64_t* total, float* kernel_ms);
 /* A copy of the last accepted run's canonical graph, in HBM: the same L and
  * labels as btc_graph_from_coo_device of the same pairs. */
 int btc_pipeline_graph(btc_pipeline p, void* stream, btc_graph* out);
@@ -414,6 +431,9 @@ int btc_pipeline_graph(btc_pipeline p, void* stream, btc_graph* out);
  * no tiny items) that btc_plan_count and the others count like any plan,
  * unaffected by later runs.  Free the plan before the graph. */
 int btc_pipeline_export(btc_pipeline p, void* stream, btc_graph* graph, btc_plan* plan);
+/* The last run of a pipeline of several graphs as a batch (btc_batch_count
+ * gives its per-graph counts again), independent of later runs. */
+int btc_pipeline_export_batch(btc_pipeline p, void* stream, btc_batch* out);
 int btc_pipeline_get_info(btc_pipeline p, btc_pipeline_info* info);
 void btc_pipeline_free(btc_pipeline p);
 

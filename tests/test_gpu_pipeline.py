@@ -174,3 +174,69 @@ def test_pipeline_tile_boundaries(gpu, orc, nnz):
         want, _ = oracle_counts(orc, host)
         assert p.count(s, d, n)[0] == want, (nnz, n)
         same_graph(host, p.graph())
+
+
+def group_graphs(gpu, k, seed):
+    """k small graphs of several kinds, an empty one and one of loops only."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        n = int(rng.integers(64, 3000))
+        kind = i % 5
+        if kind == 0:
+            s, d = gpu.gen_er(n, float(rng.uniform(4, 16)), seed=seed * 100 + i)
+        elif kind == 1:
+            sc = int(np.ceil(np.log2(n)))
+            s, d = gpu.gen_rmat(sc, 12, seed=seed * 100 + i)
+            keep = (s < n) & (d < n)
+            s, d = s[keep], d[keep]
+        elif kind == 2:
+            s, d = gpu.gen_planted(n, 48, 0.5, 2.0, seed=seed * 100 + i)
+        elif kind == 3:
+            s = d = np.zeros(0, np.int32)
+        else:
+            s = d = np.arange(n, dtype=np.int32)  # loops only
+        out.append((np.asarray(s, np.int32), np.asarray(d, np.int32), n if i % 2 == 0 else 0))
+    return out
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_pipeline_batch_matches_oracle(gpu, orc, flags):
+    """Several graphs per run: each graph's count equals the oracle's and
+    btc_batch_from_coo's, over runs of different group sizes; the exported
+    batch counts the same and is untouched by later runs."""
+    p = gpu.Pipeline(1 << 16, 1 << 20, flags, max_graphs=16)
+    pool = group_graphs(gpu, 24, seed=flags + 1)
+    want = [oracle_counts(orc, gpu.Graph.from_coo(s, d, n))[0] for s, d, n in pool]
+    rng = np.random.default_rng(9)
+    kept = None
+    for k in (16, 1, 7, 16, 3):
+        pick = rng.choice(len(pool), size=k, replace=False)
+        gs = [pool[i] for i in pick]
+        t, per, _ = p.count_batch(gs)
+        np.testing.assert_array_equal(per, [want[i] for i in pick])
+        assert t == sum(want[i] for i in pick)
+        if kept is None and k == 7:
+            kept = (p.export_batch(), [want[i] for i in pick])
+            np.testing.assert_array_equal(kept[0].count()[1], kept[1])
+    b = gpu.Batch.from_coo([pool[i] for i in range(8)], flags=flags)
+    np.testing.assert_array_equal(b.count()[1], want[:8])
+    np.testing.assert_array_equal(kept[0].count()[1], kept[1])  # after later runs
+    # single counts on a several-graph pipeline run as one range
+    s, d, n = pool[1]
+    assert p.count(s, d, n)[0] == want[1]
+
+
+def test_pipeline_batch_refuses_bad_input(gpu):
+    p = gpu.Pipeline(4096, 1 << 16, 3, max_graphs=4)
+    ok = (np.array([0, 1, 2], np.int32), np.array([1, 2, 0], np.int32), 3)
+    with pytest.raises(gpu.BtcError):
+        p.count_batch([ok, (np.array([0, 5], np.int32), np.array([1, 2], np.int32), 4)])  # id 5 >= 4
+    with pytest.raises(gpu.BtcError):
+        p.count_batch([ok] * 5)  # more graphs than the pipeline takes
+    with pytest.raises(gpu.BtcError):
+        p.count_batch([ok, (ok[0], ok[1], 4094)])  # 4097 ids in all
+    with pytest.raises(gpu.BtcError):
+        gpu.Pipeline(1 << 27, 1 << 16, 3, max_graphs=64)  # 27 + 6 key bits
+    t, per, _ = p.count_batch([ok, ok])
+    assert t == 2 and list(per) == [1, 1]
