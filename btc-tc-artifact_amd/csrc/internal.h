@@ -21,6 +21,8 @@ struct btc_bitbsr_s {
     btcx::DevBuf<unsigned long long> result;
 };
 
+struct btc_pipeline_s;
+
 struct btc_plan_s {
     btc_graph_s* g = nullptr;
     int K = 0;
@@ -286,6 +288,13 @@ inline size_t pipe_bitmap_rows(int64_t m_cap, int32_t n_cap) {
 // items; the item cut is about 64 items per CU with no upper bound on an
 // item's checks) with kernels only; counts land in *ctl.
 void plan_capacity(const PlanCap& c, hipStream_t s);
+
+// btc_pipeline_count_batch for graphs whose ids the caller has checked
+// (graph g: sizes[g] pairs at src[g] / dst[g], ids in [0, nv[g])), staged
+// straight from the callers' arrays.
+void pipeline_count_checked(btc_pipeline_s* p, int32_t count, const int32_t* const* src, const int32_t* const* dst,
+                            const int64_t* sizes, const int64_t* nv, hipStream_t s, uint64_t* per_graph,
+                            uint64_t* total);
 
 // Checks the launch-shape fields of o (BTC_ERR_ARG on a bad value).
 void check_launch_opts(const btc_count_opts& o);

@@ -44,6 +44,13 @@ struct btc_pipeline_s {
     btcx::PinnedBuf pin;  // [0, 4 KiB): control block and total; then staged pairs
     btcx::PipeCtl last{};
     size_t ws_bytes = 0;  // the sorts' and scans' workspace (zeroed at creation)
+    // a run's end is awaited on this event, which blocks the host thread
+    // instead of spinning (threads that run pipelines side by side leave
+    // their cores to each other meanwhile)
+    hipEvent_t done = nullptr;
+    ~btc_pipeline_s() {
+        if (done) (void)hipEventDestroy(done);
+    }
     bool ran = false;     // `last` holds an accepted run
     std::mutex mu;        // one run at a time
 };
@@ -182,6 +189,7 @@ void pipeline_build(btc_pipeline_s* p, hipStream_t s) {
         plan_capacity(p->plan, cs);
         pipeline_count_enqueue(p->plan, p->col16, p->opts, cs, p->pv, p->csegs, p->per_graph, p->max_graphs);
     });
+    BTC_HIP(hipEventCreateWithFlags(&p->done, hipEventBlockingSync | hipEventDisableTiming));
     size_t nodes = 0;
     BTC_HIP(hipGraphGetNodes(p->graph.graph, nullptr, &nodes));
     p->graph_nodes = (int32_t)nodes;
@@ -213,7 +221,8 @@ void replay(btc_pipeline_s* p, char* pin, uint32_t nnz, int32_t n_in, uint32_t g
     BTC_HIP(hipMemcpyAsync(herr, dsort_error_word(p->canon.ws), 4, hipMemcpyDeviceToHost, s));
     if (p->max_graphs > 1)
         BTC_HIP(hipMemcpyAsync(hpg, p->per_graph, (size_t)graphs * 8, hipMemcpyDeviceToHost, s));
-    BTC_HIP(hipStreamSynchronize(s));
+    BTC_HIP(hipEventRecord(p->done, s));
+    BTC_HIP(hipEventSynchronize(p->done));
     BTC_REQUIRE(*herr == 0, BTC_ERR_HIP, "a device sort of the pipeline failed (look-back spin limit)");
     const PipeCtl c = *h;
     BTC_REQUIRE(!(c.status & kPipeNegativeId), BTC_ERR_ARG, "negative vertex id %d", c.lo);
@@ -264,9 +273,50 @@ void pipeline_run(btc_pipeline_s* p, const int32_t* src, const int32_t* dst, int
     replay(p, pin, (uint32_t)nnz, n, 1, s, total, nullptr, kernel_ms);
 }
 
+// Several graphs in one run, their ids already checked: graph g's pairs are
+// src[g] / dst[g][0, sizes[g]) over ids [0, nv[g]); they are shifted into
+// one id space while they are staged.
+void pipeline_run_checked(btc_pipeline_s* p, int32_t count, const int32_t* const* src, const int32_t* const* dst,
+                          const int64_t* sizes, const int64_t* nv, hipStream_t s, uint64_t* per_graph,
+                          uint64_t* total, float* kernel_ms) {
+    BTC_REQUIRE(count >= 1 && count <= p->max_graphs && p->max_graphs > 1, BTC_ERR_ARG,
+                "%d graphs (the pipeline takes %d)", count, p->max_graphs);
+    std::vector<int64_t> eoff((size_t)count + 1, 0), segs((size_t)count + 1, 0);
+    for (int32_t g = 0; g < count; ++g) {
+        eoff[(size_t)g + 1] = eoff[(size_t)g] + sizes[g];
+        segs[(size_t)g + 1] = segs[(size_t)g] + nv[g];
+    }
+    const int64_t pairs = eoff[(size_t)count];
+    BTC_REQUIRE(pairs <= p->nnz_cap, BTC_ERR_ARG, "%lld pairs (pipeline capacity %lld)", (long long)pairs,
+                (long long)p->nnz_cap);
+    BTC_REQUIRE(segs[(size_t)count] <= p->n_cap, BTC_ERR_ARG, "%lld ids in all (the pipeline takes %d)",
+                (long long)segs[(size_t)count], p->n_cap);
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->ran = false;
+    const size_t pair_bytes = (size_t)pairs * 4;
+    char* pin = p->pin.reserve(kPinHeader + 2 * pair_bytes);
+    int32_t* hs = reinterpret_cast<int32_t*>(pin + kPinHeader);
+    int32_t* hd = hs + pairs;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t g = 0; g < count; ++g) {
+        const int32_t v0 = (int32_t)segs[(size_t)g];
+        const int64_t e0 = eoff[(size_t)g];
+        for (int64_t e = 0; e < sizes[g]; ++e) {
+            hs[e0 + e] = src[g][e] + v0;
+            hd[e0 + e] = dst[g][e] + v0;
+        }
+    }
+    uint32_t* hsegs = reinterpret_cast<uint32_t*>(pin + kPinSegs);
+    for (int32_t g = 0; g <= count; ++g) hsegs[g] = (uint32_t)segs[(size_t)g];
+    if (pairs) {
+        BTC_HIP(hipMemcpyAsync(p->src, hs, pair_bytes, hipMemcpyHostToDevice, s));
+        BTC_HIP(hipMemcpyAsync(p->dst, hd, pair_bytes, hipMemcpyHostToDevice, s));
+    }
+    replay(p, pin, (uint32_t)pairs, (int32_t)segs[(size_t)count], (uint32_t)count, s, total, per_graph, kernel_ms);
+}
+
 // Several graphs in one run: graph g's pairs are src/dst[eoff[g], eoff[g +
-// 1]) over ids [0, npg[g]) (<= 0: its largest id + 1), checked here and
-// shifted into one id space while they are staged.
+// 1]) over ids [0, npg[g]) (<= 0: its largest id + 1), checked here.
 void pipeline_run_batch(btc_pipeline_s* p, const int32_t* src, const int32_t* dst, const int64_t* eoff,
                         const int32_t* npg, int32_t count, hipStream_t s, uint64_t* per_graph, uint64_t* total,
                         float* kernel_ms) {
@@ -284,7 +334,8 @@ void pipeline_run_batch(btc_pipeline_s* p, const int32_t* src, const int32_t* ds
         if (per_graph) per_graph[0] = t;
         return;
     }
-    std::vector<int64_t> segs((size_t)count + 1, 0);
+    std::vector<const int32_t*> sp((size_t)count), dp((size_t)count);
+    std::vector<int64_t> sizes((size_t)count), nv((size_t)count);
     for (int32_t g = 0; g < count; ++g) {
         const int64_t e0 = eoff[g], e1 = eoff[g + 1];
         BTC_REQUIRE(e0 <= e1, BTC_ERR_ARG, "edge_offsets not ascending at graph %d", g);
@@ -295,31 +346,12 @@ void pipeline_run_batch(btc_pipeline_s* p, const int32_t* src, const int32_t* ds
         }
         const int64_t n = npg[g] > 0 ? npg[g] : (int64_t)hi + 1;
         BTC_REQUIRE(lo >= 0 && hi < n, BTC_ERR_ARG, "graph %d: vertex id out of [0, %lld)", g, (long long)n);
-        segs[(size_t)g + 1] = segs[(size_t)g] + n;
+        sp[(size_t)g] = src + e0;
+        dp[(size_t)g] = dst + e0;
+        sizes[(size_t)g] = e1 - e0;
+        nv[(size_t)g] = n;
     }
-    BTC_REQUIRE(segs[(size_t)count] <= p->n_cap, BTC_ERR_ARG, "%lld ids in all (the pipeline takes %d)",
-                (long long)segs[(size_t)count], p->n_cap);
-    std::lock_guard<std::mutex> lk(p->mu);
-    p->ran = false;
-    const size_t pair_bytes = (size_t)pairs * 4;
-    char* pin = p->pin.reserve(kPinHeader + 2 * pair_bytes);
-    int32_t* hs = reinterpret_cast<int32_t*>(pin + kPinHeader);
-    int32_t* hd = hs + pairs;
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int32_t g = 0; g < count; ++g) {
-        const int32_t v0 = (int32_t)segs[(size_t)g];
-        for (int64_t e = eoff[g]; e < eoff[g + 1]; ++e) {
-            hs[e] = src[e] + v0;
-            hd[e] = dst[e] + v0;
-        }
-    }
-    uint32_t* hsegs = reinterpret_cast<uint32_t*>(pin + kPinSegs);
-    for (int32_t g = 0; g <= count; ++g) hsegs[g] = (uint32_t)segs[(size_t)g];
-    if (pairs) {
-        BTC_HIP(hipMemcpyAsync(p->src, hs, pair_bytes, hipMemcpyHostToDevice, s));
-        BTC_HIP(hipMemcpyAsync(p->dst, hd, pair_bytes, hipMemcpyHostToDevice, s));
-    }
-    replay(p, pin, (uint32_t)pairs, (int32_t)segs[(size_t)count], (uint32_t)count, s, total, per_graph, kernel_ms);
+    pipeline_run_checked(p, count, sp.data(), dp.data(), sizes.data(), nv.data(), s, per_graph, total, kernel_ms);
 }
 
 // Copies of the last accepted run: its graph (with_plan false) or its graph
@@ -383,7 +415,8 @@ void pipeline_copy_locked(btc_pipeline_s* p, hipStream_t s, bool with_plan, btc_
         BTC_HIP(hipMemcpyAsync(q->col16.p, p->col16, (m + kColPad) * 2, hipMemcpyDeviceToDevice, s));
         BTC_HIP(hipMemsetAsync(q->scratch.p, 0, q->scratch.bytes(), s));
     }
-    BTC_HIP(hipStreamSynchronize(s));
+    BTC_HIP(hipEventRecord(p->done, s));
+    BTC_HIP(hipEventSynchronize(p->done));
     if (with_plan) {
         q->g = g.get();
         q->K = 64;
@@ -447,6 +480,13 @@ btc_batch_s* pipeline_export_batch(btc_pipeline_s* p, hipStream_t s) {
 }
 
 }  // namespace
+
+void pipeline_count_checked(btc_pipeline_s* p, int32_t count, const int32_t* const* src, const int32_t* const* dst,
+                            const int64_t* sizes, const int64_t* nv, hipStream_t s, uint64_t* per_graph,
+                            uint64_t* total) {
+    pipeline_run_checked(p, count, src, dst, sizes, nv, s, per_graph, total, nullptr);
+}
+
 }  // namespace btcx
 
 extern "C" {

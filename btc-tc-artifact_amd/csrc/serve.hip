@@ -66,15 +66,21 @@ int64_t pow2_at_least(int64_t x, int64_t lo) {
     return c;
 }
 
-// A prep thread's pipelines by size class (ids, pairs), least recently used
-// dropped first beyond kPipeCacheBytes.
+// A prep thread's pipelines by size class (ids, pairs, graphs per run),
+// least recently used dropped first beyond kPipeCacheBytes.
 struct PipeCache {
     struct Slot {
         btc_pipeline p = nullptr;
         int64_t bytes = 0;
         uint64_t used = 0;
     };
-    std::vector<std::pair<std::pair<int32_t, int64_t>, Slot>> slots;
+    struct Key {
+        int32_t ncap;
+        int64_t ecap;
+        int32_t graphs;
+        bool operator==(const Key& o) const { return ncap == o.ncap && ecap == o.ecap && graphs == o.graphs; }
+    };
+    std::vector<std::pair<Key, Slot>> slots;
     int64_t bytes = 0;
     uint64_t tick = 0;
     PipeCache() = default;
@@ -83,16 +89,18 @@ struct PipeCache {
     ~PipeCache() {
         for (auto& x : slots) btc_pipeline_free(x.second.p);
     }
-    // The pipeline of class (ncap, ecap), created on s when missing.
-    int get(int32_t ncap, int64_t ecap, uint32_t flags, const btc_count_opts& co, hipStream_t s, btc_pipeline* out) {
+    // The pipeline of class (ncap, ecap, graphs), created on s when missing.
+    int get(int32_t ncap, int64_t ecap, int32_t graphs, uint32_t flags, const btc_count_opts& co, hipStream_t s,
+            btc_pipeline* out) {
+        const Key key{ncap, ecap, graphs};
         for (auto& x : slots)
-            if (x.first.first == ncap && x.first.second == ecap) {
+            if (x.first == key) {
                 x.second.used = ++tick;
                 *out = x.second.p;
                 return BTC_OK;
             }
         btc_pipeline p = nullptr;
-        int rc = btc_pipeline_create(ncap, ecap, flags, &co, s, &p);
+        int rc = btc_pipeline_create_batch(ncap, ecap, graphs, flags, &co, s, &p);
         if (rc != BTC_OK) return rc;
         btc_pipeline_info pi;
         btc_pipeline_get_info(p, &pi);
@@ -108,7 +116,7 @@ struct PipeCache {
         sl.p = p;
         sl.bytes = pi.device_bytes;
         sl.used = ++tick;
-        slots.emplace_back(std::make_pair(ncap, ecap), sl);
+        slots.emplace_back(key, sl);
         bytes += sl.bytes;
         *out = p;
         return BTC_OK;
@@ -323,14 +331,15 @@ struct btc_server_s {
     bool prepare_captured(PrepJob& job, uint32_t cf, const btc_count_opts& co, hipStream_t s, PipeCache& cache_p,
                           std::shared_ptr<Unit>& out, int& rc) {
         const int64_t nnz = (int64_t)job.src.size();
-        const int64_t nv = job_vertices(job);
+        // with n given the pipeline checks the ids itself (on the device)
+        const int64_t nv = job.n > 0 ? (int64_t)job.n : job_vertices(job);
         if (nnz > kPipeMaxPairs || nv > kPipeMaxIds) return false;
         if (nv < 0) {
             rc = BTC_ERR_ARG;
             return true;
         }
         btc_pipeline pipe = nullptr;
-        rc = cache_p.get((int32_t)pow2_at_least(nv, 1024), pow2_at_least(nnz, 4096), cf & BTC_CANON_DEFAULT, co, s,
+        rc = cache_p.get((int32_t)pow2_at_least(nv, 1024), pow2_at_least(nnz, 4096), 1, cf & BTC_CANON_DEFAULT, co, s,
                          &pipe);
         if (rc != BTC_OK) return true;
         const double t_start = now();
@@ -382,6 +391,78 @@ struct btc_server_s {
         out->bytes = info.device_bytes + 4 * (gi.m + 2 * (int64_t)gi.n + 1);
         out->keys = {job.key};
         return true;
+    }
+
+    // Captured preparation of several checked graphs (ids within nv[k]) in
+    // one pipeline run: the per-graph counts answer their waiting requests at
+    // once; the exported batch becomes the resident unit.
+    int prepare_captured_group(std::vector<PrepJob>& jobs, const std::vector<int64_t>& nv, uint32_t cf,
+                               const btc_count_opts& co, hipStream_t s, PipeCache& cache_p,
+                               std::shared_ptr<Unit>& out) {
+        const int32_t k = (int32_t)jobs.size();
+        std::vector<const int32_t*> sp((size_t)k), dp((size_t)k);
+        std::vector<int64_t> sizes((size_t)k);
+        int64_t nsum = 0, pairs = 0;
+        for (int32_t g = 0; g < k; ++g) {
+            sp[(size_t)g] = jobs[(size_t)g].src.data();
+            dp[(size_t)g] = jobs[(size_t)g].dst.data();
+            sizes[(size_t)g] = (int64_t)jobs[(size_t)g].src.size();
+            nsum += nv[(size_t)g];
+            pairs += sizes[(size_t)g];
+        }
+        btc_pipeline pipe = nullptr;
+        int rc = cache_p.get((int32_t)pow2_at_least(nsum, 1 << 14), pow2_at_least(pairs, 1 << 16), opts.group_graphs,
+                             cf & BTC_CANON_DEFAULT, co, s, &pipe);
+        if (rc != BTC_OK) return rc;
+        const double t_start = now();
+        std::vector<uint64_t> per((size_t)k);
+        uint64_t tot = 0;
+        // ids were checked against nv when the group was formed
+        rc = btcx::guard([&] {
+            btcx::pipeline_count_checked(pipe, k, sp.data(), dp.data(), sizes.data(), nv.data(), s, per.data(), &tot);
+        });
+        for (PrepJob& j : jobs) {
+            std::vector<int32_t>().swap(j.src);
+            std::vector<int32_t>().swap(j.dst);
+        }
+        if (rc != BTC_OK) return rc;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (int32_t g = 0; g < k; ++g) {
+                Entry& e = cache[jobs[(size_t)g].key];
+                auto w = waiting.find(jobs[(size_t)g].key);
+                if (w == waiting.end()) continue;
+                for (Request& r : w->second) {
+                    --e.pins;
+                    r.t_start = t_start;
+                    finish(r, BTC_OK, per[(size_t)g], k);
+                    ++stats.launched_requests;
+                }
+                waiting.erase(w);
+            }
+            ++stats.launches;
+            cv_done.notify_all();
+        }
+        btc_pipeline_info pi;
+        btc_pipeline_get_info(pipe, &pi);
+        const int64_t est = 24 * pi.m + 36 * (int64_t)pi.work_items + 20 * (int64_t)pi.n + (1 << 16);
+        if (!admit(est)) return BTC_ERR_STATE;
+        btc_batch b = nullptr;
+        rc = btc_pipeline_export_batch(pipe, s, &b);
+        if (rc == BTC_OK) btcx::batch_set_side_streams(b, false);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            resident_bytes -= est;
+        }
+        cv_prep.notify_all();
+        if (rc != BTC_OK) return rc;
+        btc_batch_info bi;
+        btc_batch_get_info(b, &bi);
+        out = std::make_shared<Unit>();
+        out->batch = b;
+        out->bytes = bi.plan.device_bytes + 4 * (bi.m + 2 * (int64_t)bi.n + 1) + 8 * bi.graphs;
+        for (const PrepJob& j : jobs) out->keys.push_back(j.key);
+        return BTC_OK;
     }
 
     // One GPU canonicalisation and one block-diagonal plan for several graphs.
@@ -527,7 +608,11 @@ struct btc_server_s {
                         std::vector<uint64_t> keys;
                         for (const PrepJob& j : good) keys.push_back(j.key);
                         std::shared_ptr<Unit> u;
-                        const int rc = prepare_group(good, nv, cf, co, s, pin, u);
+                        int64_t pairs = 0;
+                        for (const PrepJob& j : good) pairs += (int64_t)j.src.size();
+                        const int rc = captured && pairs <= kPipeMaxPairs
+                                           ? prepare_captured_group(good, nv, cf, co, s, pipes, u)
+                                           : prepare_group(good, nv, cf, co, s, pin, u);
                         if (rc == BTC_OK) units.push_back(u);
                         else
                             for (uint64_t k : keys) failed.emplace_back(k, rc);

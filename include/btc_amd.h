@@ -465,7 +465,9 @@ typedef struct {
                                  2^23 pairs with one replay of a btc_pipeline of its size class (ids and
                                  pairs rounded up to powers of two), answers the requests waiting for it at
                                  once, then keeps a copy of its L and plan resident for later requests
-                                 (btc_pipeline_export); a longer queue is prepared in groups as with 0.
+                                 (btc_pipeline_export); a longer queue is prepared in groups of up to
+                                 group_graphs graphs, each group by one run of a pipeline for several graphs
+                                 (btc_pipeline_count_batch, then btc_pipeline_export_batch).
                                  0: eager preparation (grouped), counts by the GPU worker */
 } btc_server_opts;
 
