@@ -21,6 +21,8 @@
 // (r, q) = (lane / 4, lane % 4) owns row r of the tile and the 4 columns
 // 4q..4q+3 of each 16-column J_out slab, so every lane's partial sum belongs
 // to a single vertex 16I + r.
+#include <type_traits>
+
 #include "device_util.h"
 #include "prim.h"
 #include "internal.h"
@@ -164,6 +166,12 @@ struct RowBits<32> {
         x.w = reinterpret_cast<const uint32_t*>(T + tile * 8)[r];
         return x;
     }
+    // the row at p (4-byte aligned)
+    __device__ static RowBits at(const unsigned char* p) {
+        RowBits x;
+        x.w = *reinterpret_cast<const uint32_t*>(p);
+        return x;
+    }
     __device__ int and_popc(const RowBits& o) const { return __popc(w & o.w); }
     __device__ uint32_t bits(int c0) const { return w >> c0; }
 };
@@ -175,6 +183,12 @@ struct RowBits<64> {
     __device__ static RowBits load(const uint64_t* T, size_t tile, int r) {
         RowBits x;
         x.w = T[tile * 16 + r];
+        return x;
+    }
+    // the row at p (8-byte aligned)
+    __device__ static RowBits at(const unsigned char* p) {
+        RowBits x;
+        x.w = *reinterpret_cast<const uint64_t*>(p);
         return x;
     }
     __device__ int and_popc(const RowBits& o) const { return __popcll(w & o.w); }
@@ -193,6 +207,14 @@ struct RowBits<128> {
         const uint64_t* p = T + tile * 32 + 2 * r;
         x.w0 = p[0];
         x.w1 = p[1];
+        return x;
+    }
+    // the row at p (16-byte aligned)
+    __device__ static RowBits at(const unsigned char* p) {
+        RowBits x;
+        const uint4 v = *reinterpret_cast<const uint4*>(p);
+        x.w0 = ((uint64_t)v.y << 32) | v.x;
+        x.w1 = ((uint64_t)v.w << 32) | v.z;
         return x;
     }
     __device__ int and_popc(const RowBits& o) const {
@@ -737,9 +759,13 @@ template <int K>
 constexpr int kTileBytes = 2 * K;  // 16 rows of K bits
 template <int K>
 constexpr int kStageBytes = 2 * kStagePairs * kTileBytes<K>;
-// per wave: stage, then the match ring (uint2 each)
+// AND + popcount: a unit's mask entries (one byte each, row << 4 | column;
+// at most 16 x 16) and, for per-vertex sums, its 16 row totals
+constexpr int kEntryBytes = 256;
+constexpr int kRowSumBytes = 16 * 8;
+// per wave: stage, the match ring (uint2 each), the entries, the row totals
 template <int K>
-constexpr int kWaveDenseBytes = kStageBytes<K> + (int)kMatchRing * 8;
+constexpr int kWaveDenseBytes = kStageBytes<K> + (int)kMatchRing * 8 + kEntryBytes + kRowSumBytes;
 
 typedef const void __attribute__((address_space(1)))* GlobalVoid;
 typedef void __attribute__((address_space(3)))* LdsVoid;
@@ -755,7 +781,9 @@ typedef void __attribute__((address_space(3)))* LdsVoid;
 // stage by global_load_lds_dwordx4 (each lane 16 bytes, all loads in flight
 // together, no registers), then each pair's 16 x 16 block A B^T is taken
 // from LDS: i8 MFMA (MFMA) or 64-bit AND + v_bcnt popcount.  The mask
-// tile's slab selects which block entries count.
+// tile's slab selects which block entries count: MFMA computes all 256 and
+// masks the sums; AND + popcount at K = 128 first lists the unit's mask
+// entries (a ballot prefix sum) and computes only those, up to 4 per lane.
 template <int K, bool PV, bool MFMA>
 __global__ __launch_bounds__(256) void dense_unit_kernel(int n, const uint32_t* __restrict__ indices,
                                                          const uint64_t* __restrict__ tiles,
@@ -766,6 +794,10 @@ __global__ __launch_bounds__(256) void dense_unit_kernel(int n, const uint32_t* 
     constexpr int SLABS = K / 16;
     constexpr uint32_t TB = kTileBytes<K>;
     constexpr uint32_t WB = kWaveDenseBytes<K>;
+    constexpr uint32_t RB = K / 8;  // bytes of a tile row
+    // AND + popcount lists the mask entries at K = 128 only (planted 2^17: 39.4 -> 28.8 us there; at
+    // K = 64 31.8 against 31.9 us, at K = 32 28.5 against 23.0 us)
+    constexpr bool kCompact = !MFMA && K == 128;
     // dynamic LDS: 4 waves' stage + ring, then 4 column sets' bits, then their ranks
     extern __shared__ __align__(16) unsigned char dense_lds[];
     __shared__ unsigned long long wg_part[4];
@@ -776,6 +808,9 @@ __global__ __launch_bounds__(256) void dense_unit_kernel(int n, const uint32_t* 
     unsigned char* stage_b = dense_lds + wv * WB;
     const uint64_t* stage = reinterpret_cast<const uint64_t*>(stage_b);
     uint2* ring = reinterpret_cast<uint2*>(dense_lds + wv * WB + kStageBytes<K>);
+    uint8_t* ents = dense_lds + wv * WB + kStageBytes<K> + kMatchRing * 8;
+    unsigned long long* row_sums = reinterpret_cast<unsigned long long*>(ents + kEntryBytes);
+    if (PV && lane < 16) row_sums[lane] = 0;
     uint32_t* sets = reinterpret_cast<uint32_t*>(dense_lds + 4 * WB);
     ColumnSet cs{sets + wv * set_words, reinterpret_cast<uint16_t*>(sets + 4 * set_words) + wv * set_words, 0};
     if (set_words)
@@ -813,9 +848,62 @@ __global__ __launch_bounds__(256) void dense_unit_kernel(int n, const uint32_t* 
         }
         uint32_t head = 0, tail = 0;  // ring positions (wave-uniform)
         v4i acc = {0, 0, 0, 0};                 // MFMA: D[4 (l / 16) + v][l % 16]
-        uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // AND: row l / 4, columns 4 (l % 4) + v
+        // AND at K = 128: entry j * 64 + lane of the unit's mask entries
+        // (valid below `listed`) has its A and B rows at byte offsets
+        // oab[j] & 0xFFFF and oab[j] >> 16 of a staged pair; rounds = the
+        // wave's entry rounds (1 to 4).  AND at K = 32 and 64: row l / 4,
+        // columns 4 (l % 4) + v, every entry.
+        uint32_t oab[4] = {0, 0, 0, 0}, sj[4] = {0, 0, 0, 0};
+        uint32_t rounds = 0, listed = 0;
+        if constexpr (kCompact) {
+            const int r = lane >> 2, q = lane & 3;
+            const uint32_t m4 = (__shfl((int)mrow, r) >> (4 * q)) & 0xFu;
+            const uint32_t c4 = (uint32_t)__popc(m4);
+            uint32_t pre = 0, total = 0;  // exclusive prefix of c4 (< 8) over the lanes, bit by bit
+#pragma unroll
+            for (int bt = 0; bt < 3; ++bt) {
+                const uint64_t bal = __ballot((c4 >> bt) & 1u);
+                pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u))
+                       << bt;
+                total += (uint32_t)__popcll(bal) << bt;
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if ((m4 >> v) & 1u) ents[pre++] = (uint8_t)((r << 4) | (4 * q + v));
+            ColumnSet::wave_lds_sync();
+            listed = uniform(total);
+            rounds = (listed + 63) / 64;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t x = (uint32_t)j * 64 + (uint32_t)lane;
+                const uint32_t e = x < listed ? ents[x] : 0u;
+                oab[j] = (e >> 4) * RB | (TB + (e & 15u) * RB) << 16;
+            }
+            ColumnSet::wave_lds_sync();  // the next unit rewrites the entries
+        }
+        // the listed entries of staged pairs [0, nb) at K = 128: R rounds,
+        // full batches unrolled
+        auto and_listed = [&](auto rounds_c, uint32_t nb) __attribute__((always_inline)) {
+            constexpr int R = decltype(rounds_c)::value;
+            if (nb == (uint32_t)kStagePairs) {
+#pragma unroll
+                for (int m = 0; m < kStagePairs; ++m)
+#pragma unroll
+                    for (int j = 0; j < R; ++j)
+                        sj[j] += RowBits<K>::at(stage_b + 2 * m * TB + (oab[j] & 0xFFFFu))
+                                     .and_popc(RowBits<K>::at(stage_b + 2 * m * TB + (oab[j] >> 16)));
+            } else {
+#pragma unroll 1
+                for (uint32_t m = 0; m < nb; ++m)
+#pragma unroll
+                    for (int j = 0; j < R; ++j)
+                        sj[j] += RowBits<K>::at(stage_b + 2 * m * TB + (oab[j] & 0xFFFFu))
+                                     .and_popc(RowBits<K>::at(stage_b + 2 * m * TB + (oab[j] >> 16)));
+            }
+        };
+        uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // AND at K = 32 and 64
         // stages the tiles of ring pairs [head, head + nb) and adds their blocks
-        auto drain = [&](uint32_t nb) {
+        auto drain = [&](uint32_t nb) __attribute__((always_inline)) {
             const uint32_t bytes = 2 * nb * TB;
             for (uint32_t k = 0; k * 1024u < bytes; ++k) {
                 const uint32_t o = k * 1024u + (uint32_t)lane * 16u;
@@ -833,6 +921,13 @@ __global__ __launch_bounds__(256) void dense_unit_kernel(int n, const uint32_t* 
                 const int g = lane >> 4, c = lane & 15;
                 for (uint32_t m = 0; m < nb; ++m)
                     acc = mfma_mac<K>(mfma_load<K>(stage, 2 * m, c), mfma_load<K>(stage, 2 * m + 1, c), g, acc);
+            } else if constexpr (kCompact) {
+                switch (rounds) {
+                    case 1: and_listed(std::integral_constant<int, 1>{}, nb); break;
+                    case 2: and_listed(std::integral_constant<int, 2>{}, nb); break;
+                    case 3: and_listed(std::integral_constant<int, 3>{}, nb); break;
+                    default: and_listed(std::integral_constant<int, 4>{}, nb); break;
+                }
             } else {
                 const int r = lane >> 2, q = lane & 3;
                 for (uint32_t m = 0; m < nb; ++m) {
@@ -899,6 +994,26 @@ __global__ __launch_bounds__(256) void dense_unit_kernel(int n, const uint32_t* 
                     const int64_t vtx = (int64_t)I * 16 + 4 * g + v;
                     if (c == 0 && x && vtx < n) pv_add(pv, vtx, x);
                 }
+            }
+        } else if constexpr (kCompact) {
+            uint64_t sum = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if ((uint32_t)j * 64 + (uint32_t)lane < listed) sum += sj[j];
+            tot += sum;
+            if (PV) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if ((uint32_t)j * 64 + (uint32_t)lane < listed && sj[j])
+                        atomicAdd(&row_sums[(oab[j] & 0xFFFFu) / RB], (unsigned long long)sj[j]);
+                ColumnSet::wave_lds_sync();
+                if (lane < 16) {
+                    const unsigned long long x = row_sums[lane];
+                    row_sums[lane] = 0;
+                    const int64_t v = (int64_t)I * 16 + lane;
+                    if (x && v < n) pv_add(pv, v, x);
+                }
+                ColumnSet::wave_lds_sync();
             }
         } else {
             const int r = lane >> 2, q = lane & 3;
