@@ -121,8 +121,8 @@ int btc_autotune(btc_graph g, const btc_tune_opts* topts, void* stream, btc_coun
                     return mn;
                 };
                 // {lanes per list, loads in flight, workgroups per CU, schedule}
-                const int shapes[6][4] = {{8, 1, 0, 1}, {4, 1, 0, 1}, {8, 2, 0, 1},
-                                          {16, 1, 0, 1}, {8, 1, 6, 1}, {8, 1, 0, 0}};
+                const int shapes[6][4] = {{4, 1, 0, 1}, {8, 1, 0, 1}, {4, 2, 0, 1},
+                                          {16, 1, 0, 1}, {4, 1, 6, 1}, {4, 1, 0, 0}};
                 btc_count_opts plan_best = o;
                 float plan_ms = 1e30f;
                 for (const auto& sh : shapes) {
@@ -217,7 +217,7 @@ int btc_tune_table_load(const char* path) {
             if (r == EOF) break;
             ++line;
             if (r != 13 || std::strcmp(tag, "btc-tune-3") != 0 || !(K == 32 || K == 64 || K == 128) ||
-                !(G == 8 || G == 16 || G == 32) || !(U == 1 || U == 2 || U == 4) || w < 0 || D < 0 || D > 2 ||
+                !(G == 4 || G == 8 || G == 16 || G == 32) || !(U == 1 || U == 2 || U == 4) || w < 0 || D < 0 || D > 2 ||
                 !(core <= 0 || (core % 128 == 0 && core <= 32768))) {
                 std::fclose(f);
                 btcx::fail(BTC_ERR_ARG, "%s: bad entry on line %d", path, line);

@@ -192,11 +192,12 @@ __device__ __forceinline__ void team_sync() {
 // bitmap (and whose pivot is in an LDS bitmap) first ANDs the row's core
 // bitmap with the pivot's window word by word (popcount) and then looks up
 // only the prefix ids >= C; any other list looks up its whole clipped
-// prefix.  Prefix ids are read 4 per lane (U loads in flight) with aligned
-// loads: 16 bytes of 32-bit ids, or, when N16 (every id of the pivot below
-// kNarrowIds, so every prefix id too), 8 bytes of the 16-bit copy;
-// ids outside the prefix become the set's sentinel, or, in a FullWindow16,
-// are masked out of the four bits a lane reads.  The next
+// prefix.  Prefix ids are read with aligned loads, U in flight per lane:
+// 16 bytes of 32-bit ids, or, when N16 (every id of the pivot below
+// kNarrowIds, so every prefix id too), 8 bytes of the 16-bit copy, or, in a
+// FullWindow16, 16 bytes (8 ids) of it; ids outside the prefix become the
+// set's sentinel, or, in a FullWindow16, are masked out of the eight bits a
+// lane reads.  The next
 // list's record is fetched before the current list is scanned; records are
 // read once per launch, so they are loaded non-temporally.  L is the
 // group's first record, loaded by the caller.  For bitmap sets, wabs[w] is
@@ -242,17 +243,26 @@ __device__ __forceinline__ uint64_t scan_lists(const Set& set, const PlanDev& a,
             s0 = L.y;
         }
         if constexpr (Set::kFull16) {
-            for (uint32_t q = (s0 & ~3u) + 4 * gl; q < s1; q += STEP) {
+            // 16 bytes (8 ids) per lane per load, U loads in flight
+            for (uint32_t q = (s0 & ~7u) + 8 * gl; q < s1; q += 2 * STEP) {
+                uint4 h[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const uint32_t qq = q + 4 * G * u;
+                    const uint32_t qq = q + 8 * G * u;
+                    h[u] = make_uint4(0, 0, 0, 0);
+                    if (qq < s1) h[u] = to_u4(*(GlobalU4)(colb + byte_off<W64>(qq, 1)));
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t qq = q + 8 * G * u;
                     if (qq < s1) {
-                        const uint32_t b = set.bits4(to_u2(*(GlobalU2)(colb + byte_off<W64>(qq, 1))));
-                        // the prefix starts d0 (0..3) ids into this load and
-                        // ends d1 (1..4) ids into it (empty when d1 <= d0)
+                        const uint32_t b = set.bits4(make_uint2(h[u].x, h[u].y)) |
+                                           (set.bits4(make_uint2(h[u].z, h[u].w)) << 4);
+                        // the prefix starts d0 (0..7) ids into this load and
+                        // ends d1 (1..8) ids into it (empty when d1 <= d0)
                         const uint32_t d0 = s0 > qq ? s0 - qq : 0u;
-                        const uint32_t d1 = min(s1 - qq, 4u);
-                        hits += __popc(b & (0xFu << d0) & ((1u << d1) - 1u));
+                        const uint32_t d1 = min(s1 - qq, 8u);
+                        hits += __popc(b & (0xFFu << d0) & ((1u << d1) - 1u));
                     }
                 }
             }
@@ -447,9 +457,9 @@ __device__ __forceinline__ void flush_slot(uint64_t acc, unsigned long long* slo
 // with its own 4 KiB slice.  schedule 1 strides statically over the sorted
 // items and prefetches the next item; schedule 0 takes items from an
 // atomic cursor.
-// The default shape (G = 8, U = 1, totals only) fits 64 VGPRs without
-// spilling, which keeps 8 waves per SIMD resident; the others are left to
-// the compiler (forcing them spills to scratch).
+// Shapes with G <= 8, U = 1, totals only (the default is G = 4) are held
+// to 64 VGPRs, which keeps 8 waves per SIMD resident at the cost of a few
+// spilled registers; the others are left to the compiler.
 template <int G, int U, bool PV>
 constexpr int kMinWaves = (G <= 8 && U == 1 && !PV) ? 8 : 1;
 
@@ -895,12 +905,12 @@ void check_launch_opts(const btc_count_opts& o) {
 
 namespace {
 
-// W64: 64-bit byte offsets into L, in the default shape only (8 lanes per
+// W64: 64-bit byte offsets into L, in the default shape only (4 lanes per
 // list, one load in flight).
 WedgeFn wedge_fn(const btc_count_opts& o, bool pv, bool w64, unsigned* grid) {
-    const int G = o.group_lanes ? o.group_lanes : 8;
+    const int G = o.group_lanes ? o.group_lanes : 4;
     const int U = o.unroll ? o.unroll : 1;
-    WedgeFn fn = w64 ? (pv ? wedge_kernel<8, 1, true, true> : wedge_kernel<8, 1, false, true>) : pick_wedge(G, U, pv);
+    WedgeFn fn = w64 ? (pv ? wedge_kernel<4, 1, true, true> : wedge_kernel<4, 1, false, true>) : pick_wedge(G, U, pv);
     int per_cu = 0;
     BTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kThreads, 0));
     per_cu = std::max(1, per_cu);

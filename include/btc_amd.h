@@ -203,7 +203,7 @@ typedef struct {
     int32_t chunk_work;   /* wedge checks per work item; 0 = auto */
     int32_t waves_per_eu; /* resident workgroups (4 waves) per CU of the persistent grid; 0 = max */
     int32_t lookup;       /* 0 = auto, 1 = never the LDS bitmap, 2 = never the LDS hash */
-    int32_t group_lanes;  /* lanes streaming one prefix list: 4, 8, 16 or 32; 0 = auto (8) */
+    int32_t group_lanes;  /* lanes streaming one prefix list: 4, 8, 16 or 32; 0 = auto (4) */
     int32_t unroll;       /* 16-byte column loads in flight per lane: 1, 2 or 4; 0 = auto */
     int32_t schedule;     /* 0 = work stealing on one counter, 1 = static stride over sorted items */
     int32_t dense_kernel; /* dense tiles: 0 = auto (AND + popcount, the faster form at every K measured),
