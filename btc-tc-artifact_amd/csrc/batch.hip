@@ -5,9 +5,13 @@
 // 16 x K tile, K <= 128, holds rows or columns of two graphs), one plan is
 // built for it, and one enqueue of the dense-tile and wedge kernels counts
 // all graphs; the wedge kernel's static stride over work-sorted items
-// balances graphs of very different sizes.  Per-graph totals are segment
-// sums of the per-vertex counts, one workgroup per graph.  No reference
-// counterpart (the reference counts one graph per process).
+// balances graphs of very different sizes.  Every wedge and tiny-pivot work
+// item belongs to one graph (its pivot's), so those kernels add each item's
+// count to its graph's sum (the item record carries the graph index); the
+// dense-tile kernel keeps per-vertex sums, and a per-graph total is its
+// graph's item sum plus the segment sum of its vertices' dense counts, one
+// workgroup per graph.  No reference counterpart (the reference counts one
+// graph per process).
 #include <algorithm>
 #include <cstring>
 
@@ -26,6 +30,7 @@ struct btc_batch_s {
     btcx::DevBuf<uint32_t> d_voff;
     btcx::PvBuf d_pv;  // per-vertex sums (64-bit slots when a row is longer than 65536)
     btcx::DevBuf<unsigned long long> d_pg;
+    btcx::DevBuf<unsigned long long> d_gsum;  // item sums per graph (zero between counts)
     std::vector<uint64_t> h_pg;
     btcx::CapturedGraph count_graph;  // zero + count + per-graph sums, replayed when captured
     // counts may put the dense-tile and tiny-pivot kernels on side streams
@@ -42,8 +47,10 @@ struct btc_batch_s {
 namespace btcx {
 namespace {
 
-// per_graph[g] = sum of pv over [voff[g], voff[g + 1]) (padding vertices are 0)
+// per_graph[g] = gsum[g] + sum of pv over [voff[g], voff[g + 1]) (padding
+// vertices are 0); gsum[g] is cleared for the next count
 __global__ __launch_bounds__(256) void graph_sums_kernel(PvOut pv, const uint32_t* __restrict__ voff,
+                                                         unsigned long long* __restrict__ gsum,
                                                          unsigned long long* __restrict__ per_graph) {
     __shared__ unsigned long long part[4];
     const uint32_t g = blockIdx.x;
@@ -53,10 +60,58 @@ __global__ __launch_bounds__(256) void graph_sums_kernel(PvOut pv, const uint32_
     s = wave_sum_u64(s);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) per_graph[g] = part[0] + part[1] + part[2] + part[3];
+    if (threadIdx.x == 0) {
+        per_graph[g] = gsum[g] + part[0] + part[1] + part[2] + part[3];
+        gsum[g] = 0;
+    }
+}
+
+// lseg[g] = first position of graph g's rows in L (g = 0..count)
+__global__ void graph_starts_kernel(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ voff,
+                                    uint32_t count, uint32_t* __restrict__ lseg) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g <= count; g += gridDim.x * blockDim.x)
+        lseg[g] = rowptr[voff[g]];
+}
+
+// Writes each item's graph (the one whose rows hold its pivot row's start
+// t0: the last g with lseg[g] <= t0) into the item's B.w.
+__global__ void item_graphs_kernel(uint4* __restrict__ items, uint32_t nitems, const uint32_t* __restrict__ lseg,
+                                   uint32_t count) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nitems; k += gridDim.x * blockDim.x) {
+        const uint32_t t0 = items[2 * k].x;
+        uint32_t lo = 0, hi = count;  // lseg[lo] <= t0 < lseg[hi] when lseg[count] > t0
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (lseg[mid] <= t0) lo = mid;
+            else hi = mid;
+        }
+        items[2 * k + 1].w = lo;
+    }
 }
 
 }  // namespace
+
+// Per-graph item sums for a batch's plan: graph indices into its items and
+// the zeroed sums the wedge and tiny kernels add to.
+void batch_item_graphs(btc_batch_s* b, hipStream_t s) {
+    btc_plan_s* p = b->plan;
+    b->d_gsum.alloc((size_t)b->count);
+    BTC_HIP(hipMemsetAsync(b->d_gsum.p, 0, (size_t)b->count * 8, s));
+    if (p->nitems) {
+        TmpBuf<uint32_t> lseg;
+        lseg.alloc((size_t)b->count + 1, s);
+        const unsigned g1 = (unsigned)std::min<int64_t>(((int64_t)b->count + 256) / 256, 1024);
+        hipLaunchKernelGGL(graph_starts_kernel, dim3(g1), dim3(256), 0, s, b->graph.d_rowptr.p, b->d_voff.p,
+                           (uint32_t)b->count, lseg.p);
+        BTC_CHECK_LAUNCH("graph_starts_kernel");
+        const unsigned g2 = (unsigned)std::min<int64_t>(((int64_t)p->nitems + 255) / 256, 4096);
+        hipLaunchKernelGGL(item_graphs_kernel, dim3(g2), dim3(256), 0, s, const_cast<uint4*>(p->items_p),
+                           (uint32_t)p->nitems, lseg.p, (uint32_t)b->count);
+        BTC_CHECK_LAUNCH("item_graphs_kernel");
+    }
+    p->item_graph_sums = b->d_gsum.p;
+}
+
 }  // namespace btcx
 
 namespace btcx {
@@ -70,6 +125,7 @@ void batch_finish(btc_batch_s* b, const btc_count_opts* opts, hipStream_t s) {
     b->d_pg.alloc((size_t)b->count);
     b->h_pg.assign((size_t)b->count, 0);
     BTC_HIP(hipMemcpyAsync(b->d_voff.p, vo32.data(), vo32.size() * 4, hipMemcpyHostToDevice, s));
+    batch_item_graphs(b, s);
     BTC_HIP(hipStreamSynchronize(s));
 }
 
@@ -99,6 +155,7 @@ btc_batch_s* batch_from_plan(btc_graph_s&& g, btc_plan_s* plan, const std::vecto
     b->d_pg.alloc((size_t)b->count);
     b->h_pg.assign((size_t)b->count, 0);
     BTC_HIP(hipMemcpyAsync(b->d_voff.p, vo32.data(), vo32.size() * 4, hipMemcpyHostToDevice, s));
+    batch_item_graphs(b.get(), s);
     BTC_HIP(hipStreamSynchronize(s));
     return b.release();
 }
@@ -109,7 +166,8 @@ void batch_set_side_streams(btc_batch_s* b, bool on) { b->side_streams = on; }
 
 void btc_batch_s::enqueue_count(hipStream_t s) {
     btcx::plan_enqueue(plan, s, d_pv.out(), nullptr, nullptr, nullptr, side_streams);
-    hipLaunchKernelGGL(btcx::graph_sums_kernel, dim3((unsigned)count), dim3(256), 0, s, d_pv.out(), d_voff.p, d_pg.p);
+    hipLaunchKernelGGL(btcx::graph_sums_kernel, dim3((unsigned)count), dim3(256), 0, s, d_pv.out(), d_voff.p,
+                       d_gsum.p, d_pg.p);
     BTC_CHECK_LAUNCH("graph_sums_kernel");
 }
 
@@ -243,6 +301,8 @@ int btc_batch_autotune(btc_batch b, const btc_tune_opts* topts, void* stream, bt
         btcx::check_status(btc_plan_create(&b->graph, best, stream, &p));
         btc_plan_free(b->plan);
         b->plan = p;
+        btcx::batch_item_graphs(b, btcx::as_stream(stream));
+        BTC_HIP(hipStreamSynchronize(btcx::as_stream(stream)));
         b->count_graph.reset();
     });
 }

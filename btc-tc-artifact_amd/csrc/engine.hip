@@ -148,12 +148,15 @@ struct PlanDev {
     const uint4* lists;        // per list {s0, s_noncore, s1, row bitmap or kNoBitmap}
     const uint4* rowbits;      // core4 uint4 per row bitmap
     const int32_t* ur;
-    const uint4* items;  // 2 x uint4 per item: {t0, t1, eb, ee}, {lo, hi, work, 0}
+    const uint4* items;  // 2 x uint4 per item: {t0, t1, eb, ee}, {lo, hi, work, graph (batches) or 0}
     uint32_t n_block;    // items [0, n_block) are workgroup items, the rest wave items
     uint32_t n_items;
     unsigned long long* result;
     PvOut pv;
     uint32_t core4;  // hub-core ids / 128 (uint4 words of a row bitmap)
+    // batches: per-graph sums that the GS kernels add each item's count to,
+    // at the graph index its record holds (B.w)
+    unsigned long long* gsum = nullptr;
 };
 
 // One launch counts one plan (nplans == 1: p0) or several (plans[], with map
@@ -463,11 +466,39 @@ __device__ __forceinline__ void flush_slot(uint64_t acc, unsigned long long* slo
 template <int G, int U, bool PV>
 constexpr int kMinWaves = (G <= 8 && U == 1 && !PV) ? 8 : 1;
 
-template <int G, int U, bool PV, bool W64>
+// Adds one item's count to its graph's sum gsum[g] (batches): v is this
+// thread's part of the item, shared by NT threads (a workgroup, a wave or a
+// tiny-item group).  A workgroup reduces through part[4]; the caller's next
+// barrier orders part's reuse.
+template <int NT>
+__device__ __forceinline__ void add_item_graph(uint64_t v, unsigned long long* gsum, uint32_t g,
+                                               unsigned long long* part, uint32_t tid) {
+    if constexpr (NT == kThreads) {
+        v = wave_sum_u64(v);
+        if ((tid & 63) == 0) part[tid >> 6] = v;
+        __syncthreads();
+        if (tid == 0) {
+            const unsigned long long x = part[0] + part[1] + part[2] + part[3];
+            if (x) atomicAdd(gsum + g, x);
+        }
+    } else if constexpr (NT == 64) {
+        v = wave_sum_u64(v);
+        if ((tid & 63) == 0 && v) atomicAdd(gsum + g, (unsigned long long)v);
+    } else {
+#pragma unroll
+        for (int o = NT / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, NT);
+        if (tid % NT == 0 && v) atomicAdd(gsum + g, (unsigned long long)v);
+    }
+}
+
+// GS: every item's count also goes to its graph's sum (PV is then off).
+template <int G, int U, bool PV, bool W64, bool GS>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<G, U, PV>))) void wedge_kernel(
     WedgeArgs a) {
+    static_assert(!(GS && PV), "per-graph item sums replace per-vertex sums");
     __shared__ __align__(16) uint32_t lds[kLdsWords];  // 16-byte reads of bitmap words
     __shared__ uint32_t s_item;
+    __shared__ unsigned long long gs_part[kThreads / 64];
     const uint32_t tid = threadIdx.x;
     const int lane = tid & 63;
     const uint32_t wid = uniform(tid >> 6);
@@ -489,7 +520,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             Item nxt;
             if (nx < n_block) nxt = fetch_item(a, nx, n_block, pnb);
             const PlanDev P = item_plan(a, cur);
-            acc += process_item<kThreads, kLdsWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, lds, tid);
+            const uint64_t v = process_item<kThreads, kLdsWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, lds, tid);
+            acc += v;
+            if constexpr (GS) add_item_graph<kThreads>(v, P.gsum, cur.B.w, gs_part, tid);
             if (multi) flush(acc, P.result);
             __syncthreads();
             it = nx;
@@ -503,7 +536,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             if (it >= n_block) break;
             const Item cur = fetch_item(a, it, n_block, pnb);
             const PlanDev P = item_plan(a, cur);
-            acc += process_item<kThreads, kLdsWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, lds, tid);
+            const uint64_t v = process_item<kThreads, kLdsWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, lds, tid);
+            acc += v;
+            if constexpr (GS) add_item_graph<kThreads>(v, P.gsum, cur.B.w, gs_part, tid);
             if (multi) flush(acc, P.result);
             __syncthreads();
         }
@@ -522,7 +557,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             Item nxt;
             if (nx < nwave) nxt = fetch_item(a, n_block + nx, n_block, pnb);
             const PlanDev P = item_plan(a, cur);
-            acc += process_item<64, kWaveWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
+            const uint64_t v = process_item<64, kWaveWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, slice,
+                                                                           (uint32_t)lane);
+            acc += v;
+            if constexpr (GS) add_item_graph<64>(v, P.gsum, cur.B.w, nullptr, (uint32_t)lane);
             if (multi) flush(acc, P.result);
             team_sync<64>();
             it = nx;
@@ -536,7 +574,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
             if (it >= nwave) break;
             const Item cur = fetch_item(a, n_block + it, n_block, pnb);
             const PlanDev P = item_plan(a, cur);
-            acc += process_item<64, kWaveWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, slice, (uint32_t)lane);
+            const uint64_t v = process_item<64, kWaveWords, G, U, PV, W64>(P, a.lookup, cur.A, cur.B, slice,
+                                                                           (uint32_t)lane);
+            acc += v;
+            if constexpr (GS) add_item_graph<64>(v, P.gsum, cur.B.w, nullptr, (uint32_t)lane);
             if (multi) flush(acc, P.result);
             team_sync<64>();
         }
@@ -553,10 +594,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 // wave has 8 pivots' HBM round trips in flight where a wave item has one.
 // The groups of a wave run different items; every step of process_item
 // for a team of kTinyLanes lanes stays within the group.
-template <bool PV, bool W64>
+template <bool PV, bool W64, bool GS>
 __global__ __launch_bounds__(kThreads) void tiny_kernel(PlanDev P, int lookup, uint32_t first, uint32_t n,
                                                          uint32_t share, uint32_t shares,
                                                          unsigned long long* slots) {
+    static_assert(!(GS && PV), "per-graph item sums replace per-vertex sums");
     constexpr uint32_t kGroups = kThreads / kTinyLanes;
     __shared__ __align__(16) uint32_t lds[kGroups * kTinyWords];
     const uint32_t tid = threadIdx.x, grp = tid / kTinyLanes, gl = tid % kTinyLanes;
@@ -567,7 +609,9 @@ __global__ __launch_bounds__(kThreads) void tiny_kernel(PlanDev P, int lookup, u
     for (uint32_t t = blockIdx.x * kGroups + grp; t < n; t += gridDim.x * kGroups) {
         const uint32_t li = first + share + t * shares;
         const uint4 A = P.items[2 * li], B = P.items[2 * li + 1];
-        acc += process_item<kTinyLanes, kTinyWords, kTinyLanes, 1, PV, W64>(P, lookup, A, B, table, gl);
+        const uint64_t v = process_item<kTinyLanes, kTinyWords, kTinyLanes, 1, PV, W64>(P, lookup, A, B, table, gl);
+        acc += v;
+        if constexpr (GS) add_item_graph<kTinyLanes>(v, P.gsum, B.w, nullptr, tid);
     }
     if (slots) flush_slot(acc, slots);
     else flush(acc, P.result);
@@ -576,25 +620,26 @@ __global__ __launch_bounds__(kThreads) void tiny_kernel(PlanDev P, int lookup, u
 using WedgeFn = void (*)(WedgeArgs);
 
 template <int G, int U>
-WedgeFn pick_pv(bool pv) {
-    return pv ? wedge_kernel<G, U, true, false> : wedge_kernel<G, U, false, false>;
+WedgeFn pick_pv(bool pv, bool gs) {
+    return pv ? wedge_kernel<G, U, true, false, false>
+              : (gs ? wedge_kernel<G, U, false, false, true> : wedge_kernel<G, U, false, false, false>);
 }
 
 template <int G>
-WedgeFn pick_u(int u, bool pv) {
+WedgeFn pick_u(int u, bool pv, bool gs) {
     switch (u) {
-        case 1: return pick_pv<G, 1>(pv);
-        case 2: return pick_pv<G, 2>(pv);
-        default: return pick_pv<G, 4>(pv);
+        case 1: return pick_pv<G, 1>(pv, gs);
+        case 2: return pick_pv<G, 2>(pv, gs);
+        default: return pick_pv<G, 4>(pv, gs);
     }
 }
 
-WedgeFn pick_wedge(int g, int u, bool pv) {
+WedgeFn pick_wedge(int g, int u, bool pv, bool gs) {
     switch (g) {
-        case 4: return pick_u<4>(u, pv);
-        case 16: return pick_u<16>(u, pv);
-        case 32: return pick_u<32>(u, pv);
-        default: return pick_u<8>(u, pv);
+        case 4: return pick_u<4>(u, pv, gs);
+        case 16: return pick_u<16>(u, pv, gs);
+        case 32: return pick_u<32>(u, pv, gs);
+        default: return pick_u<8>(u, pv, gs);
     }
 }
 
@@ -907,10 +952,11 @@ namespace {
 
 // W64: 64-bit byte offsets into L, in the default shape only (4 lanes per
 // list, one load in flight).
-WedgeFn wedge_fn(const btc_count_opts& o, bool pv, bool w64, unsigned* grid) {
+WedgeFn wedge_fn(const btc_count_opts& o, bool pv, bool w64, unsigned* grid, bool gs = false) {
     const int G = o.group_lanes ? o.group_lanes : 4;
     const int U = o.unroll ? o.unroll : 1;
-    WedgeFn fn = w64 ? (pv ? wedge_kernel<4, 1, true, true> : wedge_kernel<4, 1, false, true>) : pick_wedge(G, U, pv);
+    WedgeFn fn = w64 ? (pv ? wedge_kernel<4, 1, true, true, false> : wedge_kernel<4, 1, false, true, false>)
+                     : pick_wedge(G, U, pv, gs);
     int per_cu = 0;
     BTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kThreads, 0));
     per_cu = std::max(1, per_cu);
@@ -927,8 +973,12 @@ bool wide_offsets(const btc_plan_s* p) {
 
 PlanDev plan_dev(btc_plan_s* p, unsigned long long* d_total, PvOut d_pv) {
     return PlanDev{p->g->d_col.p, p->col16.p, p->list_p, p->rowbits_p, p->urow_p, p->items_p, p->n_block_items, p->nitems,
-                   d_total, d_pv, p->core_ids / 128};
+                   d_total, d_pv, p->core_ids / 128, p->item_graph_sums};
 }
+
+// Whether the wedge and tiny kernels add each item's count to its graph's
+// sum (a batch's plan; not with 64-bit offsets, which keep per-vertex sums).
+bool graph_item_sums(const btc_plan_s* p) { return p->item_graph_sums && !wide_offsets(p); }
 
 // Items of one class (n of them) that fall to `share` of `shares`.
 uint32_t share_count(uint32_t n, uint32_t share, uint32_t shares) {
@@ -942,7 +992,9 @@ void launch_wedge(btc_plan_s* p, unsigned long long* d_total, uint32_t* d_counte
     const uint32_t nw = share_count(p->nitems - p->n_tiny_items - p->n_block_items, share, shares);
     if (nb + nw == 0) return;
     unsigned grid = 0;
-    WedgeFn fn = wedge_fn(p->launch, d_pv.on(), wide_offsets(p), &grid);
+    const bool gs = graph_item_sums(p);
+    if (gs) d_pv = PvOut{};
+    WedgeFn fn = wedge_fn(p->launch, d_pv.on(), wide_offsets(p), &grid, gs);
     WedgeArgs a{plan_dev(p, d_total, d_pv), nullptr, nullptr, 1u, nb, nb + nw, d_counters, (int)p->launch.lookup,
                 (int)p->launch.schedule, share, shares, d_slots};
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
@@ -953,8 +1005,11 @@ void launch_tiny(btc_plan_s* p, unsigned long long* d_total, PvOut d_pv, hipStre
                  uint32_t shares = 1, unsigned long long* d_slots = nullptr) {
     const uint32_t n = share_count(p->n_tiny_items, share, shares);
     if (!n) return;
-    auto fn = wide_offsets(p) ? (d_pv.on() ? tiny_kernel<true, true> : tiny_kernel<false, true>)
-                              : (d_pv.on() ? tiny_kernel<true, false> : tiny_kernel<false, false>);
+    const bool gs = graph_item_sums(p);
+    if (gs) d_pv = PvOut{};
+    auto fn = wide_offsets(p) ? (d_pv.on() ? tiny_kernel<true, true, false> : tiny_kernel<false, true, false>)
+                              : (d_pv.on() ? tiny_kernel<true, false, false>
+                                           : (gs ? tiny_kernel<false, false, true> : tiny_kernel<false, false, false>));
     int per_cu = 0;
     BTC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kThreads, 0));
     constexpr uint32_t kGroups = kThreads / kTinyLanes;

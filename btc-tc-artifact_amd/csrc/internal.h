@@ -43,6 +43,9 @@ struct btc_plan_s {
     btcx::DevBuf<unsigned char> lists;  // items, list records, row core bitmaps, list rows
     btcx::DevBuf<uint4> items_dev;      // work items of a device-built plan (kept apart from lists)
     const uint4* items_p = nullptr;
+    // a batch's per-graph sums: the wedge and tiny kernels add each item's
+    // count at its graph index (the item's B.w; see batch.hip)
+    unsigned long long* item_graph_sums = nullptr;
     const uint4* list_p = nullptr;
     const uint4* rowbits_p = nullptr;
     const int32_t* urow_p = nullptr;
