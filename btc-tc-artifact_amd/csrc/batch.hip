@@ -116,9 +116,22 @@ void batch_item_graphs(btc_batch_s* b, hipStream_t s) {
 
 namespace btcx {
 
+// A batch's plan options: opts (or the defaults) with tiny_items 0 taken as
+// 2, since the pivots of small graphs hold a few short lists each and a
+// wave then waits on one pivot's round trips where a tiny-item group of 8
+// lanes keeps 8 pivots in flight (4096-graph workload: 6.60 -> 6.23 ms).
+btc_count_opts batch_opts(const btc_count_opts* opts) {
+    btc_count_opts o;
+    if (opts) o = *opts;
+    else btc_count_opts_default(&o);
+    if (o.tiny_items == 0) o.tiny_items = 2;
+    return o;
+}
+
 // Plans the batch's block-diagonal graph and allocates its sums.
 void batch_finish(btc_batch_s* b, const btc_count_opts* opts, hipStream_t s) {
-    check_status(btc_plan_create(&b->graph, opts, s, &b->plan));
+    const btc_count_opts o = batch_opts(opts);
+    check_status(btc_plan_create(&b->graph, &o, s, &b->plan));
     std::vector<uint32_t> vo32(b->voff.begin(), b->voff.end());
     b->d_voff.alloc(vo32.size());
     b->d_pv.alloc((size_t)b->graph.n, b->graph.max_row);
@@ -298,7 +311,8 @@ int btc_batch_autotune(btc_batch b, const btc_tune_opts* topts, void* stream, bt
         btcx::check_status(btc_autotune(&b->graph, topts, stream, best, best_ms));
         // re-plan the batch with the winner (tile width and tau need a new plan)
         btc_plan p = nullptr;
-        btcx::check_status(btc_plan_create(&b->graph, best, stream, &p));
+        const btc_count_opts o = btcx::batch_opts(best);
+        btcx::check_status(btc_plan_create(&b->graph, &o, stream, &p));
         btc_plan_free(b->plan);
         b->plan = p;
         btcx::batch_item_graphs(b, btcx::as_stream(stream));

@@ -868,7 +868,7 @@ void build_wedge_lists(btc_plan_s* p, const std::vector<uint8_t>& dense, int64_t
         };
         const uint32_t nl = (uint32_t)std::min<int64_t>(e1 - e, 0xFFFFFFFFLL);
         const bool small = wave_pivot(nl, lo, hi, t1 - t0);
-        const bool tiny = p->use_tiny && small && tiny_pivot(nl, lo, hi, t1 - t0);
+        const bool tiny = p->use_tiny && small && tiny_pivot(nl, lo, hi, t1 - t0, p->tiny_all);
         while (e < e1) {
             int64_t w = 0, f = e;
             while (f < e1 && (small || w == 0 || w + cost(f) <= chunk)) {
@@ -1278,9 +1278,10 @@ btc_plan_s* plan_create(btc_graph g, const btc_count_opts* opts, hipStream_t s, 
                 BTC_ERR_ARG, "core_ids %d (expected a multiple of 128 up to %u, 0 = default, < 0 = none)",
                 o.core_ids, btcx::kCoreIdsMax);
     p->core_ids = o.core_ids == 0 ? btcx::kCoreIdsDefault : o.core_ids < 0 ? 0u : (uint32_t)o.core_ids;
-    BTC_REQUIRE(o.tiny_items >= -1 && o.tiny_items <= 1, BTC_ERR_ARG,
-                "tiny_items %d (expected -1, 0 or 1)", o.tiny_items);
+    BTC_REQUIRE(o.tiny_items >= -1 && o.tiny_items <= 2, BTC_ERR_ARG,
+                "tiny_items %d (expected -1, 0, 1 or 2)", o.tiny_items);
     p->use_tiny = o.tiny_items > 0 || (o.tiny_items == 0 && g->wedges >= btcx::kTinyMinWedges);
+    p->tiny_all = o.tiny_items == 2;
     btcx::check_launch_opts(o);
     p->launch = o;
     btcx::ensure_device(g, s);

@@ -29,6 +29,7 @@ struct btc_plan_s {
     int tau = 0;
     uint32_t core_ids = 0;  // hub-core id range C of the row bitmaps (0 = none)
     bool use_tiny = false;  // tiny pivots become tiny items (btc_count_opts.tiny_items)
+    bool tiny_all = false;  // tiny_items 2: every small pivot of at most 64 ids, whatever its spread
     btc_count_opts launch{};  // launch-only options (btc_plan_tune)
     // Bit-BSR of width K read by the dense-tile kernel (NULL when no tile is
     // dense): the plan's own, or a caller's Bit-BSR of g that must outlive
@@ -147,8 +148,11 @@ constexpr int kTinyLanes = 8;
 // 3.03 vs 3.07 ms); on planted 2^17 (9.2e7 wedges) it added 0.36 ms to a
 // 0.14 ms wedge kernel.
 constexpr int64_t kTinyMinWedges = int64_t(1) << 30;
-__host__ __device__ inline bool tiny_pivot(uint32_t lists, uint32_t lo, uint32_t hi, uint32_t d) {
-    return lists <= kWaveItemLists && hi - (lo & ~127u) >= 32u * kWaveWords - 1u && d <= hash_capacity(kTinyWords);
+// all: every such pivot, whatever the spread of its ids (batches of small
+// graphs, whose wave items hold a few short lists each).
+__host__ __device__ inline bool tiny_pivot(uint32_t lists, uint32_t lo, uint32_t hi, uint32_t d, bool all = false) {
+    return lists <= kWaveItemLists && (all || hi - (lo & ~127u) >= 32u * kWaveWords - 1u) &&
+           d <= hash_capacity(kTinyWords);
 }
 
 // A batch over an already block-diagonal canonical graph whose graphs

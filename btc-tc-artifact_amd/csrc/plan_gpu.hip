@@ -266,7 +266,7 @@ __global__ void item_start_kernel(const uint32_t* __restrict__ rowptr, const int
 __global__ void items_kernel(const uint32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                              const uint32_t* __restrict__ key, const uint32_t* __restrict__ off,
                              const uint64_t* __restrict__ P, const uint32_t* __restrict__ first, Cnt nitems_,
-                             Cnt ne_, bool use_tiny, bool coarse, uint4* __restrict__ tmp_items,
+                             Cnt ne_, bool use_tiny, bool tiny_all, bool coarse, uint4* __restrict__ tmp_items,
                              uint32_t* __restrict__ skey, uint32_t* __restrict__ sval,
                              unsigned long long* __restrict__ stats) {
     const uint32_t nitems = (uint32_t)nitems_.get();
@@ -280,7 +280,7 @@ __global__ void items_kernel(const uint32_t* __restrict__ rowptr, const int32_t*
         const uint32_t w32 = (uint32_t)min(w, (uint64_t)0xFFFFFFFFu);
         const uint32_t lo = (uint32_t)col[t0], hi = (uint32_t)col[t1 - 1];
         const bool small = pivot_small(rowptr, col, off, j);
-        const bool tiny = use_tiny && small && tiny_pivot(off[j + 1] - off[j], lo, hi, t1 - t0);
+        const bool tiny = use_tiny && small && tiny_pivot(off[j + 1] - off[j], lo, hi, t1 - t0, tiny_all);
         tmp_items[2 * k] = make_uint4(t0, t1, eb, ee);
         tmp_items[2 * k + 1] = make_uint4(lo, hi, w32, 0u);
         const uint32_t cls = tiny ? 2u : small ? 1u : 0u;
@@ -476,7 +476,7 @@ void plan_device(btc_plan_s* p, int64_t chunk, hipStream_t s) {
     sv1.alloc(nitems, s);
     if (nitems)
         BTC_LAUNCH_RED(items_kernel, nitems, s, rowptr, col, key, off.p, P.p, first.p, nitems, ne,
-                   p->use_tiny, false, tmp.p, sk0.p, sv0.p, st.p + 4);
+                   p->use_tiny, p->tiny_all, false, tmp.p, sk0.p, sv0.p, st.p + 4);
     const bool ialt = sort_pairs_u32(sk0.p, sv0.p, sk1.p, sv1.p, nitems, 32, s);
     p->items_dev.alloc((size_t)nitems * 2);
     if (nitems)
@@ -567,7 +567,7 @@ void plan_capacity(const PlanCap& c, hipStream_t s) {
                (uint64_t)64 * (uint64_t)c.cus, c.start);
     dscan_u32(c.start, c.spos, (size_t)M, &ctl->ne, &ctl->nitems, c.ws, s);
     CAP_LAUNCH(select_kernel, M, s, c.start, c.spos, ne, c.first);
-    CAP_LAUNCH_RED(items_kernel, I, s, rowptr, col, key, c.off, c.P, c.first, nitems, ne, false, true, c.tmp_items,
+    CAP_LAUNCH_RED(items_kernel, I, s, rowptr, col, key, c.off, c.P, c.first, nitems, ne, false, false, true, c.tmp_items,
                c.sk0, c.sv0, st + 4);
     const bool ialt = dsort_pairs_u32(c.sk0, c.sv0, c.sk1, c.sv1, (size_t)I, &ctl->nitems, 8, c.ws, s);
     CAP_LAUNCH(place_items_kernel, I, s, c.tmp_items, ialt ? c.sv1 : c.sv0, nitems, c.items, ctl, c.work);

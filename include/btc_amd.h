@@ -215,8 +215,9 @@ typedef struct {
                              32768; 0 = 2048, < 0 = no core bitmaps */
     int32_t tiny_items;   /* pivots with few lists, at most 64 ids and ids spread wider than a wave's
                              bitmap window counted by groups of 8 lanes in their own kernel: 0 = auto
-                             (graphs with at least 2^30 wedges), 1 = always, -1 = never (they stay
-                             wave items) */
+                             (graphs with at least 2^30 wedges; for batches 2), 1 = always, 2 = every
+                             pivot with few lists and at most 64 ids, whatever their spread, -1 =
+                             never (they stay wave items) */
     int32_t wide_offsets; /* 1 = the wedge kernel addresses L with 64-bit byte offsets (in its default
                              shape); 0 = only when L holds 2^30 ids or more, where 32-bit offsets end */
 } btc_count_opts;

@@ -256,6 +256,29 @@ def test_engine_tiny_pivot_items(gpu, orc, planner):
 
 
 @pytest.mark.parametrize("planner", [1, 2])
+def test_engine_tiny_items_all_small_pivots(gpu, orc, planner):
+    """tiny_items 2 (the batches' default) gives every pivot with few lists
+    and at most 64 ids to the tiny kernel, whatever the spread of its ids:
+    more tiny items than tiny_items 1, and the same totals and per-vertex
+    counts as the oracle, alone, in shares and replayed."""
+    for s, d, n in (gpu.gen_planted(1 << 17, 128, 0.5, 4.0, seed=11) + (1 << 17,),
+                    gpu.gen_rmat(16, 8, seed=16) + (1 << 16,)):
+        g = gpu.Graph.from_coo(s, d, n)
+        want, want_pv = oracle_counts(orc, g)
+        p1 = gpu.Plan(g, planner=planner, tiny_items=1)
+        p2 = gpu.Plan(g, planner=planner, tiny_items=2)
+        assert p2.info["tiny_items"] > p1.info["tiny_items"]
+        assert p2.info["work_items"] == p1.info["work_items"]
+        t, pv, _ = p2.count(per_vertex=True)
+        assert t == want
+        np.testing.assert_array_equal(pv, want_pv)
+        assert sum(p2.count_share(3, j)[0] for j in range(3)) == want
+        p2.capture()
+        for _ in range(2):
+            assert p2.count()[0] == want
+
+
+@pytest.mark.parametrize("planner", [1, 2])
 @pytest.mark.parametrize("core_ids", [-1, 128, 1024, 8192, 32768])
 def test_engine_core_size(gpu, orc, core_ids, planner):
     """Any hub-core size (none, one 128-id block, up to a wave's window span)
@@ -280,7 +303,7 @@ def test_engine_core_size_rejects(gpu):
     for bad in (100, 65536, 32768 + 128):
         with pytest.raises(gpu.BtcError):
             gpu.Plan(g, core_ids=bad)
-    for bad in (-2, 2):
+    for bad in (-2, 3):
         with pytest.raises(gpu.BtcError):
             gpu.Plan(g, tiny_items=bad)
 
