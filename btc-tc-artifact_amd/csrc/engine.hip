@@ -591,7 +591,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 // Tiny-pivot items [first, first + share + t * shares) of one plan (t < n):
 // each group of kTinyLanes lanes takes one item with a kTinyWords-slot LDS
 // hash table of its own (static stride over the work-sorted items), so a
-// wave has 8 pivots' HBM round trips in flight where a wave item has one.
+// wave has 8 pivots' HBM round trips in flight where a wave item has one;
+// within a group, kTinyListLanes lanes stream each prefix list, so a
+// group's lists (a few short ones each) run 4 at a time.
+// kTinyListLanes 2 against 8 (one list at a time): batched workload 6.22 ->
+// 4.38 ms (1 lane: 4.39), R-MAT s22 count 2.751 -> 2.744 ms (1 lane: 2.759).
+constexpr int kTinyListLanes = 2;
 // The groups of a wave run different items; every step of process_item
 // for a team of kTinyLanes lanes stays within the group.
 template <bool PV, bool W64, bool GS>
@@ -609,7 +614,8 @@ __global__ __launch_bounds__(kThreads) void tiny_kernel(PlanDev P, int lookup, u
     for (uint32_t t = blockIdx.x * kGroups + grp; t < n; t += gridDim.x * kGroups) {
         const uint32_t li = first + share + t * shares;
         const uint4 A = P.items[2 * li], B = P.items[2 * li + 1];
-        const uint64_t v = process_item<kTinyLanes, kTinyWords, kTinyLanes, 1, PV, W64>(P, lookup, A, B, table, gl);
+        const uint64_t v =
+            process_item<kTinyLanes, kTinyWords, kTinyListLanes, 1, PV, W64>(P, lookup, A, B, table, gl);
         acc += v;
         if constexpr (GS) add_item_graph<kTinyLanes>(v, P.gsum, B.w, nullptr, tid);
     }
