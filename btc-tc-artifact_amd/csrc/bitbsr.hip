@@ -247,15 +247,25 @@ __global__ __launch_bounds__(256) void build_kernel(int n, int n_rb, const uint3
     }
     uint64_t t = (FILL && valid) ? offs[I] : 0;
     uint32_t cnt = 0;
+    // the row's next four ids (kNoId past its end), refilled one load
+    // ahead per consumed id so a run of ids in one tile waits on memory
+    // once every four ids instead of once per id
+    constexpr uint32_t kNoId = 0xFFFFFFFFu;
+    auto at = [&](uint32_t q) { return q < e ? (uint32_t)col[q] : kNoId; };
+    uint32_t b0 = at(p), b1 = at(p + 1), b2 = at(p + 2), b3 = at(p + 3);
     for (;;) {
-        uint32_t nxt = p < e ? (uint32_t)col[p] / K : kNoTile;
+        uint32_t nxt = b0 != kNoId ? b0 / K : kNoTile;
         uint32_t cur = group_min_u32(nxt, 16);
         if (__all(cur == kNoTile)) break;
         if (cur != kNoTile) {
             RowBits<K> bits;
-            while (p < e && (uint32_t)col[p] / K == cur) {
-                bits.set((int)((uint32_t)col[p] % K));
+            while (b0 != kNoId && b0 / K == cur) {
+                bits.set((int)(b0 % K));
                 ++p;
+                b0 = b1;
+                b1 = b2;
+                b2 = b3;
+                b3 = at(p + 3);
             }
             if (FILL) {
                 if (r == 0) {
