@@ -1049,16 +1049,24 @@ __global__ __launch_bounds__(256) void dense_unit_kernel(int n, const uint32_t* 
     }
 }
 
+// pop[t] = set bits of tile t: one 64-bit word per lane, 64 / (K / 4)
+// tiles per wave, summed over each tile's lanes; waves stride over the
+// words (a wave's base is uniform, so every lane of a tile is present).
 template <int K>
 __global__ __launch_bounds__(256) void tile_popcount_kernel(const uint64_t* __restrict__ tiles, uint32_t nblk,
                                                              uint32_t* __restrict__ pop) {
     constexpr int WPT = K / 4;
+    static_assert(64 % WPT == 0, "whole tiles per wave");
     const int lane = threadIdx.x & 63;
-    const uint32_t w = uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (w >= nblk) return;
-    uint32_t c = lane < WPT ? (uint32_t)__popcll(tiles[(size_t)w * WPT + lane]) : 0u;
-    c = wave_sum_u32(c);
-    if (lane == 0) pop[w] = c;
+    const uint64_t nw = (uint64_t)nblk * WPT;
+    for (uint64_t base = ((uint64_t)blockIdx.x * 256 + (threadIdx.x & ~63u)); base < nw;
+         base += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = base + (uint64_t)lane;
+        uint32_t c = i < nw ? (uint32_t)__popcll(tiles[i]) : 0u;
+#pragma unroll
+        for (int o = WPT / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, WPT);
+        if (lane % WPT == 0 && i < nw) pop[i / WPT] = c;
+    }
 }
 
 // A dispatch holds at most 2^32 work-items, so long tile lists are split
@@ -1238,7 +1246,8 @@ void launch_zero_u32(uint32_t* p, size_t n, hipStream_t s) {
 
 void launch_tile_popcount(const btc_bitbsr_s* b, uint32_t* d_pop, hipStream_t s) {
     if (b->nblk == 0) return;
-    const unsigned grid = (b->nblk + 3) / 4;
+    const uint64_t words = (uint64_t)b->nblk * (uint64_t)(b->K / 4);
+    const unsigned grid = (unsigned)std::min<uint64_t>((words + 255) / 256, 1u << 16);
     switch (b->K) {
         case 32: hipLaunchKernelGGL(tile_popcount_kernel<32>, dim3(grid), dim3(256), 0, s, b->tiles.p, b->nblk, d_pop); break;
         case 64: hipLaunchKernelGGL(tile_popcount_kernel<64>, dim3(grid), dim3(256), 0, s, b->tiles.p, b->nblk, d_pop); break;
