@@ -112,6 +112,21 @@ void batch_item_graphs(btc_batch_s* b, hipStream_t s) {
     p->item_graph_sums = b->d_gsum.p;
 }
 
+// Best of three timed batch counts on s (after one untimed), in ms.
+float batch_count_ms(btc_batch_s* b, hipStream_t s) {
+    float best = 1e30f;
+    for (int r = 0; r < 4; ++r) {
+        EventTimer tm(true);
+        launch_zero_u32(reinterpret_cast<uint32_t*>(b->d_pv.data()), b->d_pv.bytes() / 4, s);
+        tm.start(s);
+        b->enqueue_count(s);
+        tm.stop(s);
+        const float ms = tm.ms();
+        if (r) best = std::min(best, ms);
+    }
+    return best;
+}
+
 }  // namespace btcx
 
 namespace btcx {
@@ -309,15 +324,28 @@ int btc_batch_autotune(btc_batch b, const btc_tune_opts* topts, void* stream, bt
     return btcx::guard([&] {
         BTC_REQUIRE(b && best, BTC_ERR_ARG, "NULL argument");
         btcx::check_status(btc_autotune(&b->graph, topts, stream, best, best_ms));
-        // re-plan the batch with the winner (tile width and tau need a new plan)
+        // re-plan the batch with the winner (tile width and tau need a new
+        // plan); the tuner times counts of the whole block-diagonal graph,
+        // so the batch's own counts decide whether the new plan is kept
+        hipStream_t s = btcx::as_stream(stream);
+        b->count_graph.reset();
+        const float before = btcx::batch_count_ms(b, s);
+        btc_plan old = b->plan;
         btc_plan p = nullptr;
         const btc_count_opts o = btcx::batch_opts(best);
         btcx::check_status(btc_plan_create(&b->graph, &o, stream, &p));
-        btc_plan_free(b->plan);
         b->plan = p;
-        btcx::batch_item_graphs(b, btcx::as_stream(stream));
-        BTC_HIP(hipStreamSynchronize(btcx::as_stream(stream)));
-        b->count_graph.reset();
+        btcx::batch_item_graphs(b, s);
+        const float after = btcx::batch_count_ms(b, s);
+        if (after <= before) {
+            btc_plan_free(old);
+        } else {  // the plan the batch had counts faster: keep it
+            btc_plan_free(p);
+            b->plan = old;
+            btcx::batch_item_graphs(b, s);
+            *best = old->launch;
+        }
+        BTC_HIP(hipStreamSynchronize(s));
     });
 }
 

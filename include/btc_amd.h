@@ -357,7 +357,9 @@ int btc_batch_tune(btc_batch b, const btc_count_opts* opts);
 /* per_graph: host uint64[count] or NULL; total: sum over the batch. */
 int btc_batch_count(btc_batch b, void* stream, uint64_t* per_graph, uint64_t* total, float* kernel_ms);
 /* Runs btc_autotune on the batch's block-diagonal graph and re-plans the
- * batch with the winning options (returned in best). */
+ * batch with the winning options (returned in best), unless the batch's
+ * current plan counts the batch faster (timed batch counts of both); best
+ * then returns the current plan's options. */
 int btc_batch_autotune(btc_batch b, const btc_tune_opts* topts, void* stream, btc_count_opts* best,
                        float* best_ms);
 /* Captures a batch count (zero fill, kernels, per-graph sums) into a HIP
