@@ -31,6 +31,32 @@ def test_batch_workload_matches_oracle(gpu, orc):
         assert np.array_equal(b.count()[1], want)
 
 
+def test_batch_item_sums_and_per_vertex_paths(gpu, orc):
+    """A batch's wedge and tiny kernels add each work item's count to its
+    graph (the item records carry the graph index); with 64-bit offsets they
+    fall back to per-vertex sums.  Both give the oracle's per-graph counts,
+    with every tiny-item rule, launch shape and schedule, eagerly and from a
+    captured graph, and the autotuner's choice changes none of them."""
+    work = gpu.batch_workload(count=64, seed=23, n_lo=100, n_hi=5000)
+    gs = [gpu.Graph.from_coo(s, d, n) for s, d, n, _ in work]
+    want = np.array([oracle_total(orc, g) for g in gs], np.uint64)
+    for tiny in (-1, 1, 2, 0):
+        b = gpu.Batch(gs, tiny_items=tiny)
+        for opts in ({}, {"wide_offsets": 1}, {"group_lanes": 8, "schedule": 0}, {"wide_offsets": 0}):
+            if opts:
+                b.tune(**opts)
+            for _ in range(2):  # item sums are cleared by each count
+                total, per, _ = b.count()
+                np.testing.assert_array_equal(per, want)
+                assert total == int(want.sum())
+        b.capture()
+        for _ in range(3):
+            np.testing.assert_array_equal(b.count()[1], want)
+    b.autotune(budget_ms=300)
+    for _ in range(2):
+        np.testing.assert_array_equal(b.count()[1], want)
+
+
 @pytest.mark.parametrize("tau", [-1, 0, 100000])
 def test_batch_mixed_extremes(gpu, orc, tau):
     """Empty graphs, a single triangle, cliques, stars and a dense planted
