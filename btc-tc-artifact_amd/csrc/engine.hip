@@ -1059,6 +1059,30 @@ __global__ __launch_bounds__(kThreads) void segment_sums_kernel(const unsigned l
     if (threadIdx.x == 0) per_graph[g] = x;
 }
 
+// A several-graph run's per-graph sums, cleared before its count adds into
+// them: per_graph[g] = 0 for g < max_graphs.
+__global__ void clear_graph_sums_kernel(unsigned long long* __restrict__ per_graph, uint32_t max_graphs) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < max_graphs; g += gridDim.x * blockDim.x)
+        per_graph[g] = 0;
+}
+
+// Each of the run's items (ctl->nitems) gets its graph in B.w: the last g <
+// ctl->graphs whose rows start at or before the item's pivot row in L.
+__global__ void run_item_graphs_kernel(uint4* __restrict__ items, const uint32_t* __restrict__ rowptr,
+                                       const uint32_t* __restrict__ csegs, const PipeCtl* __restrict__ c) {
+    const uint32_t n = c->nitems, graphs = c->graphs;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t t0 = items[2 * k].x;
+        uint32_t lo = 0, hi = graphs;  // rowptr[csegs[lo]] <= t0 < rowptr[csegs[hi]]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (rowptr[csegs[mid]] <= t0) lo = mid;
+            else hi = mid;
+        }
+        items[2 * k + 1].w = lo;
+    }
+}
+
 }  // namespace
 
 void pipeline_count_enqueue(const PlanCap& c, const uint16_t* col16, const btc_count_opts& o, hipStream_t s,
@@ -1066,24 +1090,34 @@ void pipeline_count_enqueue(const PlanCap& c, const uint16_t* col16, const btc_c
                             int max_graphs) {
     const int64_t span = c.m_cap + (int64_t)kColPad;
     const bool w64 = o.wide_offsets != 0 || (uint64_t)span >= (uint64_t(1) << 30);
+    // several graphs: each item's count goes to its graph (the item records
+    // get their graph first), or, with 64-bit offsets, per vertex and then
+    // summed over each graph's vertex range
+    const bool gs = pv && !w64;
     PvOut pvo{};
-    if (pv) {
+    if (gs) {
+        hipLaunchKernelGGL(clear_graph_sums_kernel, dim3(1), dim3(kThreads), 0, s, per_graph, (uint32_t)max_graphs);
+        BTC_CHECK_LAUNCH("clear_graph_sums_kernel");
+        const unsigned g = (unsigned)std::min<int64_t>(((int64_t)c.items_cap + kThreads - 1) / kThreads, 2048);
+        hipLaunchKernelGGL(run_item_graphs_kernel, dim3(g), dim3(kThreads), 0, s, c.items, c.rowptr, csegs, c.ctl);
+        BTC_CHECK_LAUNCH("run_item_graphs_kernel");
+    } else if (pv) {
         pvo.p64 = pv;
         const unsigned g = (unsigned)std::min<int64_t>((c.n_cap + kThreads - 1) / kThreads, 2048);
         hipLaunchKernelGGL(zero_pv_kernel, dim3(g), dim3(kThreads), 0, s, pv, c.ctl);
         BTC_CHECK_LAUNCH("zero_pv_kernel");
     }
     unsigned grid = 0;
-    WedgeFn fn = wedge_fn(o, pvo.on(), w64, &grid);
+    WedgeFn fn = wedge_fn(o, pvo.on(), w64, &grid, gs);
     unsigned long long* work = c.work;
     const PlanDev pd{c.col, col16, c.lists, reinterpret_cast<const uint4*>(c.rowbits), c.ur, c.items, 0u, 0u,
-                     work, pvo, c.core_ids / 128};
+                     work, pvo, c.core_ids / 128, gs ? per_graph : nullptr};
     WedgeArgs a{pd, nullptr, nullptr, 1u, 0u, 0u, reinterpret_cast<uint32_t*>(work + 1), (int)o.lookup,
                 (int)o.schedule, 0u, 1u, work + kWorkSlots, c.ctl->item_counts};
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), 0, s, a);
     BTC_CHECK_LAUNCH("wedge_kernel");
     launch_slot_sum(work + kWorkSlots, work, s);
-    if (pv) {
+    if (pv && !gs) {
         hipLaunchKernelGGL(segment_sums_kernel, dim3((unsigned)max_graphs), dim3(kThreads), 0, s, pv, csegs, c.ctl,
                            per_graph);
         BTC_CHECK_LAUNCH("segment_sums_kernel");

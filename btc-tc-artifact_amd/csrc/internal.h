@@ -309,10 +309,12 @@ void check_launch_opts(const btc_count_opts& o);
 // Enqueues the count of a plan_capacity plan (whose last kernel cleared the
 // total and cursors at c.work; the slots are left cleared by every count):
 // the wedge kernel on the item counts in ctl, then the slot sum into
-// c.work[0].  col16 is L's 16-bit copy.  With pv set (several graphs), the
-// per-vertex sums go to pv (cleared first over ctl->nc vertices) and
-// per_graph[g] gets the sum over vertices [csegs[g], csegs[g + 1]) for g <
-// ctl->graphs (launched over max_graphs workgroups).
+// c.work[0].  col16 is L's 16-bit copy.  With pv set (several graphs),
+// per_graph[g] (g < max_graphs, cleared first) gets graph g's count: each
+// work item records its graph (the graph whose vertices [csegs[g],
+// csegs[g + 1]) hold its pivot) and the wedge kernel adds the item's count
+// there; with 64-bit offsets the per-vertex sums go to pv (cleared over
+// ctl->nc vertices) and are summed per graph instead.
 void pipeline_count_enqueue(const PlanCap& c, const uint16_t* col16, const btc_count_opts& o, hipStream_t s,
                             unsigned long long* pv = nullptr, const uint32_t* csegs = nullptr,
                             unsigned long long* per_graph = nullptr, int max_graphs = 1);
