@@ -227,6 +227,22 @@ def test_pipeline_batch_matches_oracle(gpu, orc, flags):
     assert p.count(s, d, n)[0] == want[1]
 
 
+@pytest.mark.parametrize("opts", [{"wide_offsets": 1}, {"group_lanes": 8, "schedule": 0}])
+def test_pipeline_batch_sum_paths(gpu, orc, opts):
+    """Per-graph counts of a several-graph run through the per-vertex
+    fallback (64-bit offsets) and through item sums with another launch
+    shape equal the oracle's, replay after replay and after export."""
+    p = gpu.Pipeline(1 << 16, 1 << 20, 3, max_graphs=8, **opts)
+    pool = group_graphs(gpu, 10, seed=31)
+    want = [oracle_counts(orc, gpu.Graph.from_coo(s, d, n))[0] for s, d, n in pool]
+    for pick in ((0, 1, 2, 3, 4, 5, 6, 7), (9, 8), (3,), (2, 4, 6, 8, 9)):
+        t, per, _ = p.count_batch([pool[i] for i in pick])
+        np.testing.assert_array_equal(per, [want[i] for i in pick])
+        assert t == sum(want[i] for i in pick)
+    b = p.export_batch()
+    np.testing.assert_array_equal(b.count()[1], [want[i] for i in (2, 4, 6, 8, 9)])
+
+
 def test_pipeline_batch_refuses_bad_input(gpu):
     p = gpu.Pipeline(4096, 1 << 16, 3, max_graphs=4)
     ok = (np.array([0, 1, 2], np.int32), np.array([1, 2, 0], np.int32), 3)
