@@ -593,13 +593,14 @@ __device__ __forceinline__ void for_each_match(const uint32_t* __restrict__ indi
 // Dense units per wave: lists of at least kDenseRunUnits units (a dense
 // kernel beside the wedge kernel at R-MAT s22) give each wave
 // kDenseRunLong consecutive units, so column sets are reused while I stays
-// and the wedge kernel keeps most of the chip (2.81 ms per count with 16;
-// 2.86 with 4 or 8); shorter lists give each wave half a tile's slabs, at
+// and the wedge kernel keeps most of the chip (s22 count with the 4-lane
+// wedge kernel: 2.727 ms with 8, 2.740 with 16, 2.774 with 32; with the
+// 8-lane one 16 had been best); shorter lists give each wave half a tile's slabs, at
 // least 2 (planted 2^17 with AND + popcount, tiles in descending cost:
 // 1 / 2 / 4 units per wave took 50 / 46 / 40 us at 16 x 128, 39 / 32 / 36 us
 // at 16 x 64, 26 / 23 / 29 us at 16 x 32).
 constexpr uint32_t kDenseRunUnits = 65536;
-constexpr uint32_t kDenseRunLong = 16;
+constexpr uint32_t kDenseRunLong = 8;
 
 // Tile columns of one row block as an LDS bitmap over column blocks with
 // the number of set bits before each word: find(c) is the position of
