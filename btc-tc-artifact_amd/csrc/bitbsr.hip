@@ -594,7 +594,8 @@ __device__ __forceinline__ void for_each_match(const uint32_t* __restrict__ indi
 // kernel beside the wedge kernel at R-MAT s22) give each wave
 // kDenseRunLong consecutive units, so column sets are reused while I stays
 // and the wedge kernel keeps most of the chip (s22 count with the 4-lane
-// wedge kernel: 2.727 ms with 8, 2.740 with 16, 2.774 with 32; with the
+// wedge kernel: 2.727 ms with 8, 2.740 with 16, 2.774 with 32, 2.743
+// with 4; with the
 // 8-lane one 16 had been best); shorter lists give each wave half a tile's slabs, at
 // least 2 (planted 2^17 with AND + popcount, tiles in descending cost:
 // 1 / 2 / 4 units per wave took 50 / 46 / 40 us at 16 x 128, 39 / 32 / 36 us
