@@ -596,7 +596,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 // group's lists (a few short ones each) run 4 at a time.
 // kTinyListLanes 2 against 8 (one list at a time): batched workload 6.22 ->
 // 4.38 ms (1 lane: 4.39), R-MAT s22 count 2.751 -> 2.744 ms (1 lane: 2.759).
+// Each of those lanes keeps kTinyLoads loads in flight (2 against 1: 4.41
+// -> 4.31 ms on the batched workload).
 constexpr int kTinyListLanes = 2;
+constexpr int kTinyLoads = 2;
 // The groups of a wave run different items; every step of process_item
 // for a team of kTinyLanes lanes stays within the group.
 template <bool PV, bool W64, bool GS>
@@ -615,7 +618,7 @@ __global__ __launch_bounds__(kThreads) void tiny_kernel(PlanDev P, int lookup, u
         const uint32_t li = first + share + t * shares;
         const uint4 A = P.items[2 * li], B = P.items[2 * li + 1];
         const uint64_t v =
-            process_item<kTinyLanes, kTinyWords, kTinyListLanes, 1, PV, W64>(P, lookup, A, B, table, gl);
+            process_item<kTinyLanes, kTinyWords, kTinyListLanes, kTinyLoads, PV, W64>(P, lookup, A, B, table, gl);
         acc += v;
         if constexpr (GS) add_item_graph<kTinyLanes>(v, P.gsum, B.w, nullptr, tid);
     }
