@@ -23,8 +23,16 @@ def test_table_roundtrip_and_lookup(btc, tmp_path):
     q = tmp_path / "tune2.txt"
     btc.tune_table_save(q)
     assert q.read_text().split()[:12] == p.read_text().split()[:12]
+    # four lanes per list (the default shape) is a valid entry
+    p4 = tmp_path / "tune4.txt"
+    p4.write_text("btc-tune-3 %d %d %d %d 64 128 4 1 0 1 2048 0.5\n" % key)
+    btc.tune_table_load(p4)
+    assert btc.tune_lookup(g)["group_lanes"] == 4
     bad = tmp_path / "bad.txt"
     bad.write_text("btc-tune-3 1 2 3 4 96 7 8 1 0 1 0 1.0\n")  # K = 96 is not a tile width
+    with pytest.raises(btc.BtcError):
+        btc.tune_table_load(bad)
+    bad.write_text("btc-tune-3 1 2 3 4 64 7 2 1 0 1 0 1.0\n")  # 2 lanes per list is not a shape
     with pytest.raises(btc.BtcError):
         btc.tune_table_load(bad)
     btc.tune_table_clear()
